@@ -1,0 +1,28 @@
+#!/bin/bash
+# Build the reference's own CPU LJ+LK op (tmol_ljlk::ljlk_pose_scores) from the
+# sources where they lie under the reference checkout, into oracle/_ref/.
+# Test infrastructure only: used to cross-check the C oracle and as the CPU
+# baseline in bench.py.  Needs only g++, the installed torch and the headers
+# the reference vendors (tmol/extern: Eigen, moderngpu).  Nothing is copied.
+set -euo pipefail
+R=${TMOL_REFERENCE:-/root/reference}
+HERE=$(cd "$(dirname "$0")" && pwd)
+OUT="$HERE/_ref/tmol_ljlk_ref.so"
+if [ ! -d "$R/tmol/score/ljlk/potentials" ]; then
+  echo "reference checkout not found at $R; skipping" >&2
+  exit 0
+fi
+mkdir -p "$HERE/_ref"
+TI=$(python3 -c "import torch.utils.cpp_extension as c; print(' '.join('-I'+p for p in c.include_paths()))")
+TL=$(python3 -c "import torch, os; print(os.path.join(os.path.dirname(torch.__file__), 'lib'))")
+ABI=$(python3 -c "import torch; print(int(torch._C._GLIBCXX_USE_CXX11_ABI))")
+PYI=$(python3 -c "import sysconfig; print(sysconfig.get_paths()['include'])")
+g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
+  -DTORCH_EXTENSION_NAME=tmol_ljlk_ref \
+  -I"$R" -I"$R/tmol/extern" $TI -I"$PYI" \
+  "$R/tmol/score/ljlk/potentials/compiled.ops.cpp" \
+  "$R/tmol/score/ljlk/potentials/ljlk_pose_score.cpu.cpp" \
+  -L"$TL" -ltorch -ltorch_cpu -lc10 -Wl,-rpath,"$TL" \
+  -o "$OUT.tmp"
+mv "$OUT.tmp" "$OUT"
+echo "built $OUT"

@@ -1,0 +1,238 @@
+"""ctypes driver for the C oracle (liboracle_ljlk.so) -- TEST INFRASTRUCTURE.
+
+Only tests/, ``__graft_entry__.smoke()`` and bench.py's cpu_baseline leg use
+this module, and only as the checker.  ``oracle_pose_scores`` takes the same
+27 arguments as ``ljlk_pose_scores`` (tmol/score/ljlk/potentials/
+compiled.ops.cpp:533-597) and runs the serial CPU restatement.
+"""
+
+import ctypes
+import os
+import subprocess
+
+import numpy
+import torch
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "liboracle_ljlk.so")
+REF_LIB_PATH = os.path.join(HERE, "_ref", "tmol_ljlk_ref.so")
+
+_P = ctypes.POINTER(ctypes.c_int32)
+
+
+class OracleStack(ctypes.Structure):
+    _fields_ = [
+        ("n_poses", ctypes.c_int32),
+        ("max_n_blocks", ctypes.c_int32),
+        ("n_block_types", ctypes.c_int32),
+        ("max_n_block_atoms", ctypes.c_int32),
+        ("max_n_tiles", ctypes.c_int32),
+        ("max_n_conn", ctypes.c_int32),
+        ("rot_coord_offset", _P),
+        ("block_type", _P),
+        ("min_bondsep", _P),
+        ("inter_block_bondsep", _P),
+        ("bt_n_atoms", _P),
+        ("bt_n_heavy_in_tile", _P),
+        ("bt_heavy_in_tile", _P),
+        ("bt_atom_types", _P),
+        ("bt_n_conn", _P),
+        ("bt_conn_atoms", _P),
+        ("bt_path_distance", _P),
+        ("bt_is_ligand", _P),
+    ]
+
+
+_lib = None
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            build()
+        L = ctypes.CDLL(LIB_PATH)
+        for suf, ct in (("_f", ctypes.c_float), ("_d", ctypes.c_double)):
+            PR = ctypes.POINTER(ct)
+            f = getattr(L, "oracle_block_spheres" + suf)
+            f.argtypes = [PR, ctypes.c_int, _P, _P, _P, PR]
+            f = getattr(L, "oracle_block_neighbors" + suf)
+            f.argtypes = [ctypes.c_int, ctypes.c_int, _P, PR, ct, _P]
+            f = getattr(L, "oracle_pose_scores" + suf)
+            f.argtypes = [
+                ctypes.POINTER(OracleStack), PR, ctypes.c_int64, PR, PR, _P,
+                ctypes.c_int, ctypes.c_int, PR, PR,
+            ]
+            f = getattr(L, "oracle_lj_pair" + suf)
+            f.argtypes = [ct, ctypes.c_int, PR, PR, PR, ctypes.c_int, PR]
+            f = getattr(L, "oracle_lk_pair" + suf)
+            f.argtypes = [ct, ctypes.c_int, PR, PR, PR, ctypes.c_int, PR]
+        L.oracle_pair_sep.argtypes = [
+            ctypes.POINTER(OracleStack), ctypes.c_int, ctypes.c_int, ctypes.c_int,
+            ctypes.c_int, ctypes.c_int,
+        ]
+        L.oracle_pair_sep.restype = ctypes.c_int
+        _lib = L
+    return _lib
+
+
+def _np_i32(t):
+    return numpy.ascontiguousarray(t.detach().cpu().numpy().astype(numpy.int32))
+
+
+def _ptr(a, ct):
+    return a.ctypes.data_as(ctypes.POINTER(ct))
+
+
+class _Stack:
+    """Keeps the numpy buffers alive for the struct's pointers."""
+
+    def __init__(self, args):
+        (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+         first_rot_block_type, block_ind_for_rot, pose_ind_for_rot,
+         block_type_ind_for_rot, n_rots_for_pose, rot_offset_for_pose,
+         n_rots_for_block, rot_offset_for_block, max_n_rots_per_pose,
+         min_bondsep, inter_block_bondsep, bt_n_atoms, bt_n_heavy_in_tile,
+         bt_heavy_in_tile, bt_atom_types, bt_n_conn, bt_conn_atoms,
+         bt_path_distance, bt_is_ligand, type_params, global_params, max_dis,
+         block_pair) = args
+        P, B = first_rot_for_block.shape
+        self.arrays = {
+            "rot_coord_offset": _np_i32(rot_coord_offset),
+            "block_type": _np_i32(block_type_ind_for_rot),
+            "min_bondsep": _np_i32(min_bondsep),
+            "inter_block_bondsep": _np_i32(inter_block_bondsep),
+            "bt_n_atoms": _np_i32(bt_n_atoms),
+            "bt_n_heavy_in_tile": _np_i32(bt_n_heavy_in_tile),
+            "bt_heavy_in_tile": _np_i32(bt_heavy_in_tile),
+            "bt_atom_types": _np_i32(bt_atom_types),
+            "bt_n_conn": _np_i32(bt_n_conn),
+            "bt_conn_atoms": _np_i32(bt_conn_atoms),
+            "bt_path_distance": _np_i32(bt_path_distance),
+            "bt_is_ligand": _np_i32(bt_is_ligand),
+        }
+        st = OracleStack()
+        st.n_poses = P
+        st.max_n_blocks = B
+        st.n_block_types = bt_n_atoms.shape[0]
+        st.max_n_block_atoms = bt_atom_types.shape[1]
+        st.max_n_tiles = bt_n_heavy_in_tile.shape[1]
+        st.max_n_conn = bt_conn_atoms.shape[1]
+        for k, v in self.arrays.items():
+            setattr(st, k, _ptr(v, ctypes.c_int32))
+        self.struct = st
+        self.P, self.B = P, B
+        self.coords = coords
+        self.type_params = type_params
+        self.global_params = global_params
+        self.max_dis = float(max_dis)
+        self.block_pair = bool(block_pair)
+
+
+def oracle_pose_scores(*args, require_grad=False, dtype=None):
+    """Run the oracle on the op's 27 arguments.
+
+    Returns (scores, block_neighbors, dV_dx): scores is (3, P) or (3, P, B, B),
+    block_neighbors (P, B, B) int32, dV_dx (3, NA, 3) or None."""
+    S = _Stack(args)
+    L = lib()
+    if dtype is None:
+        dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
+    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+    coords = numpy.ascontiguousarray(S.coords.detach().cpu().numpy().astype(dtype))
+    tp = numpy.ascontiguousarray(S.type_params.detach().cpu().numpy().astype(dtype))
+    gp = numpy.ascontiguousarray(S.global_params.detach().cpu().numpy().astype(dtype).reshape(-1))
+    NA = coords.shape[0]
+    PB = S.P * S.B
+    spheres = numpy.zeros((PB, 4), dtype=dtype)
+    getattr(L, "oracle_block_spheres" + suf)(
+        _ptr(coords, ct), PB, _ptr(S.arrays["rot_coord_offset"], ctypes.c_int32),
+        _ptr(S.arrays["block_type"], ctypes.c_int32),
+        _ptr(S.arrays["bt_n_atoms"], ctypes.c_int32), _ptr(spheres, ct),
+    )
+    nbr = numpy.zeros((S.P, S.B, S.B), dtype=numpy.int32)
+    # reach is the host scalar max_dis cast to the coordinate type
+    getattr(L, "oracle_block_neighbors" + suf)(
+        S.P, S.B, _ptr(S.arrays["block_type"], ctypes.c_int32), _ptr(spheres, ct),
+        ct(dtype(S.max_dis)), _ptr(nbr, ctypes.c_int32),
+    )
+    if S.block_pair:
+        out = numpy.zeros((3, S.P, S.B, S.B), dtype=dtype)
+    else:
+        out = numpy.zeros((3, S.P), dtype=dtype)
+    dvdx = numpy.zeros((3, NA, 3), dtype=dtype) if require_grad else None
+    getattr(L, "oracle_pose_scores" + suf)(
+        ctypes.byref(S.struct), _ptr(coords, ct), NA, _ptr(tp, ct), _ptr(gp, ct),
+        _ptr(nbr, ctypes.c_int32), int(S.block_pair), int(require_grad),
+        _ptr(out, ct), _ptr(dvdx, ct) if dvdx is not None else None,
+    )
+    return out, nbr, dvdx, spheres
+
+
+def oracle_pair_separations(args, items):
+    """Count-pair separations for every atom pair of each (p, b1, b2) item.
+    Returns a list of int arrays [n1, n2] (-1 where not real)."""
+    S = _Stack(args)
+    L = lib()
+    out = []
+    bt_n = S.arrays["bt_n_atoms"]
+    bt = S.arrays["block_type"]
+    for p, b1, b2 in items:
+        t1, t2 = bt[p * S.B + b1], bt[p * S.B + b2]
+        n1, n2 = bt_n[t1], bt_n[t2]
+        m = numpy.zeros((n1, n2), dtype=numpy.int32)
+        for a1 in range(n1):
+            for a2 in range(n2):
+                m[a1, a2] = L.oracle_pair_sep(ctypes.byref(S.struct), p, b1, b2, a1, a2)
+        out.append(m)
+    return out
+
+
+def lj_pair(dist, sep, params_i, params_j, global_params, with_deriv=True,
+            dtype=numpy.float64):
+    """Single LJ pair: returns (atr, rep, datr/dd, drep/dd)."""
+    L = lib()
+    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+    pi = numpy.ascontiguousarray(numpy.asarray(params_i, dtype=dtype))
+    pj = numpy.ascontiguousarray(numpy.asarray(params_j, dtype=dtype))
+    gp = numpy.ascontiguousarray(numpy.asarray(global_params, dtype=dtype).reshape(-1))
+    o = numpy.zeros(4, dtype=dtype)
+    getattr(L, "oracle_lj_pair" + suf)(
+        ct(dist), int(sep), _ptr(pi, ct), _ptr(pj, ct), _ptr(gp, ct), int(with_deriv),
+        _ptr(o, ct),
+    )
+    return o
+
+
+def lk_pair(dist, sep, params_i, params_j, global_params, with_deriv=True,
+            dtype=numpy.float64):
+    """Single LK pair (both directions): returns (V, dV/dd)."""
+    L = lib()
+    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+    pi = numpy.ascontiguousarray(numpy.asarray(params_i, dtype=dtype))
+    pj = numpy.ascontiguousarray(numpy.asarray(params_j, dtype=dtype))
+    gp = numpy.ascontiguousarray(numpy.asarray(global_params, dtype=dtype).reshape(-1))
+    o = numpy.zeros(2, dtype=dtype)
+    getattr(L, "oracle_lk_pair" + suf)(
+        ct(dist), int(sep), _ptr(pi, ct), _ptr(pj, ct), _ptr(gp, ct), int(with_deriv),
+        _ptr(o, ct),
+    )
+    return o
+
+
+def reference_op_available():
+    return os.path.exists(REF_LIB_PATH)
+
+
+def load_reference_op():
+    """Loads the reference's own CPU op built by build_ref.sh; returns
+    ``torch.ops.tmol_ljlk.ljlk_pose_scores``."""
+    if not reference_op_available():
+        raise FileNotFoundError(REF_LIB_PATH)
+    if not hasattr(torch.ops, "tmol_ljlk") or not hasattr(torch.ops.tmol_ljlk, "ljlk_pose_scores"):
+        torch.ops.load_library(REF_LIB_PATH)
+    return torch.ops.tmol_ljlk.ljlk_pose_scores
