@@ -1,0 +1,165 @@
+/* C ABI of the MI355X LJ+LK pose-scoring library (libtmol_ljlk_hip.so).
+ *
+ * Replaces, for one path, the reference's op
+ *   tmol_ljlk::ljlk_pose_scores   tmol/score/ljlk/potentials/compiled.ops.cpp:533-597
+ * and its dispatch
+ *   LJLKPoseScoreDispatch<D,Real,Int>::forward   ljlk_pose_score.impl.hh:633-1158
+ *   LJLKPoseScoreOp::backward (whole pose)       compiled.ops.cpp:157-174
+ *   LJLKPoseScoreDispatch::backward (block pair) ljlk_pose_score.impl.hh:1160-1477
+ *
+ * Plain C: device pointers, sizes, an opaque hipStream_t (void*), int status.
+ * All pointers named in the structs are device pointers unless noted.
+ * No entry point allocates device memory: the caller passes a workspace of
+ * tmol_ljlk_workspace_bytes() bytes.  Everything runs on the given stream;
+ * nothing synchronises except tmol_ljlk_read_counters().
+ */
+#ifndef TMOL_LJLK_H
+#define TMOL_LJLK_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define TMOL_LJLK_OK 0
+#define TMOL_LJLK_EINVAL 1    /* bad sizes / pointers / flags */
+#define TMOL_LJLK_EOVERFLOW 2 /* work list capacity exceeded (see read_counters) */
+#define TMOL_LJLK_EHIP 3      /* HIP runtime error (launch) */
+#define TMOL_LJLK_ENOTSUP 4   /* unsupported configuration */
+
+/* coordinate / parameter dtypes */
+#define TMOL_LJLK_F32 0
+#define TMOL_LJLK_F64 1
+
+/* stack layouts */
+#define TMOL_LJLK_PADDED 0 /* tmol PoseStack: P poses x B blocks, dense count-pair tables */
+#define TMOL_LJLK_RAGGED 1 /* CSR over poses, count pair from chain positions */
+
+/* score flags */
+#define TMOL_LJLK_DVDX 2       /* per-term dV/dx [3, NA, 3] (reference whole-pose forward) */
+#define TMOL_LJLK_WGRAD 4      /* weighted gradient sum_t w[t] dV_t/dx -> [NA, 3] */
+#define TMOL_LJLK_BLOCK_PAIR 8 /* energies per block pair [3, P, B, B]; weights too */
+#define TMOL_LJLK_NO_ENERGY 16 /* skip the energy output (backward re-walk) */
+#define TMOL_LJLK_ACCUMULATE 32 /* add into dvdx / grad instead of overwriting */
+
+/* Per block-type tables: the PackedBlockTypes annotations the reference op
+ * takes (ljlk_energy_term.py:163-178).  Maximum 32 atoms per tile. */
+typedef struct {
+  int32_t n_block_types;          /* T */
+  int32_t max_n_atoms;            /* Amax */
+  int32_t max_n_tiles;            /* nt = (Amax - 1) / 32 + 1 */
+  int32_t max_n_conn;             /* C <= 4 */
+  int32_t n_atom_types;           /* rows of type_params */
+  int32_t dtype;                  /* of type_params / global_params */
+  const int32_t* n_atoms;         /* [T] */
+  const int32_t* n_heavy_in_tile; /* [T, nt] */
+  const int32_t* heavy_in_tile;   /* [T, 32 * nt], -1 padded */
+  const int32_t* atom_types;      /* [T, Amax], -1 padded */
+  const int32_t* n_conn;          /* [T] */
+  const int32_t* conn_atoms;      /* [T, C] */
+  const int32_t* path_distance;   /* [T, Amax, Amax] (ljlk_bond_separation) */
+  const int32_t* is_ligand;       /* [T] */
+  const void* type_params;        /* [n_atom_types, 10] */
+  const void* global_params;      /* [5] */
+} tmol_ljlk_block_types;
+
+/* The systems.  Blocks are addressed by a global index g.
+ *   padded: g = pose * B + block (the reference's "rot" index); G = P * B
+ *   ragged: g in [pose_block_start[p], pose_block_start[p + 1]) */
+typedef struct {
+  int32_t layout;                     /* TMOL_LJLK_PADDED / _RAGGED */
+  int32_t n_poses;                    /* P */
+  int32_t max_n_blocks;               /* padded: B; ragged: max blocks per pose */
+  int32_t n_blocks_total;             /* G */
+  int64_t n_atoms_total;              /* NA (rows of coords) */
+  const int32_t* block_atom_offset;   /* [G] rot_coord_offset */
+  const int32_t* block_type;          /* [G] block type, -1 = padding */
+  const int32_t* block_pose;          /* ragged: [G]; padded: may be NULL */
+  const int32_t* pose_block_start;    /* ragged: [P + 1]; padded: NULL */
+  const int32_t* atom_pose;           /* [NA] pose_ind_for_atom */
+  const int32_t* min_bondsep;         /* padded: [P, B, B] */
+  const int32_t* inter_block_bondsep; /* padded: [P, B, B, C, C] */
+  const int32_t* block_conn_pos;      /* ragged: [G, C] chain position, -1 = none */
+  const int32_t* block_segment;       /* ragged: [G] bonded-segment id */
+} tmol_ljlk_stack;
+
+/* Neighbor-pass tiles: 64 x 64 block tiles of each pose's upper triangle.
+ * Each tile is int32[4] = {row_start_g, col_start_g, pose_end_g, pose}. */
+int64_t tmol_ljlk_count_neighbor_tiles(const int32_t* pose_block_start_host,
+                                       int32_t n_poses);
+int tmol_ljlk_fill_neighbor_tiles(const int32_t* pose_block_start_host,
+                                  int32_t n_poses, int32_t* tiles_host);
+
+/* Workspace bytes for a stack, block-type set and work-list capacity. */
+int64_t tmol_ljlk_workspace_bytes(const tmol_ljlk_stack* stack,
+                                  const tmol_ljlk_block_types* bt,
+                                  int64_t item_capacity, int32_t dtype);
+
+/* Neighbor pass (replaces compute_block_spheres + detect_block_neighbors,
+ * sphere_overlap.impl.hh:109-196, 287-351): bounding sphere per block,
+ * then every b1 <= b2 pair of a pose with d^2 < ((r1 + r2) + max_dis)^2 is
+ * appended to the work list in the workspace.  Also prepares the per-block-
+ * type atom records.  dense_neighbors (padded layout only, may be NULL)
+ * receives the reference's [P, B, B] int32 flags. */
+int tmol_ljlk_neighbors(const tmol_ljlk_stack* stack,
+                        const tmol_ljlk_block_types* bt, const void* coords,
+                        int32_t dtype, double max_dis, const int32_t* tiles,
+                        int64_t n_tiles, void* workspace, int64_t ws_bytes,
+                        int64_t item_capacity, int32_t* dense_neighbors,
+                        void* stream);
+
+/* Score the work list (replaces eval_energies / eval_energies_by_block /
+ * eval_derivs, ljlk_pose_score.impl.hh:803-1119, 1300-1474).
+ *   energies: [3, P] (or [3, P, B, B] with BLOCK_PAIR), dtype; may be NULL
+ *             with NO_ENERGY
+ *   dvdx:     [3, NA, 3] with DVDX
+ *   weights:  [3, P] (or [3, P, B, B]) with WGRAD
+ *   grad:     [NA, 3] with WGRAD
+ * Outputs are overwritten (zero-filled by kernels) unless ACCUMULATE. */
+int tmol_ljlk_score(const tmol_ljlk_stack* stack,
+                    const tmol_ljlk_block_types* bt, const void* coords,
+                    int32_t dtype, void* workspace, int64_t ws_bytes,
+                    int64_t item_capacity, int32_t flags,
+                    const void* weights, void* energies, void* dvdx,
+                    void* grad, int32_t n_wave_groups, void* stream);
+
+/* Whole-pose backward (compiled.ops.cpp:157-174):
+ *   grad[a, k] = sum_t grad_scores[t, atom_pose[a]] * dvdx[t, a, k] */
+int tmol_ljlk_backward_combine(const void* dvdx, const void* grad_scores,
+                               const int32_t* atom_pose, int64_t n_atoms,
+                               int32_t n_poses, int32_t dtype, void* grad,
+                               void* stream);
+
+/* Copy the work-list counters to the host (synchronises the stream):
+ *   counters_host[0] = items found (may exceed capacity)
+ *   counters_host[1] = capacity overflow flag */
+int tmol_ljlk_read_counters(const void* workspace, int64_t* counters_host,
+                            void* stream);
+
+/* Device pointer to the work list (int32 pairs {g1, g2}) inside a workspace. */
+int tmol_ljlk_work_list(const tmol_ljlk_stack* stack,
+                        const tmol_ljlk_block_types* bt, void* workspace,
+                        int64_t item_capacity, int32_t dtype,
+                        int32_t** items_out);
+
+/* Debug: count-pair separation for every atom pair of each listed block pair,
+ * computed by the scoring kernel's own count-pair code (needs the workspace
+ * prepared by tmol_ljlk_neighbors).  out[item, 32 * a1 + a2] (uint8, first
+ * tile of each block; 255 = not a real atom pair).  items: [n_items, 2]
+ * global block indices {g1, g2}, g1 <= g2, same pose. */
+int tmol_ljlk_debug_pair_separations(const tmol_ljlk_stack* stack,
+                                     const tmol_ljlk_block_types* bt,
+                                     const void* workspace, int64_t item_capacity,
+                                     int32_t dtype, const int32_t* items,
+                                     int64_t n_items, uint8_t* out, void* stream);
+
+/* Number of compute units of the current device (autotuner input). */
+int tmol_ljlk_device_cu_count(int32_t* cu_count);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* TMOL_LJLK_H */

@@ -1,0 +1,535 @@
+"""Host side of the MI355X LJ+LK op: ctypes binding of the C ABI
+(include/tmol_ljlk.h), PyTorch op registration and autograd.
+
+Public drop-in: ``ljlk_pose_scores(*27 args) -> [score, block_neighbors]``,
+same arguments, shapes, dtypes and autograd behaviour as
+``torch.ops.tmol_ljlk.ljlk_pose_scores`` (tmol/score/ljlk/potentials/
+compiled.ops.cpp:29-279, 533-597):
+
+* whole-pose mode: ``score`` is ``(3, P)``; the forward computes per-term
+  dV/dx when ``rot_coords.requires_grad`` and the backward weights it by the
+  incoming ``(3, P)`` gradient (compiled.ops.cpp:157-174);
+* block-pair mode: ``score`` is ``(3, P, B, B)``; the backward re-walks the
+  neighbor list with the ``(3, P, B, B)`` gradient as per-block-pair weights
+  (ljlk_pose_score.impl.hh:1160-1477);
+* ``block_neighbors`` is the dense ``(P, B, B)`` int32 flag matrix.
+
+The device work is done only by the HIP library; there is no CPU path: a
+CPU tensor or a missing library raises.  The primitive steps are registered
+as PyTorch ops under ``torch.ops.tmol_amd``.
+"""
+
+import ctypes
+import os
+import threading
+from typing import List, Optional
+
+import torch
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libtmol_ljlk_hip.so")
+
+OK, EINVAL, EOVERFLOW, EHIP, ENOTSUP = 0, 1, 2, 3, 4
+F32, F64 = 0, 1
+PADDED, RAGGED = 0, 1
+DVDX, WGRAD, BLOCK_PAIR, NO_ENERGY, ACCUMULATE = 2, 4, 8, 16, 32
+NBR_TILE = 64
+
+_ERRORS = {
+    EINVAL: "invalid argument",
+    EOVERFLOW: "work list overflow",
+    EHIP: "HIP runtime error",
+    ENOTSUP: "unsupported configuration",
+}
+
+
+class LJLKLibraryError(RuntimeError):
+    pass
+
+
+_P32 = ctypes.POINTER(ctypes.c_int32)
+_VP = ctypes.c_void_p
+
+
+class CStack(ctypes.Structure):
+    _fields_ = [
+        ("layout", ctypes.c_int32),
+        ("n_poses", ctypes.c_int32),
+        ("max_n_blocks", ctypes.c_int32),
+        ("n_blocks_total", ctypes.c_int32),
+        ("n_atoms_total", ctypes.c_int64),
+        ("block_atom_offset", _VP),
+        ("block_type", _VP),
+        ("block_pose", _VP),
+        ("pose_block_start", _VP),
+        ("atom_pose", _VP),
+        ("min_bondsep", _VP),
+        ("inter_block_bondsep", _VP),
+        ("block_conn_pos", _VP),
+        ("block_segment", _VP),
+    ]
+
+
+class CBlockTypes(ctypes.Structure):
+    _fields_ = [
+        ("n_block_types", ctypes.c_int32),
+        ("max_n_atoms", ctypes.c_int32),
+        ("max_n_tiles", ctypes.c_int32),
+        ("max_n_conn", ctypes.c_int32),
+        ("n_atom_types", ctypes.c_int32),
+        ("dtype", ctypes.c_int32),
+        ("n_atoms", _VP),
+        ("n_heavy_in_tile", _VP),
+        ("heavy_in_tile", _VP),
+        ("atom_types", _VP),
+        ("n_conn", _VP),
+        ("conn_atoms", _VP),
+        ("path_distance", _VP),
+        ("is_ligand", _VP),
+        ("type_params", _VP),
+        ("global_params", _VP),
+    ]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def lib():
+    """The HIP library; raises if it has not been built."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise LJLKLibraryError(
+                    "libtmol_ljlk_hip.so not found at %s: build it with "
+                    "`python -c 'import __graft_entry__ as g; g.build()'`" % LIB_PATH
+                )
+            L = ctypes.CDLL(LIB_PATH)
+            SP, BP = ctypes.POINTER(CStack), ctypes.POINTER(CBlockTypes)
+            L.tmol_ljlk_count_neighbor_tiles.argtypes = [_P32, ctypes.c_int32]
+            L.tmol_ljlk_count_neighbor_tiles.restype = ctypes.c_int64
+            L.tmol_ljlk_fill_neighbor_tiles.argtypes = [_P32, ctypes.c_int32, _P32]
+            L.tmol_ljlk_workspace_bytes.argtypes = [SP, BP, ctypes.c_int64, ctypes.c_int32]
+            L.tmol_ljlk_workspace_bytes.restype = ctypes.c_int64
+            L.tmol_ljlk_neighbors.argtypes = [
+                SP, BP, _VP, ctypes.c_int32, ctypes.c_double, _VP, ctypes.c_int64, _VP,
+                ctypes.c_int64, ctypes.c_int64, _VP, _VP,
+            ]
+            L.tmol_ljlk_score.argtypes = [
+                SP, BP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64,
+                ctypes.c_int32, _VP, _VP, _VP, _VP, ctypes.c_int32, _VP,
+            ]
+            L.tmol_ljlk_backward_combine.argtypes = [
+                _VP, _VP, _VP, ctypes.c_int64, ctypes.c_int32, ctypes.c_int32, _VP, _VP,
+            ]
+            L.tmol_ljlk_read_counters.argtypes = [_VP, ctypes.POINTER(ctypes.c_int64), _VP]
+            L.tmol_ljlk_work_list.argtypes = [
+                SP, BP, _VP, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_VP),
+            ]
+            L.tmol_ljlk_debug_pair_separations.argtypes = [
+                SP, BP, _VP, ctypes.c_int64, ctypes.c_int32, _VP, ctypes.c_int64, _VP, _VP,
+            ]
+            L.tmol_ljlk_device_cu_count.argtypes = [_P32]
+            _lib = L
+        return _lib
+
+
+def _check(status, what):
+    if status != OK:
+        raise LJLKLibraryError("%s failed: %s (status %d)" % (what, _ERRORS.get(status, "?"), status))
+
+
+def _p(t: Optional[torch.Tensor]):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(device) -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream(device).cuda_stream)
+
+
+def _dtype_code(dt):
+    if dt == torch.float32:
+        return F32
+    if dt == torch.float64:
+        return F64
+    raise LJLKLibraryError("coordinates must be float32 or float64, got %s" % dt)
+
+
+def _i32(t, device):
+    return t.to(device=device, dtype=torch.int32).contiguous()
+
+
+def _require_device(t: torch.Tensor):
+    if not t.is_cuda:
+        raise LJLKLibraryError(
+            "tmol_amd LJ+LK runs on ROCm devices only; got a tensor on %s" % t.device
+        )
+
+
+class BlockTypeTables:
+    """Device copies of the block-type annotations + parameters (keeps the
+    tensors alive for the C struct's pointers)."""
+
+    def __init__(self, n_atoms, n_heavy_in_tile, heavy_in_tile, atom_types, n_conn,
+                 conn_atoms, path_distance, is_ligand, type_params, global_params, dtype,
+                 device):
+        self.device = device
+        self.t = dict(
+            n_atoms=_i32(n_atoms, device),
+            n_heavy_in_tile=_i32(n_heavy_in_tile, device),
+            heavy_in_tile=_i32(heavy_in_tile, device),
+            atom_types=_i32(atom_types, device),
+            n_conn=_i32(n_conn, device),
+            conn_atoms=_i32(conn_atoms, device),
+            path_distance=_i32(path_distance, device),
+            is_ligand=_i32(is_ligand, device),
+            type_params=type_params.to(device=device, dtype=dtype).contiguous(),
+            global_params=global_params.to(device=device, dtype=dtype).contiguous().view(-1),
+        )
+        T, Amax = self.t["atom_types"].shape
+        c = CBlockTypes()
+        c.n_block_types = T
+        c.max_n_atoms = Amax
+        c.max_n_tiles = self.t["n_heavy_in_tile"].shape[1]
+        c.max_n_conn = self.t["conn_atoms"].shape[1] if self.t["conn_atoms"].dim() == 2 else 0
+        c.n_atom_types = self.t["type_params"].shape[0]
+        c.dtype = _dtype_code(dtype)
+        for k in ("n_atoms", "n_heavy_in_tile", "heavy_in_tile", "atom_types", "n_conn",
+                  "conn_atoms", "path_distance", "is_ligand", "type_params", "global_params"):
+            setattr(c, k, _p(self.t[k]))
+        self.c = c
+        if self.t["heavy_in_tile"].shape[1] != 32 * c.max_n_tiles:
+            raise LJLKLibraryError("block_type_heavy_atoms_in_tile must be [T, 32 * n_tiles]")
+        if self.t["path_distance"].shape != (T, Amax, Amax):
+            raise LJLKLibraryError("block_type_path_distance must be [T, Amax, Amax]")
+        if self.t["type_params"].dim() != 2 or self.t["type_params"].shape[1] != 10:
+            raise LJLKLibraryError("type_params must be [n_types, 10]")
+        if self.t["global_params"].numel() != 5:
+            raise LJLKLibraryError("global_params must hold 5 values")
+
+
+_tile_cache = {}
+
+
+def neighbor_tiles(pose_block_start: torch.Tensor, device) -> torch.Tensor:
+    """Neighbor-pass tile list for a CSR pose layout (host computation,
+    cached per layout)."""
+    pbs = pose_block_start.detach().to("cpu", torch.int32).contiguous()
+    key = (tuple(pbs.tolist()) if pbs.numel() < 4096 else (pbs.numel(), int(pbs.sum())), str(device))
+    if key in _tile_cache and pbs.numel() < 4096:
+        return _tile_cache[key]
+    L = lib()
+    P = pbs.numel() - 1
+    pp = ctypes.cast(pbs.data_ptr(), _P32)
+    n = L.tmol_ljlk_count_neighbor_tiles(pp, P)
+    tiles = torch.zeros((max(n, 1), 4), dtype=torch.int32)
+    _check(L.tmol_ljlk_fill_neighbor_tiles(pp, P, ctypes.cast(tiles.data_ptr(), _P32)), "fill tiles")
+    tiles = tiles[:n].to(device)
+    if pbs.numel() < 4096:
+        _tile_cache[key] = tiles
+    return tiles
+
+
+class Workspace:
+    """Caller-owned device workspace + the work list capacity."""
+
+    def __init__(self, cstack, cbt, capacity, dtype_code, device):
+        L = lib()
+        nbytes = L.tmol_ljlk_workspace_bytes(ctypes.byref(cstack), ctypes.byref(cbt), capacity, dtype_code)
+        if nbytes < 0:
+            raise LJLKLibraryError("workspace size query failed")
+        self.buf = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
+        self.nbytes = int(nbytes)
+        self.capacity = int(capacity)
+        self.dtype_code = dtype_code
+
+    def items(self, cstack, cbt) -> torch.Tensor:
+        """View of the work list ([capacity, 2] int32) inside the workspace."""
+        ptr = _VP()
+        _check(lib().tmol_ljlk_work_list(ctypes.byref(cstack), ctypes.byref(cbt), _p(self.buf),
+                                         self.capacity, self.dtype_code, ctypes.byref(ptr)), "work list")
+        off = ptr.value - self.buf.data_ptr()
+        return self.buf[off : off + 8 * self.capacity].view(torch.int32).view(self.capacity, 2)
+
+    def counters(self, device) -> List[int]:
+        out = (ctypes.c_int64 * 2)()
+        _check(lib().tmol_ljlk_read_counters(_p(self.buf), out, _stream(device)), "read counters")
+        return [int(out[0]), int(out[1])]
+
+
+def padded_upper_triangle_capacity(P: int, B: int) -> int:
+    return P * B * (B + 1) // 2
+
+
+class PaddedCall:
+    """Everything one padded-layout op call needs on the device."""
+
+    def __init__(self, args, dtype):
+        (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+         first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+         n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+         max_n_rots_per_pose, min_bondsep, inter_block_bondsep, bt_n_atoms, bt_n_heavy_in_tile,
+         bt_heavy_in_tile, bt_atom_types, bt_n_conn, bt_conn_atoms, bt_path_distance,
+         bt_is_ligand, type_params, global_params, max_dis, block_pair) = args
+        _require_device(coords)
+        dev = coords.device
+        self.device = dev
+        P, B = first_rot_for_block.shape
+        self.P, self.B = int(P), int(B)
+        NA = coords.shape[0]
+        if coords.dim() != 2 or coords.shape[1] != 3:
+            raise LJLKLibraryError("rot_coords must be [n_atoms, 3]")
+        if rot_coord_offset.numel() != P * B or block_type_ind_for_rot.numel() != P * B:
+            raise LJLKLibraryError("rot index tensors must have P * B entries")
+        if tuple(min_bondsep.shape) != (P, B, B):
+            raise LJLKLibraryError("pose_stack_min_bond_separation must be [P, B, B]")
+        if inter_block_bondsep.dim() != 5 or tuple(inter_block_bondsep.shape[:3]) != (P, B, B):
+            raise LJLKLibraryError("pose_stack_inter_block_bondsep must be [P, B, B, C, C]")
+        self.keep = dict(
+            block_atom_offset=_i32(rot_coord_offset, dev),
+            block_type=_i32(block_type_ind_for_rot, dev),
+            atom_pose=_i32(pose_ind_for_atom, dev),
+            min_bondsep=_i32(min_bondsep, dev),
+            inter_block_bondsep=_i32(inter_block_bondsep, dev),
+        )
+        s = CStack()
+        s.layout = PADDED
+        s.n_poses = self.P
+        s.max_n_blocks = self.B
+        s.n_blocks_total = self.P * self.B
+        s.n_atoms_total = NA
+        for k, v in self.keep.items():
+            setattr(s, k, _p(v))
+        self.cstack = s
+        self.bt = BlockTypeTables(bt_n_atoms, bt_n_heavy_in_tile, bt_heavy_in_tile, bt_atom_types,
+                                  bt_n_conn, bt_conn_atoms, bt_path_distance, bt_is_ligand,
+                                  type_params, global_params, dtype, dev)
+        if self.bt.c.max_n_conn != inter_block_bondsep.shape[3]:
+            raise LJLKLibraryError("inter_block_bondsep connection dims must match conn table")
+        self.coords = coords.contiguous()
+        self.dtype = dtype
+        self.dcode = _dtype_code(dtype)
+        self.max_dis = float(max_dis)
+        self.block_pair = bool(block_pair)
+        pbs = torch.arange(0, P + 1, dtype=torch.int32) * B
+        self.tiles = neighbor_tiles(pbs, dev)
+        cap = padded_upper_triangle_capacity(self.P, self.B)
+        self.ws = Workspace(self.cstack, self.bt.c, cap, self.dcode, dev)
+
+    def neighbors(self, dense: Optional[torch.Tensor]):
+        _check(lib().tmol_ljlk_neighbors(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), _p(self.coords), self.dcode,
+            self.max_dis, _p(self.tiles), self.tiles.shape[0], _p(self.ws.buf), self.ws.nbytes,
+            self.ws.capacity, _p(dense), _stream(self.device)), "ljlk neighbors")
+
+    def score(self, flags, weights=None, energies=None, dvdx=None, grad=None, n_wave_groups=0):
+        _check(lib().tmol_ljlk_score(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), _p(self.coords), self.dcode,
+            _p(self.ws.buf), self.ws.nbytes, self.ws.capacity, flags, _p(weights), _p(energies),
+            _p(dvdx), _p(grad), n_wave_groups, _stream(self.device)), "ljlk score")
+
+
+def _forward_impl(args, require_grad: bool):
+    coords = args[0]
+    dtype = coords.dtype
+    call = PaddedCall(args, dtype)
+    P, B = call.P, call.B
+    dev = coords.device
+    nbr = torch.empty((P, B, B), dtype=torch.int32, device=dev)
+    call.neighbors(nbr)
+    if call.block_pair:
+        score = torch.empty((3, P, B, B), dtype=dtype, device=dev)
+        call.score(BLOCK_PAIR, energies=score)
+        return score, nbr, None, call
+    score = torch.empty((3, P), dtype=dtype, device=dev)
+    if require_grad:
+        dvdx = torch.empty((3, coords.shape[0], 3), dtype=dtype, device=dev)
+        call.score(DVDX, energies=score, dvdx=dvdx)
+        return score, nbr, dvdx, call
+    call.score(0, energies=score)
+    return score, nbr, None, call
+
+
+# ---------------------------------------------------------------------------
+# PyTorch ops (primitive steps), registered under torch.ops.tmol_amd
+
+
+@torch.library.custom_op("tmol_amd::ljlk_pose_forward", mutates_args=())
+def ljlk_pose_forward(
+    rot_coords: torch.Tensor, rot_coord_offset: torch.Tensor, pose_ind_for_atom: torch.Tensor,
+    first_rot_for_block: torch.Tensor, first_rot_block_type: torch.Tensor,
+    block_ind_for_rot: torch.Tensor, pose_ind_for_rot: torch.Tensor,
+    block_type_ind_for_rot: torch.Tensor, n_rots_for_pose: torch.Tensor,
+    rot_offset_for_pose: torch.Tensor, n_rots_for_block: torch.Tensor,
+    rot_offset_for_block: torch.Tensor, max_n_rots_per_pose: int,
+    pose_stack_min_bond_separation: torch.Tensor, pose_stack_inter_block_bondsep: torch.Tensor,
+    block_type_n_atoms: torch.Tensor, block_type_n_heavy_atoms_in_tile: torch.Tensor,
+    block_type_heavy_atoms_in_tile: torch.Tensor, block_type_atom_types: torch.Tensor,
+    block_type_n_interblock_bonds: torch.Tensor,
+    block_type_atoms_forming_chemical_bonds: torch.Tensor,
+    block_type_path_distance: torch.Tensor, block_type_is_ligand_fragment: torch.Tensor,
+    type_params: torch.Tensor, global_params: torch.Tensor, max_dis: float,
+    output_block_pair_energies: bool, require_gradient: bool,
+) -> List[torch.Tensor]:
+    """Forward of the LJ+LK pose score: [score, block_neighbors, dV_dx]
+    (dV_dx is empty unless require_gradient in whole-pose mode)."""
+    args = (rot_coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+            first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+            n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+            max_n_rots_per_pose, pose_stack_min_bond_separation, pose_stack_inter_block_bondsep,
+            block_type_n_atoms, block_type_n_heavy_atoms_in_tile, block_type_heavy_atoms_in_tile,
+            block_type_atom_types, block_type_n_interblock_bonds,
+            block_type_atoms_forming_chemical_bonds, block_type_path_distance,
+            block_type_is_ligand_fragment, type_params, global_params, max_dis,
+            output_block_pair_energies)
+    score, nbr, dvdx, _ = _forward_impl(args, require_gradient and not output_block_pair_energies)
+    if dvdx is None:
+        dvdx = torch.empty((0,), dtype=rot_coords.dtype, device=rot_coords.device)
+    return [score, nbr, dvdx]
+
+
+@ljlk_pose_forward.register_fake
+def _(rot_coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block, *rest):
+    P, B = first_rot_for_block.shape
+    block_pair = rest[-2]
+    require = rest[-1]
+    dt = rot_coords.dtype
+    score = rot_coords.new_empty((3, P, B, B) if block_pair else (3, P), dtype=dt)
+    nbr = rot_coords.new_empty((P, B, B), dtype=torch.int32)
+    dv = rot_coords.new_empty((3, rot_coords.shape[0], 3) if (require and not block_pair) else (0,), dtype=dt)
+    return [score, nbr, dv]
+
+
+@torch.library.custom_op("tmol_amd::ljlk_backward_combine", mutates_args=())
+def ljlk_backward_combine(dV_dx: torch.Tensor, grad_score: torch.Tensor,
+                          pose_ind_for_atom: torch.Tensor) -> torch.Tensor:
+    """grad[a] = sum_t grad_score[t, pose(a)] * dV_dx[t, a] (hand-written kernel)."""
+    _require_device(dV_dx)
+    dev = dV_dx.device
+    NA = dV_dx.shape[1]
+    grad = torch.empty((NA, 3), dtype=dV_dx.dtype, device=dev)
+    gs = grad_score.to(dV_dx.dtype).contiguous()
+    api = _i32(pose_ind_for_atom, dev)
+    _check(lib().tmol_ljlk_backward_combine(_p(dV_dx.contiguous()), _p(gs), _p(api), NA,
+                                            gs.shape[1], _dtype_code(dV_dx.dtype), _p(grad),
+                                            _stream(dev)), "backward combine")
+    return grad
+
+
+@ljlk_backward_combine.register_fake
+def _(dV_dx, grad_score, pose_ind_for_atom):
+    return dV_dx.new_empty((dV_dx.shape[1], 3))
+
+
+@torch.library.custom_op("tmol_amd::ljlk_block_pair_backward", mutates_args=())
+def ljlk_block_pair_backward(
+    grad_score: torch.Tensor, rot_coords: torch.Tensor, rot_coord_offset: torch.Tensor,
+    pose_ind_for_atom: torch.Tensor, first_rot_for_block: torch.Tensor,
+    first_rot_block_type: torch.Tensor, block_ind_for_rot: torch.Tensor,
+    pose_ind_for_rot: torch.Tensor, block_type_ind_for_rot: torch.Tensor,
+    n_rots_for_pose: torch.Tensor, rot_offset_for_pose: torch.Tensor,
+    n_rots_for_block: torch.Tensor, rot_offset_for_block: torch.Tensor,
+    max_n_rots_per_pose: int, pose_stack_min_bond_separation: torch.Tensor,
+    pose_stack_inter_block_bondsep: torch.Tensor, block_type_n_atoms: torch.Tensor,
+    block_type_n_heavy_atoms_in_tile: torch.Tensor,
+    block_type_heavy_atoms_in_tile: torch.Tensor, block_type_atom_types: torch.Tensor,
+    block_type_n_interblock_bonds: torch.Tensor,
+    block_type_atoms_forming_chemical_bonds: torch.Tensor,
+    block_type_path_distance: torch.Tensor, block_type_is_ligand_fragment: torch.Tensor,
+    type_params: torch.Tensor, global_params: torch.Tensor, max_dis: float,
+) -> torch.Tensor:
+    """Block-pair backward: re-walk the neighbor list, scatter
+    sum_t grad_score[t, p, b1, b2] * dV_t(b1, b2)/dx (ljlk_pose_score.impl.hh:
+    1160-1477)."""
+    args = (rot_coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+            first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+            n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+            max_n_rots_per_pose, pose_stack_min_bond_separation, pose_stack_inter_block_bondsep,
+            block_type_n_atoms, block_type_n_heavy_atoms_in_tile, block_type_heavy_atoms_in_tile,
+            block_type_atom_types, block_type_n_interblock_bonds,
+            block_type_atoms_forming_chemical_bonds, block_type_path_distance,
+            block_type_is_ligand_fragment, type_params, global_params, max_dis, True)
+    call = PaddedCall(args, rot_coords.dtype)
+    call.neighbors(None)
+    grad = torch.empty_like(call.coords)
+    w = grad_score.to(rot_coords.dtype).contiguous()
+    call.score(BLOCK_PAIR | WGRAD | NO_ENERGY, weights=w, grad=grad)
+    return grad
+
+
+@ljlk_block_pair_backward.register_fake
+def _(grad_score, rot_coords, *rest):
+    return torch.empty_like(rot_coords)
+
+
+# ---------------------------------------------------------------------------
+# autograd
+
+
+class LJLKPoseScoreFunction(torch.autograd.Function):
+    """Forward/backward through the registered ops (LJLKPoseScoreOp,
+    compiled.ops.cpp:29-279)."""
+
+    @staticmethod
+    def forward(ctx, *args):
+        rot_coords = args[0]
+        block_pair = bool(args[-1])
+        require = rot_coords.requires_grad
+        score, nbr, dvdx = torch.ops.tmol_amd.ljlk_pose_forward(*args, require and not block_pair)
+        ctx.block_pair = block_pair
+        ctx.n_args = len(args)
+        if block_pair:
+            ctx.args = args[1:-1]
+            ctx.save_for_backward(rot_coords)
+        else:
+            ctx.save_for_backward(dvdx, args[2])
+        ctx.mark_non_differentiable(nbr)
+        return score, nbr
+
+    @staticmethod
+    def backward(ctx, grad_score, grad_nbr):
+        if grad_score is None:
+            return (None,) * ctx.n_args
+        if not ctx.block_pair:
+            dvdx, atom_pose = ctx.saved_tensors
+            grad = torch.ops.tmol_amd.ljlk_backward_combine(dvdx, grad_score.contiguous(), atom_pose)
+        else:
+            (coords,) = ctx.saved_tensors
+            grad = torch.ops.tmol_amd.ljlk_block_pair_backward(
+                grad_score.contiguous(), coords, *ctx.args)
+        return (grad,) + (None,) * (ctx.n_args - 1)
+
+
+def ljlk_pose_scores(*args):
+    """Drop-in for ``torch.ops.tmol_ljlk.ljlk_pose_scores`` (27 arguments);
+    returns ``[score, block_neighbors]``."""
+    if len(args) != 27:
+        raise TypeError("ljlk_pose_scores takes 27 arguments (%d given)" % len(args))
+    score, nbr = LJLKPoseScoreFunction.apply(*args)
+    return [score, nbr]
+
+
+def debug_pair_separations(args, items: torch.Tensor) -> torch.Tensor:
+    """Count-pair separations computed by the kernel's code for padded-layout
+    block pairs ``items`` ([n, 2] global block indices g = p * B + b)."""
+    call = PaddedCall(args, args[0].dtype)
+    call.neighbors(None)
+    dev = call.device
+    it = _i32(items, dev)
+    out = torch.empty((it.shape[0], 32 * 32), dtype=torch.uint8, device=dev)
+    _check(lib().tmol_ljlk_debug_pair_separations(
+        ctypes.byref(call.cstack), ctypes.byref(call.bt.c), _p(call.ws.buf), call.ws.capacity,
+        call.dcode, _p(it), it.shape[0], _p(out), _stream(dev)), "debug separations")
+    return out.view(-1, 32, 32)
+
+
+def padded_work_list(args) -> torch.Tensor:
+    """Run the neighbor pass and return the work list as sorted (g1, g2)
+    pairs on the host (parity tests)."""
+    call = PaddedCall(args, args[0].dtype)
+    nbr = torch.empty((call.P, call.B, call.B), dtype=torch.int32, device=call.device)
+    call.neighbors(nbr)
+    n, overflow = call.ws.counters(call.device)
+    items = call.ws.items(call.cstack, call.bt.c)[:n].cpu()
+    return items, nbr, overflow
