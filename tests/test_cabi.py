@@ -1,0 +1,98 @@
+"""CPU checks of the C ABI: the library loads, exports every symbol the
+header declares, and its host-only helpers behave (no device calls)."""
+
+import ctypes
+import os
+import re
+
+import numpy
+import pytest
+import torch
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+HEADER = os.path.join(ROOT, "include", "tmol_ljlk.h")
+
+
+def _declared_symbols():
+    text = open(HEADER).read()
+    return sorted(set(re.findall(r"\b(tmol_ljlk_[a-z0-9_]+)\s*\(", text)))
+
+
+def test_library_exports_header_symbols():
+    from tmol_amd import ops
+
+    L = ops.lib()
+    syms = _declared_symbols()
+    assert len(syms) >= 10
+    for s in syms:
+        assert hasattr(L, s), s
+
+
+def test_neighbor_tiles_host():
+    from tmol_amd import ops
+
+    L = ops.lib()
+    pbs = numpy.array([0, 10, 150, 150, 300], dtype=numpy.int32)  # includes an empty pose
+    n = L.tmol_ljlk_count_neighbor_tiles(pbs.ctypes.data_as(ops._P32), 4)
+    t = [(nb + 63) // 64 for nb in numpy.diff(pbs)]
+    assert n == sum(k * (k + 1) // 2 for k in t)
+    tiles = numpy.zeros((n, 4), dtype=numpy.int32)
+    L.tmol_ljlk_fill_neighbor_tiles(pbs.ctypes.data_as(ops._P32), 4, tiles.ctypes.data_as(ops._P32))
+    # every (b1 <= b2) pair of every pose is covered exactly once
+    for p in range(4):
+        s, e = pbs[p], pbs[p + 1]
+        cover = numpy.zeros((e - s, e - s), dtype=int)
+        for r0, c0, end, pp in tiles[tiles[:, 3] == p]:
+            assert end == e
+            for b1 in range(r0, min(r0 + 64, e)):
+                for b2 in range(c0, min(c0 + 64, e)):
+                    if b1 <= b2:
+                        cover[b1 - s, b2 - s] += 1
+        iu = numpy.triu_indices(e - s)
+        assert (cover[iu] == 1).all()
+        assert cover.sum() == len(iu[0])
+
+
+def test_workspace_bytes_and_validation():
+    from tmol_amd import ops
+
+    L = ops.lib()
+    s = ops.CStack()
+    s.layout = ops.PADDED
+    s.n_poses = 2
+    s.max_n_blocks = 10
+    s.n_blocks_total = 20
+    s.n_atoms_total = 300
+    b = ops.CBlockTypes()
+    b.n_block_types = 3
+    b.max_n_atoms = 24
+    b.max_n_tiles = 1
+    b.max_n_conn = 2
+    b.n_atom_types = 105
+    b.dtype = ops.F32
+    n32 = L.tmol_ljlk_workspace_bytes(ctypes.byref(s), ctypes.byref(b), 1000, ops.F32)
+    n64 = L.tmol_ljlk_workspace_bytes(ctypes.byref(s), ctypes.byref(b), 1000, ops.F64)
+    assert n32 > 8 * 1000 and n64 > n32
+    # argument validation happens before any device work
+    st = L.tmol_ljlk_score(ctypes.byref(s), ctypes.byref(b), None, ops.F32, None, 0, 0, 0, None,
+                           None, None, None, 0, None)
+    assert st == ops.EINVAL
+
+
+def test_cpu_tensor_is_rejected(ubq_pdb):
+    from tmol_amd.io_pdb import pose_from_pdb
+    from tmol_amd.ops import LJLKLibraryError, ljlk_pose_scores
+    from tmol_amd.pose import PoseStack
+    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+    ps = PoseStack([pose_from_pdb(ubq_pdb, (0, 4))])
+    with pytest.raises(LJLKLibraryError):
+        ljlk_pose_scores(*pose_op_args(ps, LJLKEnergyTerm()))
+
+
+def test_op_schema_matches_reference_arity():
+    import tmol_amd.ops  # noqa: F401
+
+    schema = str(torch.ops.tmol_amd.ljlk_pose_forward.default._schema)
+    # 27 reference arguments + require_gradient
+    assert schema.count(",") + 1 == 28

@@ -289,14 +289,16 @@ def test_oracle_matches_reference_op(ubq_pdb, ljlk_term):
     rs, rnb = ref(*a2)
     assert (rnb.numpy() == nb).all()
     numpy.testing.assert_allclose(s, rs.detach().numpy(), rtol=1e-6, atol=1e-4)
-    rs.sum().backward()
-    # the reference sums the three per-term float32 planes; clash pairs give
-    # terms of opposite sign and large magnitude, so the bound scales with
-    # the summed magnitudes
-    scale = numpy.abs(dv).sum(0)
-    assert (numpy.abs(dv.sum(0) - c.grad.numpy()) <= 2e-6 * scale + 1e-5).all()
+    # per-term gradients: weight one term at a time through the backward
+    for t in range(3):
+        c.grad = None
+        r2, _ = ref(*a2)
+        (r2[t]).sum().backward()
+        # synthetic poses have clashes whose float32 pair derivatives (10^3..10^4)
+        # cancel to O(1) per atom: bound = float32 rounding of those sums
+        numpy.testing.assert_allclose(dv[t], c.grad.numpy(), rtol=1e-4, atol=1e-3)
     # block-pair mode
     bargs = pose_op_args(ps, ljlk_term, block_pair=True)
     bs, _, _, _ = orc.oracle_pose_scores(*bargs)
     rbs, _ = ref(*bargs)
-    numpy.testing.assert_allclose(bs, rbs.numpy(), rtol=1e-6, atol=1e-5)
+    numpy.testing.assert_allclose(bs, rbs.numpy(), rtol=1e-5, atol=1e-5)
