@@ -1,0 +1,227 @@
+"""Benchmark: exact LJ+LK energy + gradient throughput on one MI355X.
+
+A step = one neighbor pass + one fused scoring pass (per-pose, per-term
+energies and the weighted gradient, beta2016 weights 1.0 / 0.55 / 1.0) over
+the whole batch, inputs resident in HBM, replayed as a captured hipGraph.
+
+Default workload: 512 poses, residue counts log-uniform in [50, 1200],
+ragged stack, float32 (seeded synthetic poses, ideal geometry).
+
+Prints one JSON line (see README of the measurement contract in DESIGN.md):
+value = poses/s for the whole job; roofline of the scoring kernel from
+algorithmic FLOPs (branch census x FLOP table in tmol_amd/flops.py) over its
+measured duration; cpu_baseline = the reference's own CPU op (oracle/_ref) or,
+if absent, the C oracle, on a bounded sample of the same poses.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+METRIC = "poses/s LJ+LK energy+grad; exact vs MLP-pruned at ≤0.05 kcal/mol/res error"
+FP32_PEAK_TFLOPS = 157.3
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def cpu_baseline(poses, sample_stride, time_budget_s=25.0):
+    """Reference CPU op (single thread) on every ``sample_stride``-th pose,
+    each pose scored alone (forward + backward), until the budget is used."""
+    import torch
+
+    from oracle import oracle as orc
+    from tmol_amd.pose import PoseStack
+    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+    torch.set_num_threads(1)
+    term = LJLKEnergyTerm()
+    kind = "reference" if orc.reference_op_available() else "port"
+    ref = orc.load_reference_op() if kind == "reference" else None
+    sample = poses[::sample_stride]
+    t_total = 0.0
+    n_done = 0
+    n_res = 0
+    for p in sample:
+        ps = PoseStack([p])
+        args = pose_op_args(ps, term)
+        if kind == "reference":
+            c = args[0].clone().requires_grad_(True)
+            args[0] = c
+            t0 = time.perf_counter()
+            s, _ = ref(*args)
+            (s * torch.tensor([[1.0], [0.55], [1.0]])).sum().backward()
+            t_total += time.perf_counter() - t0
+        else:
+            t0 = time.perf_counter()
+            orc.oracle_pose_scores(*args, require_grad=True)
+            t_total += time.perf_counter() - t0
+        n_done += 1
+        n_res += p.n_blocks
+        if t_total > time_budget_s:
+            break
+    return {
+        "value": n_done / t_total,
+        "unit": "poses/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": "%d poses (every %dth of the batch, %d residues), each scored alone, "
+                  "forward+backward, torch threads = 1, %.1f s" % (n_done, sample_stride, n_res, t_total),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--workload", default="ragged_512x50_1200")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--cpu-stride", type=int, default=4)
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
+              "(poses shard across GPUs with no exchange; see DESIGN.md).", flush=True)
+        sys.exit(2)
+
+    import numpy
+    import torch
+
+    from tmol_amd.flops import algorithmic_bytes, algorithmic_flops, pair_evaluations
+    from tmol_amd.generator import workload_poses
+    from tmol_amd.pose import RaggedPoseStack
+    from tmol_amd.scorer import LJLKRaggedScorer
+    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm
+
+    dev = torch.device("cuda:0")
+    t0 = time.time()
+    poses = workload_poses(args.workload)
+    log("generated %d poses in %.1fs" % (len(poses), time.time() - t0))
+    stack = RaggedPoseStack(poses)
+    term = LJLKEnergyTerm(device=dev)
+    scorer = LJLKRaggedScorer(stack, term, device=dev)
+    coords = stack.coords.to(dev)
+    weights = torch.tensor([1.0, 0.55, 1.0])
+    log("stack: %d poses, %d blocks, %d atoms" % (stack.n_poses, stack.n_blocks_total, stack.n_atoms_total))
+
+    tune = scorer.autotune(coords, weights, cache=False)
+    log("autotune: %s" % json.dumps(tune))
+    n_items = scorer.check_capacity()
+    census = scorer.census(coords)
+    log("census: %s" % json.dumps(census))
+    flops = algorithmic_flops(census)
+    pair_evals = pair_evaluations(census)
+
+    # kernel-level timing of the scoring kernel alone (work list fixed)
+    scorer.neighbors(coords)
+    w = scorer._weights(weights)
+    stream = torch.cuda.current_stream(dev)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(3):
+        scorer.score(coords, 4, weights=w, energies=scorer.energies, grad=scorer.grad)
+    torch.cuda.synchronize(dev)
+    ev0.record(stream)
+    for _ in range(args.kernel_reps):
+        scorer.score(coords, 4, weights=w, energies=scorer.energies, grad=scorer.grad)
+    ev1.record(stream)
+    torch.cuda.synchronize(dev)
+    score_ms = ev0.elapsed_time(ev1) / args.kernel_reps  # includes 2 tiny fill + 1 finalize kernels
+    ev0.record(stream)
+    for _ in range(args.kernel_reps):
+        scorer.neighbors(coords)
+    ev1.record(stream)
+    torch.cuda.synchronize(dev)
+    nbr_ms = ev0.elapsed_time(ev1) / args.kernel_reps
+
+    # the timed step: captured graph replays
+    use_graph = not args.no_graph
+    if use_graph:
+        scorer.capture(coords, weights)
+        step = lambda: scorer.replay()  # noqa: E731
+    else:
+        step = lambda: scorer.energy_and_grad(coords, weights)  # noqa: E731
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize(dev)
+    evs = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+           for _ in range(args.steps)]
+    t_start = time.perf_counter()
+    for a, b in evs:
+        a.record(stream)
+        step()
+        b.record(stream)
+    torch.cuda.synchronize(dev)
+    wall = time.perf_counter() - t_start
+    lat = numpy.array([a.elapsed_time(b) for a, b in evs])
+    ms_per_step = wall * 1000.0 / args.steps
+    e = scorer.energies.double().cpu()
+    finite = bool(torch.isfinite(e).all())
+
+    achieved_tflops = flops / (score_ms * 1e-3) / 1e12
+    abytes = algorithmic_bytes(stack.n_atoms_total, stack.n_blocks_total, stack.n_poses, n_items)
+    traffic = None
+    prof = os.path.join(ROOT, "profiles", "score_kernel_ragged512_traffic.json")
+    if os.path.exists(prof):
+        try:
+            traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+
+    result = {
+        "metric": METRIC,
+        "value": stack.n_poses / (ms_per_step * 1e-3),
+        "unit": "poses/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator, ideal geometry, 70/30 helix/strand, rotameric chis)",
+        "config": {
+            "workload": args.workload,
+            "poses": stack.n_poses,
+            "residues": stack.n_blocks_total,
+            "atoms": stack.n_atoms_total,
+            "block_pairs": n_items,
+            "atom_pair_evals_per_step": pair_evals,
+            "atom_pair_evals_per_s": pair_evals / (ms_per_step * 1e-3),
+            "graph": use_graph,
+            "wave_groups": scorer.wave_groups,
+            "latency_ms_p50": float(numpy.percentile(lat, 50)),
+            "latency_ms_p99": float(numpy.percentile(lat, 99)),
+            "neighbors_ms": nbr_ms,
+            "score_ms": score_ms,
+            "energies_finite": finite,
+        },
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved_tflops,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved_tflops / FP32_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel": "score_kernel<float, WGRAD> (FP32 vector ALU; peak = MI355X FP32 vector)",
+            "algorithmic_flops_per_launch": flops,
+            "algorithmic_bytes_per_launch": abytes["score"],
+        },
+    }
+    if not args.no_cpu:
+        log("cpu baseline ...")
+        result["cpu_baseline"] = cpu_baseline(poses, args.cpu_stride)
+    print(json.dumps(result), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -125,6 +125,16 @@ int tmol_ljlk_score(const tmol_ljlk_stack* stack,
                     const void* weights, void* energies, void* dvdx,
                     void* grad, int32_t n_wave_groups, void* stream);
 
+/* Branch census of the work list (float32 only): counts of enumerated atom
+ * pairs per potential region, for algorithmic FLOP accounting.  census[16]
+ * (device uint64): LJ pairs, LJ weight-0, LJ beyond max_dis, LJ linear,
+ * LJ 12-6, LJ spline, LK pairs, LK weight-0, LK beyond, LK constant,
+ * LK close spline, LK f_desolv, LK far spline, items, intra-block items. */
+int tmol_ljlk_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                     const void* coords, int32_t dtype, void* workspace,
+                     int64_t ws_bytes, int64_t item_capacity, uint64_t* census,
+                     void* stream);
+
 /* Whole-pose backward (compiled.ops.cpp:157-174):
  *   grad[a, k] = sum_t grad_scores[t, atom_pose[a]] * dvdx[t, a, k] */
 int tmol_ljlk_backward_combine(const void* dvdx, const void* grad_scores,

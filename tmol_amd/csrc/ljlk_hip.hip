@@ -387,6 +387,7 @@ struct ScoreArgs {
   Real* dvdx;            // [3, NA, 3]
   Real* grad;            // [NA, 3]
   int64_t n_atoms;
+  unsigned long long* census;  // [16] with M_CENSUS
 };
 
 // compensated (Neumaier) accumulation of one lane's pair energies
@@ -438,14 +439,26 @@ constexpr int M_DVDX = TMOL_LJLK_DVDX;
 constexpr int M_WGRAD = TMOL_LJLK_WGRAD;
 constexpr int M_BP = TMOL_LJLK_BLOCK_PAIR;
 constexpr int M_NOE = TMOL_LJLK_NO_ENERGY;
+constexpr int M_CENSUS = 64;  // branch census instead of energies (FLOP accounting)
+
+// census slots (counts of enumerated atom pairs)
+enum : int {
+  C_LJ_PAIRS = 0, C_LJ_CP0, C_LJ_FAR, C_LJ_LIN, C_LJ_MID, C_LJ_SPL,
+  C_LK_PAIRS, C_LK_CP0, C_LK_FAR, C_LK_CONST, C_LK_CLOSE, C_LK_MID, C_LK_SPL,
+  C_ITEMS, C_INTRA_ITEMS, C_N
+};
 
 template <typename Real, int MODE>
 __global__ __launch_bounds__(256) void score_kernel(ScoreArgs<Real> A) {
   constexpr bool DVDX = MODE & M_DVDX;
   constexpr bool WGRAD = MODE & M_WGRAD;
   constexpr bool BP = MODE & M_BP;
-  constexpr bool NOE = MODE & M_NOE;
-  constexpr bool DERIV = DVDX || WGRAD;
+  constexpr bool CENSUS = MODE & M_CENSUS;
+  constexpr bool NOE = (MODE & M_NOE) || CENSUS;
+  constexpr bool DERIV = (DVDX || WGRAD) && !CENSUS;
+  uint32_t cnt[C_N];
+#pragma unroll
+  for (int k = 0; k < C_N; ++k) cnt[k] = 0;
   __shared__ WaveLDS<Real> lds_all[WAVES_PER_WG];
   const int lane = lane_id();
   const int wave_in_wg = threadIdx.x / WAVE;
@@ -508,6 +521,10 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs<Real> A) {
       }
     }
     KSum<Real> e_atr, e_rep, e_lk;
+    if (CENSUS && lane == 0) {
+      cnt[C_ITEMS] += 1;
+      cnt[C_INTRA_ITEMS] += intra ? 1 : 0;
+    }
     const int nti1 = (n1 - 1) / TILE + 1, nti2 = (n2 - 1) / TILE + 1;
     for (int ti = 0; ti < nti1; ++ti) {
       for (int tj = intra ? ti : 0; tj < nti2; ++tj) {
@@ -580,7 +597,21 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs<Real> A) {
               }
               Real dx = L.x[i1] - L.x[i2], dy = L.y[i1] - L.y[i2], dz = L.z[i1] - L.z[i2];
               Real d2 = dx * dx + dy * dy + dz * dz;
-              if (sep >= 4 && d2 < gp.max_dis2) {
+              if (CENSUS) {
+                cnt[C_LJ_PAIRS] += 1;
+                if (sep < 4) {
+                  cnt[C_LJ_CP0] += 1;
+                } else if (!(d2 < gp.max_dis2)) {
+                  cnt[C_LJ_FAR] += 1;
+                } else {
+                  Real dist = fast_sqrt(d2);
+                  Real sigma = lj_sigma(L.flg[i1], L.flg[i2], L.rad[i1], L.rad[i2], gp);
+                  Real d_lin = sigma * gp.dlin_factor;
+                  Real cmin = sigma > gp.spline_start ? sigma : gp.spline_start;
+                  cmin = cmin > gp.cpoly_cap ? gp.cpoly_cap : cmin;
+                  cnt[dist < d_lin ? C_LJ_LIN : (dist < cmin ? C_LJ_MID : C_LJ_SPL)] += 1;
+                }
+              } else if (sep >= 4 && d2 < gp.max_dis2) {
                 Real dist = fast_sqrt(d2);
                 int f1 = L.flg[i1], f2 = L.flg[i2];
                 Real sigma = lj_sigma(f1, f2, L.rad[i1], L.rad[i2], gp);
@@ -643,7 +674,24 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs<Real> A) {
                 }
                 Real dx = L.x[i1] - L.x[i2], dy = L.y[i1] - L.y[i2], dz = L.z[i1] - L.z[i2];
                 Real d2 = dx * dx + dy * dy + dz * dz;
-                if (sep >= 4 && d2 < gp.max_dis2) {
+                if (CENSUS) {
+                  cnt[C_LK_PAIRS] += 1;
+                  if (sep < 4) {
+                    cnt[C_LK_CP0] += 1;
+                  } else if (!(d2 < gp.max_dis2)) {
+                    cnt[C_LK_FAR] += 1;
+                  } else {
+                    Real dist = fast_sqrt(d2);
+                    int f1 = L.flg[i1], f2 = L.flg[i2];
+                    Real sigma = lj_sigma(f1, f2, L.rad[i1], L.rad[i2], gp);
+                    bool cc = (f1 & F_CARBON_LK) && (f2 & F_CARBON_LK);
+                    Real d_min, cdmin, cdmax;
+                    lk_knots(sigma, cc, d_min, cdmin, cdmax);
+                    cnt[dist < cdmin ? C_LK_CONST
+                                     : (dist < cdmax ? C_LK_CLOSE
+                                                     : (dist < gp.spline_start ? C_LK_MID : C_LK_SPL))] += 1;
+                  }
+                } else if (sep >= 4 && d2 < gp.max_dis2) {
                   Real dist = fast_sqrt(d2);
                   int f1 = L.flg[i1], f2 = L.flg[i2];
                   Real sigma = lj_sigma(f1, f2, L.rad[i1], L.rad[i2], gp);
@@ -726,6 +774,15 @@ __global__ __launch_bounds__(256) void score_kernel(ScoreArgs<Real> A) {
           unsafeAtomicAdd(&A.energies64[2 * s.P + p], el);
         }
       }
+    }
+  }
+  if (CENSUS) {
+#pragma unroll
+    for (int k = 0; k < C_N; ++k) {
+      uint32_t v = cnt[k];
+#pragma unroll
+      for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+      if (lane == 0 && v) atomicAdd(&A.census[k], (unsigned long long)v);
     }
   }
 }
@@ -1015,6 +1072,38 @@ int tmol_ljlk_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                             grad, n_wave_groups, st, L);
   return run_score<double>(s, bt, coords, workspace, capacity, flags, weights, energies, dvdx,
                            grad, n_wave_groups, st, L);
+}
+
+int tmol_ljlk_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                     const void* coords, int32_t dtype, void* workspace, int64_t ws_bytes,
+                     int64_t capacity, uint64_t* census, void* stream) {
+  int v = validate(s, bt, dtype);
+  if (v) return v;
+  if (!coords || !workspace || !census) return TMOL_LJLK_EINVAL;
+  WsLayout Lw = ws_layout(s, bt, capacity, dtype);
+  if (ws_bytes < Lw.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, st,
+                     (unsigned long long*)census, 16, 0ull);
+  char* w = (char*)workspace;
+  if (dtype == TMOL_LJLK_F32) {
+    ScoreArgs<float> a{};
+    a.s = make_stack_view(s, bt->max_n_conn);
+    a.bt = make_bt_view(bt);
+    a.coords = (const float*)coords;
+    a.records = (const float*)(w + Lw.records);
+    a.conn_pd = (const uint8_t*)(w + Lw.conn_pd);
+    a.global_params = (const float*)bt->global_params;
+    a.items = (const int2*)(w + Lw.items);
+    a.counters = (const uint32_t*)(w + Lw.counters);
+    a.capacity = capacity;
+    a.n_atoms = s->n_atoms_total;
+    a.census = (unsigned long long*)census;
+    launch_score<float, M_CENSUS>(a, 2048, st);
+  } else {
+    return TMOL_LJLK_ENOTSUP;
+  }
+  return check(hipGetLastError());
 }
 
 int tmol_ljlk_backward_combine(const void* dvdx, const void* grad_scores,

@@ -1,0 +1,76 @@
+"""Algorithmic work model of LJ+LK energy + gradient.
+
+The work unit is one atom-pair evaluation inside a neighbor block pair (LJ:
+all atom pairs, LK: heavy-atom pairs; n1*n2 between blocks, n(n-1)/2 within a
+block).  FLOPs are counted per potential region with +, -, *, /, sqrt, exp
+and floor as one each; compares and selects are free; quantities that depend
+only on the atom-type pair (sigma, epsilon, d_lin, spline knots, spline
+end-point values and Hermite coefficients, f_desolv at the knots) are
+treated as precomputed, so the table is the minimum work of the function, not
+the instruction count of any implementation.
+
+Per pair, energy + gradient:
+  distance           dx, dy, dz, d^2                         8
+  sqrt                                                       1   (d < max_dis)
+  gradient scatter   1/d, g/d, 3 mul, 3 + 3 accumulate      11   (dV != 0)
+  energy accumulate  LJ: atr, rep                            2;  LK: 1
+LJ regions (lj.hh:115-177)
+  linear             V_lin + V'_lin (d - d_lin), split, w    6
+  12-6               sigma/d, powers, V, dV, split, w       14
+  Hermite to zero    t, Horner V and dV, 1/w, w             15
+LK regions, both directions (lk_isotropic.hh:167-235, 328-360)
+  constant           sum, w                                  2   (no gradient)
+  close Hermite      2 x 13 + sum + w                       30
+  f_desolv           2 x 12 (exp) + sum + w                 28
+  far Hermite        2 x 13 + sum + w                       30
+Pairs beyond max_dis cost the distance only (8); pairs with count-pair
+weight 0 cost nothing.
+"""
+
+CENSUS_FIELDS = [
+    "lj_pairs", "lj_cp0", "lj_far", "lj_linear", "lj_126", "lj_spline",
+    "lk_pairs", "lk_cp0", "lk_far", "lk_const", "lk_close", "lk_fdesolv", "lk_spline",
+    "items", "intra_items",
+]
+
+DIST, SQRT, GRAD = 8, 1, 11
+FLOPS_ENERGY_GRAD = {
+    "lj_far": DIST,
+    "lj_linear": DIST + SQRT + 6 + GRAD + 2,
+    "lj_126": DIST + SQRT + 14 + GRAD + 2,
+    "lj_spline": DIST + SQRT + 15 + GRAD + 2,
+    "lk_far": DIST,
+    "lk_const": DIST + SQRT + 2 + 1,
+    "lk_close": DIST + SQRT + 30 + GRAD + 1,
+    "lk_fdesolv": DIST + SQRT + 28 + GRAD + 1,
+    "lk_spline": DIST + SQRT + 30 + GRAD + 1,
+}
+
+
+def census_dict(values):
+    return {k: int(v) for k, v in zip(CENSUS_FIELDS, values)}
+
+
+def algorithmic_flops(census: dict) -> int:
+    return sum(census[k] * f for k, f in FLOPS_ENERGY_GRAD.items())
+
+
+def pair_evaluations(census: dict) -> int:
+    """Atom-pair evaluations (LJ all-atom + LK heavy-atom pairs)."""
+    return census["lj_pairs"] + census["lk_pairs"]
+
+
+def algorithmic_bytes(n_atoms: int, n_blocks: int, n_poses: int, n_items: int,
+                      real_bytes: int = 4) -> dict:
+    """Compulsory HBM bytes of one energy+gradient step, per kernel:
+    neighbors: coords in, block metadata in, spheres out/in, work list out;
+    score: coords in, work list in, block metadata in, gradient out,
+    per-pose energies out."""
+    coords = 3 * real_bytes * n_atoms
+    meta = 8 * n_blocks
+    spheres = 4 * real_bytes * n_blocks
+    items = 8 * n_items
+    return {
+        "neighbors": coords + meta + 2 * spheres + items,
+        "score": coords + items + 16 * n_items + 3 * real_bytes * n_atoms + 3 * 8 * n_poses,
+    }

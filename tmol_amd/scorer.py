@@ -1,0 +1,221 @@
+"""Persistent ragged-batch scorer: one neighbor pass + one persistent scoring
+launch over a global block-pair work list for a whole ragged stack of poses,
+with preallocated buffers, optional hipGraph capture of the
+energy + gradient step and an autotuned launch shape.
+
+This is the batched path (poses of uneven length scored in one launch); the
+padded drop-in op lives in ``ops.py``.
+"""
+
+import ctypes
+import json
+import os
+import time
+from typing import Optional
+
+import torch
+
+from . import ops
+from .flops import census_dict
+from .pose import RaggedPoseStack
+
+_TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
+
+
+class LJLKRaggedScorer:
+    """Scores a ``RaggedPoseStack`` with the HIP library.
+
+    ``energy_and_grad(coords, weights)`` is the fused step: per-pose,
+    per-term energies [3, P] and the weighted gradient
+    sum_t weights[t, p] dE_t/dx [NA, 3] in one scoring pass."""
+
+    def __init__(self, stack: RaggedPoseStack, term, device="cuda", dtype=torch.float32,
+                 capacity: Optional[int] = None, wave_groups: Optional[int] = None):
+        dev = torch.device(device)
+        self.device = dev
+        self.dtype = dtype
+        self.dcode = ops._dtype_code(dtype)
+        self.stack = stack
+        pbt = stack.packed_block_types
+        self.P = stack.n_poses
+        self.NA = stack.n_atoms_total
+        self.G = stack.n_blocks_total
+        i32 = lambda t: ops._i32(t, dev)  # noqa: E731
+        self.keep = dict(
+            block_atom_offset=i32(stack.block_atom_offset),
+            block_type=i32(stack.block_type),
+            block_pose=i32(stack.block_pose),
+            pose_block_start=i32(stack.pose_block_start),
+            atom_pose=i32(stack.atom_pose),
+            block_conn_pos=i32(stack.block_conn_pos),
+            block_segment=i32(stack.block_segment),
+        )
+        s = ops.CStack()
+        s.layout = ops.RAGGED
+        s.n_poses = self.P
+        s.max_n_blocks = stack.max_n_blocks
+        s.n_blocks_total = self.G
+        s.n_atoms_total = self.NA
+        for k, v in self.keep.items():
+            setattr(s, k, ops._p(v))
+        self.cstack = s
+        self.bt = ops.BlockTypeTables(
+            pbt.n_atoms, pbt.ljlk_n_heavy_atoms_in_tile, pbt.ljlk_heavy_atoms_in_tile,
+            pbt.atom_types, pbt.n_conn, pbt.conn_atom, pbt.ljlk_bond_separation,
+            pbt.ljlk_is_ligand_fragment, term.type_params, term.global_params, dtype, dev,
+        )
+        self.max_dis = term.max_dis
+        self.tiles = ops.neighbor_tiles(stack.pose_block_start, dev)
+        nb = stack.n_blocks_per_pose.to(torch.int64)
+        upper = int((nb * (nb + 1) // 2).sum())
+        if capacity is None:
+            capacity = min(upper, 1 << 28)
+        self.capacity = int(capacity)
+        self.ws = ops.Workspace(self.cstack, self.bt.c, self.capacity, self.dcode, dev)
+        self.energies = torch.zeros((3, self.P), dtype=dtype, device=dev)
+        self.grad = torch.zeros((self.NA, 3), dtype=dtype, device=dev)
+        self.wave_groups = wave_groups or 0
+        self.graph = None
+        self._graph_coords = None
+        self._graph_weights = None
+        self._checked_capacity = capacity >= upper
+
+    # -- raw steps -----------------------------------------------------------
+    def neighbors(self, coords: torch.Tensor):
+        ops._check(ops.lib().tmol_ljlk_neighbors(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ops._p(coords), self.dcode,
+            self.max_dis, ops._p(self.tiles), self.tiles.shape[0], ops._p(self.ws.buf),
+            self.ws.nbytes, self.ws.capacity, None, ops._stream(self.device)), "neighbors")
+
+    def score(self, coords, flags, weights=None, energies=None, dvdx=None, grad=None):
+        ops._check(ops.lib().tmol_ljlk_score(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ops._p(coords), self.dcode,
+            ops._p(self.ws.buf), self.ws.nbytes, self.ws.capacity, flags, ops._p(weights),
+            ops._p(energies), ops._p(dvdx), ops._p(grad), int(self.wave_groups),
+            ops._stream(self.device)), "score")
+
+    def counters(self):
+        return self.ws.counters(self.device)
+
+    def check_capacity(self):
+        n, overflow = self.counters()
+        if overflow:
+            raise ops.LJLKLibraryError(
+                "work list overflow: %d block pairs > capacity %d" % (n, self.capacity))
+        return n
+
+    # -- fused step ------------------------------------------------------------
+    def _weights(self, weights):
+        if weights is None:
+            weights = torch.tensor([1.0, 0.55, 1.0])
+        w = torch.as_tensor(weights, dtype=self.dtype, device=self.device)
+        if w.dim() == 1:
+            w = w[:, None].expand(3, self.P)
+        return w.contiguous()
+
+    def energy_and_grad(self, coords: torch.Tensor, weights=None):
+        """Neighbor pass + fused scoring; returns views of the scorer's
+        output buffers (energies [3, P], grad [NA, 3])."""
+        w = self._weights(weights)
+        self.neighbors(coords)
+        self.score(coords, ops.WGRAD, weights=w, energies=self.energies, grad=self.grad)
+        if not self._checked_capacity:
+            self.check_capacity()
+            self._checked_capacity = True
+        return self.energies, self.grad
+
+    def energies_only(self, coords: torch.Tensor):
+        self.neighbors(coords)
+        self.score(coords, 0, energies=self.energies)
+        return self.energies
+
+    def per_term_gradients(self, coords: torch.Tensor):
+        dvdx = torch.empty((3, self.NA, 3), dtype=self.dtype, device=self.device)
+        self.neighbors(coords)
+        self.score(coords, ops.DVDX, energies=self.energies, dvdx=dvdx)
+        return self.energies, dvdx
+
+    # -- hipGraph ------------------------------------------------------------
+    def capture(self, coords: torch.Tensor, weights=None):
+        """Capture neighbors + fused scoring into a graph over static input
+        buffers; ``replay(coords)`` copies new coordinates in and replays."""
+        self._graph_coords = coords.detach().clone().contiguous()
+        self._graph_weights = self._weights(weights).clone()
+        # warm up outside capture (first launches load code objects)
+        self.energy_and_grad(self._graph_coords, self._graph_weights)
+        torch.cuda.synchronize(self.device)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(self.device)
+        s.wait_stream(torch.cuda.current_stream(self.device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                self.neighbors(self._graph_coords)
+                self.score(self._graph_coords, ops.WGRAD, weights=self._graph_weights,
+                           energies=self.energies, grad=self.grad)
+        torch.cuda.current_stream(self.device).wait_stream(s)
+        self.graph = g
+        return g
+
+    def replay(self, coords: Optional[torch.Tensor] = None):
+        if self.graph is None:
+            raise RuntimeError("capture() first")
+        if coords is not None and coords.data_ptr() != self._graph_coords.data_ptr():
+            self._graph_coords.copy_(coords)
+        self.graph.replay()
+        return self.energies, self.grad
+
+    # -- census ----------------------------------------------------------------
+    def census(self, coords: torch.Tensor) -> dict:
+        """Branch census of the current work list (float32 scorer)."""
+        out = torch.zeros(16, dtype=torch.int64, device=self.device)
+        self.neighbors(coords)
+        ops._check(ops.lib().tmol_ljlk_census(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ops._p(coords), self.dcode,
+            ops._p(self.ws.buf), self.ws.nbytes, self.ws.capacity, ops._p(out),
+            ops._stream(self.device)), "census")
+        return census_dict(out.cpu().tolist())
+
+    # -- autotuner -------------------------------------------------------------
+    def autotune(self, coords, weights=None, candidates=None, reps=5, cache=True):
+        """Pick the persistent grid size (workgroups of 4 waves) by timing the
+        fused step; results cached per device and workload signature."""
+        key = "%s|G%d|NA%d|P%d|%s" % (torch.cuda.get_device_name(self.device), self.G,
+                                      self.NA, self.P, self.dtype)
+        table = {}
+        if cache and os.path.exists(_TUNE_CACHE):
+            try:
+                table = json.load(open(_TUNE_CACHE))
+            except (OSError, ValueError):
+                table = {}
+        if key in table:
+            self.wave_groups = int(table[key]["wave_groups"])
+            return table[key]
+        cu = ctypes.c_int32(0)
+        ops._check(ops.lib().tmol_ljlk_device_cu_count(ctypes.byref(cu)), "cu count")
+        n_cu = max(int(cu.value), 1)
+        if candidates is None:
+            candidates = [n_cu * k for k in (1, 2, 3, 4, 6, 8, 12, 16)]
+        w = self._weights(weights)
+        results = {}
+        for c in candidates:
+            self.wave_groups = int(c)
+            self.energy_and_grad(coords, w)
+            torch.cuda.synchronize(self.device)
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                self.energy_and_grad(coords, w)
+                torch.cuda.synchronize(self.device)
+                ts.append(time.perf_counter() - t0)
+            results[int(c)] = min(ts)
+        best = min(results, key=results.get)
+        self.wave_groups = best
+        entry = {"wave_groups": best, "cu": n_cu, "timings_s": results}
+        if cache:
+            table[key] = entry
+            try:
+                with open(_TUNE_CACHE, "w") as f:
+                    json.dump(table, f, indent=1)
+            except OSError:
+                pass
+        return entry
