@@ -131,6 +131,9 @@ def lib():
                 SP, BP, _VP, ctypes.c_int64, ctypes.c_int32, _VP, ctypes.c_int64, _VP, _VP,
             ]
             L.tmol_ljlk_device_cu_count.argtypes = [_P32]
+            L.tmol_ljlk_census.argtypes = [
+                SP, BP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64, _VP, _VP,
+            ]
             _lib = L
         return _lib
 
