@@ -43,6 +43,7 @@ extern "C" {
 #define TMOL_LJLK_BLOCK_PAIR 8 /* energies per block pair [3, P, B, B]; weights too */
 #define TMOL_LJLK_NO_ENERGY 16 /* skip the energy output (backward re-walk) */
 #define TMOL_LJLK_ACCUMULATE 32 /* add into dvdx / grad instead of overwriting */
+#define TMOL_LJLK_GENERAL_KERNEL 128 /* force the general (any block size) kernel */
 
 /* Per block-type tables: the PackedBlockTypes annotations the reference op
  * takes (ljlk_energy_term.py:163-178).  Maximum 32 atoms per tile. */
@@ -92,22 +93,24 @@ int64_t tmol_ljlk_count_neighbor_tiles(const int32_t* pose_block_start_host,
 int tmol_ljlk_fill_neighbor_tiles(const int32_t* pose_block_start_host,
                                   int32_t n_poses, int32_t* tiles_host);
 
-/* Workspace bytes for a stack, block-type set and work-list capacity. */
+/* Workspace bytes for a stack, block-type set and work-list row capacity
+ * (the most neighbor blocks g2 >= g1 any block g1 may have; a padded stack
+ * never needs more than B). */
 int64_t tmol_ljlk_workspace_bytes(const tmol_ljlk_stack* stack,
                                   const tmol_ljlk_block_types* bt,
-                                  int64_t item_capacity, int32_t dtype);
+                                  int64_t row_capacity, int32_t dtype);
 
 /* Neighbor pass (replaces compute_block_spheres + detect_block_neighbors,
  * sphere_overlap.impl.hh:109-196, 287-351): bounding sphere per block,
  * then every b1 <= b2 pair of a pose with d^2 < ((r1 + r2) + max_dis)^2 is
- * appended to the work list in the workspace.  Also prepares the per-block-
- * type atom records.  dense_neighbors (padded layout only, may be NULL)
- * receives the reference's [P, B, B] int32 flags. */
+ * appended to row b1 of the work list in the workspace.  Also prepares the
+ * per-block-type atom records.  dense_neighbors (padded layout only, may be
+ * NULL) receives the reference's [P, B, B] int32 flags. */
 int tmol_ljlk_neighbors(const tmol_ljlk_stack* stack,
                         const tmol_ljlk_block_types* bt, const void* coords,
                         int32_t dtype, double max_dis, const int32_t* tiles,
                         int64_t n_tiles, void* workspace, int64_t ws_bytes,
-                        int64_t item_capacity, int32_t* dense_neighbors,
+                        int64_t row_capacity, int32_t* dense_neighbors,
                         void* stream);
 
 /* Score the work list (replaces eval_energies / eval_energies_by_block /
@@ -121,7 +124,7 @@ int tmol_ljlk_neighbors(const tmol_ljlk_stack* stack,
 int tmol_ljlk_score(const tmol_ljlk_stack* stack,
                     const tmol_ljlk_block_types* bt, const void* coords,
                     int32_t dtype, void* workspace, int64_t ws_bytes,
-                    int64_t item_capacity, int32_t flags,
+                    int64_t row_capacity, int32_t flags,
                     const void* weights, void* energies, void* dvdx,
                     void* grad, int32_t n_wave_groups, void* stream);
 
@@ -132,7 +135,7 @@ int tmol_ljlk_score(const tmol_ljlk_stack* stack,
  * LK close spline, LK f_desolv, LK far spline, items, intra-block items. */
 int tmol_ljlk_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
                      const void* coords, int32_t dtype, void* workspace,
-                     int64_t ws_bytes, int64_t item_capacity, uint64_t* census,
+                     int64_t ws_bytes, int64_t row_capacity, uint64_t* census,
                      void* stream);
 
 /* Whole-pose backward (compiled.ops.cpp:157-174):
@@ -143,16 +146,17 @@ int tmol_ljlk_backward_combine(const void* dvdx, const void* grad_scores,
                                void* stream);
 
 /* Copy the work-list counters to the host (synchronises the stream):
- *   counters_host[0] = items found (may exceed capacity)
- *   counters_host[1] = capacity overflow flag */
+ *   counters_host[0] = block pairs found (including any beyond row capacity)
+ *   counters_host[1] = row capacity overflow flag */
 int tmol_ljlk_read_counters(const void* workspace, int64_t* counters_host,
                             void* stream);
 
-/* Device pointer to the work list (int32 pairs {g1, g2}) inside a workspace. */
+/* Device pointers to the work list inside a workspace: row_count[G]
+ * (uint32) and row_items[G, row_capacity] (int32 g2 of row g1). */
 int tmol_ljlk_work_list(const tmol_ljlk_stack* stack,
                         const tmol_ljlk_block_types* bt, void* workspace,
-                        int64_t item_capacity, int32_t dtype,
-                        int32_t** items_out);
+                        int64_t row_capacity, int32_t dtype,
+                        uint32_t** row_count_out, int32_t** row_items_out);
 
 /* Debug: count-pair separation for every atom pair of each listed block pair,
  * computed by the scoring kernel's own count-pair code (needs the workspace
@@ -161,7 +165,7 @@ int tmol_ljlk_work_list(const tmol_ljlk_stack* stack,
  * global block indices {g1, g2}, g1 <= g2, same pose. */
 int tmol_ljlk_debug_pair_separations(const tmol_ljlk_stack* stack,
                                      const tmol_ljlk_block_types* bt,
-                                     const void* workspace, int64_t item_capacity,
+                                     const void* workspace, int64_t row_capacity,
                                      int32_t dtype, const int32_t* items,
                                      int64_t n_items, uint8_t* out, void* stream);
 

@@ -1,0 +1,291 @@
+// Shared definitions of the LJ+LK device code: constants, workspace layout,
+// views of the stack / block-type tables, wave helpers, count-pair lookups.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/tmol_ljlk.h"
+#include "ljlk_math.hh"
+
+namespace tmol_ljlk {
+
+constexpr int TILE = 32;         // atoms per block tile
+constexpr int MAX_CONN = 4;      // inter-block connections per block type
+constexpr int WAVE = 64;         // CDNA wavefront
+constexpr int WAVES_PER_WG = 4;  // scoring workgroups: 4 independent waves
+constexpr int NBR_TILE = 64;     // neighbor pass: 64 x 64 block tiles
+constexpr int REC = 8;           // Reals per prepared (block type, atom) record
+
+// prepared per (block type, atom) record:
+//   [0] lj_radius  [1] sqrt(lj_wdepth)  [2] K = dg / (2 pi^1.5 lambda)
+//   [3] 1 / lambda^2  [4] lk_volume  [5] flags (F_* bits, F_HEAVY)  [6] 1
+// per (block type, connection, atom): path distance from the connection atom
+constexpr int F_HEAVY = 32;
+
+// score modes (compile-time template bits); the public flags plus
+constexpr int M_DVDX = TMOL_LJLK_DVDX;
+constexpr int M_WGRAD = TMOL_LJLK_WGRAD;
+constexpr int M_BP = TMOL_LJLK_BLOCK_PAIR;
+constexpr int M_NOE = TMOL_LJLK_NO_ENERGY;
+constexpr int M_CENSUS = 64;  // branch census instead of energies (FLOP accounting)
+
+// census slots (counts of enumerated atom pairs)
+enum : int {
+  C_LJ_PAIRS = 0, C_LJ_CP0, C_LJ_FAR, C_LJ_LIN, C_LJ_MID, C_LJ_SPL,
+  C_LK_PAIRS, C_LK_CP0, C_LK_FAR, C_LK_CONST, C_LK_CLOSE, C_LK_MID, C_LK_SPL,
+  C_ITEMS, C_INTRA_ITEMS, C_N
+};
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+__device__ __forceinline__ int lane_id() { return threadIdx.x & (WAVE - 1); }
+
+template <typename T>
+__device__ __forceinline__ T wave_sum(T v) {
+#pragma unroll
+  for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, WAVE);
+  return v;
+}
+
+// number of set bits of `mask` below this lane
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+  return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                   __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0));
+}
+
+// ------------------------------------------------------------------ workspace
+
+struct WsLayout {
+  int64_t counters;   // uint32[16]: [0] block pairs found, [1] row overflow
+  int64_t energies;   // double[3 * P]
+  int64_t spheres;    // Real[4 * G]
+  int64_t records;    // Real[T * Amax * REC]
+  int64_t conn_pd;    // uint8[T * MAX_CONN * Amax]
+  int64_t row_count;  // uint32[G]
+  int64_t row_items;  // int32[G * row_cap]
+  int64_t total;
+};
+
+inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
+
+inline WsLayout ws_layout(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                          int64_t row_cap, int dtype) {
+  int64_t rs = dtype == TMOL_LJLK_F64 ? 8 : 4;
+  int64_t G = s->n_blocks_total;
+  WsLayout L;
+  int64_t o = 0;
+  L.counters = o;
+  o = align256(o + 64);
+  L.energies = o;
+  o = align256(o + 8 * 3 * int64_t(s->n_poses));
+  L.spheres = o;
+  o = align256(o + rs * 4 * G);
+  L.records = o;
+  o = align256(o + rs * REC * int64_t(bt->n_block_types) * bt->max_n_atoms);
+  L.conn_pd = o;
+  o = align256(o + int64_t(bt->n_block_types) * MAX_CONN * bt->max_n_atoms);
+  L.row_count = o;
+  o = align256(o + 4 * G);
+  L.row_items = o;
+  o = align256(o + 4 * G * row_cap);
+  L.total = o;
+  return L;
+}
+
+// ------------------------------------------------------------------ views
+
+struct StackView {
+  int layout, P, B, G, C;
+  const int32_t* block_atom_offset;
+  const int32_t* block_type;
+  const int32_t* block_pose;
+  const int32_t* min_bondsep;
+  const int32_t* inter_block_bondsep;
+  const int32_t* block_conn_pos;
+  const int32_t* block_segment;
+
+  __device__ __forceinline__ int pose_of(int g) const {
+    return layout == TMOL_LJLK_PADDED ? g / B : block_pose[g];
+  }
+};
+
+struct BTView {
+  int T, Amax, nt, C;
+  const int32_t* n_atoms;
+  const int32_t* n_heavy_in_tile;
+  const int32_t* heavy_in_tile;
+  const int32_t* atom_types;
+  const int32_t* n_conn;
+  const int32_t* conn_atoms;
+  const int32_t* path_distance;
+  const int32_t* is_ligand;
+};
+
+// work list: row g1 holds the g2 >= g1 neighbor blocks of g1 in the same pose
+struct WorkView {
+  const uint32_t* row_count;  // [G]
+  const int32_t* row_items;   // [G * row_cap]
+  int64_t row_cap;
+};
+
+inline StackView make_stack_view(const tmol_ljlk_stack* s, int C) {
+  StackView v;
+  v.layout = s->layout;
+  v.P = s->n_poses;
+  v.B = s->max_n_blocks;
+  v.G = s->n_blocks_total;
+  v.C = C;
+  v.block_atom_offset = s->block_atom_offset;
+  v.block_type = s->block_type;
+  v.block_pose = s->block_pose;
+  v.min_bondsep = s->min_bondsep;
+  v.inter_block_bondsep = s->inter_block_bondsep;
+  v.block_conn_pos = s->block_conn_pos;
+  v.block_segment = s->block_segment;
+  return v;
+}
+
+inline BTView make_bt_view(const tmol_ljlk_block_types* b) {
+  BTView v;
+  v.T = b->n_block_types;
+  v.Amax = b->max_n_atoms;
+  v.nt = b->max_n_tiles;
+  v.C = b->max_n_conn;
+  v.n_atoms = b->n_atoms;
+  v.n_heavy_in_tile = b->n_heavy_in_tile;
+  v.heavy_in_tile = b->heavy_in_tile;
+  v.atom_types = b->atom_types;
+  v.n_conn = b->n_conn;
+  v.conn_atoms = b->conn_atoms;
+  v.path_distance = b->path_distance;
+  v.is_ligand = b->is_ligand;
+  return v;
+}
+
+// bond separation between connection c1 of g1 and c2 of g2 (g1 != g2)
+__device__ __forceinline__ int conn_sep(const StackView& s, int g1, int g2, int c1, int c2) {
+  if (s.layout == TMOL_LJLK_PADDED) {
+    int b2 = g2 - (g2 / s.B) * s.B;
+    int64_t idx = ((int64_t(g1) * s.B + b2) * s.C + c1) * s.C + c2;
+    return s.inter_block_bondsep[idx];
+  }
+  int p1 = s.block_conn_pos[int64_t(g1) * s.C + c1];
+  int p2 = s.block_conn_pos[int64_t(g2) * s.C + c2];
+  if (p1 < 0 || p2 < 0 || s.block_segment[g1] != s.block_segment[g2]) return 6;
+  int d = p1 > p2 ? p1 - p2 : p2 - p1;
+  return d < 6 ? d : 6;
+}
+
+// minimum bond separation over connection pairs (min_block_bondsep)
+__device__ __forceinline__ int min_block_sep(const StackView& s, int g1, int g2, int nc1,
+                                             int nc2) {
+  if (s.layout == TMOL_LJLK_PADDED) {
+    int b2 = g2 - (g2 / s.B) * s.B;
+    return s.min_bondsep[int64_t(g1) * s.B + b2];
+  }
+  int m = 6;
+  for (int c1 = 0; c1 < nc1; ++c1)
+    for (int c2 = 0; c2 < nc2; ++c2) {
+      int v = conn_sep(s, g1, g2, c1, c2);
+      m = v < m ? v : m;
+    }
+  return m;
+}
+
+// Inter-block count-pair separation from LDS tables (impl.hh:47-67,
+// count_pair.hh:162-207): min over connection pairs of
+// pd1[c1][a1] + cs[c1][c2] + pd2[c2][a2], starting at 5, only when the
+// blocks' minimum separation is <= 4; both ligand fragments -> 3/4 become 5.
+__device__ __forceinline__ int inter_pair_sep(const uint8_t (*pd1)[TILE],
+                                              const uint8_t (*pd2)[TILE],
+                                              const uint8_t* cs, int min_sep, int nc1,
+                                              int nc2, int a1, int a2, bool lig) {
+  int sep = min_sep;
+  if (sep <= 4) {
+    int best = 5;
+    for (int c1 = 0; c1 < nc1; ++c1) {
+      int p1 = pd1[c1][a1];
+      for (int c2 = 0; c2 < nc2; ++c2) {
+        int v = p1 + cs[c1 * MAX_CONN + c2] + pd2[c2][a2];
+        best = v < best ? v : best;
+      }
+    }
+    sep = best;
+  }
+  if (lig && (sep == 3 || sep == 4)) sep = 5;
+  return sep;
+}
+
+// compensated (Neumaier) accumulation of one lane's pair energies
+template <typename Real>
+struct KSum {
+  Real s = Real(0), c = Real(0);
+  __device__ __forceinline__ void add(Real v) {
+    Real t = s + v;
+    c += (fabs(s) >= fabs(v)) ? ((s - t) + v) : ((v - t) + s);
+    s = t;
+  }
+  __device__ __forceinline__ double value() const { return double(s) + double(c); }
+};
+
+template <typename Real>
+struct ScoreArgs {
+  StackView s;
+  BTView bt;
+  WorkView w;
+  const Real* coords;
+  const Real* records;
+  const uint8_t* conn_pd;
+  const Real* global_params;
+  const uint32_t* counters;
+  const Real* weights;         // [3, P] or [3, P, B, B]
+  double* energies64;          // [3, P] whole pose
+  Real* energies_bp;           // [3, P, B, B] block pair
+  Real* dvdx;                  // [3, NA, 3]
+  Real* grad;                  // [NA, 3]
+  int64_t n_atoms;
+  unsigned long long* census;  // [16] with M_CENSUS
+};
+
+template <typename Real>
+__device__ __forceinline__ GlobalParams<Real> load_global_params(const Real* g) {
+  GlobalParams<Real> gp;
+  gp.max_dis = g[0];
+  gp.dlin_factor = g[1];
+  gp.hbond_dis = g[2];
+  gp.hbond_oh_dis = g[3];
+  gp.hbond_hdis = g[4];
+  gp.spline_start = gp.max_dis - Real(1.5);
+  gp.cpoly_cap = gp.max_dis - Real(0.1);
+  gp.max_dis2 = gp.max_dis * gp.max_dis;
+  return gp;
+}
+
+// XCD-aware split of [0, n) over the persistent grid: workgroups are dealt
+// round-robin to the 8 XCDs; each XCD gets a contiguous slice so that the
+// (pose-ordered) rows it walks share its L2.  Returns this wave's first index
+// and the stride.
+struct XcdSlice {
+  int64_t begin, end, first, stride;
+};
+
+__device__ __forceinline__ XcdSlice xcd_slice(int64_t n) {
+  const int n_wg = gridDim.x;
+  const int n_xcd = n_wg >= 8 ? 8 : 1;
+  const int xcd = blockIdx.x % n_xcd;
+  const int wg_in_xcd = blockIdx.x / n_xcd;
+  const int wg_per_xcd = (n_wg - xcd + n_xcd - 1) / n_xcd;
+  XcdSlice r;
+  r.begin = n * xcd / n_xcd;
+  r.end = n * (xcd + 1) / n_xcd;
+  r.first = r.begin + int64_t(wg_in_xcd) * WAVES_PER_WG + threadIdx.x / WAVE;
+  r.stride = int64_t(wg_per_xcd) * WAVES_PER_WG;
+  return r;
+}
+
+}  // namespace tmol_ljlk

@@ -32,7 +32,7 @@ LIB_PATH = os.path.join(_HERE, "libtmol_ljlk_hip.so")
 OK, EINVAL, EOVERFLOW, EHIP, ENOTSUP = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 PADDED, RAGGED = 0, 1
-DVDX, WGRAD, BLOCK_PAIR, NO_ENERGY, ACCUMULATE = 2, 4, 8, 16, 32
+DVDX, WGRAD, BLOCK_PAIR, NO_ENERGY, ACCUMULATE, GENERAL_KERNEL = 2, 4, 8, 16, 32, 128
 NBR_TILE = 64
 
 _ERRORS = {
@@ -126,6 +126,7 @@ def lib():
             L.tmol_ljlk_read_counters.argtypes = [_VP, ctypes.POINTER(ctypes.c_int64), _VP]
             L.tmol_ljlk_work_list.argtypes = [
                 SP, BP, _VP, ctypes.c_int64, ctypes.c_int32, ctypes.POINTER(_VP),
+                ctypes.POINTER(_VP),
             ]
             L.tmol_ljlk_debug_pair_separations.argtypes = [
                 SP, BP, _VP, ctypes.c_int64, ctypes.c_int32, _VP, ctypes.c_int64, _VP, _VP,
@@ -235,7 +236,8 @@ def neighbor_tiles(pose_block_start: torch.Tensor, device) -> torch.Tensor:
 
 
 class Workspace:
-    """Caller-owned device workspace + the work list capacity."""
+    """Caller-owned device workspace; ``capacity`` is the work list's row
+    capacity (most neighbor blocks g2 >= g1 of any block g1)."""
 
     def __init__(self, cstack, cbt, capacity, dtype_code, device):
         L = lib()
@@ -246,14 +248,30 @@ class Workspace:
         self.nbytes = int(nbytes)
         self.capacity = int(capacity)
         self.dtype_code = dtype_code
+        self.G = cstack.n_blocks_total
 
-    def items(self, cstack, cbt) -> torch.Tensor:
-        """View of the work list ([capacity, 2] int32) inside the workspace."""
-        ptr = _VP()
+    def rows(self, cstack, cbt):
+        """(row_count [G] int32 view, row_items [G, capacity] int32 view)."""
+        pc, pi = _VP(), _VP()
         _check(lib().tmol_ljlk_work_list(ctypes.byref(cstack), ctypes.byref(cbt), _p(self.buf),
-                                         self.capacity, self.dtype_code, ctypes.byref(ptr)), "work list")
-        off = ptr.value - self.buf.data_ptr()
-        return self.buf[off : off + 8 * self.capacity].view(torch.int32).view(self.capacity, 2)
+                                         self.capacity, self.dtype_code, ctypes.byref(pc),
+                                         ctypes.byref(pi)), "work list")
+        base = self.buf.data_ptr()
+        oc, oi = pc.value - base, pi.value - base
+        G = self.G
+        cnt = self.buf[oc : oc + 4 * G].view(torch.int32)
+        items = self.buf[oi : oi + 4 * G * self.capacity].view(torch.int32).view(G, self.capacity)
+        return cnt, items
+
+    def pairs(self, cstack, cbt) -> torch.Tensor:
+        """Work list as [n, 2] (g1, g2) pairs on the host."""
+        cnt, items = self.rows(cstack, cbt)
+        cnt = cnt.cpu().clamp(max=self.capacity)
+        items = items.cpu()
+        g1 = torch.repeat_interleave(torch.arange(self.G, dtype=torch.int32), cnt)
+        mask = torch.arange(self.capacity)[None, :] < cnt[:, None]
+        g2 = items[mask]
+        return torch.stack([g1, g2], dim=1)
 
     def counters(self, device) -> List[int]:
         out = (ctypes.c_int64 * 2)()
@@ -261,8 +279,9 @@ class Workspace:
         return [int(out[0]), int(out[1])]
 
 
-def padded_upper_triangle_capacity(P: int, B: int) -> int:
-    return P * B * (B + 1) // 2
+def padded_row_capacity(P: int, B: int) -> int:
+    # a row of a padded stack holds at most B blocks (g2 >= g1 in the pose)
+    return max(B, 1)
 
 
 class PaddedCall:
@@ -317,7 +336,7 @@ class PaddedCall:
         self.block_pair = bool(block_pair)
         pbs = torch.arange(0, P + 1, dtype=torch.int32) * B
         self.tiles = neighbor_tiles(pbs, dev)
-        cap = padded_upper_triangle_capacity(self.P, self.B)
+        cap = padded_row_capacity(self.P, self.B)
         self.ws = Workspace(self.cstack, self.bt.c, cap, self.dcode, dev)
 
     def neighbors(self, dense: Optional[torch.Tensor]):
@@ -527,12 +546,13 @@ def debug_pair_separations(args, items: torch.Tensor) -> torch.Tensor:
     return out.view(-1, 32, 32)
 
 
-def padded_work_list(args) -> torch.Tensor:
-    """Run the neighbor pass and return the work list as sorted (g1, g2)
-    pairs on the host (parity tests)."""
+def padded_work_list(args):
+    """Run the neighbor pass and return the work list as (g1, g2) pairs on
+    the host, the dense flags and the overflow flag (parity tests)."""
     call = PaddedCall(args, args[0].dtype)
     nbr = torch.empty((call.P, call.B, call.B), dtype=torch.int32, device=call.device)
     call.neighbors(nbr)
     n, overflow = call.ws.counters(call.device)
-    items = call.ws.items(call.cstack, call.bt.c)[:n].cpu()
-    return items, nbr, overflow
+    pairs = call.ws.pairs(call.cstack, call.bt.c)
+    assert pairs.shape[0] == n or overflow
+    return pairs, nbr, overflow

@@ -66,10 +66,11 @@ class LJLKRaggedScorer:
         )
         self.max_dis = term.max_dis
         self.tiles = ops.neighbor_tiles(stack.pose_block_start, dev)
-        nb = stack.n_blocks_per_pose.to(torch.int64)
-        upper = int((nb * (nb + 1) // 2).sum())
+        max_nb = int(stack.n_blocks_per_pose.max())
         if capacity is None:
-            capacity = min(upper, 1 << 28)
+            # neighbor blocks g2 >= g1 per block: bounded by the pose length;
+            # 192 covers densely packed chains (checked after the first pass)
+            capacity = min(max_nb, 192)
         self.capacity = int(capacity)
         self.ws = ops.Workspace(self.cstack, self.bt.c, self.capacity, self.dcode, dev)
         self.energies = torch.zeros((3, self.P), dtype=dtype, device=dev)
@@ -78,7 +79,7 @@ class LJLKRaggedScorer:
         self.graph = None
         self._graph_coords = None
         self._graph_weights = None
-        self._checked_capacity = capacity >= upper
+        self._checked_capacity = capacity >= max_nb
 
     # -- raw steps -----------------------------------------------------------
     def neighbors(self, coords: torch.Tensor):
