@@ -239,6 +239,7 @@ struct ScoreArgs {
   BTView bt;
   WorkView w;
   const Real* coords;
+  const Real* spheres;  // [G, 4] centre, radius (from the neighbor pass)
   const Real* records;
   const uint8_t* conn_pd;
   const Real* global_params;

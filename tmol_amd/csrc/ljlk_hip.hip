@@ -107,6 +107,7 @@ static ScoreArgs<Real> make_args(const tmol_ljlk_stack* st, const tmol_ljlk_bloc
   a.w.row_items = (const int32_t*)(w + Lw.row_items);
   a.w.row_cap = row_cap;
   a.coords = (const Real*)coords;
+  a.spheres = (const Real*)(w + Lw.spheres);
   a.records = (const Real*)(w + Lw.records);
   a.conn_pd = (const uint8_t*)(w + Lw.conn_pd);
   a.global_params = (const Real*)btp->global_params;

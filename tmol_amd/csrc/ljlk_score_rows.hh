@@ -3,49 +3,42 @@
 // neighbor block g2 >= g1 of its pose -- in two phases per block pair.
 //
 //   stage     g1's atoms once per row (LDS slots 0..31); each g2 into one of
-//             two ring slots (32..63, 64..95), its loads issued one block pair
-//             ahead
-//   phase A   all atom pairs of the block pair: squared distance and count
-//             pair only; pairs within max_dis with non-zero weight are
-//             ballot-compacted into a wave-private FIFO of LJ entries (and of
-//             LK entries when both atoms are heavy)
+//             two ring slots (32..63, 64..95)
+//   cull      atoms of g2 farther than r1 + max_dis from g1's sphere centre,
+//             and atoms of g1 farther than r2 + max_dis from g2's, cannot
+//             reach any atom of the other block: the pair loop runs over the
+//             remaining atoms only (exact: the triangle inequality, with a
+//             1e-3 A margin for rounding)
+//   phase A   remaining atom pairs: squared distance and count pair only;
+//             pairs within max_dis with non-zero weight are ballot-compacted
+//             into a wave-private FIFO of LJ entries (and of LK entries when
+//             both atoms are heavy)
 //   phase B   whenever a FIFO holds a full wave of entries (or a ring slot must
-//             be reused, or the row ends): evaluate 64 entries, one per lane,
-//             with every lane busy, block pairs mixed freely
+//             be reused, or the row ends): 64 entries, one per lane, every
+//             lane busy, block pairs mixed freely
 //
-// Energies accumulate per lane across the whole row (one pose); gradients
-// accumulate in LDS per atom slot and reach global memory once per atom,
-// term and component when a ring slot is recycled (g2) or the row ends (g1).
+// Energies accumulate per lane across the whole row (one pose).  Gradients
+// accumulate in LDS per atom slot -- R copies selected by lane so that lanes
+// sharing an atom rarely hit the same address in one instruction -- and reach
+// global memory once per atom, term and component when a ring slot is
+// recycled (g2) or the row ends (g1).  Loop bounds and FIFO counters are
+// wave-uniform scalars.
 #pragma once
 
 #include "ljlk_common.hh"
 
 namespace tmol_ljlk {
 
-constexpr int RSLOTS = 96;   // 32 (g1) + 2 x 32 (ring of g2 blocks)
-constexpr int LIST = 256;    // FIFO capacity (entries)
+constexpr int RSLOTS = 96;  // 32 (g1) + 2 x 32 (ring of g2 blocks)
+constexpr int LIST = 256;   // FIFO capacity (entries)
 
-template <typename Real, int NG>
-struct RowLDS {
-  Real pos[RSLOTS][4];  // x, y, z, lj_radius
-  Real prm[RSLOTS][4];  // sqrt(lj_wdepth), K, 1/lambda^2, lk_volume
-  int flg[RSLOTS];
-  Real grad[NG > 0 ? NG : 1][3][RSLOTS];
-  uint16_t lj[LIST];
-  uint16_t lk[LIST];
-  uint8_t pd[3][MAX_CONN][TILE];      // [0] g1, [1 + r] ring slot r
-  uint8_t cs[2][MAX_CONN * MAX_CONN];  // per ring slot
-  Real ebp[3][3];                      // block-pair energies: ring 0, ring 1, intra
-  Real wt[3][3];                       // block-pair weights:  ring 0, ring 1, intra
-};
-
-// entry: a1 (5 bits) | slot2 (7 bits) | weight-0.2 flag (bit 12)
-__device__ __forceinline__ uint16_t make_entry(int a1, int s2, bool w02) {
-  return (uint16_t)(a1 | (s2 << 5) | (w02 ? (1 << 12) : 0));
+__device__ __forceinline__ int uni(int x) { return __builtin_amdgcn_readfirstlane(x); }
+__device__ __forceinline__ int64_t uni64(int64_t x) {
+  uint32_t lo = __builtin_amdgcn_readfirstlane((uint32_t)(x & 0xffffffffu));
+  uint32_t hi = __builtin_amdgcn_readfirstlane((uint32_t)((uint64_t)x >> 32));
+  return (int64_t)(((uint64_t)hi << 32) | lo);
 }
-
-// owner of an entry's second slot: 0/1 ring slot, 2 = g1 (intra)
-__device__ __forceinline__ int entry_owner(int s2) { return s2 < TILE ? 2 : (s2 - TILE) >> 5; }
+__device__ __forceinline__ int lane_val(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
 
 template <typename Real, int MODE>
 struct RowCtx {
@@ -55,35 +48,72 @@ struct RowCtx {
   static constexpr bool NOE = MODE & M_NOE;
   static constexpr bool DERIV = DVDX || WGRAD;
   static constexpr int NG = DVDX ? 3 : (WGRAD ? 1 : 0);
+  // gradient accumulator copies: fewer when the planes are many or wide
+  static constexpr int R = !DERIV ? 1 : (DVDX ? (sizeof(Real) == 8 ? 1 : 2)
+                                              : (sizeof(Real) == 8 ? 2 : 4));
 };
 
-// Evaluate LJ entries [0, n) of the FIFO in batches of 64 (the last one may be
-// partial when `all`); returns the number consumed.
 template <typename Real, int MODE>
-__device__ __forceinline__ int drain_lj(RowLDS<Real, RowCtx<Real, MODE>::NG>& L, int n, bool all,
+struct RowLDS {
+  using C = RowCtx<Real, MODE>;
+  Real pos[RSLOTS][4];  // x, y, z, lj_radius
+  Real prm[RSLOTS][4];  // sqrt(lj_wdepth), K, 1/lambda^2, lk_volume
+  int flg[RSLOTS];
+  Real grad[C::R][C::NG > 0 ? C::NG : 1][3][C::DERIV ? RSLOTS : 1];
+  uint16_t lj[LIST];
+  uint16_t lk[LIST];
+  uint8_t idx1[TILE];                  // culled g1 atoms of the current block pair
+  uint8_t idx2[2][TILE];               // culled g2 atoms per ring slot
+  uint8_t pd[3][MAX_CONN][TILE];       // [0] g1, [1 + r] ring slot r
+  uint8_t cs[2][MAX_CONN * MAX_CONN];  // per ring slot
+  Real ebp[3][3];                      // block-pair energies: ring 0, ring 1, intra
+  Real wt[3][3];                       // block-pair weights:  ring 0, ring 1, intra
+};
+
+// entry: a1 slot (5 bits) | slot2 (7 bits) | weight-0.2 flag (bit 12)
+__device__ __forceinline__ uint16_t make_entry(int a1, int s2, bool w02) {
+  return (uint16_t)(a1 | (s2 << 5) | (w02 ? (1 << 12) : 0));
+}
+
+// owner of an entry's second slot: 0/1 ring slot, 2 = g1 (intra)
+__device__ __forceinline__ int entry_owner(int s2) { return s2 < TILE ? 2 : (s2 - TILE) >> 5; }
+
+template <typename Real, int MODE>
+__device__ __forceinline__ void grad_scatter(RowLDS<Real, MODE>& L, int term, int s1, int s2,
+                                             Real gx, Real gy, Real gz, int lane) {
+  using C = RowCtx<Real, MODE>;
+  const int c = lane & (C::R - 1);
+  atomicAdd(&L.grad[c][term][0][s1], gx);
+  atomicAdd(&L.grad[c][term][1][s1], gy);
+  atomicAdd(&L.grad[c][term][2][s1], gz);
+  atomicAdd(&L.grad[c][term][0][s2], -gx);
+  atomicAdd(&L.grad[c][term][1][s2], -gy);
+  atomicAdd(&L.grad[c][term][2][s2], -gz);
+}
+
+// Evaluate LJ entries [0, consume) of the FIFO, one per lane.
+template <typename Real, int MODE>
+__device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
                                         const GlobalParams<Real>& gp, const Real* wrow,
                                         KSum<Real>& e_atr, KSum<Real>& e_rep, int lane) {
   using C = RowCtx<Real, MODE>;
-  int consume = all ? n : (n & ~(WAVE - 1));
   for (int i0 = 0; i0 < consume; i0 += WAVE) {
-    int i = i0 + lane;
+    const int i = i0 + lane;
     if (i < consume) {
-      int e = L.lj[i];
-      int s1 = e & 31, s2 = (e >> 5) & 127;
-      Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
-      Real x1 = L.pos[s1][0], y1 = L.pos[s1][1], z1 = L.pos[s1][2], r1 = L.pos[s1][3];
-      Real x2 = L.pos[s2][0], y2 = L.pos[s2][1], z2 = L.pos[s2][2], r2 = L.pos[s2][3];
-      Real dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
-      Real d2 = dx * dx + dy * dy + dz * dz;
-      Real dist = fast_sqrt(d2);
-      int f1 = L.flg[s1], f2 = L.flg[s2];
-      Real sigma = lj_sigma(f1, f2, r1, r2, gp);
-      Real eps = L.prm[s1][0] * L.prm[s2][0];
+      const int e = L.lj[i];
+      const int s1 = e & 31, s2 = (e >> 5) & 127;
+      const Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
+      const Real x1 = L.pos[s1][0], y1 = L.pos[s1][1], z1 = L.pos[s1][2], r1 = L.pos[s1][3];
+      const Real x2 = L.pos[s2][0], y2 = L.pos[s2][1], z2 = L.pos[s2][2], r2 = L.pos[s2][3];
+      const Real dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+      const Real dist = fast_sqrt(dx * dx + dy * dy + dz * dz);
+      const Real sigma = lj_sigma(L.flg[s1], L.flg[s2], r1, r2, gp);
+      const Real eps = L.prm[s1][0] * L.prm[s2][0];
       Real atr, rep, dlj;
       lj_pair<Real, C::DERIV>(dist, sigma, eps, wcp, gp, atr, rep, dlj);
       if (!C::NOE) {
         if (C::BP) {
-          int ow = entry_owner(s2);
+          const int ow = entry_owner(s2);
           atomicAdd(&L.ebp[ow][0], atr);
           atomicAdd(&L.ebp[ow][1], rep);
         } else {
@@ -92,60 +122,45 @@ __device__ __forceinline__ int drain_lj(RowLDS<Real, RowCtx<Real, MODE>::NG>& L,
         }
       }
       if (C::DERIV && dlj != Real(0) && dist > Real(0)) {
-        bool inside = dist < sigma;
+        const bool inside = dist < sigma;
         Real gs = dlj * fast_rcp(dist);
         int term = 0;
-        if (C::DVDX) {
+        if (C::DVDX)
           term = inside ? 1 : 0;
-        } else {
-          Real wv;
-          if (C::BP)
-            wv = L.wt[entry_owner(s2)][inside ? 1 : 0];
-          else
-            wv = wrow[inside ? 1 : 0];
-          gs *= wv;
-        }
-        Real gx = gs * dx, gy = gs * dy, gz = gs * dz;
-        atomicAdd(&L.grad[term][0][s1], gx);
-        atomicAdd(&L.grad[term][1][s1], gy);
-        atomicAdd(&L.grad[term][2][s1], gz);
-        atomicAdd(&L.grad[term][0][s2], -gx);
-        atomicAdd(&L.grad[term][1][s2], -gy);
-        atomicAdd(&L.grad[term][2][s2], -gz);
+        else
+          gs *= C::BP ? L.wt[entry_owner(s2)][inside ? 1 : 0] : wrow[inside ? 1 : 0];
+        grad_scatter(L, term, s1, s2, gs * dx, gs * dy, gs * dz, lane);
       }
     }
   }
-  return consume;
 }
 
 template <typename Real, int MODE>
-__device__ __forceinline__ int drain_lk(RowLDS<Real, RowCtx<Real, MODE>::NG>& L, int n, bool all,
+__device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
                                         const GlobalParams<Real>& gp, const Real* wrow,
                                         KSum<Real>& e_lk, int lane) {
   using C = RowCtx<Real, MODE>;
-  int consume = all ? n : (n & ~(WAVE - 1));
   for (int i0 = 0; i0 < consume; i0 += WAVE) {
-    int i = i0 + lane;
+    const int i = i0 + lane;
     if (i < consume) {
-      int e = L.lk[i];
-      int s1 = e & 31, s2 = (e >> 5) & 127;
-      Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
-      Real x1 = L.pos[s1][0], y1 = L.pos[s1][1], z1 = L.pos[s1][2], r1 = L.pos[s1][3];
-      Real x2 = L.pos[s2][0], y2 = L.pos[s2][1], z2 = L.pos[s2][2], r2 = L.pos[s2][3];
-      Real dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
-      Real d2 = dx * dx + dy * dy + dz * dz;
-      Real dist = fast_sqrt(d2);
-      int f1 = L.flg[s1], f2 = L.flg[s2];
-      Real sigma = lj_sigma(f1, f2, r1, r2, gp);
-      bool cc = (f1 & F_CARBON_LK) && (f2 & F_CARBON_LK);
+      const int e = L.lk[i];
+      const int s1 = e & 31, s2 = (e >> 5) & 127;
+      const Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
+      const Real x1 = L.pos[s1][0], y1 = L.pos[s1][1], z1 = L.pos[s1][2], r1 = L.pos[s1][3];
+      const Real x2 = L.pos[s2][0], y2 = L.pos[s2][1], z2 = L.pos[s2][2], r2 = L.pos[s2][3];
+      const Real dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+      const Real dist = fast_sqrt(dx * dx + dy * dy + dz * dz);
+      const int f1 = L.flg[s1], f2 = L.flg[s2];
+      const Real sigma = lj_sigma(f1, f2, r1, r2, gp);
+      const bool cc = (f1 & F_CARBON_LK) && (f2 & F_CARBON_LK);
       Real d_min, cdmin, cdmax;
       lk_knots(sigma, cc, d_min, cdmin, cdmax);
-      Real K1 = L.prm[s1][1], il1 = L.prm[s1][2], v1 = L.prm[s1][3];
-      Real K2 = L.prm[s2][1], il2 = L.prm[s2][2], v2 = L.prm[s2][3];
+      const Real K1 = L.prm[s1][1], il1 = L.prm[s1][2], v1 = L.prm[s1][3];
+      const Real K2 = L.prm[s2][1], il2 = L.prm[s2][2], v2 = L.prm[s2][3];
       Real va, dva, vb, dvb;
       lk_dir<Real, C::DERIV>(dist, d_min, cdmin, cdmax, -v2 * K1, r1, il1, gp, va, dva);
       lk_dir<Real, C::DERIV>(dist, d_min, cdmin, cdmax, -v1 * K2, r2, il2, gp, vb, dvb);
-      Real lk = wcp * (va + vb);
+      const Real lk = wcp * (va + vb);
       if (!C::NOE) {
         if (C::BP)
           atomicAdd(&L.ebp[entry_owner(s2)][2], lk);
@@ -153,61 +168,53 @@ __device__ __forceinline__ int drain_lk(RowLDS<Real, RowCtx<Real, MODE>::NG>& L,
           e_lk.add(lk);
       }
       if (C::DERIV && dist > Real(0)) {
-        Real dlk = wcp * (dva + dvb);
+        const Real dlk = wcp * (dva + dvb);
         if (dlk != Real(0)) {
           Real gs = dlk * fast_rcp(dist);
-          int term = C::DVDX ? 2 : 0;
           if (C::WGRAD) gs *= C::BP ? L.wt[entry_owner(s2)][2] : wrow[2];
-          Real gx = gs * dx, gy = gs * dy, gz = gs * dz;
-          atomicAdd(&L.grad[term][0][s1], gx);
-          atomicAdd(&L.grad[term][1][s1], gy);
-          atomicAdd(&L.grad[term][2][s1], gz);
-          atomicAdd(&L.grad[term][0][s2], -gx);
-          atomicAdd(&L.grad[term][1][s2], -gy);
-          atomicAdd(&L.grad[term][2][s2], -gz);
+          grad_scatter(L, C::DVDX ? 2 : 0, s1, s2, gs * dx, gs * dy, gs * dz, lane);
         }
       }
     }
   }
-  return consume;
 }
 
-// move the unconsumed tail [consumed, n) of a FIFO to its front
+// move the unconsumed tail [consumed, n) of a FIFO to its front (rem < 64)
 __device__ __forceinline__ void fifo_shift(uint16_t* list, int consumed, int n, int lane) {
-  int rem = n - consumed;
+  const int rem = n - consumed;
   if (consumed == 0 || rem <= 0) return;
-  // rem < 64 except after a partial-free drain of a full list; loop for safety
-  for (int b = 0; b < rem; b += WAVE) {
-    uint16_t v = 0;
-    if (b + lane < rem) v = list[consumed + b + lane];
-    wave_sync();
-    if (b + lane < rem) list[b + lane] = v;
-    wave_sync();
-  }
+  uint16_t v = 0;
+  if (lane < rem) v = list[consumed + lane];
+  wave_sync();
+  if (lane < rem) list[lane] = v;
+  wave_sync();
 }
 
 // flush gradients of slots [slot0, slot0 + n) to atoms [atom0, atom0 + n), zero them
 template <typename Real, int MODE>
-__device__ __forceinline__ void flush_slots(RowLDS<Real, RowCtx<Real, MODE>::NG>& L,
-                                            const ScoreArgs<Real>& A, int slot0, int n,
-                                            int64_t atom0, int lane) {
+__device__ __forceinline__ void flush_slots(RowLDS<Real, MODE>& L, const ScoreArgs<Real>& A,
+                                            int slot0, int n, int64_t atom0, int lane) {
   using C = RowCtx<Real, MODE>;
   if (!C::DERIV) return;
   if (lane < n) {
-    int slot = slot0 + lane;
-    int64_t atom = atom0 + lane;
+    const int slot = slot0 + lane;
+    const int64_t atom = atom0 + lane;
 #pragma unroll
     for (int t = 0; t < C::NG; ++t)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        Real v = L.grad[t][k][slot];
+        Real v = Real(0);
+#pragma unroll
+        for (int c = 0; c < C::R; ++c) {
+          v += L.grad[c][t][k][slot];
+          L.grad[c][t][k][slot] = Real(0);
+        }
         if (v != Real(0)) {
           if (C::DVDX)
             unsafeAtomicAdd(&A.dvdx[(int64_t(t) * A.n_atoms + atom) * 3 + k], v);
           else
             unsafeAtomicAdd(&A.grad[atom * 3 + k], v);
         }
-        L.grad[t][k][slot] = Real(0);
       }
   }
 }
@@ -215,32 +222,34 @@ __device__ __forceinline__ void flush_slots(RowLDS<Real, RowCtx<Real, MODE>::NG>
 template <typename Real, int MODE>
 __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
   using C = RowCtx<Real, MODE>;
-  __shared__ RowLDS<Real, C::NG> lds_all[WAVES_PER_WG];
+  __shared__ RowLDS<Real, MODE> lds_all[WAVES_PER_WG];
   const int lane = lane_id();
-  RowLDS<Real, C::NG>& L = lds_all[threadIdx.x / WAVE];
+  RowLDS<Real, MODE>& L = lds_all[uni(threadIdx.x / WAVE)];
   const GlobalParams<Real> gp = load_global_params(A.global_params);
   const StackView& s = A.s;
   const BTView& bt = A.bt;
   const XcdSlice sl = xcd_slice(s.G);
+  const Real cull_pad = gp.max_dis + Real(1e-3);
 
-  // zero the gradient accumulators once; flushes re-zero what they read
   if (C::DERIV) {
-    for (int i = lane; i < (C::NG > 0 ? C::NG : 1) * 3 * RSLOTS; i += WAVE)
-      (&L.grad[0][0][0])[i] = Real(0);
+    Real* g = &L.grad[0][0][0][0];
+    for (int i = lane; i < C::R * (C::NG > 0 ? C::NG : 1) * 3 * RSLOTS; i += WAVE) g[i] = Real(0);
   }
   wave_sync();
 
-  for (int64_t row = sl.first; row < sl.end; row += sl.stride) {
-    int n_items = (int)min<int64_t>(A.w.row_count[row], A.w.row_cap);
+  for (int64_t row = uni64(sl.first); row < sl.end; row += sl.stride) {
+    const int n_items = uni((int)min<int64_t>(A.w.row_count[row], A.w.row_cap));
     if (n_items == 0) continue;
     const int g1 = (int)row;
-    const int t1 = s.block_type[g1];
-    const int n1 = bt.n_atoms[t1];
-    const int64_t o1 = s.block_atom_offset[g1];
-    const int p = s.pose_of(g1);
-    const int nc1 = bt.n_conn[t1];
+    const int t1 = uni(s.block_type[g1]);
+    const int n1 = uni(bt.n_atoms[t1]);
+    const int64_t o1 = uni64(s.block_atom_offset[g1]);
+    const int p = uni(s.pose_of(g1));
+    const int nc1 = uni(bt.n_conn[t1]);
     const bool lig1 = bt.is_ligand[t1] != 0;
     const int b1 = g1 - p * s.B;  // block index within the pose (padded layout)
+    const Real c1x = A.spheres[4 * int64_t(g1) + 0], c1y = A.spheres[4 * int64_t(g1) + 1];
+    const Real c1z = A.spheres[4 * int64_t(g1) + 2], c1r = A.spheres[4 * int64_t(g1) + 3];
     Real wrow[3] = {Real(0), Real(0), Real(0)};
     if (C::WGRAD && !C::BP) {
       wrow[0] = A.weights[p];
@@ -248,193 +257,236 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
       wrow[2] = A.weights[2 * s.P + p];
     }
     // ---- stage g1 into slots 0..31
-    if (lane < TILE) {
-      int a = lane;
-      if (a < n1) {
-        const Real* xyz = A.coords + (o1 + a) * 3;
-        const Real* r = A.records + (int64_t(t1) * bt.Amax + a) * REC;
-        L.pos[a][0] = xyz[0];
-        L.pos[a][1] = xyz[1];
-        L.pos[a][2] = xyz[2];
-        L.pos[a][3] = r[0];
-        L.prm[a][0] = r[1];
-        L.prm[a][1] = r[2];
-        L.prm[a][2] = r[3];
-        L.prm[a][3] = r[4];
-        L.flg[a] = int(r[5]);
-        for (int c = 0; c < MAX_CONN; ++c)
-          L.pd[0][c][a] = A.conn_pd[(int64_t(t1) * MAX_CONN + c) * bt.Amax + a];
-      }
+    bool heavy1 = false;
+    if (lane < n1) {
+      const int a = lane;
+      const Real* xyz = A.coords + (o1 + a) * 3;
+      const Real* r = A.records + (int64_t(t1) * bt.Amax + a) * REC;
+      L.pos[a][0] = xyz[0];
+      L.pos[a][1] = xyz[1];
+      L.pos[a][2] = xyz[2];
+      L.pos[a][3] = r[0];
+      L.prm[a][0] = r[1];
+      L.prm[a][1] = r[2];
+      L.prm[a][2] = r[3];
+      L.prm[a][3] = r[4];
+      const int f = int(r[5]);
+      L.flg[a] = f;
+      heavy1 = (f & F_HEAVY) != 0;
+      for (int c = 0; c < MAX_CONN; ++c)
+        L.pd[0][c][a] = A.conn_pd[(int64_t(t1) * MAX_CONN + c) * bt.Amax + a];
     }
+    const uint32_t hmask1 = (uint32_t)__ballot(heavy1);
     if (C::BP && lane < 9) (&L.ebp[0][0])[lane] = Real(0);
     KSum<Real> e_atr, e_rep, e_lk;
     // FIFO state (wave-uniform)
-    int n_lj = 0, n_lk = 0;          // pending entries
-    int64_t seq_lj = 0, seq_lk = 0;  // entries ever appended (this row)
-    int64_t done_lj = 0, done_lk = 0;
-    int64_t end_lj[2] = {0, 0}, end_lk[2] = {0, 0};  // last seq + 1 per ring slot
-    bool slot_used[2] = {false, false};
-    int64_t slot_o2[2] = {0, 0};
-    int slot_n2[2] = {0, 0};
-    int slot_b2[2] = {0, 0};
+    int n_lj = 0, n_lk = 0;    // pending entries
+    int seq_lj = 0, seq_lk = 0;  // entries appended in this row
+    int done_lj = 0, done_lk = 0;
+    int end_lj0 = 0, end_lj1 = 0, end_lk0 = 0, end_lk1 = 0;
+    bool used0 = false, used1 = false;
+    int64_t o2s0 = 0, o2s1 = 0;
+    int n2s0 = 0, n2s1 = 0, b2s0 = 0, b2s1 = 0;
     int ring = 0;
 
     auto drain = [&](bool all) {
-      int c1 = drain_lj<Real, MODE>(L, n_lj, all, gp, wrow, e_atr, e_rep, lane);
+      const int c1 = all ? n_lj : (n_lj & ~(WAVE - 1));
+      const int c2 = all ? n_lk : (n_lk & ~(WAVE - 1));
+      if (c1 > 0) eval_lj<Real, MODE>(L, c1, gp, wrow, e_atr, e_rep, lane);
+      if (c2 > 0) eval_lk<Real, MODE>(L, c2, gp, wrow, e_lk, lane);
       wave_sync();
       fifo_shift(L.lj, c1, n_lj, lane);
+      fifo_shift(L.lk, c2, n_lk, lane);
       n_lj -= c1;
       done_lj += c1;
-      int c2 = drain_lk<Real, MODE>(L, n_lk, all, gp, wrow, e_lk, lane);
-      wave_sync();
-      fifo_shift(L.lk, c2, n_lk, lane);
       n_lk -= c2;
       done_lk += c2;
     };
-    auto retire_slot = [&](int r) {
+    auto retire = [&](int r) {
       // all entries of ring slot r are consumed: flush its g2 atoms
-      flush_slots<Real, MODE>(L, A, TILE + TILE * r, slot_n2[r], slot_o2[r], lane);
+      flush_slots<Real, MODE>(L, A, TILE + TILE * r, r ? n2s1 : n2s0, r ? o2s1 : o2s0, lane);
       if (C::BP && !C::NOE && lane < 3) {
-        int64_t PBB = int64_t(s.P) * s.B * s.B;
-        int64_t base = (int64_t(p) * s.B + b1) * s.B + slot_b2[r];
+        const int64_t PBB = int64_t(s.P) * s.B * s.B;
+        const int64_t base = (int64_t(p) * s.B + b1) * s.B + (r ? b2s1 : b2s0);
         A.energies_bp[lane * PBB + base] = L.ebp[r][lane];
         L.ebp[r][lane] = Real(0);
       }
-      slot_used[r] = false;
+      if (r)
+        used1 = false;
+      else
+        used0 = false;
     };
 
     for (int c0 = 0; c0 < n_items; c0 += WAVE) {
       // ---- per-item descriptors, one item per lane
       const int n_chunk = min(WAVE, n_items - c0);
-      int d_g2 = g1, d_t2 = t1, d_n2 = 0, d_min = 6;
-      int64_t d_o2 = 0;
+      int d_g2 = g1, d_min = 6;
       if (lane < n_chunk) {
         d_g2 = A.w.row_items[row * A.w.row_cap + c0 + lane];
-        d_t2 = s.block_type[d_g2];
-        d_o2 = s.block_atom_offset[d_g2];
-        d_n2 = bt.n_atoms[d_t2];
-        d_min = d_g2 == g1 ? 0 : min_block_sep(s, g1, d_g2, nc1, bt.n_conn[d_t2]);
+        const int dt = s.block_type[d_g2];
+        d_min = d_g2 == g1 ? 0 : min_block_sep(s, g1, d_g2, nc1, bt.n_conn[dt]);
       }
       for (int j = 0; j < n_chunk; ++j) {
-        const int g2 = __shfl(d_g2, j, WAVE);
-        const int t2 = __shfl(d_t2, j, WAVE);
-        const int n2 = __shfl(d_n2, j, WAVE);
-        const int min_sep = __shfl(d_min, j, WAVE);
-        const int64_t o2 = __shfl(d_o2, j, WAVE);
+        const int g2 = lane_val(d_g2, j);
+        const int min_sep = lane_val(d_min, j);
         if (g2 == g1) {
           // ---- intra-block pairs a1 < a2, separation = path distance
           if (C::BP && C::WGRAD && lane < 3) {
-            int64_t PBB = int64_t(s.P) * s.B * s.B;
+            const int64_t PBB = int64_t(s.P) * s.B * s.B;
             L.wt[2][lane] = A.weights[lane * PBB + (int64_t(p) * s.B + b1) * s.B + b1];
           }
           wave_sync();
           const int total = n1 * n1;
+          const int q = WAVE / n1, rstep = WAVE - q * n1;
+          int a2 = lane % n1, a1 = lane / n1;
           for (int k0 = 0; k0 < total; k0 += WAVE) {
             if (n_lj > LIST - WAVE || n_lk > LIST - WAVE) drain(false);
-            int k = k0 + lane;
-            int a1 = k / n1, a2 = k - (k / n1) * n1;
             bool in = false, both_heavy = false, w02 = false;
-            if (k < total && a1 < a2) {
-              int sep = bt.path_distance[(int64_t(t1) * bt.Amax + a1) * bt.Amax + a2];
-              Real dx = L.pos[a1][0] - L.pos[a2][0];
-              Real dy = L.pos[a1][1] - L.pos[a2][1];
-              Real dz = L.pos[a1][2] - L.pos[a2][2];
-              Real d2 = dx * dx + dy * dy + dz * dz;
-              in = sep >= 4 && d2 < gp.max_dis2;
+            if (a1 < a2 && a1 < n1) {
+              const int sep = bt.path_distance[(int64_t(t1) * bt.Amax + a1) * bt.Amax + a2];
+              const Real dx = L.pos[a1][0] - L.pos[a2][0];
+              const Real dy = L.pos[a1][1] - L.pos[a2][1];
+              const Real dz = L.pos[a1][2] - L.pos[a2][2];
+              in = sep >= 4 && dx * dx + dy * dy + dz * dz < gp.max_dis2;
               w02 = sep == 4;
-              both_heavy = in && (L.flg[a1] & L.flg[a2] & F_HEAVY);
+              both_heavy = in && ((hmask1 >> a1) & (hmask1 >> a2) & 1u);
             }
-            uint64_t m = __ballot(in);
+            const uint64_t m = __ballot(in);
             if (in) L.lj[n_lj + lanes_below(m)] = make_entry(a1, a2, w02);
             n_lj += __popcll(m);
             seq_lj += __popcll(m);
-            uint64_t mh = __ballot(both_heavy);
+            const uint64_t mh = __ballot(both_heavy);
             if (both_heavy) L.lk[n_lk + lanes_below(mh)] = make_entry(a1, a2, w02);
             n_lk += __popcll(mh);
             seq_lk += __popcll(mh);
+            a2 += rstep;
+            a1 += q;
+            if (a2 >= n1) {
+              a2 -= n1;
+              a1 += 1;
+            }
             wave_sync();
           }
         } else {
           // ---- inter-block pairs through ring slot `ring`
           const int r = ring;
           ring ^= 1;
-          if (slot_used[r]) {
-            if (end_lj[r] > done_lj || end_lk[r] > done_lk) drain(true);
-            retire_slot(r);
+          if (r ? used1 : used0) {
+            const int end_lj = r ? end_lj1 : end_lj0;
+            const int end_lk = r ? end_lk1 : end_lk0;
+            if (end_lj > done_lj || end_lk > done_lk) drain(true);
+            retire(r);
           }
+          const int t2 = uni(s.block_type[g2]);
+          const int n2 = uni(bt.n_atoms[t2]);
+          const int64_t o2 = uni64(s.block_atom_offset[g2]);
           const int sb = TILE + TILE * r;
           const bool need_cp = min_sep <= 4;
-          const int nc2 = bt.n_conn[t2];
+          const int nc2 = uni(bt.n_conn[t2]);
           const bool lig = lig1 && bt.is_ligand[t2] != 0;
+          const Real c2x = A.spheres[4 * int64_t(g2) + 0], c2y = A.spheres[4 * int64_t(g2) + 1];
+          const Real c2z = A.spheres[4 * int64_t(g2) + 2], c2r = A.spheres[4 * int64_t(g2) + 3];
+          const Real reach1 = c1r + cull_pad, reach2 = c2r + cull_pad;
+          bool near2 = false, heavy2 = false;
           if (lane < n2) {
-            int a = lane;
+            const int a = lane;
             const Real* xyz = A.coords + (o2 + a) * 3;
             const Real* rr = A.records + (int64_t(t2) * bt.Amax + a) * REC;
-            L.pos[sb + a][0] = xyz[0];
-            L.pos[sb + a][1] = xyz[1];
-            L.pos[sb + a][2] = xyz[2];
+            const Real x = xyz[0], y = xyz[1], z = xyz[2];
+            L.pos[sb + a][0] = x;
+            L.pos[sb + a][1] = y;
+            L.pos[sb + a][2] = z;
             L.pos[sb + a][3] = rr[0];
             L.prm[sb + a][0] = rr[1];
             L.prm[sb + a][1] = rr[2];
             L.prm[sb + a][2] = rr[3];
             L.prm[sb + a][3] = rr[4];
-            L.flg[sb + a] = int(rr[5]);
+            const int f = int(rr[5]);
+            L.flg[sb + a] = f;
+            heavy2 = (f & F_HEAVY) != 0;
             if (need_cp)
               for (int c = 0; c < MAX_CONN; ++c)
                 L.pd[1 + r][c][a] = A.conn_pd[(int64_t(t2) * MAX_CONN + c) * bt.Amax + a];
+            const Real ex = x - c1x, ey = y - c1y, ez = z - c1z;
+            near2 = ex * ex + ey * ey + ez * ez < reach1 * reach1;
+          } else if (lane >= TILE && lane - TILE < n1) {
+            // g1 atoms within reach of g2's sphere
+            const int a = lane - TILE;
+            const Real ex = L.pos[a][0] - c2x, ey = L.pos[a][1] - c2y, ez = L.pos[a][2] - c2z;
+            near2 = ex * ex + ey * ey + ez * ez < reach2 * reach2;
+          }
+          const uint64_t mnear = __ballot(near2);
+          const uint32_t hmask2 = (uint32_t)__ballot(heavy2);
+          const uint32_t m2 = (uint32_t)mnear, m1 = (uint32_t)(mnear >> 32);
+          if (near2) {
+            if (lane < TILE)
+              L.idx2[r][__builtin_amdgcn_mbcnt_lo(m2, 0)] = (uint8_t)lane;
+            else
+              L.idx1[__builtin_amdgcn_mbcnt_hi(m1, 0)] = (uint8_t)(lane - TILE);
           }
           if (need_cp && lane >= TILE && lane < TILE + MAX_CONN * MAX_CONN) {
-            int q = lane - TILE;
-            int c1 = q / MAX_CONN, c2 = q % MAX_CONN;
-            L.cs[r][q] = (c1 < nc1 && c2 < nc2) ? (uint8_t)conn_sep(s, g1, g2, c1, c2) : 6;
+            const int qq = lane - TILE;
+            const int cc1 = qq / MAX_CONN, cc2 = qq % MAX_CONN;
+            L.cs[r][qq] = (cc1 < nc1 && cc2 < nc2) ? (uint8_t)conn_sep(s, g1, g2, cc1, cc2) : 6;
           }
           const int b2 = g2 - p * s.B;
           if (C::BP && C::WGRAD && lane < 3) {
-            int64_t PBB = int64_t(s.P) * s.B * s.B;
+            const int64_t PBB = int64_t(s.P) * s.B * s.B;
             L.wt[r][lane] = A.weights[lane * PBB + (int64_t(p) * s.B + b1) * s.B + b2];
           }
-          slot_used[r] = true;
-          slot_o2[r] = o2;
-          slot_n2[r] = n2;
-          slot_b2[r] = b2;
-          wave_sync();
-          // phase A: n1 x n2 pairs, k = lane + 64 it, a1 = k / n2, a2 = k % n2
-          const int total = n1 * n2;
-          const int q = WAVE / n2, rstep = WAVE - q * n2;
-          int a2 = lane % n2, a1 = lane / n2;
-          for (int k0 = 0; k0 < total; k0 += WAVE) {
-            if (n_lj > LIST - WAVE || n_lk > LIST - WAVE) drain(false);
-            bool in = false, both_heavy = false, w02 = false;
-            if (a1 < n1) {
-              Real dx = L.pos[a1][0] - L.pos[sb + a2][0];
-              Real dy = L.pos[a1][1] - L.pos[sb + a2][1];
-              Real dz = L.pos[a1][2] - L.pos[sb + a2][2];
-              Real d2 = dx * dx + dy * dy + dz * dz;
-              int sep = need_cp ? inter_pair_sep(L.pd[0], L.pd[1 + r], L.cs[r], min_sep, nc1,
-                                                 nc2, a1, a2, lig)
-                                : min_sep;
-              in = sep >= 4 && d2 < gp.max_dis2;
-              w02 = sep == 4;
-              both_heavy = in && (L.flg[a1] & L.flg[sb + a2] & F_HEAVY);
-            }
-            uint64_t m = __ballot(in);
-            if (in) L.lj[n_lj + lanes_below(m)] = make_entry(a1, sb + a2, w02);
-            n_lj += __popcll(m);
-            seq_lj += __popcll(m);
-            uint64_t mh = __ballot(both_heavy);
-            if (both_heavy) L.lk[n_lk + lanes_below(mh)] = make_entry(a1, sb + a2, w02);
-            n_lk += __popcll(mh);
-            seq_lk += __popcll(mh);
-            a2 += rstep;
-            a1 += q;
-            if (a2 >= n2) {
-              a2 -= n2;
-              a1 += 1;
-            }
-            wave_sync();
+          if (r) {
+            used1 = true; o2s1 = o2; n2s1 = n2; b2s1 = b2;
+          } else {
+            used0 = true; o2s0 = o2; n2s0 = n2; b2s0 = b2;
           }
-          end_lj[r] = seq_lj;
-          end_lk[r] = seq_lk;
+          wave_sync();
+          // phase A over the culled atoms: k = lane + 64 it, t = k / n2c, u = k % n2c
+          const int n1c = __popc(m1), n2c = __popc(m2);
+          const int total = n1c * n2c;
+          if (total > 0) {
+            const int q = WAVE / n2c, rstep = WAVE - q * n2c;
+            int u = lane % n2c, t = lane / n2c;
+            for (int k0 = 0; k0 < total; k0 += WAVE) {
+              if (n_lj > LIST - WAVE || n_lk > LIST - WAVE) drain(false);
+              bool in = false, both_heavy = false, w02 = false;
+              int a1 = 0, a2 = 0;
+              if (t < n1c) {
+                a1 = L.idx1[t];
+                a2 = L.idx2[r][u];
+                const Real dx = L.pos[a1][0] - L.pos[sb + a2][0];
+                const Real dy = L.pos[a1][1] - L.pos[sb + a2][1];
+                const Real dz = L.pos[a1][2] - L.pos[sb + a2][2];
+                const int sep = need_cp ? inter_pair_sep(L.pd[0], L.pd[1 + r], L.cs[r], min_sep,
+                                                         nc1, nc2, a1, a2, lig)
+                                        : min_sep;
+                in = sep >= 4 && dx * dx + dy * dy + dz * dz < gp.max_dis2;
+                w02 = sep == 4;
+                both_heavy = in && ((hmask1 >> a1) & (hmask2 >> a2) & 1u);
+              }
+              const uint64_t m = __ballot(in);
+              if (in) L.lj[n_lj + lanes_below(m)] = make_entry(a1, sb + a2, w02);
+              n_lj += __popcll(m);
+              seq_lj += __popcll(m);
+              const uint64_t mh = __ballot(both_heavy);
+              if (both_heavy) L.lk[n_lk + lanes_below(mh)] = make_entry(a1, sb + a2, w02);
+              n_lk += __popcll(mh);
+              seq_lk += __popcll(mh);
+              u += rstep;
+              t += q;
+              if (u >= n2c) {
+                u -= n2c;
+                t += 1;
+              }
+              wave_sync();
+            }
+          }
+          if (r) {
+            end_lj1 = seq_lj;
+            end_lk1 = seq_lk;
+          } else {
+            end_lj0 = seq_lj;
+            end_lk0 = seq_lk;
+          }
         }
         // evaluate whatever fills whole waves
         if (n_lj >= WAVE || n_lk >= WAVE) drain(false);
@@ -442,19 +494,19 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
     }
     // ---- row end: evaluate the rest, flush everything
     drain(true);
-    for (int r = 0; r < 2; ++r)
-      if (slot_used[r]) retire_slot(r);
+    if (used0) retire(0);
+    if (used1) retire(1);
     flush_slots<Real, MODE>(L, A, 0, n1, o1, lane);
     if (C::BP && !C::NOE && lane < 3) {
-      int64_t PBB = int64_t(s.P) * s.B * s.B;
-      int64_t base = (int64_t(p) * s.B + b1) * s.B + b1;
+      const int64_t PBB = int64_t(s.P) * s.B * s.B;
+      const int64_t base = (int64_t(p) * s.B + b1) * s.B + b1;
       A.energies_bp[lane * PBB + base] = L.ebp[2][lane];
       L.ebp[2][lane] = Real(0);
     }
     if (!C::NOE && !C::BP) {
-      double ea = wave_sum(e_atr.value());
-      double er = wave_sum(e_rep.value());
-      double el = wave_sum(e_lk.value());
+      const double ea = wave_sum(e_atr.value());
+      const double er = wave_sum(e_rep.value());
+      const double el = wave_sum(e_lk.value());
       if (lane == 0) {
         unsafeAtomicAdd(&A.energies64[p], ea);
         unsafeAtomicAdd(&A.energies64[s.P + p], er);
