@@ -221,6 +221,38 @@ __device__ __forceinline__ int inter_pair_sep(const uint8_t (*pd1)[TILE],
   return sep;
 }
 
+// Accumulator for wave-private LDS sums (atom gradients, block-pair
+// energies).  On gfx950 a float LDS atomic (ds_add_f32) costs ~190 CU cycles
+// per wave instruction against ~6 for a 64-bit integer one (ds_add_u64), so
+// float contributions are added as 64-bit fixed point with 2^-28 resolution:
+// exact and independent of summation order.  The encoding rounds to nearest
+// via the 1.5 * 2^52 magic constant and saturates a single contribution at
+// +-2^22 (4.2e6, orders of magnitude above any pair term); sums may reach
+// 2^35.  Double contributions accumulate as doubles.
+template <typename Real>
+struct LdsAcc;
+
+template <>
+struct LdsAcc<float> {
+  using T = unsigned long long;
+  __device__ static __forceinline__ T enc(float v) {
+    double d = double(v) * 268435456.0;  // 2^28
+    d = fmin(fmax(d, -1125899906842624.0), 1125899906842624.0);  // +-2^50
+    d = d + 6755399441055744.0;                                   // 1.5 * 2^52
+    return (T)(__double_as_longlong(d) - 0x4338000000000000LL);
+  }
+  __device__ static __forceinline__ float dec(T x) {
+    return float(double((long long)x) * (1.0 / 268435456.0));
+  }
+};
+
+template <>
+struct LdsAcc<double> {
+  using T = double;
+  __device__ static __forceinline__ T enc(double v) { return v; }
+  __device__ static __forceinline__ double dec(T x) { return x; }
+};
+
 // compensated (Neumaier) accumulation of one lane's pair energies
 template <typename Real>
 struct KSum {

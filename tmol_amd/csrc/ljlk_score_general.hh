@@ -11,22 +11,29 @@ namespace tmol_ljlk {
 
 template <typename Real>
 struct GenLDS {
+  typename LdsAcc<Real>::T grad[3][3][2 * TILE];  // [term][component][slot]
   Real x[2 * TILE], y[2 * TILE], z[2 * TILE];
   Real rad[2 * TILE], sw[2 * TILE], K[2 * TILE], il2[2 * TILE], vol[2 * TILE];
   int flg[2 * TILE];
-  Real grad[3][3][2 * TILE];  // [term][component][slot]
   uint8_t heavy[2][TILE];
   uint8_t pd[2][MAX_CONN][TILE];
   uint8_t cs[MAX_CONN * MAX_CONN];
   int n_heavy[2];
 };
 
+// +g to slot i1, -g to slot i2 (exact negation of the encoded value)
 template <typename Real>
-__device__ __forceinline__ void grad_add(GenLDS<Real>& L, int term, int slot,
-                                         Real gx, Real gy, Real gz) {
-  atomicAdd(&L.grad[term][0][slot], gx);
-  atomicAdd(&L.grad[term][1][slot], gy);
-  atomicAdd(&L.grad[term][2][slot], gz);
+__device__ __forceinline__ void grad_pair(GenLDS<Real>& L, int term, int i1, int i2,
+                                          Real gx, Real gy, Real gz) {
+  using Acc = LdsAcc<Real>;
+  using T = typename Acc::T;
+  const T ex = Acc::enc(gx), ey = Acc::enc(gy), ez = Acc::enc(gz);
+  atomicAdd(&L.grad[term][0][i1], ex);
+  atomicAdd(&L.grad[term][1][i1], ey);
+  atomicAdd(&L.grad[term][2][i1], ez);
+  atomicAdd(&L.grad[term][0][i2], T(0) - ex);
+  atomicAdd(&L.grad[term][1][i2], T(0) - ey);
+  atomicAdd(&L.grad[term][2][i2], T(0) - ez);
 }
 
 template <typename Real, int MODE>
@@ -130,7 +137,7 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
 #pragma unroll
             for (int t = 0; t < 3; ++t)
 #pragma unroll
-              for (int k = 0; k < 3; ++k) L.grad[t][k][slot] = Real(0);
+              for (int k = 0; k < 3; ++k) L.grad[t][k][slot] = typename LdsAcc<Real>::T(0);
           }
         }
         wave_sync();
@@ -189,12 +196,10 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
                   if (DVDX) {
                     Real gs = dlj * inv_d;
                     int term = inside ? 1 : 0;
-                    grad_add(L, term, i1, gs * dx, gs * dy, gs * dz);
-                    grad_add(L, term, i2, -gs * dx, -gs * dy, -gs * dz);
+                    grad_pair(L, term, i1, i2, gs * dx, gs * dy, gs * dz);
                   } else {
                     Real gs = dlj * inv_d * (inside ? w[1] : w[0]);
-                    grad_add(L, 0, i1, gs * dx, gs * dy, gs * dz);
-                    grad_add(L, 0, i2, -gs * dx, -gs * dy, -gs * dz);
+                    grad_pair(L, 0, i1, i2, gs * dx, gs * dy, gs * dz);
                   }
                 }
               }
@@ -273,8 +278,7 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
                       Real gs = dlk * fast_rcp(dist);
                       if (WGRAD) gs *= w[2];
                       int term = DVDX ? 2 : 0;
-                      grad_add(L, term, i1, gs * dx, gs * dy, gs * dz);
-                      grad_add(L, term, i2, -gs * dx, -gs * dy, -gs * dz);
+                      grad_pair(L, term, i1, i2, gs * dx, gs * dy, gs * dz);
                     }
                   }
                 }
@@ -303,14 +307,14 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
               for (int t = 0; t < 3; ++t)
 #pragma unroll
                 for (int k = 0; k < 3; ++k) {
-                  Real v = L.grad[t][k][slot];
+                  Real v = LdsAcc<Real>::dec(L.grad[t][k][slot]);
                   if (v != Real(0))
                     unsafeAtomicAdd(&A.dvdx[(int64_t(t) * A.n_atoms + atom) * 3 + k], v);
                 }
             } else {
 #pragma unroll
               for (int k = 0; k < 3; ++k) {
-                Real v = L.grad[0][k][slot];
+                Real v = LdsAcc<Real>::dec(L.grad[0][k][slot]);
                 if (v != Real(0)) unsafeAtomicAdd(&A.grad[atom * 3 + k], v);
               }
             }

@@ -18,8 +18,9 @@
 //             lane busy, block pairs mixed freely
 //
 // Energies accumulate per lane across the whole row (one pose).  Gradients
-// accumulate in LDS per atom slot -- R copies selected by lane so that lanes
-// sharing an atom rarely hit the same address in one instruction -- and reach
+// accumulate in LDS per atom slot as 64-bit fixed point (integer LDS atomics)
+// -- R copies selected by lane so that lanes sharing an atom rarely hit the
+// same address in one instruction -- and reach
 // global memory once per atom, term and component when a ring slot is
 // recycled (g2) or the row ends (g1).  Loop bounds and FIFO counters are
 // wave-uniform scalars.
@@ -49,24 +50,24 @@ struct RowCtx {
   static constexpr bool DERIV = DVDX || WGRAD;
   static constexpr int NG = DVDX ? 3 : (WGRAD ? 1 : 0);
   // gradient accumulator copies: fewer when the planes are many or wide
-  static constexpr int R = !DERIV ? 1 : (DVDX ? (sizeof(Real) == 8 ? 1 : 2)
-                                              : (sizeof(Real) == 8 ? 2 : 4));
+  static constexpr int R = (DERIV && !DVDX && sizeof(Real) == 4) ? 2 : 1;
 };
 
 template <typename Real, int MODE>
 struct RowLDS {
   using C = RowCtx<Real, MODE>;
+  using T = typename LdsAcc<Real>::T;
+  T grad[C::R][C::NG > 0 ? C::NG : 1][3][C::DERIV ? RSLOTS : 1];
+  T ebp[3][3];          // block-pair energies: ring 0, ring 1, intra
   Real pos[RSLOTS][4];  // x, y, z, lj_radius
   Real prm[RSLOTS][4];  // sqrt(lj_wdepth), K, 1/lambda^2, lk_volume
   int flg[RSLOTS];
-  Real grad[C::R][C::NG > 0 ? C::NG : 1][3][C::DERIV ? RSLOTS : 1];
   uint16_t lj[LIST];
   uint16_t lk[LIST];
   uint8_t idx1[TILE];                  // culled g1 atoms of the current block pair
   uint8_t idx2[2][TILE];               // culled g2 atoms per ring slot
   uint8_t pd[3][MAX_CONN][TILE];       // [0] g1, [1 + r] ring slot r
   uint8_t cs[2][MAX_CONN * MAX_CONN];  // per ring slot
-  Real ebp[3][3];                      // block-pair energies: ring 0, ring 1, intra
   Real wt[3][3];                       // block-pair weights:  ring 0, ring 1, intra
 };
 
@@ -82,13 +83,16 @@ template <typename Real, int MODE>
 __device__ __forceinline__ void grad_scatter(RowLDS<Real, MODE>& L, int term, int s1, int s2,
                                              Real gx, Real gy, Real gz, int lane) {
   using C = RowCtx<Real, MODE>;
+  using Acc = LdsAcc<Real>;
+  using T = typename Acc::T;
   const int c = lane & (C::R - 1);
-  atomicAdd(&L.grad[c][term][0][s1], gx);
-  atomicAdd(&L.grad[c][term][1][s1], gy);
-  atomicAdd(&L.grad[c][term][2][s1], gz);
-  atomicAdd(&L.grad[c][term][0][s2], -gx);
-  atomicAdd(&L.grad[c][term][1][s2], -gy);
-  atomicAdd(&L.grad[c][term][2][s2], -gz);
+  const T ex = Acc::enc(gx), ey = Acc::enc(gy), ez = Acc::enc(gz);
+  atomicAdd(&L.grad[c][term][0][s1], ex);
+  atomicAdd(&L.grad[c][term][1][s1], ey);
+  atomicAdd(&L.grad[c][term][2][s1], ez);
+  atomicAdd(&L.grad[c][term][0][s2], T(0) - ex);
+  atomicAdd(&L.grad[c][term][1][s2], T(0) - ey);
+  atomicAdd(&L.grad[c][term][2][s2], T(0) - ez);
 }
 
 // Evaluate LJ entries [0, consume) of the FIFO, one per lane.
@@ -114,8 +118,8 @@ __device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
       if (!C::NOE) {
         if (C::BP) {
           const int ow = entry_owner(s2);
-          atomicAdd(&L.ebp[ow][0], atr);
-          atomicAdd(&L.ebp[ow][1], rep);
+          atomicAdd(&L.ebp[ow][0], LdsAcc<Real>::enc(atr));
+          atomicAdd(&L.ebp[ow][1], LdsAcc<Real>::enc(rep));
         } else {
           e_atr.add(atr);
           e_rep.add(rep);
@@ -163,7 +167,7 @@ __device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
       const Real lk = wcp * (va + vb);
       if (!C::NOE) {
         if (C::BP)
-          atomicAdd(&L.ebp[entry_owner(s2)][2], lk);
+          atomicAdd(&L.ebp[entry_owner(s2)][2], LdsAcc<Real>::enc(lk));
         else
           e_lk.add(lk);
       }
@@ -203,12 +207,13 @@ __device__ __forceinline__ void flush_slots(RowLDS<Real, MODE>& L, const ScoreAr
     for (int t = 0; t < C::NG; ++t)
 #pragma unroll
       for (int k = 0; k < 3; ++k) {
-        Real v = Real(0);
+        typename LdsAcc<Real>::T acc = 0;
 #pragma unroll
         for (int c = 0; c < C::R; ++c) {
-          v += L.grad[c][t][k][slot];
-          L.grad[c][t][k][slot] = Real(0);
+          acc += L.grad[c][t][k][slot];
+          L.grad[c][t][k][slot] = 0;
         }
+        const Real v = LdsAcc<Real>::dec(acc);
         if (v != Real(0)) {
           if (C::DVDX)
             unsafeAtomicAdd(&A.dvdx[(int64_t(t) * A.n_atoms + atom) * 3 + k], v);
@@ -232,8 +237,8 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
   const Real cull_pad = gp.max_dis + Real(1e-3);
 
   if (C::DERIV) {
-    Real* g = &L.grad[0][0][0][0];
-    for (int i = lane; i < C::R * (C::NG > 0 ? C::NG : 1) * 3 * RSLOTS; i += WAVE) g[i] = Real(0);
+    auto* g = &L.grad[0][0][0][0];
+    for (int i = lane; i < C::R * (C::NG > 0 ? C::NG : 1) * 3 * RSLOTS; i += WAVE) g[i] = 0;
   }
   wave_sync();
 
@@ -277,7 +282,7 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
         L.pd[0][c][a] = A.conn_pd[(int64_t(t1) * MAX_CONN + c) * bt.Amax + a];
     }
     const uint32_t hmask1 = (uint32_t)__ballot(heavy1);
-    if (C::BP && lane < 9) (&L.ebp[0][0])[lane] = Real(0);
+    if (C::BP && lane < 9) (&L.ebp[0][0])[lane] = 0;
     KSum<Real> e_atr, e_rep, e_lk;
     // FIFO state (wave-uniform)
     int n_lj = 0, n_lk = 0;    // pending entries
@@ -308,8 +313,8 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
       if (C::BP && !C::NOE && lane < 3) {
         const int64_t PBB = int64_t(s.P) * s.B * s.B;
         const int64_t base = (int64_t(p) * s.B + b1) * s.B + (r ? b2s1 : b2s0);
-        A.energies_bp[lane * PBB + base] = L.ebp[r][lane];
-        L.ebp[r][lane] = Real(0);
+        A.energies_bp[lane * PBB + base] = LdsAcc<Real>::dec(L.ebp[r][lane]);
+        L.ebp[r][lane] = 0;
       }
       if (r)
         used1 = false;
@@ -500,8 +505,8 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
     if (C::BP && !C::NOE && lane < 3) {
       const int64_t PBB = int64_t(s.P) * s.B * s.B;
       const int64_t base = (int64_t(p) * s.B + b1) * s.B + b1;
-      A.energies_bp[lane * PBB + base] = L.ebp[2][lane];
-      L.ebp[2][lane] = Real(0);
+      A.energies_bp[lane * PBB + base] = LdsAcc<Real>::dec(L.ebp[2][lane]);
+      L.ebp[2][lane] = 0;
     }
     if (!C::NOE && !C::BP) {
       const double ea = wave_sum(e_atr.value());
