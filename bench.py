@@ -7,8 +7,7 @@ the whole batch, inputs resident in HBM, replayed as a captured hipGraph.
 Default workload: 512 poses, residue counts log-uniform in [50, 1200],
 ragged stack, float32 (seeded synthetic poses, ideal geometry).
 
-Prints one JSON line (see README of the measurement contract in DESIGN.md):
-value = poses/s for the whole job; roofline of the scoring kernel from
+Prints one JSON line: value = poses/s for the whole job; roofline of the scoring kernel from
 algorithmic FLOPs (branch census x FLOP table in tmol_amd/flops.py) over its
 measured duration; cpu_baseline = the reference's own CPU op (oracle/_ref) or,
 if absent, the C oracle, on a bounded sample of the same poses.
@@ -89,8 +88,9 @@ def main():
     ap.add_argument("--kernel-reps", type=int, default=10)
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU only: multi-GPU runs are not part of this build "
-              "(poses shard across GPUs with no exchange; see DESIGN.md).", flush=True)
+        print("bench.py measures one GPU only; --gpus %d is not supported. Poses are "
+              "independent: a multi-GPU run would shard the pose list by predicted cost with "
+              "no exchange beyond gathering the [3, P] energies." % args.gpus, flush=True)
         sys.exit(2)
 
     import numpy

@@ -20,7 +20,7 @@ constexpr int REC = 8;           // Reals per prepared (block type, atom) record
 // prepared per (block type, atom) record:
 //   [0] lj_radius  [1] sqrt(lj_wdepth)  [2] K = dg / (2 pi^1.5 lambda)
 //   [3] 1 / lambda^2  [4] lk_volume  [5] flags (F_* bits, F_HEAVY)  [6] 1
-// per (block type, connection, atom): path distance from the connection atom
+// per (block type, atom, connection): path distance from the connection atom
 constexpr int F_HEAVY = 32;
 
 // score modes (compile-time template bits); the public flags plus
@@ -62,10 +62,11 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
 
 struct WsLayout {
   int64_t counters;   // uint32[16]: [0] block pairs found, [1] row overflow
+  int64_t cursors;    // uint32[8 * 64]: row-queue cursors, one per 256-byte line
   int64_t energies;   // double[3 * P]
   int64_t spheres;    // Real[4 * G]
   int64_t records;    // Real[T * Amax * REC]
-  int64_t conn_pd;    // uint8[T * MAX_CONN * Amax]
+  int64_t conn_pd;    // uint8[T * Amax * MAX_CONN]: path distance atom -> connection atom
   int64_t row_count;  // uint32[G]
   int64_t row_items;  // int32[G * row_cap]
   int64_t total;
@@ -81,6 +82,8 @@ inline WsLayout ws_layout(const tmol_ljlk_stack* s, const tmol_ljlk_block_types*
   int64_t o = 0;
   L.counters = o;
   o = align256(o + 64);
+  L.cursors = o;
+  o = align256(o + 8 * 256);
   L.energies = o;
   o = align256(o + 8 * 3 * int64_t(s->n_poses));
   L.spheres = o;
@@ -276,6 +279,7 @@ struct ScoreArgs {
   const uint8_t* conn_pd;
   const Real* global_params;
   const uint32_t* counters;
+  uint32_t* row_cursor;        // [8 * QSTRIDE] RowQueue cursors (workspace)
   const Real* weights;         // [3, P] or [3, P, B, B]
   double* energies64;          // [3, P] whole pose
   Real* energies_bp;           // [3, P, B, B] block pair
@@ -299,26 +303,44 @@ __device__ __forceinline__ GlobalParams<Real> load_global_params(const Real* g) 
   return gp;
 }
 
-// XCD-aware split of [0, n) over the persistent grid: workgroups are dealt
-// round-robin to the 8 XCDs; each XCD gets a contiguous slice so that the
-// (pose-ordered) rows it walks share its L2.  Returns this wave's first index
-// and the stride.
-struct XcdSlice {
-  int64_t begin, end, first, stride;
-};
+// Dynamic row scheduling for the persistent scoring kernels.  Rows [0, n)
+// are split into 8 contiguous ranges, one per XCD (workgroups are dispatched
+// round-robin over the 8 XCDs of MI355X, so workgroup b runs on XCD b % 8 and
+// the pose-ordered rows of its range share that XCD's L2).  A wave takes rows
+// of its own range through an atomic cursor and, once that range is
+// exhausted, rows of the other ranges: no wave idles while rows remain.  Rows
+// are handed out one at a time (a row is tens of microseconds of one wave's
+// work, so larger grabs lengthen the tail) and each cursor sits on its own
+// 256-byte line (same-line L2 atomics serialize).  The cursors are zeroed by
+// a fill kernel before every scoring launch.
+constexpr int QROWS = 1;
+constexpr int QSTRIDE = 64;  // uint32 between cursors
 
-__device__ __forceinline__ XcdSlice xcd_slice(int64_t n) {
-  const int n_wg = gridDim.x;
-  const int n_xcd = n_wg >= 8 ? 8 : 1;
-  const int xcd = blockIdx.x % n_xcd;
-  const int wg_in_xcd = blockIdx.x / n_xcd;
-  const int wg_per_xcd = (n_wg - xcd + n_xcd - 1) / n_xcd;
-  XcdSlice r;
-  r.begin = n * xcd / n_xcd;
-  r.end = n * (xcd + 1) / n_xcd;
-  r.first = r.begin + int64_t(wg_in_xcd) * WAVES_PER_WG + threadIdx.x / WAVE;
-  r.stride = int64_t(wg_per_xcd) * WAVES_PER_WG;
-  return r;
-}
+struct RowQueue {
+  uint32_t* cursor;
+  int64_t n;
+  int64_t cur, end;
+  int xcd, left;
+  __device__ RowQueue(uint32_t* c, int64_t n_rows)
+      : cursor(c), n(n_rows), cur(0), end(0), xcd(int(blockIdx.x & 7)), left(8) {}
+  // next row of this wave (wave-uniform), -1 when every row is taken
+  __device__ int64_t next() {
+    if (cur < end) return cur++;
+    while (left > 0) {
+      const int64_t lo = n * xcd / 8, hi = n * (xcd + 1) / 8;
+      uint32_t k = 0;
+      if (lane_id() == 0) k = atomicAdd(&cursor[xcd * QSTRIDE], uint32_t(QROWS));
+      k = __builtin_amdgcn_readfirstlane(k);
+      if (lo + int64_t(k) < hi) {
+        cur = lo + int64_t(k);
+        end = cur + QROWS < hi ? cur + QROWS : hi;
+        return cur++;
+      }
+      xcd = (xcd + 1) & 7;
+      --left;
+    }
+    return -1;
+  }
+};
 
 }  // namespace tmol_ljlk

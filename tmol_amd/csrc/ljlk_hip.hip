@@ -112,17 +112,42 @@ static ScoreArgs<Real> make_args(const tmol_ljlk_stack* st, const tmol_ljlk_bloc
   a.conn_pd = (const uint8_t*)(w + Lw.conn_pd);
   a.global_params = (const Real*)btp->global_params;
   a.counters = (const uint32_t*)(w + Lw.counters);
+  a.row_cursor = (uint32_t*)(w + Lw.cursors);
   a.energies64 = (double*)(w + Lw.energies);
   a.n_atoms = st->n_atoms_total;
   return a;
 }
 
+// workgroups of `kernel` resident on the whole device at once (cached per device)
+template <typename K>
+static int resident_workgroups(K kernel) {
+  static int cache[64] = {0};
+  int dev = 0;
+  if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
+  if (dev < 64 && cache[dev] > 0) return cache[dev];
+  int cu = 0, per_cu = 0;
+  if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess)
+    per_cu = 0;
+  const int n = std::max(1, cu) * std::max(1, per_cu);
+  if (dev < 64) cache[dev] = n;
+  return n;
+}
+
+// n_wg <= 0: one workgroup per resident slot (the row queue balances the load)
 template <typename Real, int MODE>
 static void launch_score(const ScoreArgs<Real>& a, bool general, int n_wg, hipStream_t stream) {
-  if (general)
-    hipLaunchKernelGGL((score_general_kernel<Real, MODE>), dim3(n_wg), dim3(256), 0, stream, a);
-  else
-    hipLaunchKernelGGL((score_rows_kernel<Real, MODE>), dim3(n_wg), dim3(256), 0, stream, a);
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
+                     8 * QSTRIDE, 0u);
+  if (general) {
+    auto k = score_general_kernel<Real, MODE>;
+    if (n_wg <= 0) n_wg = resident_workgroups(k);
+    hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  } else {
+    auto k = score_rows_kernel<Real, MODE>;
+    if (n_wg <= 0) n_wg = resident_workgroups(k);
+    hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  }
 }
 
 template <typename Real>
@@ -149,7 +174,7 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
   }
   if ((flags & TMOL_LJLK_DVDX) && !acc) fill<Real>((Real*)dvdx, 3 * NA3, stream);
   if ((flags & TMOL_LJLK_WGRAD) && !acc) fill<Real>((Real*)grad, NA3, stream);
-  int n_wg = n_wave_groups > 0 ? n_wave_groups : 2048;
+  const int n_wg = n_wave_groups;
   int mode = flags & (TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY);
   switch (mode) {
     case 0: launch_score<Real, 0>(a, general, n_wg, stream); break;
@@ -277,7 +302,7 @@ int tmol_ljlk_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                      (unsigned long long*)census, 16, 0ull);
   ScoreArgs<float> a = make_args<float>(s, bt, coords, workspace, row_capacity, L);
   a.census = (unsigned long long*)census;
-  launch_score<float, M_CENSUS>(a, true, 2048, st);
+  launch_score<float, M_CENSUS>(a, true, 0, st);
   return check(hipGetLastError());
 }
 

@@ -52,7 +52,7 @@ __global__ void prepare_records_kernel(BTView bt, const Real* __restrict__ type_
       int ca = bt.conn_atoms[t * bt.C + c];
       v = (uint8_t)bt.path_distance[(int64_t(t) * bt.Amax + ca) * bt.Amax + a];
     }
-    conn_pd[(int64_t(t) * MAX_CONN + c) * bt.Amax + a] = v;
+    conn_pd[(int64_t(t) * bt.Amax + a) * MAX_CONN + c] = v;
   }
 }
 
@@ -241,7 +241,7 @@ __global__ __launch_bounds__(64) void debug_sep_kernel(StackView s, BTView bt,
     int a = lane - half * TILE;
     int tt = half ? t2 : t1;
     for (int c = 0; c < MAX_CONN; ++c)
-      pd[half][c][a] = conn_pd[(int64_t(tt) * MAX_CONN + c) * bt.Amax + min(a, bt.Amax - 1)];
+      pd[half][c][a] = conn_pd[(int64_t(tt) * bt.Amax + min(a, bt.Amax - 1)) * MAX_CONN + c];
   }
   __syncthreads();
   for (int k = lane; k < TILE * TILE; k += WAVE) {

@@ -55,9 +55,9 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
   const GlobalParams<Real> gp = load_global_params(A.global_params);
   const StackView& s = A.s;
   const BTView& bt = A.bt;
-  const XcdSlice sl = xcd_slice(s.G);
+  RowQueue queue(A.row_cursor, s.G);
 
-  for (int64_t row = sl.first; row < sl.end; row += sl.stride) {
+  for (int64_t row = queue.next(); row >= 0; row = queue.next()) {
   uint32_t row_n = A.w.row_count[row];
   if (row_n > A.w.row_cap) row_n = (uint32_t)A.w.row_cap;
   for (uint32_t kk = 0; kk < row_n; ++kk) {
@@ -125,7 +125,7 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
             L.flg[slot] = int(r[5]);
             if (need_cp) {
               for (int c = 0; c < MAX_CONN; ++c)
-                L.pd[half][c][a] = A.conn_pd[(int64_t(tt) * MAX_CONN + c) * bt.Amax + st + a];
+                L.pd[half][c][a] = A.conn_pd[(int64_t(tt) * bt.Amax + st + a) * MAX_CONN + c];
             }
           }
           if (a < TILE) {

@@ -40,6 +40,76 @@ __device__ __forceinline__ int64_t uni64(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 __device__ __forceinline__ int lane_val(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+__device__ __forceinline__ float lane_val(float v, int j) {
+  return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
+}
+__device__ __forceinline__ double lane_val(double v, int j) {
+  const long long b = __double_as_longlong(v);
+  const uint32_t lo = __builtin_amdgcn_readlane((uint32_t)(b & 0xffffffffll), j);
+  const uint32_t hi = __builtin_amdgcn_readlane((uint32_t)((unsigned long long)b >> 32), j);
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+// four Reals moved as one 16-byte (float) or 32-byte (double) access
+template <typename Real>
+struct V4;
+template <>
+struct V4<float> {
+  using T = float4;
+};
+template <>
+struct V4<double> {
+  using T = double4;
+};
+
+// one lane's atom of a block, held in registers between its global load and
+// its LDS staging: coordinates, prepared record, connection path distances
+template <typename Real>
+struct AtomRegs {
+  Real x, y, z;
+  typename V4<Real>::T r0;  // lj_radius, sqrt(wdepth), K, 1/lambda^2
+  Real vol;
+  int flags;
+  uint32_t pd;  // byte c: path distance to connection atom c
+};
+
+template <typename Real>
+__device__ __forceinline__ void load_atom(AtomRegs<Real>& R, const ScoreArgs<Real>& A, int t,
+                                          int o, int a) {
+  using V = typename V4<Real>::T;
+  const Real* xyz = A.coords + (int64_t(o) + a) * 3;
+  R.x = xyz[0];
+  R.y = xyz[1];
+  R.z = xyz[2];
+  const Real* rec = A.records + (int64_t(t) * A.bt.Amax + a) * REC;
+  R.r0 = *(const V*)rec;
+  const V r1 = *(const V*)(rec + 4);
+  R.vol = r1.x;
+  R.flags = int(r1.y);
+  R.pd = *(const uint32_t*)(A.conn_pd + (int64_t(t) * A.bt.Amax + a) * MAX_CONN);
+}
+
+// Count-pair separation of an inter-block atom pair from packed path
+// distances (impl.hh:47-67, count_pair.hh:162-207): min over connection pairs
+// of pd1[c1] + cs[c1][c2] + pd2[c2], starting at 5, only when the blocks'
+// minimum separation is <= 4; both ligand fragments -> 3/4 become 5.
+__device__ __forceinline__ int packed_pair_sep(uint32_t pd1, uint32_t pd2, const uint8_t* cs,
+                                               int min_sep, int nc1, int nc2, bool lig) {
+  int sep = min_sep;
+  if (sep <= 4) {
+    int best = 5;
+    for (int c1 = 0; c1 < nc1; ++c1) {
+      const int p1 = (pd1 >> (8 * c1)) & 255;
+      for (int c2 = 0; c2 < nc2; ++c2) {
+        const int v = p1 + cs[c1 * MAX_CONN + c2] + ((pd2 >> (8 * c2)) & 255);
+        best = v < best ? v : best;
+      }
+    }
+    sep = best;
+  }
+  if (lig && (sep == 3 || sep == 4)) sep = 5;
+  return sep;
+}
 
 template <typename Real, int MODE>
 struct RowCtx {
@@ -50,7 +120,7 @@ struct RowCtx {
   static constexpr bool DERIV = DVDX || WGRAD;
   static constexpr int NG = DVDX ? 3 : (WGRAD ? 1 : 0);
   // gradient accumulator copies: fewer when the planes are many or wide
-  static constexpr int R = (DERIV && !DVDX && sizeof(Real) == 4) ? 2 : 1;
+  static constexpr int R = 1;
 };
 
 template <typename Real, int MODE>
@@ -59,14 +129,15 @@ struct RowLDS {
   using T = typename LdsAcc<Real>::T;
   T grad[C::R][C::NG > 0 ? C::NG : 1][3][C::DERIV ? RSLOTS : 1];
   T ebp[3][3];          // block-pair energies: ring 0, ring 1, intra
-  Real pos[RSLOTS][4];  // x, y, z, lj_radius
-  Real prm[RSLOTS][4];  // sqrt(lj_wdepth), K, 1/lambda^2, lk_volume
+  alignas(4 * sizeof(Real)) Real pos[RSLOTS][4];  // x, y, z, lj_radius
+  alignas(4 * sizeof(Real)) Real prm[RSLOTS][4];  // sqrt(lj_wdepth), K, 1/lambda^2, lk_volume
   int flg[RSLOTS];
   uint16_t lj[LIST];
   uint16_t lk[LIST];
-  uint8_t idx1[TILE];                  // culled g1 atoms of the current block pair
-  uint8_t idx2[2][TILE];               // culled g2 atoms per ring slot
-  uint8_t pd[3][MAX_CONN][TILE];       // [0] g1, [1 + r] ring slot r
+  // culled atoms of the current block pair, compacted: x, y, z, atom index
+  alignas(4 * sizeof(Real)) Real c1[TILE][4];  // of g1
+  alignas(4 * sizeof(Real)) Real c2[TILE][4];  // of g2
+  uint32_t pdw[3][TILE];               // packed path distances: [0] g1, [1 + r] ring slot r
   uint8_t cs[2][MAX_CONN * MAX_CONN];  // per ring slot
   Real wt[3][3];                       // block-pair weights:  ring 0, ring 1, intra
 };
@@ -107,11 +178,11 @@ __device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
       const int e = L.lj[i];
       const int s1 = e & 31, s2 = (e >> 5) & 127;
       const Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
-      const Real x1 = L.pos[s1][0], y1 = L.pos[s1][1], z1 = L.pos[s1][2], r1 = L.pos[s1][3];
-      const Real x2 = L.pos[s2][0], y2 = L.pos[s2][1], z2 = L.pos[s2][2], r2 = L.pos[s2][3];
-      const Real dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+      using V = typename V4<Real>::T;
+      const V p1 = *(const V*)L.pos[s1], p2 = *(const V*)L.pos[s2];
+      const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
       const Real dist = fast_sqrt(dx * dx + dy * dy + dz * dz);
-      const Real sigma = lj_sigma(L.flg[s1], L.flg[s2], r1, r2, gp);
+      const Real sigma = lj_sigma(L.flg[s1], L.flg[s2], p1.w, p2.w, gp);
       const Real eps = L.prm[s1][0] * L.prm[s2][0];
       Real atr, rep, dlj;
       lj_pair<Real, C::DERIV>(dist, sigma, eps, wcp, gp, atr, rep, dlj);
@@ -150,20 +221,19 @@ __device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
       const int e = L.lk[i];
       const int s1 = e & 31, s2 = (e >> 5) & 127;
       const Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
-      const Real x1 = L.pos[s1][0], y1 = L.pos[s1][1], z1 = L.pos[s1][2], r1 = L.pos[s1][3];
-      const Real x2 = L.pos[s2][0], y2 = L.pos[s2][1], z2 = L.pos[s2][2], r2 = L.pos[s2][3];
-      const Real dx = x1 - x2, dy = y1 - y2, dz = z1 - z2;
+      using V = typename V4<Real>::T;
+      const V p1 = *(const V*)L.pos[s1], p2 = *(const V*)L.pos[s2];
+      const V q1 = *(const V*)L.prm[s1], q2 = *(const V*)L.prm[s2];
+      const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
       const Real dist = fast_sqrt(dx * dx + dy * dy + dz * dz);
       const int f1 = L.flg[s1], f2 = L.flg[s2];
-      const Real sigma = lj_sigma(f1, f2, r1, r2, gp);
+      const Real sigma = lj_sigma(f1, f2, p1.w, p2.w, gp);
       const bool cc = (f1 & F_CARBON_LK) && (f2 & F_CARBON_LK);
       Real d_min, cdmin, cdmax;
       lk_knots(sigma, cc, d_min, cdmin, cdmax);
-      const Real K1 = L.prm[s1][1], il1 = L.prm[s1][2], v1 = L.prm[s1][3];
-      const Real K2 = L.prm[s2][1], il2 = L.prm[s2][2], v2 = L.prm[s2][3];
       Real va, dva, vb, dvb;
-      lk_dir<Real, C::DERIV>(dist, d_min, cdmin, cdmax, -v2 * K1, r1, il1, gp, va, dva);
-      lk_dir<Real, C::DERIV>(dist, d_min, cdmin, cdmax, -v1 * K2, r2, il2, gp, vb, dvb);
+      lk_dir<Real, C::DERIV>(dist, d_min, cdmin, cdmax, -q2.w * q1.y, p1.w, q1.z, gp, va, dva);
+      lk_dir<Real, C::DERIV>(dist, d_min, cdmin, cdmax, -q1.w * q2.y, p2.w, q2.z, gp, vb, dvb);
       const Real lk = wcp * (va + vb);
       if (!C::NOE) {
         if (C::BP)
@@ -224,16 +294,37 @@ __device__ __forceinline__ void flush_slots(RowLDS<Real, MODE>& L, const ScoreAr
   }
 }
 
+// stage one lane's atom into LDS slot `slot`
 template <typename Real, int MODE>
-__global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
+__device__ __forceinline__ void stage_atom(RowLDS<Real, MODE>& L, int slot, int pd_row, int a,
+                                           const AtomRegs<Real>& R) {
+  using V = typename V4<Real>::T;
+  V p, q;
+  p.x = R.x;
+  p.y = R.y;
+  p.z = R.z;
+  p.w = R.r0.x;
+  q.x = R.r0.y;
+  q.y = R.r0.z;
+  q.z = R.r0.w;
+  q.w = R.vol;
+  *(V*)L.pos[slot] = p;
+  *(V*)L.prm[slot] = q;
+  L.flg[slot] = R.flags;
+  L.pdw[pd_row][a] = R.pd;
+}
+
+template <typename Real, int MODE>
+__global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
   using C = RowCtx<Real, MODE>;
+  using V = typename V4<Real>::T;
   __shared__ RowLDS<Real, MODE> lds_all[WAVES_PER_WG];
   const int lane = lane_id();
   RowLDS<Real, MODE>& L = lds_all[uni(threadIdx.x / WAVE)];
   const GlobalParams<Real> gp = load_global_params(A.global_params);
   const StackView& s = A.s;
   const BTView& bt = A.bt;
-  const XcdSlice sl = xcd_slice(s.G);
+  RowQueue queue(A.row_cursor, s.G);
   const Real cull_pad = gp.max_dis + Real(1e-3);
 
   if (C::DERIV) {
@@ -242,19 +333,18 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
   }
   wave_sync();
 
-  for (int64_t row = uni64(sl.first); row < sl.end; row += sl.stride) {
+  for (int64_t row = queue.next(); row >= 0; row = queue.next()) {
     const int n_items = uni((int)min<int64_t>(A.w.row_count[row], A.w.row_cap));
     if (n_items == 0) continue;
     const int g1 = (int)row;
     const int t1 = uni(s.block_type[g1]);
     const int n1 = uni(bt.n_atoms[t1]);
-    const int64_t o1 = uni64(s.block_atom_offset[g1]);
+    const int o1 = uni(s.block_atom_offset[g1]);
     const int p = uni(s.pose_of(g1));
     const int nc1 = uni(bt.n_conn[t1]);
     const bool lig1 = bt.is_ligand[t1] != 0;
     const int b1 = g1 - p * s.B;  // block index within the pose (padded layout)
-    const Real c1x = A.spheres[4 * int64_t(g1) + 0], c1y = A.spheres[4 * int64_t(g1) + 1];
-    const Real c1z = A.spheres[4 * int64_t(g1) + 2], c1r = A.spheres[4 * int64_t(g1) + 3];
+    const V sph1 = *(const V*)(A.spheres + 4 * int64_t(g1));
     Real wrow[3] = {Real(0), Real(0), Real(0)};
     if (C::WGRAD && !C::BP) {
       wrow[0] = A.weights[p];
@@ -264,33 +354,21 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
     // ---- stage g1 into slots 0..31
     bool heavy1 = false;
     if (lane < n1) {
-      const int a = lane;
-      const Real* xyz = A.coords + (o1 + a) * 3;
-      const Real* r = A.records + (int64_t(t1) * bt.Amax + a) * REC;
-      L.pos[a][0] = xyz[0];
-      L.pos[a][1] = xyz[1];
-      L.pos[a][2] = xyz[2];
-      L.pos[a][3] = r[0];
-      L.prm[a][0] = r[1];
-      L.prm[a][1] = r[2];
-      L.prm[a][2] = r[3];
-      L.prm[a][3] = r[4];
-      const int f = int(r[5]);
-      L.flg[a] = f;
-      heavy1 = (f & F_HEAVY) != 0;
-      for (int c = 0; c < MAX_CONN; ++c)
-        L.pd[0][c][a] = A.conn_pd[(int64_t(t1) * MAX_CONN + c) * bt.Amax + a];
+      AtomRegs<Real> R;
+      load_atom(R, A, t1, o1, lane);
+      stage_atom(L, lane, 0, lane, R);
+      heavy1 = (R.flags & F_HEAVY) != 0;
     }
     const uint32_t hmask1 = (uint32_t)__ballot(heavy1);
     if (C::BP && lane < 9) (&L.ebp[0][0])[lane] = 0;
     KSum<Real> e_atr, e_rep, e_lk;
     // FIFO state (wave-uniform)
-    int n_lj = 0, n_lk = 0;    // pending entries
+    int n_lj = 0, n_lk = 0;      // pending entries
     int seq_lj = 0, seq_lk = 0;  // entries appended in this row
     int done_lj = 0, done_lk = 0;
     int end_lj0 = 0, end_lj1 = 0, end_lk0 = 0, end_lk1 = 0;
     bool used0 = false, used1 = false;
-    int64_t o2s0 = 0, o2s1 = 0;
+    int o2s0 = 0, o2s1 = 0;
     int n2s0 = 0, n2s1 = 0, b2s0 = 0, b2s1 = 0;
     int ring = 0;
 
@@ -325,20 +403,40 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
     for (int c0 = 0; c0 < n_items; c0 += WAVE) {
       // ---- per-item descriptors, one item per lane
       const int n_chunk = min(WAVE, n_items - c0);
-      int d_g2 = g1, d_min = 6;
+      // d_pk: n_atoms | n_conn << 8 | is_ligand << 12 | min block separation << 16
+      int d_g2 = g1, d_t2 = t1, d_o2 = 0, d_pk = 0;
+      V d_sph;
+      d_sph.x = d_sph.y = d_sph.z = d_sph.w = Real(0);
       if (lane < n_chunk) {
         d_g2 = A.w.row_items[row * A.w.row_cap + c0 + lane];
-        const int dt = s.block_type[d_g2];
-        d_min = d_g2 == g1 ? 0 : min_block_sep(s, g1, d_g2, nc1, bt.n_conn[dt]);
+        d_t2 = s.block_type[d_g2];
+        d_o2 = s.block_atom_offset[d_g2];
+        d_sph = *(const V*)(A.spheres + 4 * int64_t(d_g2));
+        const int nc = bt.n_conn[d_t2];
+        const int ms = d_g2 == g1 ? 0 : min_block_sep(s, g1, d_g2, nc1, nc);
+        d_pk = bt.n_atoms[d_t2] | (nc << 8) | ((bt.is_ligand[d_t2] != 0) << 12) | (ms << 16);
+      }
+      // atoms of the chunk's first block, then always one block ahead
+      AtomRegs<Real> pf;
+      {
+        const int t = lane_val(d_t2, 0), o = lane_val(d_o2, 0), n = lane_val(d_pk, 0) & 255;
+        if (lane < n) load_atom(pf, A, t, o, lane);
       }
       for (int j = 0; j < n_chunk; ++j) {
         const int g2 = lane_val(d_g2, j);
-        const int min_sep = lane_val(d_min, j);
+        const int pk = lane_val(d_pk, j);
+        const int min_sep = pk >> 16;
+        const AtomRegs<Real> cur = pf;
         if (g2 == g1) {
           // ---- intra-block pairs a1 < a2, separation = path distance
           if (C::BP && C::WGRAD && lane < 3) {
             const int64_t PBB = int64_t(s.P) * s.B * s.B;
             L.wt[2][lane] = A.weights[lane * PBB + (int64_t(p) * s.B + b1) * s.B + b1];
+          }
+          if (j + 1 < n_chunk) {
+            const int t = lane_val(d_t2, j + 1), o = lane_val(d_o2, j + 1);
+            const int n = lane_val(d_pk, j + 1) & 255;
+            if (lane < n) load_atom(pf, A, t, o, lane);
           }
           wave_sync();
           const int total = n1 * n1;
@@ -349,9 +447,8 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
             bool in = false, both_heavy = false, w02 = false;
             if (a1 < a2 && a1 < n1) {
               const int sep = bt.path_distance[(int64_t(t1) * bt.Amax + a1) * bt.Amax + a2];
-              const Real dx = L.pos[a1][0] - L.pos[a2][0];
-              const Real dy = L.pos[a1][1] - L.pos[a2][1];
-              const Real dz = L.pos[a1][2] - L.pos[a2][2];
+              const V pa = *(const V*)L.pos[a1], pb = *(const V*)L.pos[a2];
+              const Real dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
               in = sep >= 4 && dx * dx + dy * dy + dz * dz < gp.max_dis2;
               w02 = sep == 4;
               both_heavy = in && ((hmask1 >> a1) & (hmask1 >> a2) & 1u);
@@ -376,68 +473,77 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
           // ---- inter-block pairs through ring slot `ring`
           const int r = ring;
           ring ^= 1;
+          const int n2 = pk & 255;
+          const int o2 = lane_val(d_o2, j);
+          const int nc2 = (pk >> 8) & 15;
+          const bool lig = lig1 && ((pk >> 12) & 1);
+          const Real c2x = lane_val(d_sph.x, j), c2y = lane_val(d_sph.y, j);
+          const Real c2z = lane_val(d_sph.z, j), c2r = lane_val(d_sph.w, j);
+          const int sb = TILE + TILE * r;
+          const bool need_cp = min_sep <= 4;
+          const int b2 = g2 - p * s.B;
+          // cull: g2 atoms within reach of g1's sphere (lanes 0..31) and g1
+          // atoms within reach of g2's sphere (lanes 32..63)
+          V mine;
+          mine.x = mine.y = mine.z = mine.w = Real(0);
+          bool near = false;
+          if (lane < n2) {
+            mine.x = cur.x;
+            mine.y = cur.y;
+            mine.z = cur.z;
+            mine.w = Real(lane);
+            const Real ex = cur.x - sph1.x, ey = cur.y - sph1.y, ez = cur.z - sph1.z;
+            const Real reach = sph1.w + cull_pad;
+            near = ex * ex + ey * ey + ez * ez < reach * reach;
+          } else if (lane >= TILE && lane - TILE < n1) {
+            mine = *(const V*)L.pos[lane - TILE];
+            mine.w = Real(lane - TILE);
+            const Real ex = mine.x - c2x, ey = mine.y - c2y, ez = mine.z - c2z;
+            const Real reach = c2r + cull_pad;
+            near = ex * ex + ey * ey + ez * ez < reach * reach;
+          }
+          const uint64_t mnear = __ballot(near);
+          const uint32_t hmask2 = (uint32_t)__ballot(lane < n2 && (cur.flags & F_HEAVY));
+          const uint32_t m2 = (uint32_t)mnear, m1 = (uint32_t)(mnear >> 32);
+          const int n1c = __popc(m1), n2c = __popc(m2);
+          if (n1c == 0 || n2c == 0) {
+            // no atom pair within reach: nothing staged, nothing to flush
+            if (j + 1 < n_chunk) {
+              const int t = lane_val(d_t2, j + 1), o = lane_val(d_o2, j + 1);
+              const int n = lane_val(d_pk, j + 1) & 255;
+              if (lane < n) load_atom(pf, A, t, o, lane);
+            }
+            ring = r;
+            if (n_lj >= WAVE || n_lk >= WAVE) drain(false);
+            continue;
+          }
           if (r ? used1 : used0) {
             const int end_lj = r ? end_lj1 : end_lj0;
             const int end_lk = r ? end_lk1 : end_lk0;
             if (end_lj > done_lj || end_lk > done_lk) drain(true);
             retire(r);
           }
-          const int t2 = uni(s.block_type[g2]);
-          const int n2 = uni(bt.n_atoms[t2]);
-          const int64_t o2 = uni64(s.block_atom_offset[g2]);
-          const int sb = TILE + TILE * r;
-          const bool need_cp = min_sep <= 4;
-          const int nc2 = uni(bt.n_conn[t2]);
-          const bool lig = lig1 && bt.is_ligand[t2] != 0;
-          const Real c2x = A.spheres[4 * int64_t(g2) + 0], c2y = A.spheres[4 * int64_t(g2) + 1];
-          const Real c2z = A.spheres[4 * int64_t(g2) + 2], c2r = A.spheres[4 * int64_t(g2) + 3];
-          const Real reach1 = c1r + cull_pad, reach2 = c2r + cull_pad;
-          bool near2 = false, heavy2 = false;
-          if (lane < n2) {
-            const int a = lane;
-            const Real* xyz = A.coords + (o2 + a) * 3;
-            const Real* rr = A.records + (int64_t(t2) * bt.Amax + a) * REC;
-            const Real x = xyz[0], y = xyz[1], z = xyz[2];
-            L.pos[sb + a][0] = x;
-            L.pos[sb + a][1] = y;
-            L.pos[sb + a][2] = z;
-            L.pos[sb + a][3] = rr[0];
-            L.prm[sb + a][0] = rr[1];
-            L.prm[sb + a][1] = rr[2];
-            L.prm[sb + a][2] = rr[3];
-            L.prm[sb + a][3] = rr[4];
-            const int f = int(rr[5]);
-            L.flg[sb + a] = f;
-            heavy2 = (f & F_HEAVY) != 0;
-            if (need_cp)
-              for (int c = 0; c < MAX_CONN; ++c)
-                L.pd[1 + r][c][a] = A.conn_pd[(int64_t(t2) * MAX_CONN + c) * bt.Amax + a];
-            const Real ex = x - c1x, ey = y - c1y, ez = z - c1z;
-            near2 = ex * ex + ey * ey + ez * ez < reach1 * reach1;
-          } else if (lane >= TILE && lane - TILE < n1) {
-            // g1 atoms within reach of g2's sphere
-            const int a = lane - TILE;
-            const Real ex = L.pos[a][0] - c2x, ey = L.pos[a][1] - c2y, ez = L.pos[a][2] - c2z;
-            near2 = ex * ex + ey * ey + ez * ez < reach2 * reach2;
-          }
-          const uint64_t mnear = __ballot(near2);
-          const uint32_t hmask2 = (uint32_t)__ballot(heavy2);
-          const uint32_t m2 = (uint32_t)mnear, m1 = (uint32_t)(mnear >> 32);
-          if (near2) {
-            if (lane < TILE)
-              L.idx2[r][__builtin_amdgcn_mbcnt_lo(m2, 0)] = (uint8_t)lane;
-            else
-              L.idx1[__builtin_amdgcn_mbcnt_hi(m1, 0)] = (uint8_t)(lane - TILE);
-          }
+          // global reads this item needs now, before the next prefetch
           if (need_cp && lane >= TILE && lane < TILE + MAX_CONN * MAX_CONN) {
             const int qq = lane - TILE;
             const int cc1 = qq / MAX_CONN, cc2 = qq % MAX_CONN;
             L.cs[r][qq] = (cc1 < nc1 && cc2 < nc2) ? (uint8_t)conn_sep(s, g1, g2, cc1, cc2) : 6;
           }
-          const int b2 = g2 - p * s.B;
           if (C::BP && C::WGRAD && lane < 3) {
             const int64_t PBB = int64_t(s.P) * s.B * s.B;
             L.wt[r][lane] = A.weights[lane * PBB + (int64_t(p) * s.B + b1) * s.B + b2];
+          }
+          if (lane < n2) stage_atom(L, sb + lane, 1 + r, lane, cur);
+          if (near) {
+            if (lane < TILE)
+              *(V*)L.c2[__builtin_amdgcn_mbcnt_lo(m2, 0)] = mine;
+            else
+              *(V*)L.c1[__builtin_amdgcn_mbcnt_hi(m1, 0)] = mine;
+          }
+          if (j + 1 < n_chunk) {
+            const int t = lane_val(d_t2, j + 1), o = lane_val(d_o2, j + 1);
+            const int n = lane_val(d_pk, j + 1) & 255;
+            if (lane < n) load_atom(pf, A, t, o, lane);
           }
           if (r) {
             used1 = true; o2s1 = o2; n2s1 = n2; b2s1 = b2;
@@ -445,45 +551,57 @@ __global__ __launch_bounds__(256) void score_rows_kernel(ScoreArgs<Real> A) {
             used0 = true; o2s0 = o2; n2s0 = n2; b2s0 = b2;
           }
           wave_sync();
-          // phase A over the culled atoms: k = lane + 64 it, t = k / n2c, u = k % n2c
-          const int n1c = __popc(m1), n2c = __popc(m2);
-          const int total = n1c * n2c;
-          if (total > 0) {
-            const int q = WAVE / n2c, rstep = WAVE - q * n2c;
-            int u = lane % n2c, t = lane / n2c;
-            for (int k0 = 0; k0 < total; k0 += WAVE) {
-              if (n_lj > LIST - WAVE || n_lk > LIST - WAVE) drain(false);
-              bool in = false, both_heavy = false, w02 = false;
-              int a1 = 0, a2 = 0;
-              if (t < n1c) {
-                a1 = L.idx1[t];
-                a2 = L.idx2[r][u];
-                const Real dx = L.pos[a1][0] - L.pos[sb + a2][0];
-                const Real dy = L.pos[a1][1] - L.pos[sb + a2][1];
-                const Real dz = L.pos[a1][2] - L.pos[sb + a2][2];
-                const int sep = need_cp ? inter_pair_sep(L.pd[0], L.pd[1 + r], L.cs[r], min_sep,
-                                                         nc1, nc2, a1, a2, lig)
-                                        : min_sep;
-                in = sep >= 4 && dx * dx + dy * dy + dz * dz < gp.max_dis2;
-                w02 = sep == 4;
-                both_heavy = in && ((hmask1 >> a1) & (hmask2 >> a2) & 1u);
-              }
-              const uint64_t m = __ballot(in);
-              if (in) L.lj[n_lj + lanes_below(m)] = make_entry(a1, sb + a2, w02);
-              n_lj += __popcll(m);
-              seq_lj += __popcll(m);
-              const uint64_t mh = __ballot(both_heavy);
-              if (both_heavy) L.lk[n_lk + lanes_below(mh)] = make_entry(a1, sb + a2, w02);
-              n_lk += __popcll(mh);
-              seq_lk += __popcll(mh);
-              u += rstep;
-              t += q;
-              if (u >= n2c) {
-                u -= n2c;
-                t += 1;
-              }
-              wave_sync();
+          // phase A over the culled atoms, two pair indices per lane and
+          // iteration: k = lane + 128 it (+ 64), t = k / n2c, u = k % n2c
+          auto test_pair = [&](int t, int u, bool& in, bool& hv, bool& w02, int& a1, int& a2) {
+            in = hv = w02 = false;
+            a1 = a2 = 0;
+            if (t < n1c) {
+              const V P1 = *(const V*)L.c1[t], P2 = *(const V*)L.c2[u];
+              a1 = int(P1.w);
+              a2 = int(P2.w);
+              const Real dx = P1.x - P2.x, dy = P1.y - P2.y, dz = P1.z - P2.z;
+              const int sep = need_cp ? packed_pair_sep(L.pdw[0][a1], L.pdw[1 + r][a2], L.cs[r],
+                                                        min_sep, nc1, nc2, lig)
+                                      : min_sep;
+              in = sep >= 4 && dx * dx + dy * dy + dz * dz < gp.max_dis2;
+              w02 = sep == 4;
+              hv = in && ((hmask1 >> a1) & (hmask2 >> a2) & 1u);
             }
+          };
+          auto append = [&](bool in, bool hv, bool w02, int a1, int a2) {
+            const uint64_t m = __ballot(in);
+            if (in) L.lj[n_lj + lanes_below(m)] = make_entry(a1, sb + a2, w02);
+            n_lj += __popcll(m);
+            seq_lj += __popcll(m);
+            const uint64_t mh = __ballot(hv);
+            if (hv) L.lk[n_lk + lanes_below(mh)] = make_entry(a1, sb + a2, w02);
+            n_lk += __popcll(mh);
+            seq_lk += __popcll(mh);
+          };
+          const int total = n1c * n2c;
+          const int q = WAVE / n2c, rstep = WAVE - q * n2c;
+          int u = lane % n2c, t = lane / n2c;
+          for (int k0 = 0; k0 < total; k0 += 2 * WAVE) {
+            if (n_lj > LIST - 2 * WAVE || n_lk > LIST - 2 * WAVE) drain(false);
+            int uB = u + rstep, tB = t + q;
+            if (uB >= n2c) {
+              uB -= n2c;
+              tB += 1;
+            }
+            bool inA, hvA, wA, inB, hvB, wB;
+            int a1A, a2A, a1B, a2B;
+            test_pair(t, u, inA, hvA, wA, a1A, a2A);
+            test_pair(tB, uB, inB, hvB, wB, a1B, a2B);
+            append(inA, hvA, wA, a1A, a2A);
+            append(inB, hvB, wB, a1B, a2B);
+            u = uB + rstep;
+            t = tB + q;
+            if (u >= n2c) {
+              u -= n2c;
+              t += 1;
+            }
+            wave_sync();
           }
           if (r) {
             end_lj1 = seq_lj;

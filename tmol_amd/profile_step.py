@@ -4,6 +4,10 @@ that ``rocprofv3`` sees a short, repeatable kernel sequence.
 
     python -m tmol_amd.profile_step --workload ragged_512x50_1200 --reps 5 \
         [--mode wgrad|dvdx|energy] [--general] [--wave-groups N]
+
+``--copy-bytes N`` instead copies an N-byte device buffer ``reps`` times with
+torch's copy kernel: the reference point that scales the HBM counters
+(FETCH_SIZE / WRITE_SIZE) of a known byte count.
 """
 
 import argparse
@@ -23,7 +27,22 @@ def main(argv=None):
     ap.add_argument("--general", action="store_true")
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--copy-bytes", type=int, default=0)
     args = ap.parse_args(argv)
+
+    if args.copy_bytes > 0:
+        dev = torch.device("cuda:0")
+        src = torch.ones(args.copy_bytes, dtype=torch.uint8, device=dev)
+        dst = torch.empty_like(src)
+        times = []
+        for _ in range(args.reps):
+            t0 = time.perf_counter()
+            dst.copy_(src)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        print(json.dumps({"copy_bytes": args.copy_bytes, "step_ms": [1e3 * t for t in times]}),
+              flush=True)
+        return 0
 
     from . import ops
     from .generator import workload_poses

@@ -178,8 +178,9 @@ class LJLKRaggedScorer:
 
     # -- autotuner -------------------------------------------------------------
     def autotune(self, coords, weights=None, candidates=None, reps=5, cache=True):
-        """Pick the persistent grid size (workgroups of 4 waves) by timing the
-        fused step; results cached per device and workload signature."""
+        """Pick the persistent grid size (workgroups of 4 waves; 0 = resident
+        slots) by timing the fused step; results cached per device and
+        workload signature."""
         key = "%s|G%d|NA%d|P%d|%s" % (torch.cuda.get_device_name(self.device), self.G,
                                       self.NA, self.P, self.dtype)
         table = {}
@@ -195,7 +196,8 @@ class LJLKRaggedScorer:
         ops._check(ops.lib().tmol_ljlk_device_cu_count(ctypes.byref(cu)), "cu count")
         n_cu = max(int(cu.value), 1)
         if candidates is None:
-            candidates = [n_cu * k for k in (1, 2, 3, 4, 6, 8, 12, 16)]
+            # 0 = one workgroup per resident slot (the library's default)
+            candidates = [0] + [n_cu * k for k in (1, 2, 3, 4, 6, 8)]
         w = self._weights(weights)
         results = {}
         for c in candidates:
