@@ -86,6 +86,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--autotune", action="store_true",
+                    help="time candidate grid sizes first (default: resident-slot grid)")
+    ap.add_argument("--wave-groups", type=int, default=0)
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported. Poses are "
@@ -113,8 +116,12 @@ def main():
     weights = torch.tensor([1.0, 0.55, 1.0])
     log("stack: %d poses, %d blocks, %d atoms" % (stack.n_poses, stack.n_blocks_total, stack.n_atoms_total))
 
-    tune = scorer.autotune(coords, weights, cache=False)
-    log("autotune: %s" % json.dumps(tune))
+    if args.autotune:
+        tune = scorer.autotune(coords, weights, cache=False)
+        log("autotune: %s" % json.dumps(tune))
+    else:
+        scorer.wave_groups = args.wave_groups  # 0: one workgroup per resident slot
+    scorer.neighbors(coords)
     n_items = scorer.check_capacity()
     census = scorer.census(coords)
     log("census: %s" % json.dumps(census))
