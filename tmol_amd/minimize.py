@@ -1,0 +1,356 @@
+"""Per-pose (segmented) L-BFGS with an Armijo line search, driving LJ+LK
+minimization.
+
+Mirrors the reference minimizer's contract (tmol/optimization/
+lbfgs_armijo.py:196-744, Rosetta-derived constants):
+
+* each pose is an independent segment: its own step length, history
+  validity, restart and convergence;
+* history of 128 (s, y) pairs; a pair enters a segment's history only when
+  s.y > 1e-6 for that segment; the search direction is -H g with H_0 = I,
+  computed in float64;
+* directional-derivative repair: flip the components with g_i d_i > 0, then
+  fall back to steepest descent (lbfgs_armijo.py:557-575);
+* Armijo search per segment (lbfgs_armijo.py:68-188): start at
+  min(2 t_prev, 1); sufficient decrease sigma = 0.1; a step that looks linear
+  (sigma = 0.8) is tried once at twice the length; backtracking by 0.25;
+  below minstep a downhill step is still taken, else the search fails;
+* a failed search restarts the segment once from steepest descent with
+  step min(1 / sqrt(|g.d|), 1); a second failure retires it;
+* convergence: max |g| <= gradtol freezes a segment; the run stops when every
+  segment is frozen, retired or changed its energy by <= atol or relative
+  <= rtol (defaults sqrt(eps) of the dtype).
+
+The inverse-Hessian product uses the compact form of Byrd, Nocedal and
+Schnabel (Math. Prog. 63, 1994): H g = g + S p1 + Y p2 with
+p2 = -R^-1 S g and p1 = R^-T ((D + Y Y^T) R^-1 S g - Y g), R the upper
+triangle of S Y^T.  Here the Gram matrices S Y^T and Y Y^T are kept up to date
+incrementally (one new row and column per accepted pair, O(m n)) instead of
+being recomputed from the history every iteration (O(m^2 n)).
+"""
+
+import math
+from dataclasses import dataclass, field
+from typing import Callable, List, Optional, Tuple
+
+import torch
+
+Evaluate = Callable[[torch.Tensor], Tuple[torch.Tensor, torch.Tensor]]
+
+
+@dataclass
+class Trace:
+    energy: List[torch.Tensor] = field(default_factory=list)    # [P] per iteration (f64, cpu)
+    grad_norm: List[torch.Tensor] = field(default_factory=list)  # [P] L2 norm of dE/dx
+    grad_max: List[torch.Tensor] = field(default_factory=list)   # [P] max |dE/dx_i|
+    evals: List[int] = field(default_factory=list)               # evaluations per iteration
+    iterations: int = 0
+    converged: Optional[torch.Tensor] = None
+    stalled: Optional[torch.Tensor] = None
+
+    def energies(self) -> torch.Tensor:
+        return torch.stack(self.energy)
+
+    def grad_norms(self) -> torch.Tensor:
+        return torch.stack(self.grad_norm)
+
+
+class SegmentedLBFGS:
+    """Minimize sum_p E_p(x) with every segment p independent.
+
+    ``segment_ids[i]`` is the segment of parameter element i.  ``evaluate(x)``
+    returns ``(E [P] float64, dE/dx [N])`` at the flat parameter vector x."""
+
+    def __init__(self, segment_ids: torch.Tensor, n_segments: Optional[int] = None,
+                 history: int = 128, max_iter: int = 200, gradtol: float = 1.0,
+                 rtol: Optional[float] = None, atol: Optional[float] = None, lr: float = 1.0,
+                 minstep: float = 1e-12, verbose: bool = False):
+        seg = segment_ids.reshape(-1).to(torch.int64)
+        self.device = seg.device
+        self.P = int(n_segments) if n_segments is not None else int(seg.max()) + 1
+        counts = torch.bincount(seg, minlength=self.P)
+        if bool((counts == 0).any()):
+            raise ValueError("every segment needs at least one parameter")
+        self.seg = seg
+        self.S = int(counts.max())
+        # position of each element inside its segment's padded row
+        order = torch.argsort(seg, stable=True)
+        start = torch.cumsum(counts, 0) - counts
+        rank = torch.empty_like(seg)
+        rank[order] = torch.arange(seg.numel(), device=seg.device) - start[seg[order]]
+        self.flat_index = seg * self.S + rank
+        self.m = int(history)
+        self.max_iter = int(max_iter)
+        self.gradtol = float(gradtol)
+        self.rtol, self.atol = rtol, atol
+        self.lr = float(lr)
+        self.minstep = float(minstep)
+        self.verbose = verbose
+
+    # -- padded <-> flat -------------------------------------------------------
+    def _padded(self, v: torch.Tensor, dtype=torch.float64) -> torch.Tensor:
+        out = torch.zeros(self.P * self.S, dtype=dtype, device=v.device)
+        out[self.flat_index] = v.to(dtype)
+        return out.view(self.P, self.S)
+
+    def _flat(self, padded: torch.Tensor) -> torch.Tensor:
+        return padded.reshape(-1)[self.flat_index]
+
+    def _seg_sum(self, v: torch.Tensor) -> torch.Tensor:
+        out = torch.zeros(self.P, dtype=torch.float64, device=v.device)
+        return out.index_add_(0, self.seg, v.to(torch.float64))
+
+    def _seg_max(self, v: torch.Tensor) -> torch.Tensor:
+        out = torch.full((self.P,), -math.inf, dtype=torch.float64, device=v.device)
+        return out.scatter_reduce(0, self.seg, v.to(torch.float64), "amax", include_self=True)
+
+    def _per_elem(self, per_seg: torch.Tensor) -> torch.Tensor:
+        return per_seg[self.seg]
+
+    # -- history ------------------------------------------------------------------
+    def _reset_history(self, dev):
+        m, P, S = self.m, self.P, self.S
+        self.Sh = torch.zeros((m, P, S), dtype=torch.float64, device=dev)  # s pairs, chronological
+        self.Yh = torch.zeros((m, P, S), dtype=torch.float64, device=dev)
+        self.SY = torch.zeros((P, m, m), dtype=torch.float64, device=dev)  # s_i . y_j
+        self.YY = torch.zeros((P, m, m), dtype=torch.float64, device=dev)  # y_i . y_j
+        self.n_hist = 0
+
+    def _push(self, s: torch.Tensor, y: torch.Tensor):
+        """Append one (s, y) pair [P, S] (rows of segments that did not keep
+        it are zero) as the newest history entry."""
+        if self.n_hist == self.m:
+            # drop the oldest entry: shift history and Gram matrices by one
+            self.Sh = torch.roll(self.Sh, -1, 0)
+            self.Yh = torch.roll(self.Yh, -1, 0)
+            self.SY = torch.roll(self.SY, (-1, -1), (1, 2))
+            self.YY = torch.roll(self.YY, (-1, -1), (1, 2))
+            k = self.m - 1
+        else:
+            k = self.n_hist
+            self.n_hist += 1
+        self.Sh[k] = s
+        self.Yh[k] = y
+        n = self.n_hist
+        Sv, Yv = self.Sh[:n], self.Yh[:n]
+        s_y = torch.einsum("ips,ps->pi", Sv, y)  # s_i . y_new
+        y_s = torch.einsum("ips,ps->pi", Yv, s)  # y_i . s_new
+        y_y = torch.einsum("ips,ps->pi", Yv, y)  # y_i . y_new
+        self.SY[:, :n, k] = s_y
+        self.SY[:, k, :n] = y_s
+        self.YY[:, :n, k] = y_y
+        self.YY[:, k, :n] = y_y
+
+    def _drop_segments(self, mask: torch.Tensor):
+        """Forget the history of the segments in ``mask``."""
+        idx = mask.nonzero(as_tuple=False).flatten()
+        if idx.numel() == 0:
+            return
+        self.Sh[:, idx] = 0.0
+        self.Yh[:, idx] = 0.0
+        self.SY[idx] = 0.0
+        self.YY[idx] = 0.0
+
+    def _hg(self, g: torch.Tensor) -> torch.Tensor:
+        """H g for padded g [P, S] (float64), compact representation."""
+        n = self.n_hist
+        if n == 0:
+            return g.clone()
+        Sv, Yv = self.Sh[:n], self.Yh[:n]
+        a = torch.einsum("ips,ps->pi", Sv, g)
+        b = torch.einsum("ips,ps->pi", Yv, g)
+        SY = self.SY[:, :n, :n]
+        D = torch.diagonal(SY, dim1=1, dim2=2)
+        # absent entries (zero rows) get a unit diagonal: they then contribute 0
+        R = torch.triu(SY) + torch.diag_embed((D == 0).to(SY.dtype))
+        u = torch.linalg.solve_triangular(R, a.unsqueeze(-1), upper=True).squeeze(-1)
+        v = torch.einsum("pij,pj->pi", self.YY[:, :n, :n], u) + D * u - b
+        p1 = torch.linalg.solve_triangular(R.transpose(1, 2), v.unsqueeze(-1),
+                                           upper=False).squeeze(-1)
+        return g + torch.einsum("pi,ips->ps", p1, Sv) - torch.einsum("pi,ips->ps", u, Yv)
+
+    # -- line search ------------------------------------------------------------------
+    def _armijo(self, phi_at, gtd, phi0, alpha0, searching):
+        """Per-segment Armijo search; returns (accepted step, energies at it,
+        evaluations, failed mask, whether the last evaluation was at the
+        accepted steps)."""
+        zero = torch.zeros_like(alpha0)
+        alpha = torch.where(searching, alpha0, zero)
+        phi = phi_at(alpha)
+        evals = 1
+        linear = phi <= phi0 + alpha * 0.8 * gtd
+        enough = phi <= phi0 + alpha * 0.1 * gtd
+        grow = searching & linear
+        shrink = searching & ~linear & ~enough
+        ok = searching & (linear | enough)
+        step = torch.where(ok, alpha, zero)
+        phi_step = torch.where(ok, phi, phi0)
+        failed = torch.zeros_like(searching)
+        while bool((grow | shrink).any()):
+            trial = torch.where(grow, alpha * 2.0, step)
+            trial = torch.where(shrink, alpha * 0.25, trial)
+            phi_t = phi_at(trial)
+            evals += 1
+            longer = grow & (phi_t < phi)
+            step = torch.where(longer, trial, step)
+            phi_step = torch.where(longer, phi_t, phi_step)
+            grow = torch.zeros_like(grow)
+            armijo = shrink & (phi_t <= phi0 + trial * 0.1 * gtd)
+            step = torch.where(armijo, trial, step)
+            phi_step = torch.where(armijo, phi_t, phi_step)
+            low = shrink & ~armijo & (trial < self.minstep)
+            downhill = low & (phi_t < phi0)
+            step = torch.where(downhill, trial, step)
+            phi_step = torch.where(downhill, phi_t, phi_step)
+            fail = low & ~downhill
+            step = torch.where(fail, zero, step)
+            phi_step = torch.where(fail, phi0, phi_step)
+            failed = failed | fail
+            shrink = shrink & ~armijo & ~low
+            alpha, phi = trial, phi_t
+        return step, phi_step, evals, failed, bool(torch.equal(alpha, step))
+
+    # -- driver ------------------------------------------------------------------------
+    def run(self, x0: torch.Tensor, evaluate: Evaluate) -> Tuple[torch.Tensor, Trace]:
+        x = x0.detach().reshape(-1).clone()
+        dev = x.device
+        eps = torch.finfo(x.dtype).eps
+        rtol = self.rtol if self.rtol is not None else math.sqrt(eps)
+        atol = self.atol if self.atol is not None else math.sqrt(eps)
+        self._reset_history(dev)
+        P = self.P
+        flags = dict(dtype=torch.bool, device=dev)
+        converged = torch.zeros(P, **flags)
+        stalled = torch.zeros(P, **flags)
+        needs_reset = torch.zeros(P, **flags)
+        trace = Trace()
+
+        E, g = evaluate(x)
+        E, g = E.to(torch.float64), g.reshape(-1).clone()
+        n_evals = [1]
+
+        def record(E, g, evals):
+            trace.energy.append(E.detach().cpu())
+            trace.grad_norm.append(self._seg_sum(g * g).sqrt().cpu())
+            trace.grad_max.append(self._seg_max(g.abs()).cpu())
+            trace.evals.append(evals)
+
+        record(E, g, 1)
+        t = torch.full((P,), self.lr, dtype=torch.float64, device=dev)
+        g_prev = None
+        x_ref = x.clone()
+        it = 0
+        while it < self.max_iter:
+            it += 1
+            # ---- search direction
+            if g_prev is None:
+                d = -g
+            else:
+                y = g - g_prev
+                s = x - x_ref
+                keep = self._seg_sum(y * s) > 1e-6
+                if bool(keep.any()):
+                    ke = self._per_elem(keep)
+                    self._push(self._padded(torch.where(ke, s, torch.zeros_like(s))),
+                               self._padded(torch.where(ke, y, torch.zeros_like(y))))
+                    x_ref = torch.where(ke, x, x_ref)
+                d = -self._flat(self._hg(self._padded(g))).to(x.dtype)
+            was_reset = needs_reset.clone()
+            if bool(needs_reset.any()):
+                self._drop_segments(needs_reset)
+                re = self._per_elem(needs_reset)
+                d = torch.where(re, -g, d)
+                x_ref = torch.where(re, x, x_ref)
+                needs_reset = torch.zeros_like(needs_reset)
+            inactive = converged | stalled
+            d = d * (~self._per_elem(inactive)).to(d.dtype)
+            g_prev = g.clone()
+            E_prev = E.clone()
+            # ---- directional derivative, with repair of uphill directions
+            gtd = self._seg_sum(g * d)
+            for repair in (1, 2):
+                bad = (gtd > -1e-5) & ~inactive
+                if not bool(bad.any()):
+                    break
+                be = self._per_elem(bad)
+                fixed = d * -torch.sign(g * d) if repair == 1 else -g
+                d = torch.where(be, fixed, d)
+                gtd = self._seg_sum(g * d)
+            # ---- line search
+            searching = ~inactive & (gtd < 0)
+            t0 = torch.clamp(t / 0.5, max=1.0)
+            x_base = x.clone()
+            last = {}
+
+            def phi_at(alpha):
+                xt = x_base + d * self._per_elem(alpha).to(x.dtype)
+                Et, gt = evaluate(xt)
+                last["x"], last["E"], last["g"] = xt, Et.to(torch.float64), gt.reshape(-1).clone()
+                return last["E"]
+
+            step, E_step, evals, failed, at_step = self._armijo(phi_at, gtd, E, t0, searching)
+            x = x_base + d * self._per_elem(step).to(x.dtype)
+            if at_step:
+                E, g = last["E"], last["g"]
+            else:
+                E, g = evaluate(x)
+                E, g = E.to(torch.float64), g.reshape(-1).clone()
+                evals += 1
+            retry = torch.clamp((-gtd).clamp(min=self.minstep).rsqrt(), max=1.0)
+            t = torch.where(failed, retry, step)
+            t = torch.where(searching, t, t0)
+            stalled = stalled | (failed & was_reset)
+            needs_reset = needs_reset | (failed & ~was_reset)
+            record(E, g, evals)
+            # ---- convergence
+            converged = converged | (self._seg_max(g.abs()) <= self.gradtol)
+            dE = (E - E_prev).abs()
+            rdiff = 2 * dE / (E.abs() + E_prev.abs() + 1e-10)
+            done = converged | stalled | (dE <= atol) | (rdiff <= rtol)
+            if self.verbose:
+                print("iter %4d  E=%.6f  evals=%d  done=%d/%d" % (it, float(E.sum()), evals,
+                                                                  int(done.sum()), P))
+            if bool(done.all()):
+                break
+        trace.iterations = it
+        trace.converged = converged.cpu()
+        trace.stalled = stalled.cpu()
+        return x, trace
+
+
+# ---------------------------------------------------------------------------
+# evaluators
+
+
+def ragged_evaluator(scorer, weights=(1.0, 0.55, 1.0)) -> Evaluate:
+    """Energy + weighted gradient of a ``LJLKRaggedScorer`` (fused kernel)."""
+    w = torch.tensor(weights, dtype=torch.float64)
+    wd = w.to(scorer.device)
+
+    def evaluate(x):
+        coords = x.view(-1, 3)
+        E, g = scorer.energy_and_grad(coords, w)
+        return (wd[:, None] * E.double()).sum(0), g.reshape(-1).clone()
+
+    return evaluate
+
+
+def op_evaluator(op_args: list, weights=(1.0, 0.55, 1.0), op=None) -> Evaluate:
+    """Energy + gradient through an ``ljlk_pose_scores``-schema op and
+    autograd (the drop-in path: forward kernel, saved dV/dx, backward
+    kernel).  ``op_args`` are the op's 27 arguments; argument 0 is replaced by
+    the coordinates being evaluated."""
+    if op is None:
+        from .ops import ljlk_pose_scores as op
+    args = list(op_args)
+    w = torch.tensor(weights, dtype=torch.float64)
+
+    def evaluate(x):
+        coords = x.detach().view(-1, 3).clone().requires_grad_(True)
+        args[0] = coords
+        score = op(*args)[0]
+        Ew = (w.to(score.device, score.dtype)[:, None] * score).sum(0)
+        Ew.sum().backward()
+        return Ew.detach().double(), coords.grad.detach().reshape(-1)
+
+    return evaluate
