@@ -40,7 +40,8 @@ extern "C" {
 /* score flags */
 #define TMOL_LJLK_DVDX 2       /* per-term dV/dx [3, NA, 3] (reference whole-pose forward) */
 #define TMOL_LJLK_WGRAD 4      /* weighted gradient sum_t w[t] dV_t/dx -> [NA, 3] */
-#define TMOL_LJLK_BLOCK_PAIR 8 /* energies per block pair [3, P, B, B]; weights too */
+#define TMOL_LJLK_BLOCK_PAIR 8 /* energies (and weights) per block pair: padded [3, P, B, B];
+                                  ragged [3, G, row_capacity] at (g1, position in row g1) */
 #define TMOL_LJLK_NO_ENERGY 16 /* skip the energy output (backward re-walk) */
 #define TMOL_LJLK_ACCUMULATE 32 /* add into dvdx / grad instead of overwriting */
 #define TMOL_LJLK_GENERAL_KERNEL 128 /* force the general (any block size) kernel */
@@ -99,6 +100,12 @@ int tmol_ljlk_fill_neighbor_tiles(const int32_t* pose_block_start_host,
 int64_t tmol_ljlk_workspace_bytes(const tmol_ljlk_stack* stack,
                                   const tmol_ljlk_block_types* bt,
                                   int64_t row_capacity, int32_t dtype);
+
+/* Device address of the block spheres [G, 4] (dtype) inside the workspace,
+ * written by tmol_ljlk_neighbors (centre x, y, z, radius). */
+int tmol_ljlk_spheres(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                      void* workspace, int64_t row_capacity, int32_t dtype,
+                      void** spheres_out);
 
 /* Neighbor pass (replaces compute_block_spheres + detect_block_neighbors,
  * sphere_overlap.impl.hh:109-196, 287-351): bounding sphere per block,
@@ -171,6 +178,86 @@ int tmol_ljlk_debug_pair_separations(const tmol_ljlk_stack* stack,
 
 /* Number of compute units of the current device (autotuner input). */
 int tmol_ljlk_device_cu_count(int32_t* cu_count);
+
+/* ---------------------------------------------------------------------
+ * Learned block-pair pruner (no counterpart in the reference: its kernels
+ * train and run the MLP that predicts which neighbor block pairs carry
+ * |E| < 1e-3 kcal/mol).  All arrays are device pointers; bf16 values are
+ * passed as uint16_t bit patterns. */
+
+/* Per-block inputs of the pruner MLP (all device, float32 / int32). */
+typedef struct {
+  const float* sphere;    /* [G, 4] centre, radius */
+  const float* sincos;    /* [G, 10] sin, cos of phi, psi, omega, chi1, chi2 */
+  const int32_t* emb_row; /* [G] embedding row of the block's residue type */
+  const int32_t* seqpos;  /* [G] position of the block within its pose */
+  const float* emb;       /* [n_emb, 8] residue-type embedding */
+} tmol_ljlk_prune_blocks;
+
+/* sin, cos of five torsions per block from global atom indices
+ * torsion_atoms [G, 5, 4] (-1: torsion absent -> 0, 0). */
+int tmol_ljlk_prune_block_torsions(int32_t n_blocks, const int32_t* torsion_atoms,
+                                   const void* coords, int32_t dtype, float* sincos,
+                                   void* stream);
+
+/* Feature rows [n, 48] (bf16; columns 40..47 zero) of block pairs [n, 2]. */
+int tmol_ljlk_prune_pair_features(int64_t n_pairs, const int32_t* pairs,
+                                  const tmol_ljlk_prune_blocks* blocks, uint16_t* features,
+                                  void* stream);
+
+/* C = A B on the matrix cores, bf16 operands, fp32 accumulation.
+ * a_km: A stored [K][M] (else [M][K]); b_kn: B stored [K][N] (else [N][K]).
+ * epilogue 0: C fp32 [M, N] (split_k > 1: atomically added, C pre-zeroed);
+ *          1: Z = C + bias[n] (fp32), H = SiLU(Z) (bf16);
+ *          2: D = C * SiLU'(Z) (fp32, Z read), H = D (bf16). */
+int tmol_ljlk_prune_gemm(int32_t epilogue, int32_t a_km, int32_t b_kn, const uint16_t* A,
+                         const uint16_t* B, int32_t M, int32_t N, int32_t K, int32_t lda,
+                         int32_t ldb, int32_t ldc, int32_t split_k, float* C, const float* bias,
+                         float* Z, uint16_t* H, float* D, void* stream);
+
+/* Output layer (128 -> 1) and, with train, the weighted logistic loss and its
+ * backward: loss (+=), dZ3 = dlogit w4 SiLU'(Z3) (fp32 and bf16), dw4 (+=),
+ * db4 (+=).  labels: 1 = negligible pair. */
+int tmol_ljlk_prune_head(int32_t n, const uint16_t* H3, const float* Z3, const float* w4,
+                         const float* b4, const float* labels, float pos_weight, int32_t train,
+                         float* logits, float* loss, float* dZ3, uint16_t* dZ3_bf16, float* dw4,
+                         float* db4, void* stream);
+
+/* out[c] += sum over rows of D[r, c] (bias gradients). */
+int tmol_ljlk_prune_colsum(int32_t n, int32_t ncol, int32_t ld, const float* D, float* out,
+                           void* stream);
+
+/* Embedding gradient: demb[emb_row[g]] += dX[pair, 3..19] of each side. */
+int tmol_ljlk_prune_emb_grad(int64_t n_pairs, const int32_t* pairs, const int32_t* emb_row,
+                             const float* dX, float* demb, void* stream);
+
+/* Bytes of device scratch tmol_ljlk_prune_flatten needs for n_blocks rows. */
+int64_t tmol_ljlk_prune_scratch_bytes(int32_t n_blocks);
+
+/* Flatten the work list: row_offset [G + 1] (exclusive offsets of each row's
+ * pairs), pairs [n, 2] (g1, g2) in row order.  The total n is left on the
+ * device at scratch[(G + 1023) / 1024] (int64) for tmol_ljlk_prune_infer.
+ * pairs must hold G * row_capacity entries. */
+int tmol_ljlk_prune_flatten(int32_t n_blocks, const uint32_t* row_count,
+                            const int32_t* row_items, int64_t row_capacity, int64_t* scratch,
+                            int64_t* row_offset, int32_t* pairs, void* stream);
+
+/* Fused inference on the matrix cores: features, three BF16 layers, head.
+ * keep[i] = 0 for a pair predicted negligible (logit > logit_cut) unless it
+ * is a block with itself; logits [n] may be NULL.  n is read from *n_pairs
+ * (device).  n_workgroups <= 0: one per resident slot. */
+int tmol_ljlk_prune_infer(const int64_t* n_pairs, const int32_t* pairs,
+                          const tmol_ljlk_prune_blocks* blocks, const uint16_t* W1,
+                          const uint16_t* W2, const uint16_t* W3, const float* b1,
+                          const float* b2, const float* b3, const float* w4, const float* b4,
+                          float logit_cut, uint8_t* keep, float* logits,
+                          int32_t n_workgroups, void* stream);
+
+/* Remove the pairs with keep == 0 from the work list, in place, keeping the
+ * order of each row; row_count is updated. */
+int tmol_ljlk_prune_compact(int32_t n_blocks, uint32_t* row_count, int32_t* row_items,
+                            int64_t row_capacity, const int64_t* row_offset,
+                            const uint8_t* keep, void* stream);
 
 #ifdef __cplusplus
 }

@@ -139,11 +139,16 @@ def oracle_pose_scores(*args, require_grad=False, dtype=None):
     Returns (scores, block_neighbors, dV_dx): scores is (3, P) or (3, P, B, B),
     block_neighbors (P, B, B) int32, dV_dx (3, NA, 3) or None."""
     S = _Stack(args)
-    L = lib()
     if dtype is None:
         dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
-    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
     coords = numpy.ascontiguousarray(S.coords.detach().cpu().numpy().astype(dtype))
+    return _scores(S, coords, require_grad, dtype)
+
+
+def _scores(S, coords, require_grad, dtype):
+    """The oracle on a prepared stack and host coordinates [NA, 3]."""
+    L = lib()
+    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
     tp = numpy.ascontiguousarray(S.type_params.detach().cpu().numpy().astype(dtype))
     gp = numpy.ascontiguousarray(S.global_params.detach().cpu().numpy().astype(dtype).reshape(-1))
     NA = coords.shape[0]
@@ -236,3 +241,48 @@ def load_reference_op():
     if not hasattr(torch.ops, "tmol_ljlk") or not hasattr(torch.ops.tmol_ljlk, "ljlk_pose_scores"):
         torch.ops.load_library(REF_LIB_PATH)
     return torch.ops.tmol_ljlk.ljlk_pose_scores
+
+
+def pose_oracle_evaluator(poses, weights=(1.0, 0.55, 1.0), threads=8, dtype=numpy.float64):
+    """Energy + weighted gradient of a list of poses by the oracle, one pose
+    per call (poses are independent), spread over ``threads`` host threads.
+
+    The parameter vector is the poses' atoms concatenated in order (the
+    ragged stack's layout), flattened; returns (E [P] float64, dE/dx [3 NA])
+    as torch tensors.  Used to drive the minimizer from the oracle."""
+    from concurrent.futures import ThreadPoolExecutor
+
+    from tmol_amd.pose import PoseStack
+    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+    tdt = torch.float64 if dtype == numpy.float64 else torch.float32
+    term = LJLKEnergyTerm()
+    per_pose = []
+    offset = 0
+    for pose in poses:
+        a = pose_op_args(PoseStack([pose], dtype=tdt), term)
+        a[23] = a[23].to(tdt)
+        a[24] = a[24].to(tdt)
+        per_pose.append((offset, pose.n_atoms, a))
+        offset += pose.n_atoms
+    w = numpy.asarray(weights, dtype=numpy.float64)
+    pool = ThreadPoolExecutor(max_workers=threads)
+
+    prepared = [(off, n, _Stack(a)) for off, n, a in per_pose]
+
+    def one(item, xn):
+        off, n, S = item
+        c = numpy.ascontiguousarray(xn[3 * off: 3 * (off + n)].reshape(n, 3).astype(dtype))
+        s, _, dv, _ = _scores(S, c, True, dtype)
+        e = float((w * s[:, 0].astype(numpy.float64)).sum())
+        g = (w[:, None, None] * dv.astype(numpy.float64)).sum(0).reshape(-1)
+        return e, g
+
+    def evaluate(x):
+        xn = x.detach().cpu().numpy().astype(numpy.float64)
+        res = list(pool.map(lambda it: one(it, xn), prepared))
+        E = torch.tensor([r[0] for r in res], dtype=torch.float64)
+        G = torch.from_numpy(numpy.concatenate([r[1] for r in res]))
+        return E.to(x.device), G.to(device=x.device, dtype=x.dtype)
+
+    return evaluate

@@ -165,3 +165,33 @@ def test_rows_kernel_matches_general_kernel(ubq_pdb, dtype):
     tol = dict(rtol=1e-10, atol=1e-9) if dtype == torch.float64 else dict(rtol=2e-5, atol=2e-4)
     for a, b in zip(out["rows"], out["general"]):
         torch.testing.assert_close(a, b, **tol)
+
+
+@pytest.mark.parametrize("general", [False, True])
+def test_item_energies_sum_to_pose_energies(ubq_pdb, general):
+    """Per-block-pair energies of a ragged stack ([3, G, capacity], one entry
+    per work-list pair) sum per pose to the whole-pose energies, and are zero
+    past each row's count."""
+    from tmol_amd import ops
+    from tmol_amd.scorer import LJLKRaggedScorer
+
+    poses = _poses(ubq_pdb)
+    _say("item energies general=%s" % general)
+    stack = RaggedPoseStack(poses, dtype=torch.float64)
+    sc = LJLKRaggedScorer(stack, LJLKEnergyTerm(device=DEV), device=DEV, dtype=torch.float64)
+    x = stack.coords.to(DEV)
+    if general:
+        out = torch.empty((3, sc.G, sc.capacity), dtype=torch.float64, device=DEV)
+        sc.neighbors(x)
+        sc.score(x, ops.BLOCK_PAIR | ops.GENERAL_KERNEL, energies=out)
+    else:
+        out = sc.item_energies(x)
+    cnt, _ = sc.ws.rows(sc.cstack, sc.bt.c)
+    cnt = cnt.cpu().long().clamp(max=sc.capacity)
+    past = torch.arange(sc.capacity)[None, :] >= cnt[:, None]
+    assert float(out.cpu()[:, past].abs().max()) == 0.0
+    per_block = out.sum(2).cpu()  # [3, G]
+    per_pose = torch.zeros(3, sc.P, dtype=torch.float64).index_add_(
+        1, stack.block_pose.long(), per_block)
+    e = sc.energies_only(x).cpu()
+    torch.testing.assert_close(per_pose, e, rtol=1e-10, atol=1e-8)

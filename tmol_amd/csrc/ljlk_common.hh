@@ -170,6 +170,22 @@ inline BTView make_bt_view(const tmol_ljlk_block_types* b) {
   return v;
 }
 
+// Block-pair outputs and weights (BLOCK_PAIR mode):
+//   padded: [3, P, B, B] at (pose, b1, b2), the reference's layout;
+//   ragged: [3, G, row_cap] at (g1, k), k = the pair's position in row g1 of
+//           the work list (one entry per scored block pair).
+__device__ __forceinline__ int64_t bp_plane(const StackView& s, const WorkView& w) {
+  return s.layout == TMOL_LJLK_PADDED ? int64_t(s.P) * s.B * s.B : int64_t(s.G) * w.row_cap;
+}
+__device__ __forceinline__ int64_t bp_index(const StackView& s, const WorkView& w, int g1, int g2,
+                                            int64_t k) {
+  if (s.layout == TMOL_LJLK_PADDED) {
+    const int p = g1 / s.B;
+    return (int64_t(p) * s.B + (g1 - p * s.B)) * s.B + (g2 - p * s.B);
+  }
+  return int64_t(g1) * w.row_cap + k;
+}
+
 // bond separation between connection c1 of g1 and c2 of g2 (g1 != g2)
 __device__ __forceinline__ int conn_sep(const StackView& s, int g1, int g2, int c1, int c2) {
   if (s.layout == TMOL_LJLK_PADDED) {

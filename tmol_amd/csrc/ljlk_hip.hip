@@ -22,6 +22,7 @@
 #include "ljlk_neighbors.hh"
 #include "ljlk_score_general.hh"
 #include "ljlk_score_rows.hh"
+#include "ljlk_prune.hh"
 
 namespace tmol_ljlk {
 
@@ -168,7 +169,11 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
   const int64_t NA3 = st->n_atoms_total * 3;
   if (!noe) {
     if (bp)
-      fill<Real>((Real*)energies, 3 * int64_t(P) * st->max_n_blocks * st->max_n_blocks, stream);
+      fill<Real>((Real*)energies,
+                 3 * (st->layout == TMOL_LJLK_PADDED
+                          ? int64_t(P) * st->max_n_blocks * st->max_n_blocks
+                          : int64_t(st->n_blocks_total) * row_cap),
+                 stream);
     else
       fill<double>(a.energies64, int64_t(3) * P, stream);
   }
@@ -244,6 +249,14 @@ int tmol_ljlk_work_list(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* b
   return TMOL_LJLK_OK;
 }
 
+int tmol_ljlk_spheres(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                      void* workspace, int64_t row_capacity, int32_t dtype, void** spheres_out) {
+  if (!s || !bt || !workspace || !spheres_out) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  *spheres_out = (char*)workspace + L.spheres;
+  return TMOL_LJLK_OK;
+}
+
 int tmol_ljlk_neighbors(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                         const void* coords, int32_t dtype, double max_dis, const int32_t* tiles,
                         int64_t n_tiles, void* workspace, int64_t ws_bytes,
@@ -277,7 +290,6 @@ int tmol_ljlk_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
   if ((flags & TMOL_LJLK_DVDX) && !dvdx) return TMOL_LJLK_EINVAL;
   if ((flags & TMOL_LJLK_WGRAD) && (!grad || !weights)) return TMOL_LJLK_EINVAL;
   if ((flags & TMOL_LJLK_DVDX) && (flags & TMOL_LJLK_WGRAD)) return TMOL_LJLK_EINVAL;
-  if ((flags & TMOL_LJLK_BLOCK_PAIR) && s->layout != TMOL_LJLK_PADDED) return TMOL_LJLK_ENOTSUP;
   if ((flags & TMOL_LJLK_BLOCK_PAIR) && (flags & TMOL_LJLK_DVDX)) return TMOL_LJLK_ENOTSUP;
   if (n_wave_groups < 0 || n_wave_groups > (1 << 20)) return TMOL_LJLK_EINVAL;
   hipStream_t st = (hipStream_t)stream;
@@ -364,6 +376,156 @@ int tmol_ljlk_device_cu_count(int32_t* cu) {
   if (hipGetDeviceProperties(&prop, dev) != hipSuccess) return TMOL_LJLK_EHIP;
   *cu = prop.multiProcessorCount;
   return TMOL_LJLK_OK;
+}
+
+
+// ------------------------------------------------------------------ pruner
+
+int tmol_ljlk_prune_block_torsions(int32_t n_blocks, const int32_t* torsion_atoms,
+                                   const void* coords, int32_t dtype, float* sincos,
+                                   void* stream) {
+  if (n_blocks < 0 || !torsion_atoms || !coords || !sincos) return TMOL_LJLK_EINVAL;
+  if (n_blocks == 0) return TMOL_LJLK_OK;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    hipLaunchKernelGGL(prune_block_torsions_kernel<float>, dim3(grid_for(n_blocks)), dim3(256), 0,
+                       st, n_blocks, torsion_atoms, (const float*)coords, sincos);
+  else if (dtype == TMOL_LJLK_F64)
+    hipLaunchKernelGGL(prune_block_torsions_kernel<double>, dim3(grid_for(n_blocks)), dim3(256), 0,
+                       st, n_blocks, torsion_atoms, (const double*)coords, sincos);
+  else
+    return TMOL_LJLK_EINVAL;
+  return check(hipGetLastError());
+}
+
+static PruneBlocks prune_blocks(const tmol_ljlk_prune_blocks* b) {
+  PruneBlocks p;
+  p.sphere = b->sphere;
+  p.sincos = b->sincos;
+  p.emb_row = b->emb_row;
+  p.seqpos = b->seqpos;
+  p.emb = b->emb;
+  return p;
+}
+
+int tmol_ljlk_prune_pair_features(int64_t n_pairs, const int32_t* pairs,
+                                  const tmol_ljlk_prune_blocks* blocks, uint16_t* features,
+                                  void* stream) {
+  if (n_pairs < 0 || !pairs || !blocks || !features) return TMOL_LJLK_EINVAL;
+  if (n_pairs == 0) return TMOL_LJLK_OK;
+  const int64_t n = n_pairs * PRUNE_K1;
+  hipLaunchKernelGGL(prune_pair_features_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n_pairs, pairs, prune_blocks(blocks), features);
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_prune_gemm(int32_t epilogue, int32_t a_km, int32_t b_kn, const uint16_t* A,
+                         const uint16_t* B, int32_t M, int32_t N, int32_t K, int32_t lda,
+                         int32_t ldb, int32_t ldc, int32_t split_k, float* C, const float* bias,
+                         float* Z, uint16_t* H, float* D, void* stream) {
+  if (!A || !B || M <= 0 || N <= 0 || K <= 0 || split_k <= 0) return TMOL_LJLK_EINVAL;
+  if (epilogue == EPI_F32 && !C) return TMOL_LJLK_EINVAL;
+  if (epilogue == EPI_BIAS_SILU && (!bias || !Z || !H)) return TMOL_LJLK_EINVAL;
+  if (epilogue == EPI_DSILU && (!Z || !H || !D)) return TMOL_LJLK_EINVAL;
+  if (epilogue != EPI_F32 && split_k != 1) return TMOL_LJLK_EINVAL;
+  GemmArgs g{A, B, M, N, K, lda, ldb, ldc, 0, C, bias, Z, H, D};
+  const int kc = ((K + split_k - 1) / split_k + GK - 1) / GK * GK;
+  g.k_chunk = kc;
+  const int nz = (K + kc - 1) / kc;
+  dim3 grid((N + GT - 1) / GT, (M + GT - 1) / GT, nz);
+  hipStream_t st = (hipStream_t)stream;
+#define GEMM_CASE(AK, BK, E) \
+  if (a_km == AK && b_kn == BK && epilogue == E) { \
+    hipLaunchKernelGGL((prune_gemm_kernel<AK, BK, E>), grid, dim3(256), 0, st, g); \
+    return check(hipGetLastError()); \
+  }
+  GEMM_CASE(0, 0, EPI_BIAS_SILU)
+  GEMM_CASE(0, 0, EPI_F32)
+  GEMM_CASE(0, 1, EPI_DSILU)
+  GEMM_CASE(0, 1, EPI_F32)
+  GEMM_CASE(1, 1, EPI_F32)
+#undef GEMM_CASE
+  return TMOL_LJLK_ENOTSUP;
+}
+
+int tmol_ljlk_prune_head(int32_t n, const uint16_t* H3, const float* Z3, const float* w4,
+                         const float* b4, const float* labels, float pos_weight, int32_t train,
+                         float* logits, float* loss, float* dZ3, uint16_t* dZ3_bf16, float* dw4,
+                         float* db4, void* stream) {
+  if (n <= 0 || !H3 || !w4 || !b4 || !logits) return TMOL_LJLK_EINVAL;
+  if (train && (!Z3 || !labels || !loss || !dZ3 || !dZ3_bf16 || !dw4 || !db4))
+    return TMOL_LJLK_EINVAL;
+  hipLaunchKernelGGL(prune_head_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
+                     n, H3, Z3, w4, b4, labels, pos_weight, train, logits, loss, dZ3, dZ3_bf16,
+                     dw4, db4);
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_prune_colsum(int32_t n, int32_t ncol, int32_t ld, const float* D, float* out,
+                           void* stream) {
+  if (n <= 0 || ncol <= 0 || ld < ncol || !D || !out) return TMOL_LJLK_EINVAL;
+  const int ychunks = std::min(64, (n + 255) / 256);
+  hipLaunchKernelGGL(prune_colsum_kernel, dim3(ncol, ychunks), dim3(256), 0, (hipStream_t)stream,
+                     n, ncol, ld, D, out);
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_prune_emb_grad(int64_t n_pairs, const int32_t* pairs, const int32_t* emb_row,
+                             const float* dX, float* demb, void* stream) {
+  if (n_pairs < 0 || !pairs || !emb_row || !dX || !demb) return TMOL_LJLK_EINVAL;
+  if (n_pairs == 0) return TMOL_LJLK_OK;
+  const int64_t n = n_pairs * 2 * PRUNE_EMB;
+  hipLaunchKernelGGL(prune_emb_grad_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0,
+                     (hipStream_t)stream, n_pairs, pairs, emb_row, dX, demb);
+  return check(hipGetLastError());
+}
+
+
+int64_t tmol_ljlk_prune_scratch_bytes(int32_t n_blocks) {
+  if (n_blocks < 0) return -1;
+  return 8 * (int64_t(n_blocks + 1023) / 1024 + 1);
+}
+
+int tmol_ljlk_prune_flatten(int32_t n_blocks, const uint32_t* row_count, const int32_t* row_items,
+                            int64_t row_capacity, int64_t* scratch, int64_t* row_offset,
+                            int32_t* pairs, void* stream) {
+  if (n_blocks <= 0 || !row_count || !row_items || row_capacity <= 0 || !scratch || !row_offset ||
+      !pairs)
+    return TMOL_LJLK_EINVAL;
+  const int64_t nb = (int64_t(n_blocks) + 1023) / 1024;
+  if (nb > (int64_t(1) << 20)) return TMOL_LJLK_ENOTSUP;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(prune_count_kernel, dim3((unsigned)nb), dim3(1024), 0, st, int64_t(n_blocks),
+                     row_count, row_capacity, scratch);
+  hipLaunchKernelGGL(prune_scan_blocks_kernel, dim3(1), dim3(1024), 0, st, nb, scratch);
+  hipLaunchKernelGGL(prune_flatten_kernel, dim3((unsigned)nb), dim3(1024), 0, st, int64_t(n_blocks),
+                     row_count, row_items, row_capacity, (const int64_t*)scratch, row_offset, pairs);
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_prune_infer(const int64_t* n_pairs, const int32_t* pairs,
+                          const tmol_ljlk_prune_blocks* blocks, const uint16_t* W1,
+                          const uint16_t* W2, const uint16_t* W3, const float* b1, const float* b2,
+                          const float* b3, const float* w4, const float* b4, float logit_cut,
+                          uint8_t* keep, float* logits, int32_t n_workgroups, void* stream) {
+  if (!n_pairs || !pairs || !blocks || !W1 || !W2 || !W3 || !b1 || !b2 || !b3 || !w4 || !b4 || !keep)
+    return TMOL_LJLK_EINVAL;
+  PruneNet net{W1, W2, W3, b1, b2, b3, w4, b4};
+  int nwg = n_workgroups;
+  if (nwg <= 0) nwg = resident_workgroups(prune_infer_kernel);
+  hipLaunchKernelGGL(prune_infer_kernel, dim3(nwg), dim3(PI_WAVES * 64), 0, (hipStream_t)stream,
+                     n_pairs, pairs, prune_blocks(blocks), net, logit_cut, keep, logits);
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_prune_compact(int32_t n_blocks, uint32_t* row_count, int32_t* row_items,
+                            int64_t row_capacity, const int64_t* row_offset, const uint8_t* keep,
+                            void* stream) {
+  if (n_blocks <= 0 || !row_count || !row_items || !row_offset || !keep) return TMOL_LJLK_EINVAL;
+  hipLaunchKernelGGL(prune_compact_kernel, dim3(grid_for(n_blocks)), dim3(256), 0,
+                     (hipStream_t)stream, int64_t(n_blocks), row_count, row_items, row_capacity,
+                     row_offset, keep);
+  return check(hipGetLastError());
 }
 
 }  // extern "C"

@@ -81,9 +81,7 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
     Real w[3] = {Real(0), Real(0), Real(0)};
     if (WGRAD) {
       if (BP) {
-        int b1 = g1 - p * s.B, b2 = g2 - p * s.B;
-        int64_t PBB = int64_t(s.P) * s.B * s.B;
-        int64_t base = (int64_t(p) * s.B + b1) * s.B + b2;
+        const int64_t PBB = bp_plane(s, A.w), base = bp_index(s, A.w, g1, g2, kk);
         for (int t = 0; t < 3; ++t) w[t] = A.weights[t * PBB + base];
       } else {
         for (int t = 0; t < 3; ++t) w[t] = A.weights[t * s.P + p];
@@ -328,9 +326,7 @@ __global__ __launch_bounds__(256) void score_general_kernel(ScoreArgs<Real> A) {
       double el = wave_sum(e_lk.value());
       if (lane == 0) {
         if (BP) {
-          int b1 = g1 - p * s.B, b2 = g2 - p * s.B;
-          int64_t PBB = int64_t(s.P) * s.B * s.B;
-          int64_t base = (int64_t(p) * s.B + b1) * s.B + b2;
+          const int64_t PBB = bp_plane(s, A.w), base = bp_index(s, A.w, g1, g2, kk);
           A.energies_bp[base] = Real(ea);
           A.energies_bp[PBB + base] = Real(er);
           A.energies_bp[2 * PBB + base] = Real(el);

@@ -344,7 +344,6 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
     const int p = uni(s.pose_of(g1));
     const int nc1 = uni(bt.n_conn[t1]);
     const bool lig1 = bt.is_ligand[t1] != 0;
-    const int b1 = g1 - p * s.B;  // block index within the pose (padded layout)
     const V sph1 = *(const V*)(A.spheres + 4 * int64_t(g1));
     Real wrow[3] = {Real(0), Real(0), Real(0)};
     if (C::WGRAD && !C::BP) {
@@ -370,7 +369,9 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
     int end_lj0 = 0, end_lj1 = 0, end_lk0 = 0, end_lk1 = 0;
     bool used0 = false, used1 = false;
     int o2s0 = 0, o2s1 = 0;
-    int n2s0 = 0, n2s1 = 0, b2s0 = 0, b2s1 = 0;
+    int n2s0 = 0, n2s1 = 0;
+    int64_t bps0 = 0, bps1 = 0, bp_intra = -1;  // block-pair output indices
+    const int64_t PBB = C::BP ? bp_plane(s, A.w) : 0;
     int ring = 0;
 
     auto drain = [&](bool all) {
@@ -390,9 +391,7 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
       // all entries of ring slot r are consumed: flush its g2 atoms
       flush_slots<Real, MODE>(L, A, TILE + TILE * r, r ? n2s1 : n2s0, r ? o2s1 : o2s0, lane);
       if (C::BP && !C::NOE && lane < 3) {
-        const int64_t PBB = int64_t(s.P) * s.B * s.B;
-        const int64_t base = (int64_t(p) * s.B + b1) * s.B + (r ? b2s1 : b2s0);
-        A.energies_bp[lane * PBB + base] = LdsAcc<Real>::dec(L.ebp[r][lane]);
+        A.energies_bp[lane * PBB + (r ? bps1 : bps0)] = LdsAcc<Real>::dec(L.ebp[r][lane]);
         L.ebp[r][lane] = 0;
       }
       if (r)
@@ -430,10 +429,8 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
         const AtomRegs<Real> cur = pf;
         if (g2 == g1) {
           // ---- intra-block pairs a1 < a2, separation = path distance
-          if (C::BP && C::WGRAD && lane < 3) {
-            const int64_t PBB = int64_t(s.P) * s.B * s.B;
-            L.wt[2][lane] = A.weights[lane * PBB + (int64_t(p) * s.B + b1) * s.B + b1];
-          }
+          if (C::BP) bp_intra = bp_index(s, A.w, g1, g1, c0 + j);
+          if (C::BP && C::WGRAD && lane < 3) L.wt[2][lane] = A.weights[lane * PBB + bp_intra];
           if (j + 1 < n_chunk) {
             const int t = lane_val(d_t2, j + 1), o = lane_val(d_o2, j + 1);
             const int n = lane_val(d_pk, j + 1) & 255;
@@ -482,7 +479,7 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
           const Real c2z = lane_val(d_sph.z, j), c2r = lane_val(d_sph.w, j);
           const int sb = TILE + TILE * r;
           const bool need_cp = min_sep <= 4;
-          const int b2 = g2 - p * s.B;
+          const int64_t bpi = C::BP ? bp_index(s, A.w, g1, g2, c0 + j) : 0;
           // cull: g2 atoms within reach of g1's sphere (lanes 0..31) and g1
           // atoms within reach of g2's sphere (lanes 32..63)
           V mine;
@@ -530,10 +527,7 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
             const int cc1 = qq / MAX_CONN, cc2 = qq % MAX_CONN;
             L.cs[r][qq] = (cc1 < nc1 && cc2 < nc2) ? (uint8_t)conn_sep(s, g1, g2, cc1, cc2) : 6;
           }
-          if (C::BP && C::WGRAD && lane < 3) {
-            const int64_t PBB = int64_t(s.P) * s.B * s.B;
-            L.wt[r][lane] = A.weights[lane * PBB + (int64_t(p) * s.B + b1) * s.B + b2];
-          }
+          if (C::BP && C::WGRAD && lane < 3) L.wt[r][lane] = A.weights[lane * PBB + bpi];
           if (lane < n2) stage_atom(L, sb + lane, 1 + r, lane, cur);
           if (near) {
             if (lane < TILE)
@@ -547,9 +541,9 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
             if (lane < n) load_atom(pf, A, t, o, lane);
           }
           if (r) {
-            used1 = true; o2s1 = o2; n2s1 = n2; b2s1 = b2;
+            used1 = true; o2s1 = o2; n2s1 = n2; bps1 = bpi;
           } else {
-            used0 = true; o2s0 = o2; n2s0 = n2; b2s0 = b2;
+            used0 = true; o2s0 = o2; n2s0 = n2; bps0 = bpi;
           }
           wave_sync();
           // phase A over the culled atoms, two pair indices per lane and
@@ -621,10 +615,8 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
     if (used0) retire(0);
     if (used1) retire(1);
     flush_slots<Real, MODE>(L, A, 0, n1, o1, lane);
-    if (C::BP && !C::NOE && lane < 3) {
-      const int64_t PBB = int64_t(s.P) * s.B * s.B;
-      const int64_t base = (int64_t(p) * s.B + b1) * s.B + b1;
-      A.energies_bp[lane * PBB + base] = LdsAcc<Real>::dec(L.ebp[2][lane]);
+    if (C::BP && !C::NOE && lane < 3 && bp_intra >= 0) {
+      A.energies_bp[lane * PBB + bp_intra] = LdsAcc<Real>::dec(L.ebp[2][lane]);
       L.ebp[2][lane] = 0;
     }
     if (!C::NOE && !C::BP) {

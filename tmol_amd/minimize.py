@@ -354,3 +354,74 @@ def op_evaluator(op_args: list, weights=(1.0, 0.55, 1.0), op=None) -> Evaluate:
         return Ew.detach().double(), coords.grad.detach().reshape(-1)
 
     return evaluate
+
+
+def main(argv=None):
+    """Minimize a workload's poses with LJ+LK on the GPU; print a JSON summary.
+
+        python -m tmol_amd.minimize --workload diff_32x400 --iters 200 \\
+            [--dtype f32|f64] [--path fused|autograd] [--trace out.json]
+    """
+    import argparse
+    import json
+    import time
+
+    from .generator import workload_poses
+    from .pose import RaggedPoseStack
+    from .score.ljlk_energy_term import LJLKEnergyTerm
+    from .scorer import LJLKRaggedScorer, ragged_pose_scores
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--workload", default="diff_32x400")
+    ap.add_argument("--iters", type=int, default=200)
+    ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
+    ap.add_argument("--path", default="autograd", choices=["fused", "autograd"])
+    ap.add_argument("--trace", default=None)
+    args = ap.parse_args(argv)
+
+    dev = torch.device("cuda:0")
+    dtype = torch.float32 if args.dtype == "f32" else torch.float64
+    poses = workload_poses(args.workload)
+    stack = RaggedPoseStack(poses, dtype=dtype)
+    scorer = LJLKRaggedScorer(stack, LJLKEnergyTerm(device=dev), device=dev, dtype=dtype)
+    x0 = stack.coords.to(dev).reshape(-1)
+    seg = stack.atom_pose.to(dev).repeat_interleave(3).long()
+    w = torch.tensor([1.0, 0.55, 1.0], dtype=torch.float64, device=dev)
+    if args.path == "fused":
+        evaluate = ragged_evaluator(scorer)
+    else:
+        def evaluate(x):
+            c = x.detach().view(-1, 3).clone().requires_grad_(True)
+            E = ragged_pose_scores(scorer, c)
+            Ew = (w.to(E.dtype)[:, None] * E).sum(0)
+            Ew.sum().backward()
+            return Ew.detach().double(), c.grad.reshape(-1)
+    opt = SegmentedLBFGS(seg, max_iter=args.iters)
+    evaluate(x0)  # warm up
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    x, tr = opt.run(x0, evaluate)
+    torch.cuda.synchronize()
+    secs = time.perf_counter() - t0
+    E = tr.energies()
+    n_res = torch.tensor([p.n_blocks for p in poses], dtype=torch.float64)
+    out = {
+        "workload": args.workload, "dtype": args.dtype, "path": args.path,
+        "poses": len(poses), "atoms": stack.n_atoms_total,
+        "iterations": tr.iterations, "evaluations": int(sum(tr.evals)),
+        "seconds": secs, "ms_per_iteration": 1e3 * secs / max(tr.iterations, 1),
+        "energy_start": float(E[0].sum()), "energy_end": float(E[-1].sum()),
+        "energy_per_residue_end": (E[-1] / n_res).tolist(),
+        "grad_norm_start": float(tr.grad_norms()[0].norm()),
+        "grad_norm_end": float(tr.grad_norms()[-1].norm()),
+        "converged": int(tr.converged.sum()), "stalled": int(tr.stalled.sum()),
+    }
+    if args.trace:
+        with open(args.trace, "w") as f:
+            json.dump({**out, "energy": [e.tolist() for e in tr.energy],
+                       "grad_norm": [g.tolist() for g in tr.grad_norm], "evals": tr.evals}, f)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

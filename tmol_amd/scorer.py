@@ -130,6 +130,15 @@ class LJLKRaggedScorer:
         self.score(coords, 0, energies=self.energies)
         return self.energies
 
+    def item_energies(self, coords: torch.Tensor) -> torch.Tensor:
+        """Energies of every work-list block pair, [3, G, capacity]: entry
+        (t, g1, k) is term t of the pair (g1, row_items[g1, k]); entries past
+        row_count[g1] are zero."""
+        out = torch.empty((3, self.G, self.capacity), dtype=self.dtype, device=self.device)
+        self.neighbors(coords)
+        self.score(coords, ops.BLOCK_PAIR, energies=out)
+        return out
+
     def per_term_gradients(self, coords: torch.Tensor):
         dvdx = torch.empty((3, self.NA, 3), dtype=self.dtype, device=self.device)
         self.neighbors(coords)
@@ -222,3 +231,33 @@ class LJLKRaggedScorer:
             except OSError:
                 pass
         return entry
+
+
+class RaggedScoreFunction(torch.autograd.Function):
+    """Per-pose, per-term energies [3, P] of a ragged stack with a hand-written
+    backward: the forward saves the per-term dV/dx [3, NA, 3] (pre-reduced
+    per atom in LDS by the scoring kernel) and the backward combines it with
+    the incoming [3, P] gradient in one kernel, as the reference's
+    LJLKPoseScoreOp does for the padded op (compiled.ops.cpp:157-174)."""
+
+    @staticmethod
+    def forward(ctx, coords: torch.Tensor, scorer: "LJLKRaggedScorer"):
+        if coords.requires_grad:
+            energies, dvdx = scorer.per_term_gradients(coords.detach())
+            ctx.save_for_backward(dvdx, scorer.keep["atom_pose"])
+        else:
+            energies = scorer.energies_only(coords)
+        return energies.clone()
+
+    @staticmethod
+    def backward(ctx, grad_energies):
+        dvdx, atom_pose = ctx.saved_tensors
+        grad = torch.ops.tmol_amd.ljlk_backward_combine(dvdx, grad_energies.contiguous(), atom_pose)
+        return grad, None
+
+
+def ragged_pose_scores(scorer: "LJLKRaggedScorer", coords: torch.Tensor) -> torch.Tensor:
+    """[3, P] LJ+LK energies (atr, rep, lk) of the scorer's stack at
+    ``coords``, differentiable with respect to ``coords``."""
+    return RaggedScoreFunction.apply(coords, scorer)
+
