@@ -108,34 +108,46 @@ class SegmentedLBFGS:
         return per_seg[self.seg]
 
     # -- history ------------------------------------------------------------------
+    # (s, y) pairs live in a ring of m physical slots; ``order`` lists the
+    # slots from oldest to newest.  The small Gram matrices are kept in that
+    # chronological order, so only they move when the oldest entry is dropped.
     def _reset_history(self, dev):
         m, P, S = self.m, self.P, self.S
-        self.Sh = torch.zeros((m, P, S), dtype=torch.float64, device=dev)  # s pairs, chronological
+        self.Sh = torch.zeros((m, P, S), dtype=torch.float64, device=dev)
         self.Yh = torch.zeros((m, P, S), dtype=torch.float64, device=dev)
         self.SY = torch.zeros((P, m, m), dtype=torch.float64, device=dev)  # s_i . y_j
         self.YY = torch.zeros((P, m, m), dtype=torch.float64, device=dev)  # y_i . y_j
+        self.order = []
         self.n_hist = 0
+
+    def _used(self):
+        """Physical slots in use (a prefix until the ring wraps) and the
+        chronological order as indices into them."""
+        n_phys = self.m if self.n_hist == self.m else self.n_hist
+        idx = torch.tensor(self.order, dtype=torch.int64, device=self.Sh.device)
+        return n_phys, idx
 
     def _push(self, s: torch.Tensor, y: torch.Tensor):
         """Append one (s, y) pair [P, S] (rows of segments that did not keep
         it are zero) as the newest history entry."""
         if self.n_hist == self.m:
-            # drop the oldest entry: shift history and Gram matrices by one
-            self.Sh = torch.roll(self.Sh, -1, 0)
-            self.Yh = torch.roll(self.Yh, -1, 0)
+            phys = self.order.pop(0)
             self.SY = torch.roll(self.SY, (-1, -1), (1, 2))
             self.YY = torch.roll(self.YY, (-1, -1), (1, 2))
             k = self.m - 1
         else:
+            phys = self.n_hist
             k = self.n_hist
             self.n_hist += 1
-        self.Sh[k] = s
-        self.Yh[k] = y
+        self.order.append(phys)
+        self.Sh[phys] = s
+        self.Yh[phys] = y
+        n_phys, idx = self._used()
+        Sv, Yv = self.Sh[:n_phys], self.Yh[:n_phys]
+        s_y = torch.einsum("ips,ps->pi", Sv, y)[:, idx]  # s_i . y_new, chronological
+        y_s = torch.einsum("ips,ps->pi", Yv, s)[:, idx]  # y_i . s_new
+        y_y = torch.einsum("ips,ps->pi", Yv, y)[:, idx]  # y_i . y_new
         n = self.n_hist
-        Sv, Yv = self.Sh[:n], self.Yh[:n]
-        s_y = torch.einsum("ips,ps->pi", Sv, y)  # s_i . y_new
-        y_s = torch.einsum("ips,ps->pi", Yv, s)  # y_i . s_new
-        y_y = torch.einsum("ips,ps->pi", Yv, y)  # y_i . y_new
         self.SY[:, :n, k] = s_y
         self.SY[:, k, :n] = y_s
         self.YY[:, :n, k] = y_y
@@ -143,22 +155,23 @@ class SegmentedLBFGS:
 
     def _drop_segments(self, mask: torch.Tensor):
         """Forget the history of the segments in ``mask``."""
-        idx = mask.nonzero(as_tuple=False).flatten()
-        if idx.numel() == 0:
+        sel = mask.nonzero(as_tuple=False).flatten()
+        if sel.numel() == 0:
             return
-        self.Sh[:, idx] = 0.0
-        self.Yh[:, idx] = 0.0
-        self.SY[idx] = 0.0
-        self.YY[idx] = 0.0
+        self.Sh[:, sel] = 0.0
+        self.Yh[:, sel] = 0.0
+        self.SY[sel] = 0.0
+        self.YY[sel] = 0.0
 
     def _hg(self, g: torch.Tensor) -> torch.Tensor:
         """H g for padded g [P, S] (float64), compact representation."""
         n = self.n_hist
         if n == 0:
             return g.clone()
-        Sv, Yv = self.Sh[:n], self.Yh[:n]
-        a = torch.einsum("ips,ps->pi", Sv, g)
-        b = torch.einsum("ips,ps->pi", Yv, g)
+        n_phys, idx = self._used()
+        Sv, Yv = self.Sh[:n_phys], self.Yh[:n_phys]
+        a = torch.einsum("ips,ps->pi", Sv, g)[:, idx]
+        b = torch.einsum("ips,ps->pi", Yv, g)[:, idx]
         SY = self.SY[:, :n, :n]
         D = torch.diagonal(SY, dim1=1, dim2=2)
         # absent entries (zero rows) get a unit diagonal: they then contribute 0
@@ -167,7 +180,12 @@ class SegmentedLBFGS:
         v = torch.einsum("pij,pj->pi", self.YY[:, :n, :n], u) + D * u - b
         p1 = torch.linalg.solve_triangular(R.transpose(1, 2), v.unsqueeze(-1),
                                            upper=False).squeeze(-1)
-        return g + torch.einsum("pi,ips->ps", p1, Sv) - torch.einsum("pi,ips->ps", u, Yv)
+        # back to physical slot order for the products with the history
+        p1_phys = torch.zeros((self.P, n_phys), dtype=g.dtype, device=g.device)
+        u_phys = torch.zeros_like(p1_phys)
+        p1_phys[:, idx] = p1
+        u_phys[:, idx] = u
+        return g + torch.einsum("pi,ips->ps", p1_phys, Sv) - torch.einsum("pi,ips->ps", u_phys, Yv)
 
     # -- line search ------------------------------------------------------------------
     def _armijo(self, phi_at, gtd, phi0, alpha0, searching):

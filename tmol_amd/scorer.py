@@ -68,9 +68,12 @@ class LJLKRaggedScorer:
         self.tiles = ops.neighbor_tiles(stack.pose_block_start, dev)
         max_nb = int(stack.n_blocks_per_pose.max())
         if capacity is None:
-            # neighbor blocks g2 >= g1 per block: bounded by the pose length;
-            # 192 covers densely packed chains (checked after the first pass)
-            capacity = min(max_nb, 192)
+            # a row holds the neighbor blocks g2 >= g1 of one pose: never more
+            # than the longest pose's block count, so the default cannot
+            # overflow whatever the coordinates (e.g. line-search trial steps
+            # that scatter atoms); a smaller capacity is checked after the
+            # first neighbor pass
+            capacity = max_nb
         self.capacity = int(capacity)
         self.ws = ops.Workspace(self.cstack, self.bt.c, self.capacity, self.dcode, dev)
         self.energies = torch.zeros((3, self.P), dtype=dtype, device=dev)
