@@ -76,6 +76,44 @@ def cpu_baseline(poses, sample_stride, time_budget_s=25.0):
     }
 
 
+def cpu_baseline_all_cores(poses, sample_stride, time_budget_s=15.0):
+    """The C restatement (oracle) on every host core at once: one pose per
+    task, tasks spread over a thread pool (the C code runs without the GIL)."""
+    import os
+    from concurrent.futures import ThreadPoolExecutor
+
+    import numpy
+    import torch
+
+    from oracle import oracle as orc
+    from tmol_amd.pose import PoseStack
+    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+    cores = os.cpu_count() or 1
+    term = LJLKEnergyTerm()
+    sample = poses[::sample_stride]
+    prepared = []
+    for p in sample:
+        a = pose_op_args(PoseStack([p]), term)
+        prepared.append(a)
+
+    def one(a):
+        orc.oracle_pose_scores(*a, require_grad=True, dtype=numpy.float32)
+        return 1
+
+    done = 0
+    t0 = time.perf_counter()
+    with ThreadPoolExecutor(max_workers=cores) as ex:
+        for k in range(0, len(prepared), cores):
+            done += sum(ex.map(one, prepared[k:k + cores]))
+            if time.perf_counter() - t0 > time_budget_s:
+                break
+    dt = time.perf_counter() - t0
+    return {"value": done / dt, "unit": "poses/s", "cores": cores, "kind": "port",
+            "sample": "%d poses (every %dth), f32 C restatement, forward+backward, %d threads, %.1f s"
+                      % (done, sample_stride, cores, dt)}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -89,6 +127,8 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="time candidate grid sizes first (default: resident-slot grid)")
     ap.add_argument("--wave-groups", type=int, default=0)
+    ap.add_argument("--no-pruner", action="store_true",
+                    help="skip the learned-pruner leg (1024 x 300-residue batch)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported. Poses are "
@@ -224,9 +264,21 @@ def main():
             "algorithmic_bytes_per_launch": abytes["score"],
         },
     }
+    if not args.no_pruner:
+        from tmol_amd import prune_bench
+
+        log("pruner: train, calibrate, measure ...")
+        pr = prune_bench.run(dev, epochs=6, steps=10, warmup=3, log=log)
+        result["pruned"] = {k: pr[k] for k in (
+            "workload", "poses", "threshold", "exact_poses_per_s", "pruned_poses_per_s", "speedup",
+            "pruned_fraction", "max_err_per_res", "mean_err_per_res", "within_bound", "fallback",
+            "effective_poses_per_s")}
+        result["pruned"]["train_seconds"] = pr["training"]["train_seconds"]
+        result["pruned"]["train_accuracy"] = pr["training"]["train_accuracy"]
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(poses, args.cpu_stride)
+        result["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(poses, args.cpu_stride)
     print(json.dumps(result), flush=True)
 
 

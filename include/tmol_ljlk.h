@@ -217,11 +217,11 @@ int tmol_ljlk_prune_gemm(int32_t epilogue, int32_t a_km, int32_t b_kn, const uin
 
 /* Output layer (128 -> 1) and, with train, the weighted logistic loss and its
  * backward: loss (+=), dZ3 = dlogit w4 SiLU'(Z3) (fp32 and bf16), dw4 (+=),
- * db4 (+=).  labels: 1 = negligible pair. */
+ * db4 (+=).  labels: 1 = negligible pair; sample_weights [n] may be NULL. */
 int tmol_ljlk_prune_head(int32_t n, const uint16_t* H3, const float* Z3, const float* w4,
-                         const float* b4, const float* labels, float pos_weight, int32_t train,
-                         float* logits, float* loss, float* dZ3, uint16_t* dZ3_bf16, float* dw4,
-                         float* db4, void* stream);
+                         const float* b4, const float* labels, const float* sample_weights,
+                         float pos_weight, int32_t train, float* logits, float* loss, float* dZ3,
+                         uint16_t* dZ3_bf16, float* dw4, float* db4, void* stream);
 
 /* out[c] += sum over rows of D[r, c] (bias gradients). */
 int tmol_ljlk_prune_colsum(int32_t n, int32_t ncol, int32_t ld, const float* D, float* out,

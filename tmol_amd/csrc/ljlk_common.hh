@@ -272,6 +272,17 @@ struct LdsAcc<double> {
   __device__ static __forceinline__ double dec(T x) { return x; }
 };
 
+// one lane's running energy sum, accumulated in FP64 (each FP32 pair term is
+// added exactly; FP64 adds run at the FP32 rate on MI355X)
+struct LaneSum {
+  double s = 0.0;
+  template <typename Real>
+  __device__ __forceinline__ void add(Real v) {
+    s += double(v);
+  }
+  __device__ __forceinline__ double value() const { return s; }
+};
+
 // compensated (Neumaier) accumulation of one lane's pair energies
 template <typename Real>
 struct KSum {

@@ -449,15 +449,15 @@ int tmol_ljlk_prune_gemm(int32_t epilogue, int32_t a_km, int32_t b_kn, const uin
 }
 
 int tmol_ljlk_prune_head(int32_t n, const uint16_t* H3, const float* Z3, const float* w4,
-                         const float* b4, const float* labels, float pos_weight, int32_t train,
-                         float* logits, float* loss, float* dZ3, uint16_t* dZ3_bf16, float* dw4,
-                         float* db4, void* stream) {
+                         const float* b4, const float* labels, const float* sample_weights,
+                         float pos_weight, int32_t train, float* logits, float* loss, float* dZ3,
+                         uint16_t* dZ3_bf16, float* dw4, float* db4, void* stream) {
   if (n <= 0 || !H3 || !w4 || !b4 || !logits) return TMOL_LJLK_EINVAL;
   if (train && (!Z3 || !labels || !loss || !dZ3 || !dZ3_bf16 || !dw4 || !db4))
     return TMOL_LJLK_EINVAL;
   hipLaunchKernelGGL(prune_head_kernel, dim3((n + 255) / 256), dim3(256), 0, (hipStream_t)stream,
-                     n, H3, Z3, w4, b4, labels, pos_weight, train, logits, loss, dZ3, dZ3_bf16,
-                     dw4, db4);
+                     n, H3, Z3, w4, b4, labels, sample_weights, pos_weight, train, logits, loss,
+                     dZ3, dZ3_bf16, dw4, db4);
   return check(hipGetLastError());
 }
 

@@ -63,6 +63,18 @@ __device__ __forceinline__ double fast_sqrt<double>(double x) {
   return __builtin_sqrt(x);
 }
 
+// 1 / sqrt(x): hardware v_rsq_f32 for float, correctly rounded ops for double
+template <typename Real>
+__device__ __forceinline__ Real fast_rsqrt(Real x);
+template <>
+__device__ __forceinline__ float fast_rsqrt<float>(float x) {
+  return __builtin_amdgcn_rsqf(x);
+}
+template <>
+__device__ __forceinline__ double fast_rsqrt<double>(double x) {
+  return 1.0 / __builtin_sqrt(x);
+}
+
 template <typename Real>
 __device__ __forceinline__ Real fast_exp(Real x);
 template <>

@@ -243,14 +243,16 @@ __global__ __launch_bounds__(256) void prune_gemm_kernel(GemmArgs g) {
 // ------------------------------------------------------------------ head
 
 // Output layer and logistic loss for a batch:
-//   logit = H3 . w4 + b4;  p = sigmoid(logit);  label y in {0, 1}
-//   loss_b = pos_w y log(1 + e^-l) + (1 - y) log(1 + e^l)  (summed into *loss)
-//   with train: dlogit = (pos_w y (p - 1) + (1 - y) p) / n;
+//   logit = H3 . w4 + b4;  p = sigmoid(logit);  label y in {0, 1}; sample
+//   weight s_b (1 when sample_w is null)
+//   loss_b = s_b [pos_w y log(1 + e^-l) + (1 - y) log(1 + e^l)]  (summed into *loss)
+//   with train: dlogit = s_b (pos_w y (p - 1) + (1 - y) p) / n;
 //   dZ3 = dlogit w4 silu'(Z3) (fp32 + bf16); dw4 += dlogit H3; db4 += dlogit
 __global__ __launch_bounds__(256) void prune_head_kernel(
     int n, const uint16_t* __restrict__ H3, const float* __restrict__ Z3,
     const float* __restrict__ w4, const float* __restrict__ b4, const float* __restrict__ labels,
-    float pos_w, int train, float* __restrict__ logits, float* __restrict__ loss,
+    const float* __restrict__ sample_w, float pos_w, int train, float* __restrict__ logits,
+    float* __restrict__ loss,
     float* __restrict__ dZ3, uint16_t* __restrict__ dZ3b, float* __restrict__ dw4,
     float* __restrict__ db4) {
   const int lane = lane_id();
@@ -268,9 +270,10 @@ __global__ __launch_bounds__(256) void prune_head_kernel(
     const float y = labels[b];
     const float sp_neg = fmaxf(-l, 0.f) + log1pf(__expf(-fabsf(l)));  // log(1 + e^-l)
     const float sp_pos = sp_neg + l;                                   // log(1 + e^l)
-    ls = pos_w * y * sp_neg + (1.f - y) * sp_pos;
+    const float sw = sample_w ? sample_w[b] : 1.f;
+    ls = sw * (pos_w * y * sp_neg + (1.f - y) * sp_pos);
     const float p = sigmoidf_(l);
-    dl = (pos_w * y * (p - 1.f) + (1.f - y) * p) / float(n);
+    dl = sw * (pos_w * y * (p - 1.f) + (1.f - y) * p) / float(n);
     for (int i = 0; i < PRUNE_H; ++i) {
       const int64_t o = int64_t(b) * PRUNE_H + i;
       const float d = dl * w4[i] * dsilu_(Z3[o]);
@@ -420,9 +423,46 @@ __global__ __launch_bounds__(PI_WAVES * 64) void prune_infer_kernel(
       g1 = pairs[2 * i];
       g2 = pairs[2 * i + 1];
     }
-    // features: lane covers row r, columns [24 half, 24 half + 24)
-    for (int f = 24 * half; f < 24 * half + 24; ++f)
-      act[r][f] = valid ? to_bf16(pair_feature(B, g1, g2, f)) : uint16_t(0);
+    // features: lane r writes the columns of block g1 of row r, lane r + 32
+    // those of g2 (same code on both halves, only the offsets differ); the
+    // centroid distance and sequence separation use the partner lane's block
+    {
+      const int gb = half ? g2 : g1;
+      const float4 sph = *(const float4*)(B.sphere + 4 * int64_t(gb));
+      const float4 e0 = *(const float4*)(B.emb + B.emb_row[gb] * PRUNE_EMB);
+      const float4 e1 = *(const float4*)(B.emb + B.emb_row[gb] * PRUNE_EMB + 4);
+      const float* sc = B.sincos + int64_t(gb) * 2 * PRUNE_TORS;
+      float t[2 * PRUNE_TORS];
+#pragma unroll
+      for (int k = 0; k < 2 * PRUNE_TORS; k += 2) {
+        const float2 v = *(const float2*)(sc + k);
+        t[k] = v.x;
+        t[k + 1] = v.y;
+      }
+      const int sp = B.seqpos[gb];
+      const float ox = __shfl_xor(sph.x, 32, WAVE), oy = __shfl_xor(sph.y, 32, WAVE);
+      const float oz = __shfl_xor(sph.z, 32, WAVE);
+      const int osp = __shfl_xor(sp, 32, WAVE);
+      const float dx = sph.x - ox, dy = sph.y - oy, dz = sph.z - oz;
+      const float ev[PRUNE_EMB] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+      const uint16_t z = 0;
+      // columns: 0 distance, 1 + half radius, 3 + 8 half embedding,
+      // 19 + 10 half torsions, 39 sequence separation, 40..47 zero
+      if (half == 0) {
+        act[r][0] = valid ? to_bf16(sqrtf(dx * dx + dy * dy + dz * dz) * 0.1f) : z;
+        int d = sp - osp;
+        d = d < 0 ? -d : d;
+        act[r][39] = valid ? to_bf16(float(d < 32 ? d : 32) * (1.0f / 32.0f)) : z;
+      } else {
+#pragma unroll
+        for (int k = PRUNE_IN; k < PRUNE_K1; ++k) act[r][k] = z;
+      }
+      act[r][1 + half] = valid ? to_bf16(sph.w * 0.1f) : z;
+#pragma unroll
+      for (int k = 0; k < PRUNE_EMB; ++k) act[r][3 + 8 * half + k] = valid ? to_bf16(ev[k]) : z;
+#pragma unroll
+      for (int k = 0; k < 2 * PRUNE_TORS; ++k) act[r][19 + 10 * half + k] = valid ? to_bf16(t[k]) : z;
+    }
     wave_sync();
     prune_layer<PRUNE_K1, PS1>(act, L.w1, L.b[0], lane);
     prune_layer<PRUNE_H, PSH>(act, L.w2, L.b[1], lane);

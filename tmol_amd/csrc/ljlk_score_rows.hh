@@ -170,7 +170,7 @@ __device__ __forceinline__ void grad_scatter(RowLDS<Real, MODE>& L, int term, in
 template <typename Real, int MODE>
 __device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
                                         const GlobalParams<Real>& gp, const Real* wrow,
-                                        KSum<Real>& e_atr, KSum<Real>& e_rep, int lane) {
+                                        LaneSum& e_atr, LaneSum& e_rep, int lane) {
   using C = RowCtx<Real, MODE>;
   for (int i0 = 0; i0 < consume; i0 += WAVE) {
     const int i = i0 + lane;
@@ -181,7 +181,9 @@ __device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
       using V = typename V4<Real>::T;
       const V p1 = *(const V*)L.pos[s1], p2 = *(const V*)L.pos[s2];
       const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
-      const Real dist = fast_sqrt(dx * dx + dy * dy + dz * dz);
+      const Real d2 = dx * dx + dy * dy + dz * dz;
+      const Real inv_d = fast_rsqrt(d2);
+      const Real dist = d2 > Real(0) ? d2 * inv_d : Real(0);  // coincident atoms: d = 0
       const Real sigma = lj_sigma(L.flg[s1], L.flg[s2], p1.w, p2.w, gp);
       const Real eps = L.prm[s1][0] * L.prm[s2][0];
       Real atr, rep, dlj;
@@ -198,7 +200,7 @@ __device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
       }
       if (C::DERIV && dlj != Real(0) && dist > Real(0)) {
         const bool inside = dist < sigma;
-        Real gs = dlj * fast_rcp(dist);
+        Real gs = dlj * inv_d;
         int term = 0;
         if (C::DVDX)
           term = inside ? 1 : 0;
@@ -213,7 +215,7 @@ __device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
 template <typename Real, int MODE>
 __device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
                                         const GlobalParams<Real>& gp, const Real* wrow,
-                                        KSum<Real>& e_lk, int lane) {
+                                        LaneSum& e_lk, int lane) {
   using C = RowCtx<Real, MODE>;
   for (int i0 = 0; i0 < consume; i0 += WAVE) {
     const int i = i0 + lane;
@@ -225,7 +227,9 @@ __device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
       const V p1 = *(const V*)L.pos[s1], p2 = *(const V*)L.pos[s2];
       const V q1 = *(const V*)L.prm[s1], q2 = *(const V*)L.prm[s2];
       const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
-      const Real dist = fast_sqrt(dx * dx + dy * dy + dz * dz);
+      const Real d2 = dx * dx + dy * dy + dz * dz;
+      const Real inv_d = fast_rsqrt(d2);
+      const Real dist = d2 > Real(0) ? d2 * inv_d : Real(0);  // coincident atoms: d = 0
       const int f1 = L.flg[s1], f2 = L.flg[s2];
       const Real sigma = lj_sigma(f1, f2, p1.w, p2.w, gp);
       const bool cc = (f1 & F_CARBON_LK) && (f2 & F_CARBON_LK);
@@ -244,7 +248,7 @@ __device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
       if (C::DERIV && dist > Real(0)) {
         const Real dlk = wcp * (dva + dvb);
         if (dlk != Real(0)) {
-          Real gs = dlk * fast_rcp(dist);
+          Real gs = dlk * inv_d;
           if (C::WGRAD) gs *= C::BP ? L.wt[entry_owner(s2)][2] : wrow[2];
           grad_scatter(L, C::DVDX ? 2 : 0, s1, s2, gs * dx, gs * dy, gs * dz, lane);
         }
@@ -361,7 +365,7 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
     }
     const uint32_t hmask1 = (uint32_t)__ballot(heavy1);
     if (C::BP && lane < 9) (&L.ebp[0][0])[lane] = 0;
-    KSum<Real> e_atr, e_rep, e_lk;
+    LaneSum e_atr, e_rep, e_lk;
     // FIFO state (wave-uniform)
     int n_lj = 0, n_lk = 0;      // pending entries
     int seq_lj = 0, seq_lk = 0;  // entries appended in this row

@@ -49,8 +49,8 @@ def _lib():
         L.tmol_ljlk_prune_pair_features.argtypes = [I64, VP, ctypes.POINTER(CPruneBlocks), VP, VP]
         L.tmol_ljlk_prune_gemm.argtypes = [I32, I32, I32, VP, VP, I32, I32, I32, I32, I32, I32,
                                            I32, VP, VP, VP, VP, VP, VP]
-        L.tmol_ljlk_prune_head.argtypes = [I32, VP, VP, VP, VP, VP, F32, I32, VP, VP, VP, VP, VP,
-                                           VP, VP]
+        L.tmol_ljlk_prune_head.argtypes = [I32, VP, VP, VP, VP, VP, VP, F32, I32, VP, VP, VP, VP,
+                                           VP, VP, VP]
         L.tmol_ljlk_prune_colsum.argtypes = [I32, I32, I32, VP, VP, VP]
         L.tmol_ljlk_prune_emb_grad.argtypes = [I64, VP, VP, VP, VP, VP]
         L.tmol_ljlk_spheres.argtypes = [ctypes.POINTER(ops.CStack), ctypes.POINTER(ops.CBlockTypes),
@@ -239,7 +239,7 @@ def _gemm(epi, a_km, b_kn, A, B, M, N, K, lda, ldb, ldc, C=None, bias=None, Z=No
 
 def mlp_loss_and_grads(params: PrunerParams, X: torch.Tensor, pairs: torch.Tensor,
                        emb_row: torch.Tensor, labels: Optional[torch.Tensor], pos_weight=1.0,
-                       train=True):
+                       train=True, sample_weights: Optional[torch.Tensor] = None):
     """Forward (and with ``train`` the backward) of the MLP on feature rows X
     [n, 48] (bf16 bits).  Returns (summed loss or None, logits [n], grads)."""
     dev = X.device
@@ -256,8 +256,9 @@ def mlp_loss_and_grads(params: PrunerParams, X: torch.Tensor, pairs: torch.Tenso
     logits = torch.empty(n, **f32)
     if not train:
         ops._check(_lib().tmol_ljlk_prune_head(n, _p(H[2]), None, _p(params.t["w4"]),
-                                               _p(params.t["b4"]), None, 1.0, 0, _p(logits), None,
-                                               None, None, None, None, ops._stream(dev)), "head")
+                                               _p(params.t["b4"]), None, None, 1.0, 0, _p(logits),
+                                               None, None, None, None, None, ops._stream(dev)),
+                   "head")
         return None, logits, None
     g = {k: torch.zeros_like(v) for k, v in params.t.items()}
     loss = torch.zeros(1, **f32)
@@ -265,7 +266,8 @@ def mlp_loss_and_grads(params: PrunerParams, X: torch.Tensor, pairs: torch.Tenso
     Db = [torch.empty((n, HIDDEN), **i16) for _ in range(3)]
     ops._check(_lib().tmol_ljlk_prune_head(
         n, _p(H[2]), _p(Z[2]), _p(params.t["w4"]), _p(params.t["b4"]), _p(labels),
-        float(pos_weight), 1, _p(logits), _p(loss), _p(D[2]), _p(Db[2]), _p(g["w4"]),
+        _p(sample_weights), float(pos_weight), 1, _p(logits), _p(loss), _p(D[2]), _p(Db[2]),
+        _p(g["w4"]),
         _p(g["b4"]), ops._stream(dev)), "head")
     split = max(1, min(64, n // 2048))
     # layer 3: dW3 = dZ3^T H2, db3 = sum dZ3, dZ2 = (dZ3 W3) * silu'(Z2)
@@ -419,9 +421,18 @@ def build_training_set(scorer, coords):
     return pairs, labels, Ew
 
 
+def energy_sample_weights(Ew: torch.Tensor, scale: float = 1.0) -> torch.Tensor:
+    """Per-pair loss weights: 1 for negligible pairs, 1 + log(1 + |E| / 1e-3)
+    (times ``scale``) otherwise, so that calling a high-energy pair
+    negligible -- an error the whole pose total carries -- costs more."""
+    a = Ew.abs().float()
+    w = 1.0 + scale * torch.log1p(a / NEGLIGIBLE)
+    return torch.where(a < NEGLIGIBLE, torch.ones_like(w), w).contiguous()
+
+
 def train(params: PrunerParams, inputs: PruneInputs, pairs: torch.Tensor, labels: torch.Tensor,
           epochs: int = 4, batch: int = 8192, lr: float = 2e-3, pos_weight: float = 1.0, seed: int = 0,
-          log=None):
+          log=None, sample_weights: Optional[torch.Tensor] = None):
     """Adam on the fp32 master parameters with gradients from the library's
     kernels.  Returns the mean loss of each epoch."""
     opt = torch.optim.Adam(params.tensors(), lr=lr)
@@ -435,8 +446,10 @@ def train(params: PrunerParams, inputs: PruneInputs, pairs: torch.Tensor, labels
             idx = perm[b0:b0 + batch]
             pb = pairs[idx].contiguous()
             yb = labels[idx].contiguous()
+            wb = sample_weights[idx].contiguous() if sample_weights is not None else None
             X = pair_features(inputs, params, pb)
-            loss, _, g = mlp_loss_and_grads(params, X, pb, inputs.emb_row, yb, pos_weight)
+            loss, _, g = mlp_loss_and_grads(params, X, pb, inputs.emb_row, yb, pos_weight,
+                                            sample_weights=wb)
             for k, t in params.t.items():
                 t.grad = g[k]
             opt.step()

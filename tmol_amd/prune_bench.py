@@ -49,11 +49,12 @@ def train_pruner(dev, n_pairs=1_000_000, epochs=6, seed=0, log=print):
     n_all = pairs.shape[0]
     g = torch.Generator().manual_seed(seed)
     sel = torch.randperm(n_all, generator=g)[:min(n_pairs, n_all)].to(dev)
-    pairs, labels = pairs[sel].contiguous(), labels[sel].contiguous()
+    pairs, labels, Ew = pairs[sel].contiguous(), labels[sel].contiguous(), Ew[sel].contiguous()
+    sw = prune.energy_sample_weights(Ew)
     log("training pairs %d of %d (negligible %.3f)" % (pairs.shape[0], n_all, float(labels.mean())))
     params = prune.PrunerParams(dev, seed=seed)
     t0 = time.perf_counter()
-    hist = prune.train(params, inputs, pairs, labels, epochs=epochs, log=log)
+    hist = prune.train(params, inputs, pairs, labels, epochs=epochs, log=log, sample_weights=sw)
     torch.cuda.synchronize(dev)
     t_train = time.perf_counter() - t0
     _, acc = prune.evaluate_classifier(params, inputs, pairs, labels)
