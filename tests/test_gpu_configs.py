@@ -1,0 +1,96 @@
+"""GPU parity at the workload sizes: the 64-pose padded drop-in batch
+(residues uniform in [150, 600], float32) and the single 100-residue pose.
+
+Bars: neighbor flags bit-exact against the oracle; count-pair separations
+bit-exact on a sample of block pairs (all sequence-adjacent ones included);
+per-pose energies within 1e-4 relative of the FP64 run of the same kernels;
+FP64 run against the FP64 oracle at 1e-9; gradients of the drop-in op's
+backward against the FP64 run (1e-4 of the gradient norm per pose)."""
+
+import os
+import sys
+
+import numpy
+import pytest
+import torch
+
+from oracle import oracle as orc
+from tmol_amd.generator import workload_poses
+from tmol_amd.pose import PoseStack
+from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+W = torch.tensor([1.0, 0.55, 1.0], dtype=torch.float64)
+
+
+def _say(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _args(poses, dtype, device):
+    ps = PoseStack(poses, device=device, dtype=dtype)
+    a = pose_op_args(ps, LJLKEnergyTerm(device=device))
+    if dtype == torch.float64:
+        a[23], a[24] = a[23].double(), a[24].double()
+    return ps, a
+
+
+@pytest.mark.parametrize("workload", ["padded_64x150_600", "cpu_single_100"])
+def test_workload_parity(workload):
+    from tmol_amd.ops import debug_pair_separations, ljlk_pose_scores, padded_work_list
+
+    poses = workload_poses(workload)
+    _say("%s: %d poses" % (workload, len(poses)))
+    ps32, a32 = _args(poses, torch.float32, DEV)
+    ps64, a64 = _args(poses, torch.float64, DEV)
+    c32 = a32[0].clone().requires_grad_(True)
+    a32[0] = c32
+    s32, nbr32 = ljlk_pose_scores(*a32)
+    (W.to(DEV, torch.float32)[:, None] * s32).sum().backward()
+    c64 = a64[0].clone().requires_grad_(True)
+    a64[0] = c64
+    s64, _ = ljlk_pose_scores(*a64)
+    (W.to(DEV)[:, None] * s64).sum().backward()
+    # neighbor flags: bit-exact against the oracle in the coordinate dtype
+    _, ca32 = _args(poses, torch.float32, "cpu")
+    o32, onbr, _, _ = orc.oracle_pose_scores(*ca32, require_grad=False)
+    assert (nbr32.cpu().numpy() == onbr).all()
+    # energies: f32 within 1e-4 of the f64 kernels
+    rel = ((s32.double() - s64).abs() / s64.abs().clamp(min=1.0)).max()
+    _say("max rel f32 vs f64 %.3g" % float(rel))
+    assert float(rel) < 1e-4
+    # f64 kernels against the f64 oracle, pose by pose (threads over poses)
+    ev = orc.pose_oracle_evaluator(poses, threads=min(16, os.cpu_count() or 1))
+    keep = numpy.concatenate([p * ps64.max_n_atoms + numpy.arange(pose.n_atoms)
+                              for p, pose in enumerate(poses)])
+    x_ragged = c64.detach()[torch.from_numpy(keep).to(DEV)].reshape(-1).cpu()
+    Eo, go = ev(x_ragged)
+    Ew = (W.to(DEV)[:, None] * s64.detach()).sum(0).cpu()
+    torch.testing.assert_close(Ew, Eo, rtol=1e-9, atol=1e-7)
+    g64 = c64.grad[torch.from_numpy(keep).to(DEV)].cpu()
+    assert float((g64.reshape(-1) - go).norm() / go.norm()) < 1e-9
+    # gradients: f32 backward against the f64 run, per pose
+    g32 = c32.grad[torch.from_numpy(keep).to(DEV)].double().cpu()
+    off = 0
+    for pose in poses:
+        sl = slice(off, off + pose.n_atoms)
+        assert float((g32[sl] - g64[sl]).norm() / g64[sl].norm()) < 1e-4
+        off += pose.n_atoms
+    # count pair: every sequence-adjacent neighbor pair plus a sample of others
+    items, _, overflow = padded_work_list(a32)
+    assert overflow == 0
+    B = ps32.max_n_blocks
+    it = items.numpy()
+    near = numpy.abs((it[:, 1] % B) - (it[:, 0] % B)) <= 1
+    rng = numpy.random.default_rng(0)
+    far = rng.choice(numpy.nonzero(~near)[0], size=min(2000, int((~near).sum())), replace=False)
+    sel = numpy.concatenate([numpy.nonzero(near)[0][:4000], far])
+    sub = torch.from_numpy(it[sel])
+    seps = debug_pair_separations(a32, sub).cpu().numpy()
+    trip = [(int(g1) // B, int(g1) % B, int(g2) % B) for g1, g2 in it[sel]]
+    ref = orc.oracle_pair_separations(ca32, trip)
+    for k, m in enumerate(ref):
+        n1, n2 = m.shape
+        assert (seps[k, :n1, :n2] == m).all(), trip[k]

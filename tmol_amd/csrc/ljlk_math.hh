@@ -92,15 +92,19 @@ __device__ __forceinline__ Real cp_weight(int sep) {
   return sep > 4 ? Real(1) : (sep == 4 ? Real(0.2) : Real(0));
 }
 
-// hbond-aware sigma (common.hh:24-44)
+// hbond-aware sigma (common.hh:24-44): donor-acceptor pairs use the hbond
+// distance (the hydroxyl-donor one if either atom is a hydroxyl donor),
+// polar-hydrogen-acceptor pairs the hbond-H distance, others r_i + r_j.
+// With the flag bits D = 1, OH = 2, PH = 4, A = 8: bit 0 of
+// (Ai & (Dj | PHj)) | (Aj & (Di | PHi)) is the donor-acceptor test and bit 2
+// the polar-H-acceptor test, each the OR of both orientations.
 template <typename Real>
 __device__ __forceinline__ Real lj_sigma(int fi, int fj, Real ri, Real rj,
                                          const GlobalParams<Real>& g) {
-  bool da = ((fi & F_DONOR) && (fj & F_ACCEPTOR)) ||
-            ((fj & F_DONOR) && (fi & F_ACCEPTOR));
-  bool oh = (fi & F_OH_DONOR) || (fj & F_OH_DONOR);
-  bool ha = ((fi & F_POLARH) && (fj & F_ACCEPTOR)) ||
-            ((fj & F_POLARH) && (fi & F_ACCEPTOR));
+  const int both = ((fi >> 3) & 1 ? fj : 0) | ((fj >> 3) & 1 ? fi : 0);
+  const bool da = both & F_DONOR;
+  const bool ha = both & F_POLARH;
+  const bool oh = (fi | fj) & F_OH_DONOR;
   Real s = ri + rj;
   s = ha ? g.hbond_hdis : s;
   s = da ? (oh ? g.hbond_oh_dis : g.hbond_dis) : s;
