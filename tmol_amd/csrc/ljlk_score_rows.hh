@@ -40,6 +40,14 @@ __device__ __forceinline__ int64_t uni64(int64_t x) {
   return (int64_t)(((uint64_t)hi << 32) | lo);
 }
 __device__ __forceinline__ int lane_val(int v, int j) { return __builtin_amdgcn_readlane(v, j); }
+
+// a / b for 0 <= a < 2^10, 1 <= b <= 64 through the float reciprocal: (a + 1/2) / b
+// lies at least 1/(2b) from every integer, far beyond the reciprocal's error,
+// so the truncation is exact (a handful of VALU instead of the integer
+// division sequence)
+__device__ __forceinline__ int small_div(int a, int b) {
+  return int((float(a) + 0.5f) * __builtin_amdgcn_rcpf(float(b)));
+}
 __device__ __forceinline__ float lane_val(float v, int j) {
   return __int_as_float(__builtin_amdgcn_readlane(__float_as_int(v), j));
 }
@@ -442,8 +450,8 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
           }
           wave_sync();
           const int total = n1 * n1;
-          const int q = WAVE / n1, rstep = WAVE - q * n1;
-          int a2 = lane % n1, a1 = lane / n1;
+          const int q = small_div(WAVE, n1), rstep = WAVE - q * n1;
+          int a1 = small_div(lane, n1), a2 = lane - a1 * n1;
           for (int k0 = 0; k0 < total; k0 += WAVE) {
             if (n_lj > LIST - WAVE || n_lk > LIST - WAVE) drain(false);
             bool in = false, both_heavy = false, w02 = false;
@@ -579,8 +587,8 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
             seq_lk += __popcll(mh);
           };
           const int total = n1c * n2c;
-          const int q = WAVE / n2c, rstep = WAVE - q * n2c;
-          int u = lane % n2c, t = lane / n2c;
+          const int q = small_div(WAVE, n2c), rstep = WAVE - q * n2c;
+          int t = small_div(lane, n2c), u = lane - t * n2c;
           for (int k0 = 0; k0 < total; k0 += 2 * WAVE) {
             if (n_lj > LIST - 2 * WAVE || n_lk > LIST - 2 * WAVE) drain(false);
             int uB = u + rstep, tB = t + q;
