@@ -524,7 +524,7 @@ int tmol_ljlk_prune_compact(int32_t n_blocks, uint32_t* row_count, int32_t* row_
   if (n_blocks <= 0 || !row_count || !row_items || !row_offset || !keep) return TMOL_LJLK_EINVAL;
   hipLaunchKernelGGL(prune_compact_kernel, dim3(grid_for(n_blocks)), dim3(256), 0,
                      (hipStream_t)stream, int64_t(n_blocks), row_count, row_items, row_capacity,
-                     row_offset, keep);
+                     row_offset, keep);  // one wave per 64 rows: n_blocks / 64 waves
   return check(hipGetLastError());
 }
 

@@ -32,15 +32,25 @@ constexpr int PRUNE_TORS = 5;
 typedef short bf16x4 __attribute__((ext_vector_type(4)));
 typedef float f32x16 __attribute__((ext_vector_type(16)));
 
-// float -> bf16 bits, round to nearest even
+// float -> bf16 bits, round to nearest even (v_cvt_pk_bf16_f32)
 __device__ __forceinline__ uint16_t to_bf16(float x) {
-  uint32_t u = __float_as_uint(x);
-  u += 0x7fffu + ((u >> 16) & 1u);
-  return (uint16_t)(u >> 16);
+  return __builtin_bit_cast(uint16_t, (__bf16)x);
+}
+// two floats -> packed bf16 pair (lo, hi)
+__device__ __forceinline__ uint32_t to_bf16x2(float lo, float hi) {
+  typedef __bf16 bf16x2_t __attribute__((ext_vector_type(2)));
+  bf16x2_t v;
+  v[0] = (__bf16)lo;
+  v[1] = (__bf16)hi;
+  return __builtin_bit_cast(uint32_t, v);
 }
 __device__ __forceinline__ float from_bf16(uint16_t h) { return __uint_as_float(uint32_t(h) << 16); }
 
-__device__ __forceinline__ float sigmoidf_(float z) { return 1.0f / (1.0f + __expf(-z)); }
+// logistic function with the hardware exp and reciprocal (inputs beyond
+// +-88 saturate to exactly 0 / 1)
+__device__ __forceinline__ float sigmoidf_(float z) {
+  return __builtin_amdgcn_rcpf(1.0f + __expf(-z));
+}
 __device__ __forceinline__ float silu_(float z) { return z * sigmoidf_(z); }
 __device__ __forceinline__ float dsilu_(float z) {
   const float s = sigmoidf_(z);
@@ -353,7 +363,10 @@ struct PruneLDS {
   uint16_t act[PI_WAVES][32][PSH];  // per wave: features, then hidden activations
 };
 
-// one hidden layer for the wave's 32 rows: act[:, :K] -> act[:, :128]
+// one hidden layer for the wave's 32 pairs: act[:, :K] -> act[:, :128].
+// The product is computed transposed, D[out][pair] = W[out, :] . act[pair, :]
+// (weights as the A operand), so that each lane ends up holding 4
+// consecutive outputs of one pair and stores them with one 8-byte write.
 template <int K, int WS>
 __device__ __forceinline__ void prune_layer(uint16_t (*act)[PSH], const uint16_t (*W)[WS],
                                             const float* bias, int lane) {
@@ -365,26 +378,27 @@ __device__ __forceinline__ void prune_layer(uint16_t (*act)[PSH], const uint16_t
   const int r = lane & 31, kh = 4 * (lane >> 5);
 #pragma unroll
   for (int ks = 0; ks < K; ks += 8) {
-    const bf16x4 a = *(const bf16x4*)&act[r][ks + kh];
+    const bf16x4 x = *(const bf16x4*)&act[r][ks + kh];
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
-      const bf16x4 b = *(const bf16x4*)&W[32 * c + r][ks + kh];
-      acc[c] = mfma_32x32x8(a, b, acc[c]);
+      const bf16x4 w = *(const bf16x4*)&W[32 * c + r][ks + kh];
+      acc[c] = mfma_32x32x8(w, x, acc[c]);
     }
   }
   wave_sync();
 #pragma unroll
-  for (int c = 0; c < 4; ++c) {
-    const int col = 32 * c + r;
-    const float bb = bias[col];
+  for (int c = 0; c < 4; ++c)
 #pragma unroll
-    for (int i = 0; i < 4; ++i)
-#pragma unroll
-      for (int q = 0; q < 4; ++q) {
-        const int row = 8 * i + kh + q;
-        act[row][col] = to_bf16(silu_(acc[c][4 * i + q] + bb));
-      }
-  }
+    for (int i = 0; i < 4; ++i) {
+      const int o = 32 * c + 8 * i + kh;  // outputs o .. o + 3 of pair r
+      const float4 bb = *(const float4*)&bias[o];
+      const float z0 = acc[c][4 * i] + bb.x, z1 = acc[c][4 * i + 1] + bb.y;
+      const float z2 = acc[c][4 * i + 2] + bb.z, z3 = acc[c][4 * i + 3] + bb.w;
+      uint2 packed;
+      packed.x = to_bf16x2(silu_(z0), silu_(z1));
+      packed.y = to_bf16x2(silu_(z2), silu_(z3));
+      *(uint2*)&act[r][o] = packed;
+    }
   wave_sync();
 }
 
@@ -530,49 +544,74 @@ __global__ __launch_bounds__(1024) void prune_scan_blocks_kernel(int64_t nb, int
   }
 }
 
-// row offsets (exclusive) and the flat pair list
+// row offsets (exclusive) and the flat pair list: lane j of a wave owns row
+// base + j for the scan; the wave then writes each of its 64 rows' pairs with
+// all lanes (coalesced)
 __global__ __launch_bounds__(1024) void prune_flatten_kernel(int64_t G, const uint32_t* __restrict__ row_count,
                                                               const int32_t* __restrict__ row_items, int64_t cap,
                                                               const int64_t* __restrict__ block_sums,
                                                               int64_t* __restrict__ row_offset,
                                                               int32_t* __restrict__ pairs) {
   __shared__ int64_t part[16];
+  const int lane = lane_id();
   const int64_t g = int64_t(blockIdx.x) * 1024 + threadIdx.x;
   const int64_t c = g < G ? min<int64_t>(row_count[g], cap) : 0;
   // inclusive scan within the wave, then across the 16 waves
   int64_t x = c;
   for (int o = 1; o < 64; o <<= 1) {
     const int64_t y = __shfl_up(x, o, WAVE);
-    if (lane_id() >= o) x += y;
+    if (lane >= o) x += y;
   }
-  if (lane_id() == 63) part[threadIdx.x >> 6] = x;
+  if (lane == 63) part[threadIdx.x >> 6] = x;
   __syncthreads();
   int64_t before = block_sums[blockIdx.x];
   for (int w = 0; w < int(threadIdx.x >> 6); ++w) before += part[w];
   const int64_t start = before + x - c;
   if (g < G) {
     row_offset[g] = start;
-    for (int64_t k = 0; k < c; ++k) {
-      pairs[2 * (start + k)] = int32_t(g);
-      pairs[2 * (start + k) + 1] = row_items[g * cap + k];
-    }
     if (g == G - 1) row_offset[G] = start + c;
+  }
+  const int64_t base = g - lane;
+  for (int j = 0; j < 64; ++j) {
+    const int64_t gj = base + j;
+    if (gj >= G) break;
+    const int64_t cj = __shfl(c, j, WAVE), sj = __shfl(start, j, WAVE);
+    for (int64_t k = lane; k < cj; k += 64) {
+      pairs[2 * (sj + k)] = int32_t(gj);
+      pairs[2 * (sj + k) + 1] = row_items[gj * cap + k];
+    }
   }
 }
 
-// keep only flagged pairs in each row of the work list (in place, order kept)
-__global__ void prune_compact_kernel(int64_t G, uint32_t* __restrict__ row_count,
-                                     int32_t* __restrict__ row_items, int64_t cap,
-                                     const int64_t* __restrict__ row_offset,
-                                     const uint8_t* __restrict__ keep) {
-  const int64_t g = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
-  if (g >= G) return;
-  const int64_t c = min<int64_t>(row_count[g], cap);
-  const int64_t o = row_offset[g];
-  int64_t w = 0;
-  for (int64_t k = 0; k < c; ++k)
-    if (keep[o + k]) row_items[g * cap + w++] = row_items[g * cap + k];
-  row_count[g] = uint32_t(w);
+// keep only flagged pairs in each row of the work list (in place, order
+// kept): one wave per 64 rows, ballot compaction of each row's pairs
+__global__ __launch_bounds__(256) void prune_compact_kernel(int64_t G, uint32_t* __restrict__ row_count,
+                                                            int32_t* __restrict__ row_items, int64_t cap,
+                                                            const int64_t* __restrict__ row_offset,
+                                                            const uint8_t* __restrict__ keep) {
+  const int lane = lane_id();
+  const int64_t base = (int64_t(blockIdx.x) * blockDim.x + threadIdx.x) - lane;
+  for (int j = 0; j < 64; ++j) {
+    const int64_t g = base + j;
+    if (g >= G) break;
+    const int64_t c = min<int64_t>(row_count[g], cap);
+    const int64_t o = row_offset[g];
+    int64_t w = 0;
+    for (int64_t k0 = 0; k0 < c; k0 += 64) {
+      const int64_t k = k0 + lane;
+      int32_t item = 0;
+      bool kept = false;
+      if (k < c) {
+        item = row_items[g * cap + k];
+        kept = keep[o + k] != 0;
+      }
+      const uint64_t m = __ballot(kept);
+      wave_sync();
+      if (kept) row_items[g * cap + w + lanes_below(m)] = item;
+      w += __popcll(m);
+    }
+    if (lane == 0) row_count[g] = uint32_t(w);
+  }
 }
 
 }  // namespace tmol_ljlk

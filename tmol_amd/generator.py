@@ -367,10 +367,11 @@ WORKLOADS = {
     "ragged_512x50_1200": dict(n_poses=512, res_range=(50, 1200), log_uniform=True, seed=MASTER_SEED + 2),
     "diff_32x400": dict(n_poses=32, res_range=(400, 400), log_uniform=False, seed=MASTER_SEED + 3),
     "pruner_1024x300": dict(n_poses=1024, res_range=(300, 300), log_uniform=False, seed=MASTER_SEED + 4),
-    # pruner training and threshold calibration sets: the 512-pose batch's
-    # distribution (log-uniform 50-1200 residues), their own seeds
+    # pruner training set: the 512-pose batch's distribution (log-uniform
+    # 50-1200 residues); calibration set: poses like the held-out batch (300
+    # residues); each with its own seed
     "pruner_train": dict(n_poses=160, res_range=(50, 1200), log_uniform=True, seed=MASTER_SEED + 5),
-    "pruner_calib": dict(n_poses=48, res_range=(50, 1200), log_uniform=True, seed=MASTER_SEED + 6),
+    "pruner_calib": dict(n_poses=128, res_range=(300, 300), log_uniform=False, seed=MASTER_SEED + 6),
 }
 
 

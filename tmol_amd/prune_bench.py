@@ -5,8 +5,9 @@
 1. Training set: poses of the 512-pose batch's distribution (own seed); the
    exact kernel labels every neighbor block pair (|E| < 1e-3 kcal/mol);
    1e6 pairs are sampled and the MLP is trained by the library's kernels.
-2. Calibration: on another such set, the largest pruning threshold whose
-   worst per-pose total-energy error stays within half the bound is chosen.
+2. Calibration: on 128 other poses of the held-out batch's size, the
+   smallest threshold (most pruning) whose worst per-pose total-energy error
+   stays within a quarter of the bound is chosen.
 3. Held-out batch: 1024 poses of 300 residues.  Exact and pruned
    energy + gradient steps are each captured in a hipGraph and timed; the
    exact kernel judges the pruned per-pose energies; a batch whose error
@@ -27,6 +28,7 @@ from .score.ljlk_energy_term import LJLKEnergyTerm
 from .scorer import LJLKRaggedScorer
 
 BOUND = 0.05  # kcal/mol per residue, total energy
+MARGIN = 0.25  # calibration: worst calibration pose within a quarter of the bound
 W = torch.tensor(prune.WEIGHTS, dtype=torch.float64)
 
 
@@ -78,7 +80,8 @@ def pose_errors(pruner, sc, x, n_res):
     return err, 1.0 - n_after / max(n_before, 1)
 
 
-def calibrate(params, dev, candidates=(0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.99), log=print):
+def calibrate(params, dev, candidates=(0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.99, 0.995, 0.999),
+              log=print):
     poses = workload_poses("pruner_calib")
     st, sc = _scorer(poses, dev)
     x = st.coords.to(dev)
@@ -92,8 +95,9 @@ def calibrate(params, dev, candidates=(0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.99), lo
         table.append({"threshold": tau, "max_err_per_res": float(err.max()),
                       "mean_err_per_res": float(err.mean()), "pruned_fraction": frac})
         log("calibration tau %.2f: max err/res %.4f, pruned %.3f" % (tau, float(err.max()), frac))
-        if float(err.max()) <= 0.5 * BOUND:
-            best = min(best, tau)
+        if float(err.max()) > MARGIN * BOUND:
+            break  # lower thresholds prune more: stop at the first failure
+        best = tau
     return best, table
 
 
