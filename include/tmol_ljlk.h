@@ -259,6 +259,80 @@ int tmol_ljlk_prune_compact(int32_t n_blocks, uint32_t* row_count, int32_t* row_
                             int64_t row_capacity, const int64_t* row_offset,
                             const uint8_t* keep, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Rotamer-pair scoring for packing.  Replaces the reference's op
+ *   tmol_ljlk::ljlk_rotamer_scores   compiled.ops.cpp:282-530, 597-663
+ * and its dispatch
+ *   LJLKRotamerScoreDispatch::forward / ::backward
+ *                                    ljlk_pose_score.impl.hh:1479-2126
+ *   compute_rot_spheres, compute_block_spheres_from_rot_spheres,
+ *   detect_block_neighbors, rot_neighbor_indices_from_block_neighbors
+ *                                    sphere_overlap.impl.hh:23-107, 287-351, 455-607
+ * Rotamers are numbered globally, grouped by pose and block; the pose
+ * stack's bond-separation tables give the count pair between blocks.
+ * Block types of at most 64 atoms (ENOTSUP beyond). */
+typedef struct {
+  int32_t n_poses;                       /* P */
+  int32_t max_n_blocks;                  /* B */
+  int32_t n_rots;                        /* R */
+  int32_t reserved;                      /* 0 */
+  int64_t n_atoms_total;                 /* rows of the rotamer coordinates */
+  const int32_t* rot_coord_offset;       /* [R] first atom of each rotamer */
+  const int32_t* block_ind_for_rot;      /* [R] */
+  const int32_t* block_type_ind_for_rot; /* [R] */
+  const int32_t* n_rots_for_block;       /* [P, B] */
+  const int32_t* rot_offset_for_block;   /* [P, B] first rotamer of each block */
+  const int32_t* first_rot_block_type;   /* [P, B] -1: block without rotamers */
+  const int32_t* min_bondsep;            /* [P, B, B] */
+  const int32_t* inter_block_bondsep;    /* [P, B, B, C, C] */
+} tmol_ljlk_rotamer_set;
+
+/* bytes of one work unit (tmol_ljlk_rotamer_dispatch's `units`) */
+#define TMOL_LJLK_ROT_UNIT_BYTES 32
+
+/* Workspace bytes for a rotamer set (spheres, rows, scans, atom records). */
+int64_t tmol_ljlk_rotamer_workspace_bytes(const tmol_ljlk_rotamer_set* rs,
+                                          const tmol_ljlk_block_types* bt, int32_t dtype);
+
+/* Rotamer spheres, block spheres enclosing each block's rotamers, the block
+ * neighbor test (d^2 < ((r1 + r2) + max_dis)^2, bit-identical to the
+ * reference), and each row's counts of rotamer pairs and work units, scanned. */
+int tmol_ljlk_rotamer_neighbors(const tmol_ljlk_rotamer_set* rs,
+                                const tmol_ljlk_block_types* bt, const void* coords,
+                                int32_t dtype, double max_dis, void* workspace,
+                                int64_t ws_bytes, void* stream);
+
+/* totals_host[0] = rotamer pairs (dispatch entries), [1] = work units.
+ * Synchronises the stream (the caller sizes its outputs from them). */
+int tmol_ljlk_rotamer_totals(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                             int32_t dtype, const void* workspace, int64_t* totals_host,
+                             void* stream);
+
+/* Dispatch indices [3, n_pairs] int32 = (pose, rot1, rot2), in the
+ * reference's order (neighbor cells (b1 <= b2) row-major; all n1 x n2 pairs
+ * rot1-major off the diagonal, self pairs on it), and n_units work units of
+ * TMOL_LJLK_ROT_UNIT_BYTES each.  max_dis as given to _neighbors. */
+int tmol_ljlk_rotamer_dispatch(const tmol_ljlk_rotamer_set* rs,
+                               const tmol_ljlk_block_types* bt, int32_t dtype, double max_dis,
+                               void* workspace, int64_t ws_bytes, int32_t* indices,
+                               int64_t n_pairs, void* units, int64_t n_units, void* stream);
+
+/* Score the dispatch list.  flags 0: energies [3, n_pairs] (atr, rep, lk of
+ * each rotamer pair, unweighted).  flags WGRAD | NO_ENERGY: the backward,
+ * grad [NA, 3] = sum_i sum_t dTdV[t, i] dV_t(pair i)/dx (zeroed first unless
+ * ACCUMULATE).  n_workgroups <= 0: one per resident slot. */
+int tmol_ljlk_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                            const void* coords, int32_t dtype, void* workspace,
+                            int64_t ws_bytes, const void* units, int64_t n_units,
+                            int64_t n_pairs, int32_t flags, const void* dTdV, void* energies,
+                            void* grad, int32_t n_workgroups, void* stream);
+
+/* Debug: device addresses of the rotamer spheres [R, 4] and block spheres
+ * [P, B, 4] (dtype) inside the workspace, written by _neighbors. */
+int tmol_ljlk_rotamer_spheres(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                              int32_t dtype, void* workspace, void** rot_spheres,
+                              void** block_spheres);
+
 #ifdef __cplusplus
 }
 #endif

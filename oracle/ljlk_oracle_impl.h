@@ -461,47 +461,55 @@ static int FN(inter_sep)(const oracle_stack* st, int p, int b1, int b2,
 }
 
 
-static void FN(add_grad)(REAL* dvdx, int64_t atom, int term, int64_t na,
-                         REAL g, const REAL* u) {
+/* Where gradients go: per-term planes dvdx [3, NA, 3] (w == NULL, the
+ * whole-pose forward), or one plane [NA, 3] with term t scaled by w[t] (the
+ * dTdV-weighted backward of the block-pair and rotamer-pair dispatches,
+ * lj_atom_derivs / lk_atom_derivs in ljlk.hh:531-600, 686-738). */
+typedef struct {
+  REAL* g;
+  int64_t na;
+  const REAL* w;
+} FN(gsink);
+
+static void FN(sink_add)(const FN(gsink) * s, int term, int64_t atom, REAL dv,
+                         const REAL* u) {
   for (int k = 0; k < 3; ++k) {
-    REAL v = g * u[k];
-    if (v != 0) dvdx[((int64_t)term * na + atom) * 3 + k] += v;
+    REAL v = s->w ? (s->w[term] * dv) * u[k] : dv * u[k];
+    if (v == 0) continue;
+    if (s->w)
+      s->g[atom * 3 + k] += v;
+    else
+      s->g[((int64_t)term * s->na + atom) * 3 + k] += v;
   }
 }
 
-/* one atom pair: LJ or LK, energy (+derivative) */
-static void FN(eval_pair)(const REAL* coords, int64_t na, int64_t i1,
-                          int64_t i2, int sep, const FN(tparams) * t1,
-                          const FN(tparams) * t2, const FN(gparams) * g,
-                          int lk, int require_grad, REAL* e, REAL* dvdx) {
+/* one atom pair: LJ or LK, energy (+derivative into sink) */
+static void FN(eval_pair)(const REAL* coords, int64_t i1, int64_t i2, int sep,
+                          const FN(tparams) * t1, const FN(tparams) * t2,
+                          const FN(gparams) * g, int lk, int require_grad,
+                          REAL* e, const FN(gsink) * sink) {
   const REAL* x1 = coords + 3 * i1;
   const REAL* x2 = coords + 3 * i2;
   REAL d[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
   REAL dist = SQRT(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+  REAL u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
+  if (require_grad && sink && dist != 0)
+    for (int k = 0; k < 3; ++k) {
+      u1[k] = d[k] / dist;
+      u2[k] = -d[k] / dist;
+    }
   if (!lk) {
     if (require_grad) {
       REAL a, r, da, dr;
       FN(lj_V_dV)(dist, sep, t1, t2, g, &a, &r, &da, &dr);
       e[0] = a;
       e[1] = r;
-      if (dvdx) {
-        REAL u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
-        if (dist != 0)
-          for (int k = 0; k < 3; ++k) {
-            u1[k] = d[k] / dist;
-            u2[k] = -d[k] / dist;
-          }
+      if (sink) {
         /* impl order (ljlk.hh:629-664): atom 1 atr/rep, then atom 2 */
-        for (int k = 0; k < 3; ++k) {
-          REAL va = da * u1[k], vr = dr * u1[k];
-          if (va != 0) dvdx[((int64_t)0 * na + i1) * 3 + k] += va;
-          if (vr != 0) dvdx[((int64_t)1 * na + i1) * 3 + k] += vr;
-        }
-        for (int k = 0; k < 3; ++k) {
-          REAL va = da * u2[k], vr = dr * u2[k];
-          if (va != 0) dvdx[((int64_t)0 * na + i2) * 3 + k] += va;
-          if (vr != 0) dvdx[((int64_t)1 * na + i2) * 3 + k] += vr;
-        }
+        FN(sink_add)(sink, 0, i1, da, u1);
+        FN(sink_add)(sink, 1, i1, dr, u1);
+        FN(sink_add)(sink, 0, i2, da, u2);
+        FN(sink_add)(sink, 1, i2, dr, u2);
       }
     } else {
       FN(lj_V)(dist, sep, t1, t2, g, &e[0], &e[1]);
@@ -511,15 +519,9 @@ static void FN(eval_pair)(const REAL* coords, int64_t na, int64_t i1,
       REAL v, dv;
       FN(lk_V_dV)(dist, sep, t1, t2, g, &v, &dv);
       e[0] = v;
-      if (dvdx) {
-        REAL u1[3] = {0, 0, 0}, u2[3] = {0, 0, 0};
-        if (dist != 0)
-          for (int k = 0; k < 3; ++k) {
-            u1[k] = d[k] / dist;
-            u2[k] = -d[k] / dist;
-          }
-        FN(add_grad)(dvdx, i1, 2, na, dv, u1);
-        FN(add_grad)(dvdx, i2, 2, na, dv, u2);
+      if (sink) {
+        FN(sink_add)(sink, 2, i1, dv, u1);
+        FN(sink_add)(sink, 2, i2, dv, u2);
       }
     } else {
       e[0] = FN(lk_V)(dist, sep, t1, t2, g);
@@ -527,11 +529,114 @@ static void FN(eval_pair)(const REAL* coords, int64_t na, int64_t i1,
   }
 }
 
+/* One block pair (b1 != b2: inter-block; b1 == b2: intra-block) with the
+ * given block types and first-atom offsets, in the reference CPU order:
+ * tiles of 32 atoms, pairs row-major inside a tile pair
+ * (tile_atom_pair_evaluation.hh:88-173, 377-505).  Energies accumulate in
+ * REAL: pair -> tile-pair partial -> block-pair total (tot[3]). */
+static void FN(score_block_pair)(const oracle_stack* st, const REAL* coords,
+                                 const REAL* type_params, const FN(gparams) * g,
+                                 int p, int b1, int b2, int bt1, int bt2,
+                                 int64_t o1, int64_t o2, int require_grad,
+                                 const FN(gsink) * sink, REAL* tot) {
+  const int Am = st->max_n_block_atoms, nt = st->max_n_tiles;
+  const int TILE = 32;
+  int n1 = st->bt_n_atoms[bt1], n2 = st->bt_n_atoms[bt2];
+  REAL tot_atr = 0, tot_rep = 0, tot_lk = 0;
+  if (b1 != b2) {
+    int nti1 = (n1 - 1) / TILE + 1, nti2 = (n2 - 1) / TILE + 1;
+    for (int ti = 0; ti < nti1; ++ti) {
+      int s1 = TILE * ti;
+      int m1 = n1 - s1 < TILE ? n1 - s1 : TILE;
+      for (int tj = 0; tj < nti2; ++tj) {
+        int s2 = TILE * tj;
+        int m2 = n2 - s2 < TILE ? n2 - s2 : TILE;
+        REAL pa = 0, pr = 0, pl = 0;
+        for (int k = 0; k < m1 * m2; ++k) {
+          int a1 = s1 + k / m2, a2 = s2 + k % m2;
+          FN(tparams) t1, t2;
+          FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
+          FN(load_params)(type_params, st->bt_atom_types[bt2 * Am + a2], &t2);
+          int sep = FN(inter_sep)(st, p, b1, b2, bt1, bt2, a1, a2);
+          REAL e[2];
+          FN(eval_pair)(coords, o1 + a1, o2 + a2, sep, &t1, &t2, g, 0,
+                        require_grad, e, sink);
+          pa += e[0];
+          pr += e[1];
+        }
+        int h1 = st->bt_n_heavy_in_tile[bt1 * nt + ti];
+        int h2 = st->bt_n_heavy_in_tile[bt2 * nt + tj];
+        for (int k = 0; k < h1 * h2; ++k) {
+          int a1 = s1 + st->bt_heavy_in_tile[bt1 * TILE * nt + TILE * ti + k / h2];
+          int a2 = s2 + st->bt_heavy_in_tile[bt2 * TILE * nt + TILE * tj + k % h2];
+          FN(tparams) t1, t2;
+          FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
+          FN(load_params)(type_params, st->bt_atom_types[bt2 * Am + a2], &t2);
+          int sep = FN(inter_sep)(st, p, b1, b2, bt1, bt2, a1, a2);
+          REAL e[1];
+          FN(eval_pair)(coords, o1 + a1, o2 + a2, sep, &t1, &t2, g, 1,
+                        require_grad, e, sink);
+          pl += e[0];
+        }
+        tot_atr += pa;
+        tot_rep += pr;
+        tot_lk += pl;
+      }
+    }
+  } else {
+    int nti = (n1 - 1) / TILE + 1;
+    const int32_t* pd = st->bt_path_distance + (int64_t)bt1 * Am * Am;
+    for (int ti = 0; ti < nti; ++ti) {
+      int s1 = TILE * ti;
+      int m1 = n1 - s1 < TILE ? n1 - s1 : TILE;
+      for (int tj = ti; tj < nti; ++tj) {
+        int s2 = TILE * tj;
+        int m2 = n1 - s2 < TILE ? n1 - s2 : TILE;
+        REAL pa = 0, pr = 0, pl = 0;
+        for (int k = 0; k < m1 * m2; ++k) {
+          int a1 = s1 + k / m2, a2 = s2 + k % m2;
+          if (a1 >= a2) continue;
+          FN(tparams) t1, t2;
+          FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
+          FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a2], &t2);
+          int sep = pd[a1 * Am + a2];
+          REAL e[2];
+          FN(eval_pair)(coords, o1 + a1, o1 + a2, sep, &t1, &t2, g, 0,
+                        require_grad, e, sink);
+          pa += e[0];
+          pr += e[1];
+        }
+        int h1 = st->bt_n_heavy_in_tile[bt1 * nt + ti];
+        int h2 = st->bt_n_heavy_in_tile[bt1 * nt + tj];
+        for (int k = 0; k < h1 * h2; ++k) {
+          int i1 = k / h2, i2 = k % h2;
+          /* positions compared, as in IntraResBlockEvaluation */
+          if (s1 + i1 >= s2 + i2) continue;
+          int a1 = s1 + st->bt_heavy_in_tile[bt1 * TILE * nt + TILE * ti + i1];
+          int a2 = s2 + st->bt_heavy_in_tile[bt1 * TILE * nt + TILE * tj + i2];
+          FN(tparams) t1, t2;
+          FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
+          FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a2], &t2);
+          int sep = pd[a1 * Am + a2];
+          REAL e[1];
+          FN(eval_pair)(coords, o1 + a1, o1 + a2, sep, &t1, &t2, g, 1,
+                        require_grad, e, sink);
+          pl += e[0];
+        }
+        tot_atr += pa;
+        tot_rep += pr;
+        tot_lk += pl;
+      }
+    }
+  }
+  tot[0] = tot_atr;
+  tot[1] = tot_rep;
+  tot[2] = tot_lk;
+}
+
 /* Score every neighbor block pair of a padded stack, in the reference CPU
- * order: pose-major, upper triangle (b1 <= b2) row by row, tiles of 32 atoms,
- * pairs row-major inside a tile pair (impl.hh:633-1158,
- * tile_atom_pair_evaluation.hh:88-173, 377-505).  Energies accumulate in
- * REAL: pair -> tile-pair partial -> block-pair total -> pose total.
+ * order: pose-major, upper triangle (b1 <= b2) row by row
+ * (impl.hh:633-1158).  Block-pair totals are added to the pose total.
  *   out:       [3, P] (whole pose) or [3, P, B, B] (block_pair != 0)
  *   dvdx:      [3, NA, 3] or NULL, filled only when require_grad
  *   neighbors: [P, B, B] dense flags from oracle_block_neighbors
@@ -542,16 +647,15 @@ void FN(oracle_pose_scores)(const oracle_stack* st, const REAL* coords,
                             int block_pair, int require_grad, REAL* out,
                             REAL* dvdx) {
   const int P = st->n_poses, B = st->max_n_blocks;
-  const int Am = st->max_n_block_atoms, nt = st->max_n_tiles;
   const FN(gparams)* g = (const FN(gparams)*)global_params;
-  const int TILE = 32;
   if (block_pair)
     for (int64_t k = 0; k < 3LL * P * B * B; ++k) out[k] = 0;
   else
     for (int64_t k = 0; k < 3LL * P; ++k) out[k] = 0;
   if (dvdx)
     for (int64_t k = 0; k < 9LL * n_atoms; ++k) dvdx[k] = 0;
-  REAL* gradp = require_grad ? dvdx : 0;
+  FN(gsink) sink = {dvdx, n_atoms, 0};
+  const FN(gsink)* sp = (require_grad && dvdx) ? &sink : 0;
 
   for (int p = 0; p < P; ++p) {
     for (int b1 = 0; b1 < B; ++b1) {
@@ -561,106 +665,105 @@ void FN(oracle_pose_scores)(const oracle_stack* st, const REAL* coords,
         if (bt1 < 0 || bt2 < 0) continue;
         int64_t o1 = st->rot_coord_offset[p * B + b1];
         int64_t o2 = st->rot_coord_offset[p * B + b2];
-        int n1 = st->bt_n_atoms[bt1], n2 = st->bt_n_atoms[bt2];
-        REAL tot_atr = 0, tot_rep = 0, tot_lk = 0;
-        if (b1 != b2) {
-          int nti1 = (n1 - 1) / TILE + 1, nti2 = (n2 - 1) / TILE + 1;
-          for (int ti = 0; ti < nti1; ++ti) {
-            int s1 = TILE * ti;
-            int m1 = n1 - s1 < TILE ? n1 - s1 : TILE;
-            for (int tj = 0; tj < nti2; ++tj) {
-              int s2 = TILE * tj;
-              int m2 = n2 - s2 < TILE ? n2 - s2 : TILE;
-              REAL pa = 0, pr = 0, pl = 0;
-              for (int k = 0; k < m1 * m2; ++k) {
-                int a1 = s1 + k / m2, a2 = s2 + k % m2;
-                FN(tparams) t1, t2;
-                FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
-                FN(load_params)(type_params, st->bt_atom_types[bt2 * Am + a2], &t2);
-                int sep = FN(inter_sep)(st, p, b1, b2, bt1, bt2, a1, a2);
-                REAL e[2];
-                FN(eval_pair)(coords, n_atoms, o1 + a1, o2 + a2, sep, &t1, &t2,
-                              g, 0, require_grad, e, gradp);
-                pa += e[0];
-                pr += e[1];
-              }
-              int h1 = st->bt_n_heavy_in_tile[bt1 * nt + ti];
-              int h2 = st->bt_n_heavy_in_tile[bt2 * nt + tj];
-              for (int k = 0; k < h1 * h2; ++k) {
-                int a1 = s1 + st->bt_heavy_in_tile[bt1 * TILE * nt + TILE * ti + k / h2];
-                int a2 = s2 + st->bt_heavy_in_tile[bt2 * TILE * nt + TILE * tj + k % h2];
-                FN(tparams) t1, t2;
-                FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
-                FN(load_params)(type_params, st->bt_atom_types[bt2 * Am + a2], &t2);
-                int sep = FN(inter_sep)(st, p, b1, b2, bt1, bt2, a1, a2);
-                REAL e[1];
-                FN(eval_pair)(coords, n_atoms, o1 + a1, o2 + a2, sep, &t1, &t2,
-                              g, 1, require_grad, e, gradp);
-                pl += e[0];
-              }
-              tot_atr += pa;
-              tot_rep += pr;
-              tot_lk += pl;
-            }
-          }
-        } else {
-          int nti = (n1 - 1) / TILE + 1;
-          const int32_t* pd = st->bt_path_distance + (int64_t)bt1 * Am * Am;
-          for (int ti = 0; ti < nti; ++ti) {
-            int s1 = TILE * ti;
-            int m1 = n1 - s1 < TILE ? n1 - s1 : TILE;
-            for (int tj = ti; tj < nti; ++tj) {
-              int s2 = TILE * tj;
-              int m2 = n1 - s2 < TILE ? n1 - s2 : TILE;
-              REAL pa = 0, pr = 0, pl = 0;
-              for (int k = 0; k < m1 * m2; ++k) {
-                int a1 = s1 + k / m2, a2 = s2 + k % m2;
-                if (a1 >= a2) continue;
-                FN(tparams) t1, t2;
-                FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
-                FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a2], &t2);
-                int sep = pd[a1 * Am + a2];
-                REAL e[2];
-                FN(eval_pair)(coords, n_atoms, o1 + a1, o1 + a2, sep, &t1, &t2,
-                              g, 0, require_grad, e, gradp);
-                pa += e[0];
-                pr += e[1];
-              }
-              int h1 = st->bt_n_heavy_in_tile[bt1 * nt + ti];
-              int h2 = st->bt_n_heavy_in_tile[bt1 * nt + tj];
-              for (int k = 0; k < h1 * h2; ++k) {
-                int i1 = k / h2, i2 = k % h2;
-                /* positions compared, as in IntraResBlockEvaluation */
-                if (s1 + i1 >= s2 + i2) continue;
-                int a1 = s1 + st->bt_heavy_in_tile[bt1 * TILE * nt + TILE * ti + i1];
-                int a2 = s2 + st->bt_heavy_in_tile[bt1 * TILE * nt + TILE * tj + i2];
-                FN(tparams) t1, t2;
-                FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a1], &t1);
-                FN(load_params)(type_params, st->bt_atom_types[bt1 * Am + a2], &t2);
-                int sep = pd[a1 * Am + a2];
-                REAL e[1];
-                FN(eval_pair)(coords, n_atoms, o1 + a1, o1 + a2, sep, &t1, &t2,
-                              g, 1, require_grad, e, gradp);
-                pl += e[0];
-              }
-              tot_atr += pa;
-              tot_rep += pr;
-              tot_lk += pl;
-            }
-          }
-        }
+        REAL tot[3];
+        FN(score_block_pair)(st, coords, type_params, g, p, b1, b2, bt1, bt2,
+                             o1, o2, require_grad, sp, tot);
         if (block_pair) {
           int64_t base = ((int64_t)p * B + b1) * B + b2;
           int64_t PBB = (int64_t)P * B * B;
-          out[base] = tot_atr;
-          out[PBB + base] = tot_rep;
-          out[2 * PBB + base] = tot_lk;
+          out[base] = tot[0];
+          out[PBB + base] = tot[1];
+          out[2 * PBB + base] = tot[2];
         } else {
-          out[p] += tot_atr;
-          out[P + p] += tot_rep;
-          out[2 * P + p] += tot_lk;
+          out[p] += tot[0];
+          out[P + p] += tot[1];
+          out[2 * P + p] += tot[2];
         }
       }
+    }
+  }
+}
+
+/* ---------- rotamer-pair scoring (packing) ---------------------------- */
+
+/* Block spheres enclosing every rotamer of a block, from the rotamer
+ * spheres (sphere_overlap.impl.hh:455-514): centre = sequential mean of the
+ * rotamer centres, radius = max over rotamers of |c_r - c| + r_r.  Blocks
+ * without rotamers keep zeros.  Rotamer spheres themselves are
+ * oracle_block_spheres over rotamers (compute_rot_spheres, :23-107, nt = 1). */
+void FN(oracle_rot_block_spheres)(int n_poses, int max_n_blocks,
+                                  const REAL* rot_spheres,
+                                  const int32_t* n_rots_for_block,
+                                  const int32_t* rot_offset_for_block,
+                                  REAL* block_spheres) {
+  for (int64_t i = 0; i < (int64_t)n_poses * max_n_blocks; ++i) {
+    REAL* s = block_spheres + 4 * i;
+    s[0] = s[1] = s[2] = s[3] = 0;
+    int n = n_rots_for_block[i];
+    int r0 = rot_offset_for_block[i];
+    if (n <= 0 || r0 < 0) continue;
+    REAL cx = 0, cy = 0, cz = 0;
+    for (int r = 0; r < n; ++r) {
+      cx += rot_spheres[4 * ((int64_t)r0 + r) + 0];
+      cy += rot_spheres[4 * ((int64_t)r0 + r) + 1];
+      cz += rot_spheres[4 * ((int64_t)r0 + r) + 2];
+    }
+    cx /= n;
+    cy /= n;
+    cz /= n;
+    REAL rmax = 0;
+    for (int r = 0; r < n; ++r) {
+      const REAL* q = rot_spheres + 4 * ((int64_t)r0 + r);
+      REAL dx = q[0] - cx, dy = q[1] - cy, dz = q[2] - cz;
+      REAL d = SQRT(dx * dx + dy * dy + dz * dz) + q[3];
+      if (d > rmax) rmax = d;
+    }
+    s[0] = cx;
+    s[1] = cy;
+    s[2] = cz;
+    s[3] = rmax;
+  }
+}
+
+/* Score (dTdV == NULL: energies out[3, n]) or differentiate (dTdV [3, n]:
+ * grad [NA, 3] += sum_t dTdV[t, i] dV_t(pair i)/dx) every rotamer pair of a
+ * dispatch list indices [3, n] = (pose, rot1, rot2)
+ * (ljlk_pose_score.impl.hh:1479-2126).  A pair of rotamers of one block is
+ * the rotamer's intra-block energy.  Energies use the energy-only potentials
+ * (lj_atom_energy / lk_atom_energy), as the reference forward does. */
+void FN(oracle_rotamer_scores)(const oracle_stack* st, const REAL* coords,
+                               int64_t n_atoms, const REAL* type_params,
+                               const REAL* global_params, int64_t n_pairs,
+                               const int32_t* indices,
+                               const int32_t* rot_coord_offset,
+                               const int32_t* block_ind_for_rot,
+                               const int32_t* block_type_ind_for_rot,
+                               const REAL* dTdV, REAL* out, REAL* grad) {
+  const FN(gparams)* g = (const FN(gparams)*)global_params;
+  if (out)
+    for (int64_t k = 0; k < 3 * n_pairs; ++k) out[k] = 0;
+  if (grad)
+    for (int64_t k = 0; k < 3 * n_atoms; ++k) grad[k] = 0;
+  for (int64_t i = 0; i < n_pairs; ++i) {
+    int p = indices[i], r1 = indices[n_pairs + i], r2 = indices[2 * n_pairs + i];
+    if (p < 0 || r1 < 0 || r2 < 0) continue;
+    int bt1 = block_type_ind_for_rot[r1], bt2 = block_type_ind_for_rot[r2];
+    if (bt1 < 0 || bt2 < 0) continue;
+    int b1 = block_ind_for_rot[r1], b2 = block_ind_for_rot[r2];
+    REAL tot[3];
+    if (dTdV) {
+      REAL w[3] = {dTdV[i], dTdV[n_pairs + i], dTdV[2 * n_pairs + i]};
+      FN(gsink) sink = {grad, n_atoms, w};
+      FN(score_block_pair)(st, coords, type_params, g, p, b1, b2, bt1, bt2,
+                           rot_coord_offset[r1], rot_coord_offset[r2], 1,
+                           &sink, tot);
+    } else {
+      FN(score_block_pair)(st, coords, type_params, g, p, b1, b2, bt1, bt2,
+                           rot_coord_offset[r1], rot_coord_offset[r2], 0, 0,
+                           tot);
+      out[i] = tot[0];
+      out[n_pairs + i] = tot[1];
+      out[2 * n_pairs + i] = tot[2];
     }
   }
 }

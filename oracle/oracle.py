@@ -71,6 +71,16 @@ def lib():
             f.argtypes = [ct, ctypes.c_int, PR, PR, PR, ctypes.c_int, PR]
             f = getattr(L, "oracle_lk_pair" + suf)
             f.argtypes = [ct, ctypes.c_int, PR, PR, PR, ctypes.c_int, PR]
+            f = getattr(L, "oracle_rot_block_spheres" + suf)
+            f.argtypes = [ctypes.c_int, ctypes.c_int, PR, _P, _P, PR]
+            f = getattr(L, "oracle_rotamer_scores" + suf)
+            f.argtypes = [
+                ctypes.POINTER(OracleStack), PR, ctypes.c_int64, PR, PR, ctypes.c_int64, _P,
+                _P, _P, _P, PR, PR, PR,
+            ]
+        L.oracle_rot_dispatch.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P,
+                                          ctypes.c_int64]
+        L.oracle_rot_dispatch.restype = ctypes.c_int64
         L.oracle_pair_sep.argtypes = [
             ctypes.POINTER(OracleStack), ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_int, ctypes.c_int,
@@ -176,6 +186,103 @@ def _scores(S, coords, require_grad, dtype):
         _ptr(out, ct), _ptr(dvdx, ct) if dvdx is not None else None,
     )
     return out, nbr, dvdx, spheres
+
+
+class _RotOracle:
+    """Host arrays of one rotamer-op call (the 27 arguments of
+    ``ljlk_rotamer_scores``) for the C oracle."""
+
+    def __init__(self, args, dtype=None):
+        (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+         first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+         n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+         max_n_rots_per_pose, min_bondsep, inter_block_bondsep, *rest) = args
+        self.S = S = _Stack(args)
+        if dtype is None:
+            dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
+        self.dtype = dtype
+        self.suf, self.ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+        self.x = numpy.ascontiguousarray(S.coords.detach().cpu().numpy().astype(dtype))
+        self.tp = numpy.ascontiguousarray(S.type_params.detach().cpu().numpy().astype(dtype))
+        self.gp = numpy.ascontiguousarray(
+            S.global_params.detach().cpu().numpy().astype(dtype).reshape(-1))
+        self.rco = _np_i32(rot_coord_offset)
+        self.btr = _np_i32(block_type_ind_for_rot)
+        self.bir = _np_i32(block_ind_for_rot)
+        self.nrb = _np_i32(n_rots_for_block)
+        self.rob = _np_i32(rot_offset_for_block)
+        self.frt = _np_i32(first_rot_block_type)
+
+    def dispatch(self):
+        """(indices [3, n] int32, rot_spheres [R, 4], block_spheres [P, B, 4],
+        block_neighbors [P, B, B])."""
+        L, S, ct, suf = lib(), self.S, self.ct, self.suf
+        I = lambda a: _ptr(a, ctypes.c_int32)
+        R = self.rco.shape[0]
+        P, B = self.nrb.shape
+        rot_sph = numpy.zeros((R, 4), dtype=self.dtype)
+        getattr(L, "oracle_block_spheres" + suf)(
+            _ptr(self.x, ct), R, I(self.rco), I(self.btr), I(S.arrays["bt_n_atoms"]),
+            _ptr(rot_sph, ct))
+        blk_sph = numpy.zeros((P, B, 4), dtype=self.dtype)
+        getattr(L, "oracle_rot_block_spheres" + suf)(P, B, _ptr(rot_sph, ct), I(self.nrb),
+                                                     I(self.rob), _ptr(blk_sph, ct))
+        nbr = numpy.zeros((P, B, B), dtype=numpy.int32)
+        getattr(L, "oracle_block_neighbors" + suf)(P, B, I(self.frt), _ptr(blk_sph, ct),
+                                                   ct(self.dtype(S.max_dis)), I(nbr))
+        n = int(L.oracle_rot_dispatch(P, B, I(nbr), I(self.nrb), I(self.rob), None, 0))
+        idx = numpy.full((3, max(n, 1)), -1, dtype=numpy.int32)
+        L.oracle_rot_dispatch(P, B, I(nbr), I(self.nrb), I(self.rob), I(idx), n)
+        return numpy.ascontiguousarray(idx[:, :n]), rot_sph, blk_sph, nbr
+
+    def score(self, idx, dTdV=None):
+        """Energies [3, n] of the listed pairs idx [3, n], or with dTdV [3, n]
+        the weighted gradient [NA, 3]."""
+        L, S, ct, suf = lib(), self.S, self.ct, self.suf
+        I = lambda a: _ptr(a, ctypes.c_int32)
+        idx = numpy.ascontiguousarray(numpy.asarray(idx, dtype=numpy.int32))
+        n = idx.shape[1]
+        NA = self.x.shape[0]
+        out = numpy.zeros((3, n), dtype=self.dtype)
+        grad = w = None
+        if dTdV is not None:
+            w = numpy.ascontiguousarray(numpy.asarray(dTdV, dtype=self.dtype).reshape(3, n))
+            grad = numpy.zeros((NA, 3), dtype=self.dtype)
+        getattr(L, "oracle_rotamer_scores" + suf)(
+            ctypes.byref(S.struct), _ptr(self.x, ct), NA, _ptr(self.tp, ct), _ptr(self.gp, ct), n,
+            I(idx) if n else None, I(self.rco), I(self.bir), I(self.btr),
+            _ptr(w, ct) if w is not None else None,
+            _ptr(out, ct) if w is None else None,
+            _ptr(grad, ct) if grad is not None else None)
+        return grad if dTdV is not None else out
+
+
+def oracle_rotamer_scores(*args, dTdV=None, dtype=None):
+    """Run the oracle on the 27 arguments of ``ljlk_rotamer_scores``
+    (compiled.ops.cpp:282-530, ljlk_pose_score.impl.hh:1479-2126).
+
+    Returns (scores [3, n], dispatch_indices [3, n] int32, rot_spheres [R, 4],
+    block_spheres [P, B, 4], block_neighbors [P, B, B]); with dTdV [3, n]
+    also the weighted gradient [NA, 3] as a sixth item."""
+    o = _RotOracle(args, dtype)
+    idx, rot_sph, blk_sph, nbr = o.dispatch()
+    out = o.score(idx)
+    res = (out, idx, rot_sph, blk_sph, nbr)
+    return res + (o.score(idx, dTdV),) if dTdV is not None else res
+
+
+def oracle_rotamer_dispatch(*args, dtype=None):
+    """(oracle, indices, rot_spheres, block_spheres, block_neighbors) without
+    scoring; ``oracle.score(idx_subset)`` scores any listed pairs."""
+    o = _RotOracle(args, dtype)
+    return (o,) + o.dispatch()
+
+
+def load_reference_rotamer_op():
+    """The reference's own CPU ``tmol_ljlk::ljlk_rotamer_scores`` built by
+    build_ref.sh."""
+    load_reference_op()
+    return torch.ops.tmol_ljlk.ljlk_rotamer_scores
 
 
 def oracle_pair_separations(args, items):

@@ -1,0 +1,251 @@
+"""CPU tests of the rotamer-pair oracle: the C restatement of
+LJLKRotamerScoreDispatch (tmol/score/ljlk/potentials/ljlk_pose_score.impl.hh:
+1479-2126) and of the rotamer sphere / dispatch helpers
+(tmol/score/common/sphere_overlap.impl.hh:23-107, 287-351, 455-607).
+
+Pinned by:
+* the reference's own CPU op (oracle/_ref, where it builds) -- identical
+  dispatch lists, energies and gradients;
+* golden vectors written by that op (tests/golden/rotamer_ref_single100.json);
+* the block-pair oracle, itself pinned by tmol's 1ubq goldens: a set of one
+  rotamer per block (the pose's own residues) scores exactly the block-pair
+  energies, and any rotamer pair scores what its two blocks score in a pose
+  built from those two rotamers;
+* the culling invariant on brute force: every rotamer pair with an atom pair
+  inside max_dis is dispatched;
+* central differences of the weighted energy for the backward.
+"""
+
+import json
+import os
+
+import numpy
+import pytest
+import torch
+
+from oracle import oracle as orc
+from tmol_amd.rotamers import (current_rotamers, dispatch_checksum, repack_workload,
+                               rotamer_op_args)
+from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _args(ps, rs, dt):
+    a = rotamer_op_args(ps, rs, LJLKEnergyTerm())
+    if dt == torch.float64:
+        a[23], a[24] = a[23].double(), a[24].double()
+    return a
+
+
+@pytest.fixture(scope="module")
+def small():
+    """A 100-residue pose (the cpu_single_100 workload) cut to its first 24
+    residues, with repacking rotamers."""
+    from tmol_amd.generator import build_chains_fast, workload_specs
+    from tmol_amd.pose import Pose, PoseStack
+    from tmol_amd.rotamers import build_repack_rotamers
+
+    specs = workload_specs("cpu_single_100")
+    rts, phis, psis, omegas, chis = specs[0]
+    n = 24
+    db_rts = list(rts[:n])
+    from tmol_amd.chemical import default_chemical_database
+
+    db = default_chemical_database()
+    # re-terminate the cut chain
+    db_rts[-1] = db.residue_type(db_rts[-1].base_name, cterm=True)
+    sp = [(db_rts, phis[:n], psis[:n], omegas[:n], chis[:n])]
+    coords = build_chains_fast(sp)
+    poses = [Pose(sp[0][0], coords[0])]
+    out = {}
+    for dt in (torch.float32, torch.float64):
+        ps = PoseStack(poses, dtype=dt)
+        out[dt] = (ps, build_repack_rotamers(sp, ps, seed=3))
+    return out
+
+
+def test_dispatch_structure(small):
+    ps, rs = small[torch.float64]
+    args = _args(ps, rs, torch.float64)
+    E, idx, rsph, bsph, nbr = orc.oracle_rotamer_scores(*args)
+    blk = rs.block_ind_for_rot.numpy()
+    p, r1, r2 = idx
+    assert (p == 0).all()
+    b1, b2 = blk[r1], blk[r2]
+    assert (b1 <= b2).all()
+    assert (r1[b1 == b2] == r2[b1 == b2]).all()
+    # entries sorted by (b1, b2, r1, r2) with every pair of a flagged cell once
+    key = (b1.astype(numpy.int64) * 10**6 + b2) * 10**8 + r1.astype(numpy.int64) * 10**4 + r2
+    assert (numpy.diff(key) > 0).all()
+    nr = rs.n_rots_for_block.numpy()[0]
+    expect = sum(nr[i] if i == j else nr[i] * nr[j] for i, j in zip(*numpy.nonzero(nbr[0])))
+    assert idx.shape[1] == expect
+    # block spheres enclose every rotamer sphere
+    off = rs.rot_offset_for_block.numpy()[0]
+    for b in range(ps.max_n_blocks):
+        c = bsph[0, b, :3]
+        for r in range(off[b], off[b] + nr[b]):
+            assert numpy.linalg.norm(rsph[r, :3] - c) + rsph[r, 3] <= bsph[0, b, 3] + 1e-9
+
+
+def test_culling_keeps_every_interacting_rotamer_pair(small):
+    """Brute force: any two rotamers (of different blocks) with an atom pair
+    closer than max_dis, and every rotamer with itself, are dispatched."""
+    ps, rs = small[torch.float64]
+    args = _args(ps, rs, torch.float64)
+    _, idx, _, _, _ = orc.oracle_rotamer_scores(*args)
+    have = set(zip(idx[1].tolist(), idx[2].tolist()))
+    x = rs.coords.numpy()
+    off = rs.coord_offset_for_rot.numpy()
+    nat = ps.packed_block_types.n_atoms.numpy()[rs.block_type_ind_for_rot.numpy()]
+    blk = rs.block_ind_for_rot.numpy()
+    R = rs.n_rotamers_total
+    lo = numpy.array([x[off[r]: off[r] + nat[r]].min(0) for r in range(R)])
+    hi = numpy.array([x[off[r]: off[r] + nat[r]].max(0) for r in range(R)])
+    md = float(LJLKEnergyTerm().max_dis)
+    n_close = 0
+    for r1 in range(R):
+        assert (r1, r1) in have
+        gap = numpy.maximum(0, numpy.maximum(lo - hi[r1], lo[r1] - hi)).max(1)
+        for r2 in numpy.nonzero((gap < md) & (blk > blk[r1]))[0]:
+            a = x[off[r1]: off[r1] + nat[r1]]
+            b = x[off[r2]: off[r2] + nat[r2]]
+            d2 = ((a[:, None, :] - b[None, :, :]) ** 2).sum(-1)
+            if d2.min() < md * md:
+                n_close += 1
+                assert (r1, int(r2)) in have
+    assert n_close > 1000
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_single_rotamer_set_equals_block_pair_energies(dt):
+    """One rotamer per block, the pose's own residues: every dispatched pair
+    scores exactly the block-pair oracle's (b1, b2) entry and the dispatch
+    covers every non-zero block pair."""
+    from tmol_amd.generator import workload_poses
+    from tmol_amd.pose import PoseStack
+
+    poses = workload_poses("cpu_single_100") + workload_poses("padded_64x150_600", 1)
+    ps = PoseStack(poses, dtype=dt)
+    rs = current_rotamers(ps)
+    args = _args(ps, rs, dt)
+    E, idx, _, _, _ = orc.oracle_rotamer_scores(*args)
+    pa = pose_op_args(ps, LJLKEnergyTerm(), block_pair=True)
+    if dt == torch.float64:
+        pa[23], pa[24] = pa[23].double(), pa[24].double()
+    bp, nbr, _, _ = orc.oracle_pose_scores(*pa)
+    p, r1, r2 = idx
+    b1 = rs.block_ind_for_rot.numpy()[r1]
+    b2 = rs.block_ind_for_rot.numpy()[r2]
+    numpy.testing.assert_array_equal(E, bp[:, p, b1, b2])
+    assert idx.shape[1] == int(nbr.sum())
+    # nothing outside the dispatched cells
+    assert numpy.abs(bp.astype(numpy.float64)).sum() == pytest.approx(
+        numpy.abs(E.astype(numpy.float64)).sum(), rel=1e-12)
+
+
+def test_rotamer_pair_equals_block_pair_of_hybrid_pose(small):
+    """A rotamer pair scores what its blocks score in a pose whose two
+    blocks carry those rotamers' coordinates (sampled pairs)."""
+    ps, rs = small[torch.float64]
+    args = _args(ps, rs, torch.float64)
+    E, idx, _, _, _ = orc.oracle_rotamer_scores(*args)
+    rng = numpy.random.default_rng(11)
+    n = idx.shape[1]
+    pick = rng.choice(n, 40, replace=False)
+    off = rs.coord_offset_for_rot.numpy()
+    blk = rs.block_ind_for_rot.numpy()
+    nat = ps.packed_block_types.n_atoms.numpy()[rs.block_type_ind_for_rot.numpy()]
+    bco = ps.block_coord_offset.numpy()[0]
+    base = ps.coords.clone()
+    pa = pose_op_args(ps, LJLKEnergyTerm(), block_pair=True)
+    pa[23], pa[24] = pa[23].double(), pa[24].double()
+    for k in pick:
+        r1, r2 = int(idx[1, k]), int(idx[2, k])
+        c = base.clone()
+        for r in (r1, r2):
+            b = blk[r]
+            c[0, bco[b]: bco[b] + nat[r]] = rs.coords[off[r]: off[r] + nat[r]]
+        pa[0] = c.flatten(0, 1)
+        bp, _, _, _ = orc.oracle_pose_scores(*pa)
+        numpy.testing.assert_allclose(E[:, k], bp[:, 0, blk[r1], blk[r2]], rtol=1e-12, atol=1e-12)
+
+
+def test_backward_central_differences(small):
+    """Central differences (h = 1e-5 A) of sum_i w_i . E_i against the
+    dTdV-weighted backward; only the pairs of the perturbed atom's rotamer
+    change, so only those are re-scored."""
+    ps, rs = small[torch.float64]
+    args = _args(ps, rs, torch.float64)
+    o, idx, *_ = orc.oracle_rotamer_dispatch(*args)
+    w = numpy.random.default_rng(5).random((3, idx.shape[1]))
+    grad = o.score(idx, dTdV=w)
+    rng = numpy.random.default_rng(9)
+    atoms = rng.choice(o.x.shape[0], 16, replace=False)
+    off = rs.coord_offset_for_rot.numpy()
+    rot_of_atom = numpy.searchsorted(off, numpy.arange(o.x.shape[0]), side="right") - 1
+    x0 = o.x.copy()
+    h = 1e-5
+    for a in atoms:
+        r = rot_of_atom[a]
+        sel = numpy.nonzero((idx[1] == r) | (idx[2] == r))[0]
+        for k in range(3):
+            vals = []
+            for s in (1, -1):
+                o.x[...] = x0
+                o.x[a, k] += s * h
+                vals.append(float((w[:, sel] * o.score(idx[:, sel])).sum()))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - grad[a, k]) <= 1e-5 * max(1.0, abs(fd)), (a, k, fd, grad[a, k])
+    o.x[...] = x0
+
+
+def _golden():
+    return json.load(open(os.path.join(HERE, "golden", "rotamer_ref_single100.json")))
+
+
+@pytest.mark.parametrize("name,dt", [("float32", torch.float32), ("float64", torch.float64)])
+def test_oracle_matches_reference_golden(name, dt):
+    g = _golden()[name]
+    ps, rs = repack_workload("cpu_single_100", seed=1, dtype=dt)
+    args = _args(ps, rs, dt)
+    assert abs(float(args[0].double().sum()) - g["coords_sum"]) < 1e-9 * abs(g["coords_sum"])
+    E, idx, *_ = orc.oracle_rotamer_scores(*args)
+    assert idx.shape[1] == g["n_pairs"]
+    assert dispatch_checksum(idx) == int(g["dispatch_checksum"])
+    pick = numpy.array(g["sample_pairs"])
+    numpy.testing.assert_array_equal(idx[:, pick], numpy.array(g["sample_indices"]))
+    tol = 1e-12 if dt == torch.float64 else 2e-5
+    ref = numpy.array(g["sample_energies"])
+    numpy.testing.assert_allclose(E[:, pick], ref, rtol=tol, atol=tol)
+    numpy.testing.assert_allclose(E.astype(numpy.float64).sum(1), g["totals"], rtol=1e-6 if dt == torch.float32 else 1e-12)
+    gen = torch.Generator().manual_seed(20261203)
+    w = torch.rand(E.shape, generator=gen, dtype=torch.float64).to(dt).numpy()
+    *_, grad = orc.oracle_rotamer_scores(*args, dTdV=w)
+    atoms = numpy.array(g["sample_atoms"])
+    gr = numpy.array(g["sample_grad"])
+    err = numpy.abs(grad[atoms] - gr).max() / max(1.0, numpy.abs(gr).max())
+    assert err < (1e-12 if dt == torch.float64 else 1e-5)
+
+
+@pytest.mark.skipif(not orc.reference_op_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_oracle_matches_reference_op(small, dt):
+    ps, rs = small[dt]
+    args = _args(ps, rs, dt)
+    ref = orc.load_reference_rotamer_op()
+    c = args[0].clone().requires_grad_(True)
+    a2 = list(args)
+    a2[0] = c
+    s, ridx = ref(*a2)
+    E, idx, *_ = orc.oracle_rotamer_scores(*args)
+    numpy.testing.assert_array_equal(ridx.numpy(), idx)
+    tol = 0.0 if dt == torch.float64 else 1e-5
+    d = numpy.abs(s.detach().numpy() - E) / numpy.maximum(1.0, numpy.abs(E))
+    assert d.max() <= tol
+    w = torch.rand(s.shape, generator=torch.Generator().manual_seed(2), dtype=dt)
+    (s * w).sum().backward()
+    *_, grad = orc.oracle_rotamer_scores(*args, dTdV=w.numpy())
+    gr = c.grad.numpy()
+    assert numpy.linalg.norm(grad - gr) <= (1e-14 if dt == torch.float64 else 1e-6) * numpy.linalg.norm(gr)

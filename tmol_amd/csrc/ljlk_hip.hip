@@ -23,6 +23,7 @@
 #include "ljlk_score_general.hh"
 #include "ljlk_score_rows.hh"
 #include "ljlk_prune.hh"
+#include "ljlk_rotamer.hh"
 
 namespace tmol_ljlk {
 
@@ -119,19 +120,34 @@ static ScoreArgs<Real> make_args(const tmol_ljlk_stack* st, const tmol_ljlk_bloc
   return a;
 }
 
-// workgroups of `kernel` resident on the whole device at once (cached per device)
+// workgroups of `kernel` (256 threads) resident on the whole device at once,
+// cached per (kernel, device)
+struct ResidencyCache {
+  const void* kernel[64];
+  int dev[64];
+  int n[64];
+  int used;
+};
+
 template <typename K>
 static int resident_workgroups(K kernel) {
-  static int cache[64] = {0};
+  static ResidencyCache cache{};
   int dev = 0;
   if (hipGetDevice(&dev) != hipSuccess || dev < 0) dev = 0;
-  if (dev < 64 && cache[dev] > 0) return cache[dev];
+  const void* key = (const void*)kernel;
+  for (int i = 0; i < cache.used; ++i)
+    if (cache.kernel[i] == key && cache.dev[i] == dev) return cache.n[i];
   int cu = 0, per_cu = 0;
   if (hipDeviceGetAttribute(&cu, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) cu = 0;
   if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, kernel, 256, 0) != hipSuccess)
     per_cu = 0;
   const int n = std::max(1, cu) * std::max(1, per_cu);
-  if (dev < 64) cache[dev] = n;
+  if (cache.used < 64) {
+    cache.kernel[cache.used] = key;
+    cache.dev[cache.used] = dev;
+    cache.n[cache.used] = n;
+    ++cache.used;
+  }
   return n;
 }
 
@@ -197,6 +213,138 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
     hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(3 * P)), dim3(256), 0, stream,
                        (const double*)a.energies64, (Real*)energies, 3 * P);
   return check(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ rotamers
+
+struct RotWsLayout {
+  int64_t cursors, rot_spheres, block_spheres, records, conn_pd;
+  int64_t row_pairs, row_units, pair_off, unit_off, total;
+};
+
+static RotWsLayout rot_ws_layout(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_block_types* bt,
+                                 int dtype) {
+  const int64_t rs = dtype == TMOL_LJLK_F64 ? 8 : 4;
+  const int64_t G = int64_t(r->n_poses) * r->max_n_blocks;
+  RotWsLayout L;
+  int64_t o = 0;
+  L.cursors = o;
+  o = align256(o + 8 * 256);
+  L.rot_spheres = o;
+  o = align256(o + rs * 4 * int64_t(r->n_rots));
+  L.block_spheres = o;
+  o = align256(o + rs * 4 * G);
+  L.records = o;
+  o = align256(o + rs * REC * int64_t(bt->n_block_types) * bt->max_n_atoms);
+  L.conn_pd = o;
+  o = align256(o + int64_t(bt->n_block_types) * MAX_CONN * bt->max_n_atoms);
+  L.row_pairs = o;
+  o = align256(o + 8 * G);
+  L.row_units = o;
+  o = align256(o + 8 * G);
+  L.pair_off = o;
+  o = align256(o + 8 * (G + 1));
+  L.unit_off = o;
+  o = align256(o + 8 * (G + 1));
+  L.total = o;
+  return L;
+}
+
+static int validate_rot(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_block_types* bt,
+                        int dtype) {
+  if (!r || !bt) return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32 && dtype != TMOL_LJLK_F64) return TMOL_LJLK_EINVAL;
+  if (bt->dtype != dtype) return TMOL_LJLK_EINVAL;
+  if (bt->max_n_conn > MAX_CONN || bt->max_n_conn < 0) return TMOL_LJLK_ENOTSUP;
+  if (bt->n_block_types <= 0 || bt->max_n_atoms <= 0) return TMOL_LJLK_EINVAL;
+  if (bt->max_n_tiles != (bt->max_n_atoms - 1) / TILE + 1) return TMOL_LJLK_EINVAL;
+  if (bt->max_n_atoms > WAVE) return TMOL_LJLK_ENOTSUP;  // one lane per atom of a rotamer
+  if (r->n_poses <= 0 || r->max_n_blocks <= 0 || r->n_rots < 0) return TMOL_LJLK_EINVAL;
+  if (int64_t(r->n_poses) * r->max_n_blocks >= (int64_t(1) << 31)) return TMOL_LJLK_ENOTSUP;
+  if (r->n_atoms_total >= (int64_t(1) << 31)) return TMOL_LJLK_ENOTSUP;
+  if (!r->rot_coord_offset || !r->block_ind_for_rot || !r->block_type_ind_for_rot ||
+      !r->n_rots_for_block || !r->rot_offset_for_block || !r->first_rot_block_type ||
+      !r->min_bondsep || !r->inter_block_bondsep)
+    return TMOL_LJLK_EINVAL;
+  return TMOL_LJLK_OK;
+}
+
+static RotView make_rot_view(const tmol_ljlk_rotamer_set* r, int C) {
+  RotView v;
+  v.P = r->n_poses;
+  v.B = r->max_n_blocks;
+  v.R = r->n_rots;
+  v.C = C;
+  v.rot_coord_offset = r->rot_coord_offset;
+  v.block_ind_for_rot = r->block_ind_for_rot;
+  v.block_type_for_rot = r->block_type_ind_for_rot;
+  v.n_rots_for_block = r->n_rots_for_block;
+  v.rot_offset_for_block = r->rot_offset_for_block;
+  v.first_rot_block_type = r->first_rot_block_type;
+  v.min_bondsep = r->min_bondsep;
+  v.inter_block_bondsep = r->inter_block_bondsep;
+  return v;
+}
+
+template <typename Real>
+static int run_rot_neighbors(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_block_types* btp,
+                             const void* coords, double max_dis, void* ws, hipStream_t stream,
+                             const RotWsLayout& Lw) {
+  RotView v = make_rot_view(r, btp->max_n_conn);
+  BTView bt = make_bt_view(btp);
+  char* w = (char*)ws;
+  Real* rsph = (Real*)(w + Lw.rot_spheres);
+  Real* bsph = (Real*)(w + Lw.block_spheres);
+  const int64_t G = int64_t(v.P) * v.B;
+  const int nrec = bt.T * bt.Amax;
+  hipLaunchKernelGGL(prepare_records_kernel<Real>, dim3(grid_for(nrec)), dim3(256), 0, stream, bt,
+                     (const Real*)btp->type_params, btp->n_atom_types, (Real*)(w + Lw.records),
+                     (uint8_t*)(w + Lw.conn_pd));
+  if (v.R > 0) {
+    // rotamer spheres: the block-sphere pass with rotamers as blocks
+    StackView s{};
+    s.G = v.R;
+    s.block_atom_offset = v.rot_coord_offset;
+    s.block_type = v.block_type_for_rot;
+    hipLaunchKernelGGL(spheres_kernel<Real>, dim3((unsigned)((v.R + 255) / 256)), dim3(256), 0,
+                       stream, s, bt, (const Real*)coords, rsph);
+  }
+  hipLaunchKernelGGL(rot_block_spheres_kernel<Real>, dim3((unsigned)((G + 255) / 256)),
+                     dim3(256), 0, stream, v, (const Real*)rsph, bsph);
+  int64_t* rp = (int64_t*)(w + Lw.row_pairs);
+  int64_t* ru = (int64_t*)(w + Lw.row_units);
+  hipLaunchKernelGGL(rot_rows_count_kernel<Real>,
+                     dim3((unsigned)((G + WAVES_PER_WG - 1) / WAVES_PER_WG)), dim3(256), 0,
+                     stream, v, (const Real*)bsph, (Real)max_dis, rp, ru);
+  hipLaunchKernelGGL(scan2_kernel, dim3(1), dim3(1024), 0, stream, (const int64_t*)rp,
+                     (const int64_t*)ru, G, (int64_t*)(w + Lw.pair_off),
+                     (int64_t*)(w + Lw.unit_off));
+  return check(hipGetLastError());
+}
+
+template <typename Real>
+static RotArgs<Real> make_rot_args(const tmol_ljlk_rotamer_set* r,
+                                   const tmol_ljlk_block_types* btp, const void* coords, void* ws,
+                                   const RotWsLayout& Lw) {
+  RotArgs<Real> a{};
+  a.v = make_rot_view(r, btp->max_n_conn);
+  a.bt = make_bt_view(btp);
+  char* w = (char*)ws;
+  a.coords = (const Real*)coords;
+  a.records = (const Real*)(w + Lw.records);
+  a.conn_pd = (const uint8_t*)(w + Lw.conn_pd);
+  a.global_params = (const Real*)btp->global_params;
+  a.cursor = (uint32_t*)(w + Lw.cursors);
+  return a;
+}
+
+template <typename Real, int MODE>
+static void launch_rot_score(const RotArgs<Real>& a, int n_wg, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.cursor,
+                     int64_t(8 * QSTRIDE), 0u);
+  auto k = rotamer_score_kernel<Real, MODE>;
+  if (n_wg <= 0) n_wg = resident_workgroups(k);
+  hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
 }
 
 }  // namespace tmol_ljlk
@@ -526,6 +674,139 @@ int tmol_ljlk_prune_compact(int32_t n_blocks, uint32_t* row_count, int32_t* row_
                      (hipStream_t)stream, int64_t(n_blocks), row_count, row_items, row_capacity,
                      row_offset, keep);  // one wave per 64 rows: n_blocks / 64 waves
   return check(hipGetLastError());
+}
+
+int64_t tmol_ljlk_rotamer_workspace_bytes(const tmol_ljlk_rotamer_set* rs,
+                                          const tmol_ljlk_block_types* bt, int32_t dtype) {
+  if (validate_rot(rs, bt, dtype)) return -1;
+  return rot_ws_layout(rs, bt, dtype).total;
+}
+
+int tmol_ljlk_rotamer_neighbors(const tmol_ljlk_rotamer_set* rs,
+                                const tmol_ljlk_block_types* bt, const void* coords,
+                                int32_t dtype, double max_dis, void* workspace,
+                                int64_t ws_bytes, void* stream) {
+  int v = validate_rot(rs, bt, dtype);
+  if (v) return v;
+  if (!coords || !workspace) return TMOL_LJLK_EINVAL;
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_rot_neighbors<float>(rs, bt, coords, max_dis, workspace, st, L);
+  return run_rot_neighbors<double>(rs, bt, coords, max_dis, workspace, st, L);
+}
+
+int tmol_ljlk_rotamer_totals(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                             int32_t dtype, const void* workspace, int64_t* totals_host,
+                             void* stream) {
+  int v = validate_rot(rs, bt, dtype);
+  if (v) return v;
+  if (!workspace || !totals_host) return TMOL_LJLK_EINVAL;
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  const int64_t G = int64_t(rs->n_poses) * rs->max_n_blocks;
+  hipStream_t st = (hipStream_t)stream;
+  const char* w = (const char*)workspace;
+  hipError_t e = hipMemcpyAsync(&totals_host[0], w + L.pair_off + 8 * G, 8,
+                                hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(&totals_host[1], w + L.unit_off + 8 * G, 8, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  return check(e);
+}
+
+int tmol_ljlk_rotamer_dispatch(const tmol_ljlk_rotamer_set* rs,
+                               const tmol_ljlk_block_types* bt, int32_t dtype, double max_dis,
+                               void* workspace, int64_t ws_bytes, int32_t* indices,
+                               int64_t n_pairs, void* units, int64_t n_units, void* stream) {
+  int v = validate_rot(rs, bt, dtype);
+  if (v) return v;
+  if (!workspace || n_pairs < 0 || n_units < 0) return TMOL_LJLK_EINVAL;
+  if ((n_pairs > 0 && !indices) || (n_units > 0 && !units)) return TMOL_LJLK_EINVAL;
+  if (n_pairs >= (int64_t(1) << 31)) return TMOL_LJLK_EOVERFLOW;  // int32 dispatch indices
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  const int64_t G = int64_t(rs->n_poses) * rs->max_n_blocks;
+  hipStream_t st = (hipStream_t)stream;
+  char* w = (char*)workspace;
+  RotView rv = make_rot_view(rs, bt->max_n_conn);
+  const dim3 grid((unsigned)((G + WAVES_PER_WG - 1) / WAVES_PER_WG));
+  const int64_t* po = (const int64_t*)(w + L.pair_off);
+  const int64_t* uo = (const int64_t*)(w + L.unit_off);
+  if (dtype == TMOL_LJLK_F32)
+    hipLaunchKernelGGL(rot_rows_fill_kernel<float>, grid, dim3(256), 0, st, rv,
+                       (const float*)(w + L.block_spheres), (float)max_dis, po, uo, indices,
+                       n_pairs, (RotUnit*)units);
+  else
+    hipLaunchKernelGGL(rot_rows_fill_kernel<double>, grid, dim3(256), 0, st, rv,
+                       (const double*)(w + L.block_spheres), max_dis, po, uo, indices, n_pairs,
+                       (RotUnit*)units);
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                            const void* coords, int32_t dtype, void* workspace,
+                            int64_t ws_bytes, const void* units, int64_t n_units,
+                            int64_t n_pairs, int32_t flags, const void* dTdV, void* energies,
+                            void* grad, int32_t n_workgroups, void* stream) {
+  int v = validate_rot(rs, bt, dtype);
+  if (v) return v;
+  if (!coords || !workspace || n_units < 0 || n_pairs < 0 || (n_units > 0 && !units))
+    return TMOL_LJLK_EINVAL;
+  if (n_workgroups < 0 || n_workgroups > (1 << 20)) return TMOL_LJLK_EINVAL;
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  const int mode = flags & ~TMOL_LJLK_ACCUMULATE;
+  const bool bwd = mode == (TMOL_LJLK_WGRAD | TMOL_LJLK_NO_ENERGY);
+  if (mode != 0 && !bwd) return TMOL_LJLK_ENOTSUP;
+  if (!bwd && n_pairs > 0 && !energies) return TMOL_LJLK_EINVAL;
+  if (bwd && (!grad || (n_pairs > 0 && !dTdV))) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t NA3 = rs->n_atoms_total * 3;
+  if (dtype == TMOL_LJLK_F32) {
+    RotArgs<float> a = make_rot_args<float>(rs, bt, coords, workspace, L);
+    a.units = (const RotUnit*)units;
+    a.n_units = n_units;
+    a.n_pairs = n_pairs;
+    a.energies = (float*)energies;
+    a.dTdV = (const float*)dTdV;
+    a.grad = (float*)grad;
+    if (bwd) {
+      if (!(flags & TMOL_LJLK_ACCUMULATE)) fill<float>((float*)grad, NA3, st);
+      if (n_units > 0) launch_rot_score<float, M_WGRAD | M_NOE>(a, n_workgroups, st);
+    } else {
+      fill<float>((float*)energies, 3 * n_pairs, st);
+      if (n_units > 0) launch_rot_score<float, 0>(a, n_workgroups, st);
+    }
+  } else {
+    RotArgs<double> a = make_rot_args<double>(rs, bt, coords, workspace, L);
+    a.units = (const RotUnit*)units;
+    a.n_units = n_units;
+    a.n_pairs = n_pairs;
+    a.energies = (double*)energies;
+    a.dTdV = (const double*)dTdV;
+    a.grad = (double*)grad;
+    if (bwd) {
+      if (!(flags & TMOL_LJLK_ACCUMULATE)) fill<double>((double*)grad, NA3, st);
+      if (n_units > 0) launch_rot_score<double, M_WGRAD | M_NOE>(a, n_workgroups, st);
+    } else {
+      fill<double>((double*)energies, 3 * n_pairs, st);
+      if (n_units > 0) launch_rot_score<double, 0>(a, n_workgroups, st);
+    }
+  }
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_rotamer_spheres(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                              int32_t dtype, void* workspace, void** rot_spheres,
+                              void** block_spheres) {
+  int v = validate_rot(rs, bt, dtype);
+  if (v) return v;
+  if (!workspace || !rot_spheres || !block_spheres) return TMOL_LJLK_EINVAL;
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  *rot_spheres = (char*)workspace + L.rot_spheres;
+  *block_spheres = (char*)workspace + L.block_spheres;
+  return TMOL_LJLK_OK;
 }
 
 }  // extern "C"

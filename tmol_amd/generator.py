@@ -343,9 +343,10 @@ def random_pose(n_res: int, rng, db=None) -> Pose:
     return Pose(rts, build_chain(rts, phis, psis, omegas, chis))
 
 
-def generate_poses(n_poses: int, res_range, seed: int = MASTER_SEED,
-                   log_uniform: bool = False, db=None) -> List[Pose]:
-    """``n_poses`` poses; residue counts uniform (or log-uniform) over the
+def generate_pose_specs(n_poses: int, res_range, seed: int = MASTER_SEED,
+                        log_uniform: bool = False, db=None):
+    """Residue types and torsions (rts, phis, psis, omegas, chis) of
+    ``n_poses`` poses; residue counts uniform (or log-uniform) over the
     inclusive ``res_range``; pose k uses its own sub-seed."""
     lo, hi = res_range
     rng = numpy.random.default_rng(seed)
@@ -353,8 +354,14 @@ def generate_poses(n_poses: int, res_range, seed: int = MASTER_SEED,
         sizes = numpy.round(numpy.exp(rng.uniform(numpy.log(lo), numpy.log(hi), n_poses))).astype(int)
     else:
         sizes = rng.integers(lo, hi + 1, n_poses)
-    specs = [random_pose_spec(int(s), numpy.random.default_rng([seed, k]), db)
-             for k, s in enumerate(sizes)]
+    return [random_pose_spec(int(s), numpy.random.default_rng([seed, k]), db)
+            for k, s in enumerate(sizes)]
+
+
+def generate_poses(n_poses: int, res_range, seed: int = MASTER_SEED,
+                   log_uniform: bool = False, db=None) -> List[Pose]:
+    """``n_poses`` poses built from generate_pose_specs."""
+    specs = generate_pose_specs(n_poses, res_range, seed, log_uniform, db)
     coords = build_chains_fast(specs)
     return [Pose(sp[0], xyz) for sp, xyz in zip(specs, coords)]
 
@@ -372,6 +379,9 @@ WORKLOADS = {
     # residues); each with its own seed
     "pruner_train": dict(n_poses=160, res_range=(50, 1200), log_uniform=True, seed=MASTER_SEED + 5),
     "pruner_calib": dict(n_poses=128, res_range=(300, 300), log_uniform=False, seed=MASTER_SEED + 6),
+    # rotamer-pair scoring (repacking): poses whose every residue gets
+    # chi-grid rotamers (tmol_amd.rotamers)
+    "repack_16x300": dict(n_poses=16, res_range=(300, 300), log_uniform=False, seed=MASTER_SEED + 7),
 }
 
 
@@ -380,3 +390,11 @@ def workload_poses(name: str, n_poses: Optional[int] = None) -> List[Pose]:
     if n_poses is not None:
         w["n_poses"] = n_poses
     return generate_poses(**w)
+
+
+def workload_specs(name: str, n_poses: Optional[int] = None):
+    """The torsion specs workload_poses builds its poses from."""
+    w = dict(WORKLOADS[name])
+    if n_poses is not None:
+        w["n_poses"] = n_poses
+    return generate_pose_specs(**w)

@@ -556,3 +556,217 @@ def padded_work_list(args):
     pairs = call.ws.pairs(call.cstack, call.bt.c)
     assert pairs.shape[0] == n or overflow
     return pairs, nbr, overflow
+
+
+# ---------------------------------------------------------------------------
+# rotamer-pair scoring (packing): tmol_ljlk::ljlk_rotamer_scores
+# (compiled.ops.cpp:282-530, 597-663)
+
+ROT_UNIT_BYTES = 32
+
+
+class CRotamerSet(ctypes.Structure):
+    _fields_ = [
+        ("n_poses", ctypes.c_int32),
+        ("max_n_blocks", ctypes.c_int32),
+        ("n_rots", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+        ("n_atoms_total", ctypes.c_int64),
+        ("rot_coord_offset", _VP),
+        ("block_ind_for_rot", _VP),
+        ("block_type_ind_for_rot", _VP),
+        ("n_rots_for_block", _VP),
+        ("rot_offset_for_block", _VP),
+        ("first_rot_block_type", _VP),
+        ("min_bondsep", _VP),
+        ("inter_block_bondsep", _VP),
+    ]
+
+
+_rot_bound = False
+
+
+def _rot_lib():
+    global _rot_bound
+    L = lib()
+    if not _rot_bound:
+        RP, BP = ctypes.POINTER(CRotamerSet), ctypes.POINTER(CBlockTypes)
+        L.tmol_ljlk_rotamer_workspace_bytes.argtypes = [RP, BP, ctypes.c_int32]
+        L.tmol_ljlk_rotamer_workspace_bytes.restype = ctypes.c_int64
+        L.tmol_ljlk_rotamer_neighbors.argtypes = [
+            RP, BP, _VP, ctypes.c_int32, ctypes.c_double, _VP, ctypes.c_int64, _VP]
+        L.tmol_ljlk_rotamer_totals.argtypes = [
+            RP, BP, ctypes.c_int32, _VP, ctypes.POINTER(ctypes.c_int64), _VP]
+        L.tmol_ljlk_rotamer_dispatch.argtypes = [
+            RP, BP, ctypes.c_int32, ctypes.c_double, _VP, ctypes.c_int64, _VP, ctypes.c_int64,
+            _VP, ctypes.c_int64, _VP]
+        L.tmol_ljlk_rotamer_score.argtypes = [
+            RP, BP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, _VP, ctypes.c_int64,
+            ctypes.c_int64, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int32, _VP]
+        L.tmol_ljlk_rotamer_spheres.argtypes = [
+            RP, BP, ctypes.c_int32, _VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP)]
+        _rot_bound = True
+    return L
+
+
+class RotamerCall:
+    """Everything one ``ljlk_rotamer_scores`` call needs on the device: the
+    rotamer set and block-type tables as C structs, the workspace, and
+    (after ``dispatch``) the dispatch list and work units."""
+
+    def __init__(self, args, dtype):
+        (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+         first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+         n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+         max_n_rots_per_pose, min_bondsep, inter_block_bondsep, bt_n_atoms, bt_n_heavy_in_tile,
+         bt_heavy_in_tile, bt_atom_types, bt_n_conn, bt_conn_atoms, bt_path_distance,
+         bt_is_ligand, type_params, global_params, max_dis, block_pair) = args
+        _require_device(coords)
+        if not block_pair:
+            raise LJLKLibraryError(
+                "ljlk_rotamer_scores computes rotamer-pair energies only "
+                "(output_block_pair_energies must be True)")
+        dev = coords.device
+        self.device = dev
+        if coords.dim() != 2 or coords.shape[1] != 3:
+            raise LJLKLibraryError("rot_coords must be [n_atoms, 3]")
+        P, B = n_rots_for_block.shape
+        R = rot_coord_offset.numel()
+        if block_ind_for_rot.numel() != R or block_type_ind_for_rot.numel() != R:
+            raise LJLKLibraryError("per-rotamer tensors must have n_rots entries")
+        if tuple(rot_offset_for_block.shape) != (P, B) or tuple(first_rot_block_type.shape) != (P, B):
+            raise LJLKLibraryError("per-block rotamer tensors must be [P, B]")
+        if tuple(min_bondsep.shape) != (P, B, B):
+            raise LJLKLibraryError("pose_stack_min_bond_separation must be [P, B, B]")
+        if inter_block_bondsep.dim() != 5 or tuple(inter_block_bondsep.shape[:3]) != (P, B, B):
+            raise LJLKLibraryError("pose_stack_inter_block_bondsep must be [P, B, B, C, C]")
+        self.P, self.B, self.R = int(P), int(B), int(R)
+        self.keep = dict(
+            rot_coord_offset=_i32(rot_coord_offset, dev),
+            block_ind_for_rot=_i32(block_ind_for_rot, dev),
+            block_type_ind_for_rot=_i32(block_type_ind_for_rot, dev),
+            n_rots_for_block=_i32(n_rots_for_block, dev),
+            rot_offset_for_block=_i32(rot_offset_for_block, dev),
+            first_rot_block_type=_i32(first_rot_block_type, dev),
+            min_bondsep=_i32(min_bondsep, dev),
+            inter_block_bondsep=_i32(inter_block_bondsep, dev),
+        )
+        c = CRotamerSet()
+        c.n_poses, c.max_n_blocks, c.n_rots, c.reserved = self.P, self.B, self.R, 0
+        c.n_atoms_total = coords.shape[0]
+        for k, v in self.keep.items():
+            setattr(c, k, _p(v))
+        self.crs = c
+        self.bt = BlockTypeTables(bt_n_atoms, bt_n_heavy_in_tile, bt_heavy_in_tile, bt_atom_types,
+                                  bt_n_conn, bt_conn_atoms, bt_path_distance, bt_is_ligand,
+                                  type_params, global_params, dtype, dev)
+        if self.bt.c.max_n_conn != inter_block_bondsep.shape[3]:
+            raise LJLKLibraryError("inter_block_bondsep connection dims must match conn table")
+        self.coords = coords.contiguous()
+        self.dtype = dtype
+        self.dcode = _dtype_code(dtype)
+        self.max_dis = float(max_dis)
+        L = _rot_lib()
+        nbytes = L.tmol_ljlk_rotamer_workspace_bytes(ctypes.byref(c), ctypes.byref(self.bt.c),
+                                                     self.dcode)
+        if nbytes < 0:
+            raise LJLKLibraryError("rotamer workspace size query failed")
+        self.ws = torch.empty(int(nbytes), dtype=torch.uint8, device=dev)
+        self.indices = None
+        self.units = None
+        self.n_pairs = self.n_units = 0
+
+    def _s(self):
+        return _stream(self.device)
+
+    def neighbors(self):
+        _check(_rot_lib().tmol_ljlk_rotamer_neighbors(
+            ctypes.byref(self.crs), ctypes.byref(self.bt.c), _p(self.coords), self.dcode,
+            self.max_dis, _p(self.ws), self.ws.numel(), self._s()), "rotamer neighbors")
+
+    def totals(self):
+        out = (ctypes.c_int64 * 2)()
+        _check(_rot_lib().tmol_ljlk_rotamer_totals(
+            ctypes.byref(self.crs), ctypes.byref(self.bt.c), self.dcode, _p(self.ws), out,
+            self._s()), "rotamer totals")
+        return int(out[0]), int(out[1])
+
+    def dispatch(self):
+        """Neighbor pass, totals (one host sync), dispatch list and units."""
+        self.neighbors()
+        self.n_pairs, self.n_units = self.totals()
+        self.indices = torch.empty((3, self.n_pairs), dtype=torch.int32, device=self.device)
+        self.units = torch.empty((max(self.n_units, 1) * ROT_UNIT_BYTES,), dtype=torch.uint8,
+                                 device=self.device)
+        _check(_rot_lib().tmol_ljlk_rotamer_dispatch(
+            ctypes.byref(self.crs), ctypes.byref(self.bt.c), self.dcode, self.max_dis,
+            _p(self.ws), self.ws.numel(), _p(self.indices), self.n_pairs, _p(self.units),
+            self.n_units, self._s()), "rotamer dispatch")
+        return self.indices
+
+    def score(self, energies=None, dTdV=None, grad=None, flags=0, n_workgroups=0):
+        _check(_rot_lib().tmol_ljlk_rotamer_score(
+            ctypes.byref(self.crs), ctypes.byref(self.bt.c), _p(self.coords), self.dcode,
+            _p(self.ws), self.ws.numel(), _p(self.units), self.n_units, self.n_pairs, flags,
+            _p(dTdV), _p(energies), _p(grad), n_workgroups, self._s()), "rotamer score")
+
+    def energies(self, out=None, n_workgroups=0):
+        if out is None:
+            out = torch.empty((3, self.n_pairs), dtype=self.dtype, device=self.device)
+        self.score(energies=out, n_workgroups=n_workgroups)
+        return out
+
+    def backward(self, dTdV, out=None, n_workgroups=0):
+        w = dTdV.to(self.dtype).contiguous()
+        if out is None:
+            out = torch.empty_like(self.coords)
+        self.score(dTdV=w, grad=out, flags=WGRAD | NO_ENERGY, n_workgroups=n_workgroups)
+        return out
+
+    def spheres(self):
+        """(rot_spheres [R, 4], block_spheres [P, B, 4]) views of the workspace."""
+        pr, pb = _VP(), _VP()
+        _check(_rot_lib().tmol_ljlk_rotamer_spheres(
+            ctypes.byref(self.crs), ctypes.byref(self.bt.c), self.dcode, _p(self.ws),
+            ctypes.byref(pr), ctypes.byref(pb)), "rotamer spheres")
+        base = self.ws.data_ptr()
+        es = 8 if self.dtype == torch.float64 else 4
+        o1, o2 = pr.value - base, pb.value - base
+        rs = self.ws[o1: o1 + es * 4 * self.R].view(self.dtype).view(self.R, 4)
+        bs = self.ws[o2: o2 + es * 4 * self.P * self.B].view(self.dtype).view(self.P, self.B, 4)
+        return rs, bs
+
+
+class LJLKRotamerScoreFunction(torch.autograd.Function):
+    """Forward/backward of the rotamer-pair op (LJLKRotamerScoreOp,
+    compiled.ops.cpp:282-530): the backward weights every rotamer pair's
+    atom-pair derivatives by the incoming [3, n] gradient
+    (LJLKRotamerScoreDispatch::backward, impl.hh:1868-2126)."""
+
+    @staticmethod
+    def forward(ctx, *args):
+        call = RotamerCall(args, args[0].dtype)
+        idx = call.dispatch()
+        score = call.energies()
+        ctx.call = call
+        ctx.n_args = len(args)
+        ctx.mark_non_differentiable(idx)
+        return score, idx
+
+    @staticmethod
+    def backward(ctx, grad_score, grad_idx):
+        if grad_score is None:
+            return (None,) * ctx.n_args
+        grad = ctx.call.backward(grad_score)
+        return (grad,) + (None,) * (ctx.n_args - 1)
+
+
+def ljlk_rotamer_scores(*args):
+    """Drop-in for ``torch.ops.tmol_ljlk.ljlk_rotamer_scores`` (27 arguments,
+    the rotamer set in place of the pose stack's rotamer tensors); returns
+    ``[scores [3, n], dispatch_indices [3, n] int32]`` with one column per
+    rotamer pair (pose, rot1, rot2)."""
+    if len(args) != 27:
+        raise TypeError("ljlk_rotamer_scores takes 27 arguments (%d given)" % len(args))
+    score, idx = LJLKRotamerScoreFunction.apply(*args)
+    return [score, idx]
