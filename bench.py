@@ -129,6 +129,8 @@ def main():
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--no-pruner", action="store_true",
                     help="skip the learned-pruner leg (1024 x 300-residue batch)")
+    ap.add_argument("--no-rotamer", action="store_true",
+                    help="skip the rotamer-pair scoring leg (16 x 300-residue repacking)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported. Poses are "
@@ -275,6 +277,12 @@ def main():
             "effective_poses_per_s")}
         result["pruned"]["train_seconds"] = pr["training"]["train_seconds"]
         result["pruned"]["train_accuracy"] = pr["training"]["train_accuracy"]
+    if not args.no_rotamer:
+        from tmol_amd import rotamer_bench
+
+        log("rotamer pairs ...")
+        result["rotamer"] = rotamer_bench.run(dev, steps=10, warmup=2, cpu=not args.no_cpu, log=log)
+        result["rotamer"].pop("census", None)
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(poses, args.cpu_stride)

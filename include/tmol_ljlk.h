@@ -327,6 +327,14 @@ int tmol_ljlk_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
                             int64_t n_pairs, int32_t flags, const void* dTdV, void* energies,
                             void* grad, int32_t n_workgroups, void* stream);
 
+/* Branch census of the dispatch list (float32 only): counts of enumerated
+ * atom pairs per potential region, in tmol_ljlk_census's layout (items =
+ * rotamer pairs, intra-block items = self pairs). */
+int tmol_ljlk_rotamer_census(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                             const void* coords, int32_t dtype, void* workspace,
+                             int64_t ws_bytes, const void* units, int64_t n_units,
+                             int64_t n_pairs, uint64_t* census, void* stream);
+
 /* Debug: device addresses of the rotamer spheres [R, 4] and block spheres
  * [P, B, 4] (dtype) inside the workspace, written by _neighbors. */
 int tmol_ljlk_rotamer_spheres(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,

@@ -759,6 +759,7 @@ int tmol_ljlk_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
   const int mode = flags & ~TMOL_LJLK_ACCUMULATE;
   const bool bwd = mode == (TMOL_LJLK_WGRAD | TMOL_LJLK_NO_ENERGY);
   if (mode != 0 && !bwd) return TMOL_LJLK_ENOTSUP;
+  if (n_pairs >= (int64_t(1) << 31)) return TMOL_LJLK_EOVERFLOW;
   if (!bwd && n_pairs > 0 && !energies) return TMOL_LJLK_EINVAL;
   if (bwd && (!grad || (n_pairs > 0 && !dTdV))) return TMOL_LJLK_EINVAL;
   hipStream_t st = (hipStream_t)stream;
@@ -775,7 +776,6 @@ int tmol_ljlk_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
       if (!(flags & TMOL_LJLK_ACCUMULATE)) fill<float>((float*)grad, NA3, st);
       if (n_units > 0) launch_rot_score<float, M_WGRAD | M_NOE>(a, n_workgroups, st);
     } else {
-      fill<float>((float*)energies, 3 * n_pairs, st);
       if (n_units > 0) launch_rot_score<float, 0>(a, n_workgroups, st);
     }
   } else {
@@ -790,10 +790,32 @@ int tmol_ljlk_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
       if (!(flags & TMOL_LJLK_ACCUMULATE)) fill<double>((double*)grad, NA3, st);
       if (n_units > 0) launch_rot_score<double, M_WGRAD | M_NOE>(a, n_workgroups, st);
     } else {
-      fill<double>((double*)energies, 3 * n_pairs, st);
       if (n_units > 0) launch_rot_score<double, 0>(a, n_workgroups, st);
     }
   }
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_rotamer_census(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                             const void* coords, int32_t dtype, void* workspace,
+                             int64_t ws_bytes, const void* units, int64_t n_units,
+                             int64_t n_pairs, uint64_t* census, void* stream) {
+  int v = validate_rot(rs, bt, dtype);
+  if (v) return v;
+  if (!coords || !workspace || !census || n_units < 0 || (n_units > 0 && !units))
+    return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32) return TMOL_LJLK_ENOTSUP;
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, st,
+                     (unsigned long long*)census, int64_t(16), 0ull);
+  RotArgs<float> a = make_rot_args<float>(rs, bt, coords, workspace, L);
+  a.units = (const RotUnit*)units;
+  a.n_units = n_units;
+  a.n_pairs = n_pairs;
+  a.census = (unsigned long long*)census;
+  if (n_units > 0) launch_rot_score<float, M_CENSUS>(a, 0, st);
   return check(hipGetLastError());
 }
 

@@ -48,7 +48,8 @@ struct alignas(16) RotUnit {
   int32_t g1;        // pose * B + b1
   int32_t b2;
   int32_t r1_first;  // global rotamer index
-  int32_t r1_count;  // 0: a cell whose rotamer offsets are negative (skipped)
+  int32_t r1_count;  // < 0: -count of a cell whose rotamer offsets are negative
+                     // (its entries get zero energies)
   int64_t entry;     // dispatch entry of (r1_first, first rotamer of b2)
   int64_t pad;
 };
@@ -249,7 +250,7 @@ __global__ __launch_bounds__(256) void rot_rows_fill_kernel(
         U.g1 = int(g1);
         U.b2 = b2;
         U.r1_first = off1 + k * UNIT_R1;
-        U.r1_count = ok ? min(UNIT_R1, nr1 - k * UNIT_R1) : 0;
+        U.r1_count = (ok ? 1 : -1) * min(UNIT_R1, nr1 - k * UNIT_R1);
         U.entry = e + (diag ? int64_t(k) * UNIT_R1 : int64_t(k) * UNIT_R1 * nr2);
         U.pad = 0;
         units[u + k] = U;
@@ -277,7 +278,44 @@ struct RotArgs {
   Real* energies;     // [3, n_pairs]
   const Real* dTdV;   // [3, n_pairs] (backward)
   Real* grad;         // [NA, 3] (backward)
+  unsigned long long* census;  // [16] with M_CENSUS
 };
+
+// Branch census of one enumerated atom pair (the reference enumerates every
+// atom pair of a rotamer pair for LJ and every heavy pair for LK): weight-0,
+// beyond max_dis, or the potential region the distance falls in.
+template <typename Real>
+__device__ __forceinline__ void census_pair(uint32_t* cnt, int f1, Real rad1,
+                                            const AtomRegs<Real>& R2, Real d2, Real wcp,
+                                            const GlobalParams<Real>& gp) {
+  const bool lk = (f1 & F_HEAVY) && (R2.flags & F_HEAVY);
+  cnt[C_LJ_PAIRS] += 1;
+  cnt[C_LK_PAIRS] += lk ? 1 : 0;
+  if (wcp == Real(0)) {
+    cnt[C_LJ_CP0] += 1;
+    cnt[C_LK_CP0] += lk ? 1 : 0;
+    return;
+  }
+  if (!(d2 < gp.max_dis2)) {
+    cnt[C_LJ_FAR] += 1;
+    cnt[C_LK_FAR] += lk ? 1 : 0;
+    return;
+  }
+  const Real dist = fast_sqrt(d2);
+  const Real sigma = lj_sigma(f1, R2.flags, rad1, R2.r0.x, gp);
+  const Real d_lin = sigma * gp.dlin_factor;
+  Real cmin = sigma > gp.spline_start ? sigma : gp.spline_start;
+  cmin = cmin > gp.cpoly_cap ? gp.cpoly_cap : cmin;
+  cnt[dist < d_lin ? C_LJ_LIN : (dist < cmin ? C_LJ_MID : C_LJ_SPL)] += 1;
+  if (lk) {
+    const bool cc = (f1 & F_CARBON_LK) && (R2.flags & F_CARBON_LK);
+    Real d_min, cdmin, cdmax;
+    lk_knots(sigma, cc, d_min, cdmin, cdmax);
+    cnt[dist < cdmin ? C_LK_CONST
+                     : (dist < cdmax ? C_LK_CLOSE
+                                     : (dist < gp.spline_start ? C_LK_MID : C_LK_SPL))] += 1;
+  }
+}
 
 // wave-private LDS: the staged atoms of the current b1 rotamer (64 at a time)
 template <typename Real>
@@ -404,15 +442,28 @@ __device__ __forceinline__ void flush_a1(const RotArgs<Real>& A, int64_t atom, R
   }
 }
 
+// zero energies of entries [e0, e0 + n) (pairs that are not scored)
+template <typename Real>
+__device__ __forceinline__ void zero_entries(const RotArgs<Real>& A, int64_t e0, int64_t n,
+                                             int lane) {
+  for (int64_t k = lane; k < n; k += WAVE) {
+    A.energies[e0 + k] = Real(0);
+    A.energies[A.n_pairs + e0 + k] = Real(0);
+    A.energies[2 * A.n_pairs + e0 + k] = Real(0);
+  }
+}
+
 // inter-block unit: rotamers [r1_first, +r1_count) of g1 x every rotamer of g2
 template <typename Real, int MODE>
 __device__ void rot_inter(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalParams<Real>& gp,
                           int g1, int g2, int b2, int r1_first, int r1_count, int64_t entry,
-                          int lane) {
+                          int lane, uint32_t* cnt) {
   constexpr bool BWD = (MODE & M_WGRAD) != 0;
+  constexpr bool CENSUS = (MODE & M_CENSUS) != 0;
   const RotView& v = A.v;
   const int nr2 = uni(v.n_rots_for_block[g2]), off2 = uni(v.rot_offset_for_block[g2]);
   const int min_sep = uni(v.min_bondsep[int64_t(g1) * v.B + b2]);
+  if (CENSUS && lane == 0) cnt[C_ITEMS] += uint32_t(r1_count) * uint32_t(nr2);
   const bool cp = min_sep <= 4;
   if (cp) {
     wave_sync();
@@ -474,7 +525,17 @@ __device__ void rot_inter(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalP
     for (int i = 0; i < r1_count; ++i) {
       const int r1 = r1_first + i;
       const int t1 = uni(v.block_type_for_rot[r1]);
-      if (t1 < 0) continue;
+      if (t1 < 0) {
+        if (!BWD && !CENSUS) zero_entries(A, entry + int64_t(i) * nr2 + ps, k, lane);
+        continue;
+      }
+      // rotamers of the pack without atoms (no block type) score zero
+      if (!BWD && !CENSUS && lane < k && nat == 0) {
+        const int64_t e0 = entry + int64_t(i) * nr2 + ps + lane;
+        A.energies[e0] = Real(0);
+        A.energies[n + e0] = Real(0);
+        A.energies[2 * n + e0] = Real(0);
+      }
       const int o1 = uni(v.rot_coord_offset[r1]);
       const int n1 = uni(A.bt.n_atoms[t1]);
       const int nc1 = uni(A.bt.n_conn[t1]);
@@ -494,12 +555,16 @@ __device__ void rot_inter(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalP
         const Real dx = p1.x - R2.x, dy = p1.y - R2.y, dz = p1.z - R2.z;
         const Real d2 = dx * dx + dy * dy + dz * dz;
         bool in = active && d2 < gp.max_dis2;
-        if (__ballot(in) == 0) continue;
+        if (!CENSUS && __ballot(in) == 0) continue;
         Real wcp = Real(1);
         if (cp) {
           const int sep = packed_pair_sep(L.pdw[j], R2.pd, L.cs, min_sep, nc1, nc2, lig);
           wcp = cp_weight<Real>(sep);
           in = in && wcp != Real(0);
+        }
+        if (CENSUS) {
+          if (active) census_pair(cnt, L.flg[j], p1.w, R2, d2, wcp, gp);
+          continue;
         }
         const bool lk_any = (L.flg[j] & F_HEAVY) && __ballot(in && (R2.flags & F_HEAVY)) != 0;
         const Real gs = rot_pair<Real, BWD>(L, j, R2, dx, dy, dz, d2, wcp, in, lk_any, gp, ea, er,
@@ -512,7 +577,7 @@ __device__ void rot_inter(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalP
             flush_a1(A, int64_t(o1) + j, gs * dx, gs * dy, gs * dz, lane);
         }
       }
-      if (!BWD) {
+      if (!BWD && !CENSUS) {
         ea = segment_scan(ea, seg_start, lane);
         er = segment_scan(er, seg_start, lane);
         el = segment_scan(el, seg_start, lane);
@@ -538,14 +603,22 @@ __device__ void rot_inter(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalP
 // one lane per atom (at most 64)
 template <typename Real, int MODE>
 __device__ void rot_intra(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalParams<Real>& gp,
-                          int r1_first, int r1_count, int64_t entry, int lane) {
+                          int r1_first, int r1_count, int64_t entry, int lane, uint32_t* cnt) {
   constexpr bool BWD = (MODE & M_WGRAD) != 0;
+  constexpr bool CENSUS = (MODE & M_CENSUS) != 0;
+  if (CENSUS && lane == 0) {
+    cnt[C_ITEMS] += uint32_t(r1_count);
+    cnt[C_INTRA_ITEMS] += uint32_t(r1_count);
+  }
   const RotView& v = A.v;
   const int64_t n = A.n_pairs;
   for (int i = 0; i < r1_count; ++i) {
     const int r = r1_first + i;
     const int t = uni(v.block_type_for_rot[r]);
-    if (t < 0) continue;
+    if (t < 0) {
+      if (!BWD && !CENSUS) zero_entries(A, entry + i, 1, lane);
+      continue;
+    }
     const int o = uni(v.rot_coord_offset[r]);
     const int na = uni(A.bt.n_atoms[t]);
     const int64_t e = entry + i;
@@ -570,10 +643,14 @@ __device__ void rot_intra(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalP
       const Real dx = p1.x - R2.x, dy = p1.y - R2.y, dz = p1.z - R2.z;
       const Real d2 = dx * dx + dy * dy + dz * dz;
       bool in = active && j < a2 && d2 < gp.max_dis2;
-      if (__ballot(in) == 0) continue;
+      if (!CENSUS && __ballot(in) == 0) continue;
       Real wcp = Real(0);
-      if (in) wcp = cp_weight<Real>(pd[int64_t(j) * A.bt.Amax + a2]);
+      if (active && j < a2) wcp = cp_weight<Real>(pd[int64_t(j) * A.bt.Amax + a2]);
       in = in && wcp != Real(0);
+      if (CENSUS) {
+        if (active && j < a2) census_pair(cnt, L.flg[j], p1.w, R2, d2, wcp, gp);
+        continue;
+      }
       const bool lk_any = (L.flg[j] & F_HEAVY) && __ballot(in && (R2.flags & F_HEAVY)) != 0;
       const Real gs = rot_pair<Real, BWD>(L, j, R2, dx, dy, dz, d2, wcp, in, lk_any, gp, ea, er,
                                           el, w0, w1, w2);
@@ -591,7 +668,7 @@ __device__ void rot_intra(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalP
       if (g2y != Real(0)) unsafeAtomicAdd(&A.grad[at + 1], g2y);
       if (g2z != Real(0)) unsafeAtomicAdd(&A.grad[at + 2], g2z);
     }
-    if (!BWD) {
+    if (!BWD && !CENSUS) {
       ea = wave_sum(ea);
       er = wave_sum(er);
       el = wave_sum(el);
@@ -605,26 +682,45 @@ __device__ void rot_intra(const RotArgs<Real>& A, RotLDS<Real>& L, const GlobalP
 }
 
 // Persistent rotamer-pair kernel: 4 independent waves per workgroup, units
-// from the per-XCD queues.  MODE 0: energies [3, n_pairs];
-// M_WGRAD | M_NOE: grad [NA, 3] += sum_t dTdV[t, i] dV_t(pair i)/dx.
+// from the per-XCD queues.  MODE 0: energies [3, n_pairs] (every entry
+// written, zero for pairs that are not scored);
+// M_WGRAD | M_NOE: grad [NA, 3] += sum_t dTdV[t, i] dV_t(pair i)/dx;
+// M_CENSUS: counts of enumerated atom pairs per potential region.
 template <typename Real, int MODE>
 __global__ __launch_bounds__(256) void rotamer_score_kernel(RotArgs<Real> A) {
   __shared__ RotLDS<Real> lds[WAVES_PER_WG];
   const int lane = lane_id();
   RotLDS<Real>& L = lds[threadIdx.x / WAVE];
   const GlobalParams<Real> gp = load_global_params(A.global_params);
+  uint32_t cnt[C_N];
+#pragma unroll
+  for (int k = 0; k < C_N; ++k) cnt[k] = 0;
   RowQueue queue(A.cursor, A.n_units);
   for (int64_t u = queue.next(); u >= 0; u = queue.next()) {
     const RotUnit* up = A.units + u;
     const int g1 = uni(up->g1), b2 = uni(up->b2);
     const int r1_first = uni(up->r1_first), r1_count = uni(up->r1_count);
     const int64_t entry = uni64(up->entry);
-    if (r1_count <= 0) continue;
     const int p = g1 / A.v.B, b1 = g1 - p * A.v.B;
+    if (r1_count <= 0) {
+      if (MODE == 0 && r1_count < 0)
+        zero_entries(A, entry,
+                     int64_t(-r1_count) * (b1 == b2 ? 1 : uni(A.v.n_rots_for_block[p * A.v.B + b2])),
+                     lane);
+      continue;
+    }
     if (b1 == b2)
-      rot_intra<Real, MODE>(A, L, gp, r1_first, r1_count, entry, lane);
+      rot_intra<Real, MODE>(A, L, gp, r1_first, r1_count, entry, lane, cnt);
     else
-      rot_inter<Real, MODE>(A, L, gp, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry, lane);
+      rot_inter<Real, MODE>(A, L, gp, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry, lane,
+                            cnt);
+  }
+  if (MODE & M_CENSUS) {
+#pragma unroll
+    for (int k = 0; k < C_N; ++k) {
+      const unsigned long long v = wave_sum((unsigned long long)cnt[k]);
+      if (lane == 0 && v) atomicAdd(&A.census[k], v);
+    }
   }
 }
 

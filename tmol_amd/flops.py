@@ -47,12 +47,31 @@ FLOPS_ENERGY_GRAD = {
 }
 
 
+# Energy only (the rotamer-pair forward, lj_atom_energy / lk_atom_energy):
+# no gradient scatter; LJ linear: V_lin + V'_lin (d - d_lin), split, w (6);
+# 12-6: sigma/d, powers, V, split, w (11); Hermite: t, Horner V, w (10);
+# LK per direction 8 (close/far Hermite: t, Horner; f_desolv: 1/d^2,
+# (d - r)^2 / lambda^2, exp, scale), both + sum + w = 18.
+FLOPS_ENERGY = {
+    "lj_far": DIST,
+    "lj_linear": DIST + SQRT + 6 + 2,
+    "lj_126": DIST + SQRT + 11 + 2,
+    "lj_spline": DIST + SQRT + 10 + 2,
+    "lk_far": DIST,
+    "lk_const": DIST + SQRT + 2 + 1,
+    "lk_close": DIST + SQRT + 18 + 1,
+    "lk_fdesolv": DIST + SQRT + 18 + 1,
+    "lk_spline": DIST + SQRT + 18 + 1,
+}
+
+
 def census_dict(values):
     return {k: int(v) for k, v in zip(CENSUS_FIELDS, values)}
 
 
-def algorithmic_flops(census: dict) -> int:
-    return sum(census[k] * f for k, f in FLOPS_ENERGY_GRAD.items())
+def algorithmic_flops(census: dict, energy_only: bool = False) -> int:
+    table = FLOPS_ENERGY if energy_only else FLOPS_ENERGY_GRAD
+    return sum(census[k] * f for k, f in table.items())
 
 
 def pair_evaluations(census: dict) -> int:

@@ -603,6 +603,9 @@ def _rot_lib():
         L.tmol_ljlk_rotamer_score.argtypes = [
             RP, BP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, _VP, ctypes.c_int64,
             ctypes.c_int64, ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int32, _VP]
+        L.tmol_ljlk_rotamer_census.argtypes = [
+            RP, BP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, _VP, ctypes.c_int64,
+            ctypes.c_int64, _VP, _VP]
         L.tmol_ljlk_rotamer_spheres.argtypes = [
             RP, BP, ctypes.c_int32, _VP, ctypes.POINTER(_VP), ctypes.POINTER(_VP)]
         _rot_bound = True
@@ -722,6 +725,16 @@ class RotamerCall:
             out = torch.empty_like(self.coords)
         self.score(dTdV=w, grad=out, flags=WGRAD | NO_ENERGY, n_workgroups=n_workgroups)
         return out
+
+    def census(self):
+        """Branch census of the dispatch list (float32): counts of enumerated
+        atom pairs per potential region (tmol_amd.flops.CENSUS_FIELDS)."""
+        out = torch.zeros(16, dtype=torch.int64, device=self.device)
+        _check(_rot_lib().tmol_ljlk_rotamer_census(
+            ctypes.byref(self.crs), ctypes.byref(self.bt.c), _p(self.coords), self.dcode,
+            _p(self.ws), self.ws.numel(), _p(self.units), self.n_units, self.n_pairs, _p(out),
+            self._s()), "rotamer census")
+        return out.cpu().tolist()
 
     def spheres(self):
         """(rot_spheres [R, 4], block_spheres [P, B, 4]) views of the workspace."""

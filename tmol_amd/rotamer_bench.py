@@ -1,0 +1,170 @@
+"""Rotamer-pair scoring benchmark (packing's energy table).
+
+Workload: 16 poses x 300 residues (the repack_16x300 generator workload),
+every residue with repacking rotamers (tmol_amd.rotamers, seed 1): 66.5k
+rotamers, 25.1M rotamer pairs.  One step = the whole forward of
+``ljlk_rotamer_scores``: rotamer and block spheres, neighbor rows, the host
+read of the totals, the dispatch list and the scoring kernel, inputs
+resident in HBM.
+
+Reported: rotamer pairs/s of the step; the scoring kernel alone (HIP events
+on its stream around repeated launches over a fixed dispatch list) with its
+roofline from the branch census x energy-only FLOP table; the reference's
+own CPU op (oracle/_ref, one thread) on the first pose as the CPU baseline.
+
+    python -m tmol_amd.rotamer_bench [--steps 10] [--no-cpu]
+"""
+
+import argparse
+import json
+import sys
+import time
+
+FP32_PEAK_TFLOPS = 157.3
+
+
+def _args(ps, rs, dt, device):
+    import torch
+
+    from .rotamers import rotamer_op_args
+    from .score.ljlk_energy_term import LJLKEnergyTerm
+
+    a = rotamer_op_args(ps, rs, LJLKEnergyTerm())
+    if dt == torch.float64:
+        a[23], a[24] = a[23].double(), a[24].double()
+    return [x.to(device) if isinstance(x, torch.Tensor) else x for x in a]
+
+
+def cpu_baseline():
+    """The reference CPU op (one thread) on the first pose of the workload
+    with its rotamers (the same pose and rotamers as in the batch: poses and
+    rotamer noise are drawn pose by pose)."""
+    import torch
+
+    from oracle import oracle as orc
+
+    from .rotamers import repack_workload
+
+    torch.set_num_threads(1)
+    ps, sub = repack_workload("repack_16x300", n_poses=1, seed=1)
+    a = _args(ps, sub, torch.float32, "cpu")
+    if orc.reference_op_available():
+        kind, f = "reference", orc.load_reference_rotamer_op()
+    else:
+        kind = "port"
+
+        def f(*x):
+            return orc.oracle_rotamer_scores(*x)[:2]
+    t0 = time.perf_counter()
+    s, idx = f(*a)
+    dt = time.perf_counter() - t0
+    n = int(idx.shape[1])
+    return {
+        "value": n / dt,
+        "unit": "rotamer pairs/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": "pose 0 of the batch (300 residues, %d rotamers, %d rotamer pairs), "
+                  "forward, torch threads = 1, %.1f s" % (sub.n_rotamers_total, n, dt),
+    }
+
+
+def run(device, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
+    import numpy
+    import torch
+
+    from .flops import CENSUS_FIELDS, algorithmic_flops, pair_evaluations
+    from .ops import RotamerCall, ljlk_rotamer_scores
+    from .rotamers import repack_workload
+
+    t0 = time.time()
+    ps, rs = repack_workload("repack_16x300", seed=1)
+    log("rotamers: %d poses, %d rotamers, %d atoms (%.1fs)"
+        % (ps.n_poses, rs.n_rotamers_total, rs.coords.shape[0], time.time() - t0))
+    a = _args(ps, rs, torch.float32, device)
+    stream = torch.cuda.current_stream(device)
+
+    # the whole forward, as the op runs it
+    for _ in range(warmup):
+        s, idx = ljlk_rotamer_scores(*a)
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s, idx = ljlk_rotamer_scores(*a)
+    torch.cuda.synchronize(device)
+    op_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    n_pairs = int(idx.shape[1])
+    finite = bool(torch.isfinite(s).all())
+
+    # the scoring kernel alone over a fixed dispatch list
+    call = RotamerCall(a, torch.float32)
+    call.dispatch()
+    out = torch.empty((3, call.n_pairs), dtype=torch.float32, device=device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    for _ in range(2):
+        call.energies(out)
+    torch.cuda.synchronize(device)
+    ev0.record(stream)
+    for _ in range(kernel_reps):
+        call.energies(out)
+    ev1.record(stream)
+    torch.cuda.synchronize(device)
+    score_ms = ev0.elapsed_time(ev1) / kernel_reps  # + one 2 KB cursor fill per launch
+    ev0.record(stream)
+    for _ in range(kernel_reps):
+        call.dispatch()
+    ev1.record(stream)
+    torch.cuda.synchronize(device)
+    dispatch_ms = ev0.elapsed_time(ev1) / kernel_reps  # includes the host read of the totals
+    census = dict(zip(CENSUS_FIELDS, call.census()))
+    flops = algorithmic_flops(census, energy_only=True)
+    evals = pair_evaluations(census)
+    achieved = flops / (score_ms * 1e-3) / 1e12
+    res = {
+        "workload": "repack_16x300",
+        "poses": ps.n_poses,
+        "rotamers": rs.n_rotamers_total,
+        "rotamer_pairs": n_pairs,
+        "value": n_pairs / (op_ms * 1e-3),
+        "unit": "rotamer pairs/s",
+        "ms_per_step": op_ms,
+        "dispatch_ms": dispatch_ms,
+        "score_ms": score_ms,
+        "atom_pair_evals": evals,
+        "atom_pair_evals_per_s": evals / (score_ms * 1e-3),
+        "energies_finite": finite,
+        "dtype": "f32",
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": None,
+            "kernel": "rotamer_score_kernel<float, 0> (FP32 vector ALU)",
+            "algorithmic_flops_per_launch": flops,
+        },
+        "census": census,
+    }
+    if cpu:
+        log("rotamer cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline()
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--kernel-reps", type=int, default=10)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    import torch
+
+    res = run(torch.device("cuda:0"), args.steps, args.warmup, args.kernel_reps,
+              cpu=not args.no_cpu, log=lambda m: print("[rotamer] " + m, file=sys.stderr, flush=True))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -56,11 +56,11 @@ class RotamerSet:
         if self.max_n_rots_per_pose is None:
             self.max_n_rots_per_pose = int(self.n_rots_for_pose.max())
         if self.pose_ind_for_atom is None:
+            # atoms [offset_r, offset_{r+1}) belong to rotamer r's pose
             n = self.coords.shape[0]
-            pifa = torch.zeros((n,), dtype=torch.int32)
-            starts = self.coord_offset_for_rot[self.rot_offset_for_pose[1:].long()].long()
-            pifa[starts] = 1
-            self.pose_ind_for_atom = torch.cumsum(pifa, 0).to(torch.int32)
+            off = torch.cat([self.coord_offset_for_rot.long(), torch.tensor([n])])
+            self.pose_ind_for_atom = torch.repeat_interleave(
+                self.pose_for_rot.to(torch.int32), off[1:] - off[:-1])
 
     @property
     def n_rotamers_total(self):
