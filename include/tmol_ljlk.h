@@ -267,7 +267,7 @@ int tmol_ljlk_prune_compact(int32_t n_blocks, uint32_t* row_count, int32_t* row_
  *                                    ljlk_pose_score.impl.hh:1479-2126
  *   compute_rot_spheres, compute_block_spheres_from_rot_spheres,
  *   detect_block_neighbors, rot_neighbor_indices_from_block_neighbors
- *                                    sphere_overlap.impl.hh:23-107, 287-351, 455-607
+ *                                    sphere_overlap.impl.hh:23-107, 287-351, 455-610
  * Rotamers are numbered globally, grouped by pose and block; the pose
  * stack's bond-separation tables give the count pair between blocks.
  * Block types of at most 64 atoms (ENOTSUP beyond). */

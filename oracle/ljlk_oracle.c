@@ -59,7 +59,7 @@ int oracle_pair_sep(const oracle_stack* st, int p, int b1, int b2, int a1,
 }
 
 /* Rotamer-pair dispatch list from block neighbor flags
- * (rot_neighbor_indices_from_block_neighbors, sphere_overlap.impl.hh:516-607):
+ * (rot_neighbor_indices_from_block_neighbors, sphere_overlap.impl.hh:516-610):
  * cells (p, b1, b2) in row-major order; a flagged diagonal cell contributes
  * its rotamers' self pairs (r, r), an off-diagonal one all n1 * n2 pairs,
  * rot1-major.  Returns the number of entries.  With indices != NULL (room

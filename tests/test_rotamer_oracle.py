@@ -1,7 +1,7 @@
 """CPU tests of the rotamer-pair oracle: the C restatement of
 LJLKRotamerScoreDispatch (tmol/score/ljlk/potentials/ljlk_pose_score.impl.hh:
 1479-2126) and of the rotamer sphere / dispatch helpers
-(tmol/score/common/sphere_overlap.impl.hh:23-107, 287-351, 455-607).
+(tmol/score/common/sphere_overlap.impl.hh:23-107, 287-351, 455-610).
 
 Pinned by:
 * the reference's own CPU op (oracle/_ref, where it builds) -- identical
