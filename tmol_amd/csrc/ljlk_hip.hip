@@ -334,6 +334,7 @@ static RotArgs<Real> make_rot_args(const tmol_ljlk_rotamer_set* r,
   a.records = (const Real*)(w + Lw.records);
   a.conn_pd = (const uint8_t*)(w + Lw.conn_pd);
   a.global_params = (const Real*)btp->global_params;
+  a.rot_spheres = (const Real*)(w + Lw.rot_spheres);
   a.cursor = (uint32_t*)(w + Lw.cursors);
   return a;
 }
@@ -342,9 +343,15 @@ template <typename Real, int MODE>
 static void launch_rot_score(const RotArgs<Real>& a, int n_wg, hipStream_t stream) {
   hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.cursor,
                      int64_t(8 * QSTRIDE), 0u);
-  auto k = rotamer_score_kernel<Real, MODE>;
-  if (n_wg <= 0) n_wg = resident_workgroups(k);
-  hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  if (MODE == 0) {
+    auto k = rotamer_energy_kernel<Real>;
+    if (n_wg <= 0) n_wg = resident_workgroups(k);
+    hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  } else {
+    auto k = rotamer_score_kernel<Real, MODE>;
+    if (n_wg <= 0) n_wg = resident_workgroups(k);
+    hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  }
 }
 
 }  // namespace tmol_ljlk

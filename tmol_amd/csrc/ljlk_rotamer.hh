@@ -271,6 +271,7 @@ struct RotArgs {
   const Real* records;
   const uint8_t* conn_pd;
   const Real* global_params;
+  const Real* rot_spheres;  // [R, 4] centre, radius (from the neighbor pass)
   const RotUnit* units;
   int64_t n_units;
   int64_t n_pairs;
@@ -721,6 +722,366 @@ __global__ __launch_bounds__(256) void rotamer_score_kernel(RotArgs<Real> A) {
       const unsigned long long v = wave_sum((unsigned long long)cnt[k]);
       if (lane == 0 && v) atomicAdd(&A.census[k], v);
     }
+  }
+}
+
+
+// ------------------------------------------------------------------ energies, culled
+
+// Energy forward with culling and compaction.  Per b1 rotamer r1 of a unit:
+//   cull      every atom of b2's rotamers (the candidates, taken 64 at a time
+//             in rotamer order) is tested against r1's sphere: an atom
+//             farther than R1 + max_dis from r1's centre reaches no atom of
+//             r1 (triangle inequality; 1e-3 A margin for rounding).
+//             Survivors are compacted, in order, into a 64-lane chunk.
+//   phase A   for a full (or the last) chunk: each lane holds one surviving
+//             atom, a1 runs over r1's atoms (LDS broadcasts); pairs within
+//             max_dis and of non-zero count-pair weight are ballot-compacted
+//             into an LJ FIFO, heavy-heavy ones also into an LK FIFO.
+//   phase B   whenever a FIFO holds 64 entries, and at the end of a chunk:
+//             one entry per lane, every lane busy.  Energies go to the
+//             chunk lane's 64-bit fixed-point LDS accumulator (integer LDS
+//             atomics; exact, order-independent).
+//   fold      at the end of a chunk, each lane's sums are added per rotamer
+//             (segments of contiguous lanes, segmented scan) into LDS sums
+//             of the rotamers of the current window of 64 b2 rotamers.
+//   write     after the window, one store per rotamer pair and term (zero
+//             for pairs with no surviving atoms).
+
+constexpr int RFIFO = 2 * WAVE;  // FIFO capacity: < 64 left over + 64 appended
+
+template <typename Real>
+struct RotLDS2 {
+  RotLDS<Real> r1;  // staged atoms of the b1 rotamer, cell separations, marks
+  alignas(4 * sizeof(Real)) Real pos2[WAVE][4];  // chunk: x, y, z, lj_radius
+  alignas(4 * sizeof(Real)) Real prm2[WAVE][4];  // sqrt(lj_wdepth), K, 1/lambda^2, lk_volume
+  int meta2[WAVE];       // flags | n_conn << 8 | ligand << 11 | slot << 16
+  uint32_t pd2[WAVE];    // packed path distances to the connection atoms
+  uint16_t lj[RFIFO];    // entry: a1 | lane << 6 | weight-0.2 << 12
+  uint16_t lk[RFIFO];
+  typename LdsAcc<Real>::T lacc[3][WAVE];  // per chunk lane: atr, rep, lk
+  Real sum[3][WAVE];     // per rotamer of the window
+};
+
+template <typename Real>
+__device__ __forceinline__ void rot_fifo_lj(RotLDS2<Real>& L, int consume,
+                                            const GlobalParams<Real>& gp, int lane) {
+  using V = typename V4<Real>::T;
+  using Acc = LdsAcc<Real>;
+  if (lane < consume) {
+    const int e = L.lj[lane];
+    const int a1 = e & 63, l2 = (e >> 6) & 63;
+    const Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
+    const V p1 = *(const V*)L.r1.pos[a1], p2 = *(const V*)L.pos2[l2];
+    const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
+    const Real d2 = dx * dx + dy * dy + dz * dz;
+    const Real inv_d = fast_rsqrt(d2);
+    const Real dist = d2 > Real(0) ? d2 * inv_d : Real(0);
+    const Real sigma = lj_sigma(L.r1.flg[a1], L.meta2[l2] & 255, p1.w, p2.w, gp);
+    const Real eps = L.r1.prm[a1][0] * L.prm2[l2][0];
+    Real atr, rep, dlj;
+    lj_pair<Real, false>(dist, sigma, eps, wcp, gp, atr, rep, dlj);
+    atomicAdd(&L.lacc[0][l2], Acc::enc(atr));
+    atomicAdd(&L.lacc[1][l2], Acc::enc(rep));
+  }
+}
+
+template <typename Real>
+__device__ __forceinline__ void rot_fifo_lk(RotLDS2<Real>& L, int consume,
+                                            const GlobalParams<Real>& gp, int lane) {
+  using V = typename V4<Real>::T;
+  using Acc = LdsAcc<Real>;
+  if (lane < consume) {
+    const int e = L.lk[lane];
+    const int a1 = e & 63, l2 = (e >> 6) & 63;
+    const Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
+    const V p1 = *(const V*)L.r1.pos[a1], p2 = *(const V*)L.pos2[l2];
+    const V q1 = *(const V*)L.r1.prm[a1], q2 = *(const V*)L.prm2[l2];
+    const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
+    const Real d2 = dx * dx + dy * dy + dz * dz;
+    const Real inv_d = fast_rsqrt(d2);
+    const Real dist = d2 > Real(0) ? d2 * inv_d : Real(0);
+    const int f1 = L.r1.flg[a1], f2 = L.meta2[l2] & 255;
+    const Real sigma = lj_sigma(f1, f2, p1.w, p2.w, gp);
+    const bool cc = (f1 & F_CARBON_LK) && (f2 & F_CARBON_LK);
+    Real d_min, cdmin, cdmax;
+    lk_knots(sigma, cc, d_min, cdmin, cdmax);
+    Real va, dva, vb, dvb;
+    lk_dir<Real, false>(dist, d_min, cdmin, cdmax, -q2.w * q1.y, p1.w, q1.z, gp, va, dva);
+    lk_dir<Real, false>(dist, d_min, cdmin, cdmax, -q1.w * q2.y, p2.w, q2.z, gp, vb, dvb);
+    atomicAdd(&L.lacc[2][l2], Acc::enc(wcp * (va + vb)));
+  }
+}
+
+// consume the first `c` entries of a FIFO of n and move the rest to the front
+__device__ __forceinline__ void rot_fifo_shift(uint16_t* f, int c, int n, int lane) {
+  const int rem = n - c;
+  if (rem <= 0) return;
+  uint16_t v = 0;
+  if (lane < rem) v = f[c + lane];
+  wave_sync();
+  if (lane < rem) f[lane] = v;
+  wave_sync();
+}
+
+// phase A + B over the chunk's nb lanes against r1 (n1 atoms staged), then
+// fold the lanes' sums into the window's per-rotamer sums
+template <typename Real>
+__device__ void rot_chunk(RotLDS2<Real>& L, const GlobalParams<Real>& gp, int nb, int n1,
+                          bool cp, int min_sep, int nc1, bool lig1, const AtomRegs<Real>& R1,
+                          int lane) {
+  using V = typename V4<Real>::T;
+  using Acc = LdsAcc<Real>;
+  wave_sync();
+  const bool valid = lane < nb;
+  V p2 = {};
+  int meta = 0;
+  uint32_t pd2 = 0;
+  if (valid) {
+    p2 = *(const V*)L.pos2[lane];
+    meta = L.meta2[lane];
+    pd2 = L.pd2[lane];
+  }
+  const bool heavy2 = meta & F_HEAVY;
+  const int nc2 = (meta >> 8) & 7;
+  const bool lig = lig1 && ((meta >> 11) & 1);
+  L.lacc[0][lane] = 0;
+  L.lacc[1][lane] = 0;
+  L.lacc[2][lane] = 0;
+  int nlj = 0, nlk = 0;  // wave-uniform FIFO fill
+  // a1's data come from the registers of the lane that staged it
+  // (wave-uniform readlane), so the test chain has no LDS latency
+  for (int j = 0; j < n1; ++j) {
+    const Real dx = lane_val(R1.x, j) - p2.x, dy = lane_val(R1.y, j) - p2.y,
+               dz = lane_val(R1.z, j) - p2.z;
+    const Real d2 = dx * dx + dy * dy + dz * dz;
+    bool in = valid && d2 < gp.max_dis2;
+    bool w02 = false;
+    if (cp) {
+      const int sep = packed_pair_sep((uint32_t)lane_val((int)R1.pd, j), pd2, L.r1.cs, min_sep,
+                                      nc1, nc2, lig);
+      in = in && sep >= 4;
+      w02 = sep == 4;
+    }
+    const uint64_t m = __ballot(in);
+    if (m == 0) continue;
+    const uint16_t ent = (uint16_t)(j | (lane << 6) | (w02 ? (1 << 12) : 0));
+    if (in) L.lj[nlj + lanes_below(m)] = ent;
+    nlj += __popcll(m);
+    if (lane_val(R1.flags, j) & F_HEAVY) {
+      const uint64_t mh = __ballot(in && heavy2);
+      if (in && heavy2) L.lk[nlk + lanes_below(mh)] = ent;
+      nlk += __popcll(mh);
+    }
+    if (nlj >= WAVE) {
+      wave_sync();
+      rot_fifo_lj(L, WAVE, gp, lane);
+      rot_fifo_shift(L.lj, WAVE, nlj, lane);
+      nlj -= WAVE;
+    }
+    if (nlk >= WAVE) {
+      wave_sync();
+      rot_fifo_lk(L, WAVE, gp, lane);
+      rot_fifo_shift(L.lk, WAVE, nlk, lane);
+      nlk -= WAVE;
+    }
+  }
+  wave_sync();
+  if (nlj > 0) rot_fifo_lj(L, nlj, gp, lane);
+  if (nlk > 0) rot_fifo_lk(L, nlk, gp, lane);
+  wave_sync();
+  // fold: lanes of one rotamer (slot) are contiguous
+  const int slot = meta >> 16;
+  const int prev = lane > 0 ? L.meta2[lane - 1] >> 16 : -1;
+  const uint64_t starts = __ballot(valid && (lane == 0 || slot != prev));
+  const uint64_t below = starts & ((2ull << lane) - 1ull);
+  const int seg_start = below ? 63 - __builtin_clzll(below) : 0;
+  const bool last = valid && (lane == nb - 1 || (L.meta2[lane + 1 < WAVE ? lane + 1 : lane] >> 16) != slot);
+  Real e0 = valid ? Acc::dec(L.lacc[0][lane]) : Real(0);
+  Real e1 = valid ? Acc::dec(L.lacc[1][lane]) : Real(0);
+  Real e2 = valid ? Acc::dec(L.lacc[2][lane]) : Real(0);
+  e0 = segment_scan(e0, seg_start, lane);
+  e1 = segment_scan(e1, seg_start, lane);
+  e2 = segment_scan(e2, seg_start, lane);
+  if (last) {
+    L.sum[0][slot] += e0;
+    L.sum[1][slot] += e1;
+    L.sum[2][slot] += e2;
+  }
+  wave_sync();
+}
+
+// inter-block unit, energies only
+template <typename Real>
+__device__ void rot_inter_energy(const RotArgs<Real>& A, RotLDS2<Real>& L,
+                                 const GlobalParams<Real>& gp, int g1, int g2, int b2,
+                                 int r1_first, int r1_count, int64_t entry, int lane) {
+  using V = typename V4<Real>::T;
+  const RotView& v = A.v;
+  const int nr2 = uni(v.n_rots_for_block[g2]), off2 = uni(v.rot_offset_for_block[g2]);
+  const int min_sep = uni(v.min_bondsep[int64_t(g1) * v.B + b2]);
+  const bool cp = min_sep <= 4;
+  if (cp) {
+    wave_sync();
+    if (lane < MAX_CONN * MAX_CONN) {
+      const int c1 = lane / MAX_CONN, c2 = lane - c1 * MAX_CONN;
+      L.r1.cs[lane] = (c1 < v.C && c2 < v.C)
+                          ? (uint8_t)v.inter_block_bondsep[((int64_t(g1) * v.B + b2) * v.C + c1) *
+                                                               v.C + c2]
+                          : (uint8_t)6;
+    }
+    wave_sync();
+  }
+  const int64_t n = A.n_pairs;
+  for (int w0 = 0; w0 < nr2; w0 += WAVE) {
+    const int wn = min(WAVE, nr2 - w0);
+    // the window's rotamers (lane = rotamer): type, first atom, atom count,
+    // first candidate; candidates fetch theirs by lane shuffle
+    int nat = 0, t2l = -1, o2l = 0;
+    if (lane < wn) {
+      t2l = v.block_type_for_rot[off2 + w0 + lane];
+      o2l = v.rot_coord_offset[off2 + w0 + lane];
+      nat = t2l >= 0 ? A.bt.n_atoms[t2l] : 0;
+    }
+    int incl = nat;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int t = __shfl_up(incl, o, WAVE);
+      if (lane >= o) incl += t;
+    }
+    const int excl = incl - nat;
+    const int total = lane_val(incl, wn - 1);
+    for (int i = 0; i < r1_count; ++i) {
+      const int r1 = r1_first + i;
+      const int t1 = uni(v.block_type_for_rot[r1]);
+      if (t1 < 0) {
+        zero_entries(A, entry + int64_t(i) * nr2 + w0, wn, lane);
+        continue;
+      }
+      const int o1 = uni(v.rot_coord_offset[r1]);
+      const int n1 = uni(A.bt.n_atoms[t1]);
+      const int nc1 = uni(A.bt.n_conn[t1]);
+      const bool lig1 = A.bt.is_ligand[t1];
+      const Real* sph = A.rot_spheres + int64_t(r1) * 4;
+      const Real cx = sph[0], cy = sph[1], cz = sph[2];
+      const Real reach = sph[3] + gp.max_dis + Real(1e-3);
+      const Real reach2 = reach * reach;
+      AtomRegs<Real> R1{};
+      wave_sync();
+      if (lane < n1) {
+        load_rot_atom(R1, A, t1, o1, lane);
+        V p, q;
+        p.x = R1.x;
+        p.y = R1.y;
+        p.z = R1.z;
+        p.w = R1.r0.x;
+        q.x = R1.r0.y;
+        q.y = R1.r0.z;
+        q.z = R1.r0.w;
+        q.w = R1.vol;
+        *(V*)L.r1.pos[lane] = p;
+        *(V*)L.r1.prm[lane] = q;
+        L.r1.flg[lane] = R1.flags;
+      }
+      L.sum[0][lane] = 0;
+      L.sum[1][lane] = 0;
+      L.sum[2][lane] = 0;
+      int nb = 0;  // surviving candidates in the chunk (wave-uniform)
+      for (int c0 = 0; c0 < total; c0 += WAVE) {
+        // candidate c = c0 + lane: its rotamer (slot) and atom
+        const int c = c0 + lane;
+        const uint64_t before = __ballot(lane < wn && excl <= c0 && nat > 0);
+        const int carry = before ? 63 - __builtin_clzll(before) : 0;
+        wave_sync();
+        L.r1.mark[lane] = 0;
+        wave_sync();
+        if (lane < wn && nat > 0 && excl > c0 && excl < c0 + WAVE) L.r1.mark[excl - c0] = (uint8_t)(lane + 1);
+        wave_sync();
+        const uint64_t st = __ballot(L.r1.mark[lane] != 0);
+        const uint64_t bl = st & ((2ull << lane) - 1ull);
+        const int slot = bl ? int(L.r1.mark[63 - __builtin_clzll(bl)]) - 1 : carry;
+        const bool cand = c < total;
+        const int a2 = c - __shfl(excl, slot, WAVE);
+        const int t2 = __shfl(t2l, slot, WAVE);
+        const int o2 = __shfl(o2l, slot, WAVE);
+        Real x = 0, y = 0, z = 0;
+        bool keep = false;
+        if (cand) {
+          const Real* xyz = A.coords + (int64_t(o2) + a2) * 3;
+          x = xyz[0];
+          y = xyz[1];
+          z = xyz[2];
+          const Real ex = x - cx, ey = y - cy, ez = z - cz;
+          keep = ex * ex + ey * ey + ez * ez < reach2;
+        }
+        const uint64_t mk = __ballot(keep);
+        const int nk = __popcll(mk);
+        if (nk == 0) continue;
+        if (nb + nk > WAVE) {
+          rot_chunk(L, gp, nb, n1, cp, min_sep, nc1, lig1, R1, lane);
+          nb = 0;
+        }
+        if (keep) {
+          const int pos = nb + lanes_below(mk);
+          const Real* rec = A.records + (int64_t(t2) * A.bt.Amax + a2) * REC;
+          const V r0 = *(const V*)rec;
+          const V r1v = *(const V*)(rec + 4);
+          V p, q;
+          p.x = x;
+          p.y = y;
+          p.z = z;
+          p.w = r0.x;
+          q.x = r0.y;
+          q.y = r0.z;
+          q.z = r0.w;
+          q.w = r1v.x;
+          *(V*)L.pos2[pos] = p;
+          *(V*)L.prm2[pos] = q;
+          L.meta2[pos] = int(r1v.y) | (A.bt.n_conn[t2] << 8) | ((A.bt.is_ligand[t2] ? 1 : 0) << 11) |
+                         (slot << 16);
+          L.pd2[pos] = *(const uint32_t*)(A.conn_pd + (int64_t(t2) * A.bt.Amax + a2) * MAX_CONN);
+        }
+        nb += nk;
+      }
+      if (nb > 0) rot_chunk(L, gp, nb, n1, cp, min_sep, nc1, lig1, R1, lane);
+      wave_sync();
+      if (lane < wn) {
+        const int64_t e = entry + int64_t(i) * nr2 + w0 + lane;
+        A.energies[e] = L.sum[0][lane];
+        A.energies[n + e] = L.sum[1][lane];
+        A.energies[2 * n + e] = L.sum[2][lane];
+      }
+    }
+  }
+}
+
+// Energy kernel: inter-block units culled and compacted, self pairs as in
+// rotamer_score_kernel.
+template <typename Real>
+__global__ __launch_bounds__(256) void rotamer_energy_kernel(RotArgs<Real> A) {
+  __shared__ RotLDS2<Real> lds[WAVES_PER_WG];
+  const int lane = lane_id();
+  RotLDS2<Real>& L = lds[threadIdx.x / WAVE];
+  const GlobalParams<Real> gp = load_global_params(A.global_params);
+  uint32_t cnt[1];
+  RowQueue queue(A.cursor, A.n_units);
+  for (int64_t u = queue.next(); u >= 0; u = queue.next()) {
+    const RotUnit* up = A.units + u;
+    const int g1 = uni(up->g1), b2 = uni(up->b2);
+    const int r1_first = uni(up->r1_first), r1_count = uni(up->r1_count);
+    const int64_t entry = uni64(up->entry);
+    const int p = g1 / A.v.B, b1 = g1 - p * A.v.B;
+    if (r1_count <= 0) {
+      if (r1_count < 0)
+        zero_entries(A, entry,
+                     int64_t(-r1_count) * (b1 == b2 ? 1 : uni(A.v.n_rots_for_block[p * A.v.B + b2])),
+                     lane);
+      continue;
+    }
+    if (b1 == b2)
+      rot_intra<Real, 0>(A, L.r1, gp, r1_first, r1_count, entry, lane, cnt);
+    else
+      rot_inter_energy<Real>(A, L, gp, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry, lane);
   }
 }
 
