@@ -141,7 +141,7 @@ def run(device, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
             "traffic": None,
-            "kernel": "rotamer_score_kernel<float, 0> (FP32 vector ALU)",
+            "kernel": "rotamer_energy_kernel<float> (FP32 vector ALU)",
             "algorithmic_flops_per_launch": flops,
         },
         "census": census,
