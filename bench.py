@@ -283,6 +283,10 @@ def main():
         log("rotamer pairs ...")
         result["rotamer"] = rotamer_bench.run(dev, steps=10, warmup=2, cpu=not args.no_cpu, log=log)
         result["rotamer"].pop("census", None)
+    from tmol_amd import topology_bench
+
+    log("topology ...")
+    result["topology"] = topology_bench.run(dev, cpu=not args.no_cpu, log=log)
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = cpu_baseline(poses, args.cpu_stride)

@@ -341,6 +341,24 @@ int tmol_ljlk_rotamer_spheres(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_b
                               int32_t dtype, void* workspace, void** rot_spheres,
                               void** block_spheres);
 
+/* ---------------------------------------------------------------------
+ * Pose topology on the device.  inter_block_bondsep [P, B, B, C, C] (int32;
+ * 6 = six bonds or more, and for connection pairs that do not exist) and,
+ * if non-NULL, min_bondsep [P, B, B] (minimum over the [C, C] entries), from
+ * block_type [P, B] (-1 = no block) and inter_residue_connections
+ * [P, B, C, 2] ((block, connection) partner, -1 = none; both directions
+ * stored): shortest bond paths between connection atoms, capped at 6.
+ * C = bt->max_n_conn.  Replaces the reference's connection-point graph and
+ * all-pairs shortest paths (pose_stack_builder.py:459-599, 1230-1349;
+ * compiled/apsp.cpu.cpp) and the min over connection pairs
+ * (bond_dependent_term.py:91-100).  overflow (device, may be NULL) is set
+ * when a connection point reaches more than 48 points within 5 bonds. */
+int tmol_ljlk_inter_block_bondsep(const tmol_ljlk_block_types* bt, int32_t n_poses,
+                                  int32_t max_n_blocks, const int32_t* block_type,
+                                  const int32_t* inter_residue_connections,
+                                  int32_t* inter_block_bondsep, int32_t* min_bondsep,
+                                  uint32_t* overflow, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

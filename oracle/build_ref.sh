@@ -1,6 +1,8 @@
 #!/bin/bash
-# Build the reference's own CPU LJ+LK op (tmol_ljlk::ljlk_pose_scores) from the
-# sources where they lie under the reference checkout, into oracle/_ref/.
+# Build the reference's own CPU LJ+LK op (tmol_ljlk::ljlk_pose_scores and
+# ljlk_rotamer_scores) and its all-pairs-shortest-paths op (tmol_apsp::apsp_op)
+# from the sources where they lie under the reference checkout, into
+# oracle/_ref/.
 # Test infrastructure only: used to cross-check the C oracle and as the CPU
 # baseline in bench.py.  Needs only g++, the installed torch and the headers
 # the reference vendors (tmol/extern: Eigen, moderngpu).  Nothing is copied.
@@ -26,3 +28,15 @@ g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
   -o "$OUT.tmp"
 mv "$OUT.tmp" "$OUT"
 echo "built $OUT"
+# the reference's all-pairs-shortest-paths op (tmol_apsp::apsp_op), which
+# turns the connection-point graph into inter_block_bondsep
+OUT2="$HERE/_ref/tmol_apsp_ref.so"
+g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
+  -DTORCH_EXTENSION_NAME=tmol_apsp_ref \
+  -I"$R" -I"$R/tmol/extern" $TI -I"$PYI" \
+  "$R/tmol/pose/compiled/apsp_vestibule.ops.cpp" \
+  "$R/tmol/pose/compiled/apsp.cpu.cpp" \
+  -L"$TL" -ltorch -ltorch_cpu -lc10 -Wl,-rpath,"$TL" \
+  -o "$OUT2.tmp"
+mv "$OUT2.tmp" "$OUT2"
+echo "built $OUT2"

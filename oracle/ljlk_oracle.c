@@ -4,6 +4,7 @@
  * (no fused multiply-add, like the reference's x86-64 CPU build). */
 #include <math.h>
 #include <stdint.h>
+#include <stdlib.h>
 
 typedef struct {
   int32_t n_poses, max_n_blocks;
@@ -101,4 +102,90 @@ int64_t oracle_rot_dispatch(int P, int B, const int32_t* neighbors,
         total += cnt;
       }
   return total;
+}
+
+/* inter_block_bondsep [P, B, B, C, C] and min_bondsep [P, B, B] from block
+ * connectivity, restating the reference's pose-stack construction
+ * (tmol/pose/pose_stack_builder.py:459-599, 1230-1349;
+ * tmol/score/bond_dependent_term.py:91-100):
+ *  - one graph node per connection point of every real block (numbered
+ *    block by block);
+ *  - two connection points of one block: the path distance between their
+ *    atoms; two points joined by an inter-residue connection: 1; otherwise
+ *    6 (no edge); the weight of a node pair is the smaller of its two
+ *    directions;
+ *  - shortest paths capped at 6 (the reference runs a cutoff-6 Dijkstra,
+ *    tmol/pose/compiled/apsp.cpu.cpp; Floyd-Warshall here, which gives the
+ *    same capped distances);
+ *  - entries of non-existent connection pairs stay 6; min_bondsep is the
+ *    minimum over all [C, C] entries.
+ * irc: inter-residue connections [P, B, C, 2] = (block, connection), -1 = none. */
+void oracle_inter_block_bondsep(int P, int B, int C, int Amax, const int32_t* block_type,
+                                const int32_t* irc, const int32_t* bt_n_conn,
+                                const int32_t* bt_conn_atoms,
+                                const int32_t* bt_path_distance, int32_t* ibs,
+                                int32_t* min_sep) {
+  const int CUT = 6;
+  int* off = (int*)malloc(sizeof(int) * (B + 1));
+  for (int p = 0; p < P; ++p) {
+    int n = 0;
+    for (int b = 0; b < B; ++b) {
+      off[b] = n;
+      int bt = block_type[p * B + b];
+      n += bt >= 0 ? bt_n_conn[bt] : 0;
+    }
+    off[B] = n;
+    int* W = (int*)malloc(sizeof(int) * (size_t)(n > 0 ? n * n : 1));
+    for (int64_t k = 0; k < (int64_t)n * n; ++k) W[k] = CUT;
+    for (int b = 0; b < B; ++b) {
+      int bt = block_type[p * B + b];
+      if (bt < 0) continue;
+      int nc = bt_n_conn[bt];
+      for (int c1 = 0; c1 < nc; ++c1)
+        for (int c2 = 0; c2 < nc; ++c2) {
+          int a1 = bt_conn_atoms[bt * C + c1], a2 = bt_conn_atoms[bt * C + c2];
+          W[(off[b] + c1) * n + off[b] + c2] =
+              bt_path_distance[((int64_t)bt * Amax + a1) * Amax + a2];
+        }
+      for (int c = 0; c < C; ++c) {
+        const int32_t* e = irc + (((int64_t)p * B + b) * C + c) * 2;
+        if (e[0] < 0) continue;
+        W[(off[b] + c) * n + off[e[0]] + e[1]] = 1;
+      }
+    }
+    for (int i = 0; i < n; ++i)
+      for (int j = i + 1; j < n; ++j) {
+        int w = W[i * n + j] < W[j * n + i] ? W[i * n + j] : W[j * n + i];
+        W[i * n + j] = W[j * n + i] = w;
+      }
+    for (int i = 0; i < n; ++i) W[i * n + i] = 0;
+    for (int k = 0; k < n; ++k)
+      for (int i = 0; i < n; ++i) {
+        int wik = W[i * n + k];
+        if (wik >= CUT) continue;
+        for (int j = 0; j < n; ++j) {
+          int v = wik + W[k * n + j];
+          if (v < W[i * n + j]) W[i * n + j] = v;
+        }
+      }
+    for (int b1 = 0; b1 < B; ++b1)
+      for (int b2 = 0; b2 < B; ++b2) {
+        int m = CUT;
+        int bt1 = block_type[p * B + b1], bt2 = block_type[p * B + b2];
+        int nc1 = bt1 >= 0 ? bt_n_conn[bt1] : 0, nc2 = bt2 >= 0 ? bt_n_conn[bt2] : 0;
+        for (int c1 = 0; c1 < C; ++c1)
+          for (int c2 = 0; c2 < C; ++c2) {
+            int v = CUT;
+            if (c1 < nc1 && c2 < nc2) {
+              v = W[(off[b1] + c1) * n + off[b2] + c2];
+              if (v > CUT) v = CUT;
+            }
+            ibs[((((int64_t)p * B + b1) * B + b2) * C + c1) * C + c2] = v;
+            if (v < m) m = v;
+          }
+        min_sep[((int64_t)p * B + b1) * B + b2] = m;
+      }
+    free(W);
+  }
+  free(off);
 }

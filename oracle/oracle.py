@@ -81,6 +81,8 @@ def lib():
         L.oracle_rot_dispatch.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P,
                                           ctypes.c_int64]
         L.oracle_rot_dispatch.restype = ctypes.c_int64
+        L.oracle_inter_block_bondsep.argtypes = [
+            ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _P, _P, _P, _P, _P, _P, _P]
         L.oracle_pair_sep.argtypes = [
             ctypes.POINTER(OracleStack), ctypes.c_int, ctypes.c_int, ctypes.c_int,
             ctypes.c_int, ctypes.c_int,
@@ -276,6 +278,39 @@ def oracle_rotamer_dispatch(*args, dtype=None):
     scoring; ``oracle.score(idx_subset)`` scores any listed pairs."""
     o = _RotOracle(args, dtype)
     return (o,) + o.dispatch()
+
+
+def oracle_inter_block_bondsep(pbt, block_type, irc):
+    """(inter_block_bondsep [P, B, B, C, C], min_block_bondsep [P, B, B]) of
+    the connection graph, restating the reference's pose-stack construction
+    (pose_stack_builder.py:459-599, 1230-1349)."""
+    L = lib()
+    bt = _np_i32(block_type)
+    ir = _np_i32(irc)
+    P, B = bt.shape
+    n_conn = _np_i32(pbt.n_conn)
+    conn = _np_i32(pbt.conn_atom)
+    pd = _np_i32(pbt.ljlk_bond_separation)
+    C = conn.shape[1]
+    Amax = pd.shape[1]
+    ibs = numpy.zeros((P, B, B, C, C), dtype=numpy.int32)
+    mins = numpy.zeros((P, B, B), dtype=numpy.int32)
+    I = lambda a: _ptr(a, ctypes.c_int32)
+    L.oracle_inter_block_bondsep(P, B, C, Amax, I(bt), I(ir), I(n_conn), I(conn), I(pd), I(ibs),
+                                 I(mins))
+    return ibs, mins
+
+
+def load_reference_apsp_op():
+    """The reference's own all-pairs-shortest-paths op
+    (tmol_apsp::apsp_op, tmol/pose/compiled/apsp.cpu.cpp) built by
+    build_ref.sh."""
+    path = os.path.join(HERE, "_ref", "tmol_apsp_ref.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    if not hasattr(torch.ops, "tmol_apsp") or not hasattr(torch.ops.tmol_apsp, "apsp_op"):
+        torch.ops.load_library(path)
+    return torch.ops.tmol_apsp.apsp_op
 
 
 def load_reference_rotamer_op():

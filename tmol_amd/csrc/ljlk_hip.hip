@@ -24,6 +24,7 @@
 #include "ljlk_score_rows.hh"
 #include "ljlk_prune.hh"
 #include "ljlk_rotamer.hh"
+#include "ljlk_topology.hh"
 
 namespace tmol_ljlk {
 
@@ -836,6 +837,33 @@ int tmol_ljlk_rotamer_spheres(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_b
   *rot_spheres = (char*)workspace + L.rot_spheres;
   *block_spheres = (char*)workspace + L.block_spheres;
   return TMOL_LJLK_OK;
+}
+
+int tmol_ljlk_inter_block_bondsep(const tmol_ljlk_block_types* bt, int32_t n_poses,
+                                  int32_t max_n_blocks, const int32_t* block_type,
+                                  const int32_t* inter_residue_connections,
+                                  int32_t* inter_block_bondsep, int32_t* min_bondsep,
+                                  uint32_t* overflow, void* stream) {
+  if (!bt || !block_type || !inter_residue_connections || !inter_block_bondsep)
+    return TMOL_LJLK_EINVAL;
+  if (n_poses <= 0 || max_n_blocks <= 0) return TMOL_LJLK_EINVAL;
+  if (bt->max_n_conn <= 0 || bt->max_n_conn > MAX_CONN) return TMOL_LJLK_ENOTSUP;
+  if (int64_t(max_n_blocks) >= (int64_t(1) << 27)) return TMOL_LJLK_ENOTSUP;
+  TopoView v{n_poses, max_n_blocks, bt->max_n_conn, block_type, inter_residue_connections};
+  BTView b = make_bt_view(bt);
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t PB = int64_t(n_poses) * max_n_blocks;
+  const int64_t n_ibs = PB * max_n_blocks * v.C * v.C;
+  hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(n_ibs)), dim3(256), 0, st,
+                     inter_block_bondsep, n_ibs, int32_t(BS_CUT));
+  if (overflow)
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, st, overflow, int64_t(1), 0u);
+  hipLaunchKernelGGL(bondsep_rows_kernel, dim3(grid_for(PB * v.C)), dim3(256), 0, st, v, b,
+                     inter_block_bondsep, (int32_t*)overflow);
+  if (min_bondsep)
+    hipLaunchKernelGGL(bondsep_min_kernel, dim3(grid_for(PB * max_n_blocks)), dim3(256), 0, st, v,
+                       (const int32_t*)inter_block_bondsep, min_bondsep);
+  return check(hipGetLastError());
 }
 
 }  // extern "C"
