@@ -17,6 +17,7 @@ own CPU op (oracle/_ref, one thread) on the first pose as the CPU baseline.
 
 import argparse
 import json
+import os
 import sys
 import time
 
@@ -120,6 +121,14 @@ def run(device, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
     flops = algorithmic_flops(census, energy_only=True)
     evals = pair_evaluations(census)
     achieved = flops / (score_ms * 1e-3) / 1e12
+    traffic = None
+    prof = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
+                        "rotamer_kernel_repack16x300_traffic.json")
+    if os.path.exists(prof):
+        try:
+            traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
     res = {
         "workload": "repack_16x300",
         "poses": ps.n_poses,
@@ -140,7 +149,7 @@ def run(device, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "rotamer_energy_kernel<float> (FP32 vector ALU)",
             "algorithmic_flops_per_launch": flops,
         },
