@@ -92,7 +92,7 @@ static int run_neighbors(const tmol_ljlk_stack* st, const tmol_ljlk_block_types*
     hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream, dense, n, 0);
   }
   if (n_tiles > 0)
-    hipLaunchKernelGGL(neighbors_kernel<Real>, dim3((unsigned)n_tiles), dim3(256), 0, stream, s,
+    hipLaunchKernelGGL(neighbor_rows_kernel<Real>, dim3((unsigned)n_tiles), dim3(256), 0, stream, s,
                        (const int4*)tiles, n_tiles, spheres, (Real)max_dis, row_count, row_items,
                        row_cap, counters, dense);
   return check(hipGetLastError());

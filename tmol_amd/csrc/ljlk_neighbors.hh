@@ -184,6 +184,130 @@ __global__ __launch_bounds__(256) void neighbors_kernel(
   if (total > 0 && lane == 0) atomicAdd(&counters[0], (uint32_t)total);
 }
 
+// One workgroup per 64-block chunk of a pose: the diagonal tiles of the tile
+// list do the work, the other tiles return at once.  The chunk's rows are
+// tested against every column chunk from itself to the end of the pose, in
+// column order.  A column chunk is skipped when the bounding boxes of the
+// two chunks' spheres (centres grown by radii) are farther apart than reach
+// + 1e-3 A: then every sphere pair is, and no test can pass.  Wave w owns
+// rows 16w..16w+15 and appends to them in column order with counters in
+// registers, so the lists need no atomics and come out sorted by g2.
+// Same test, same work-list contents and dense flags as neighbors_kernel.
+template <typename Real>
+__device__ __forceinline__ void chunk_box(const Real (*sph)[4], const int* typ, int lane,
+                                          Real* lo, Real* hi) {
+  const bool ok = typ[lane] >= 0;
+  const Real big = Real(1e30);
+#pragma unroll
+  for (int k = 0; k < 3; ++k) {
+    Real l = ok ? sph[lane][k] - sph[lane][3] : big;
+    Real h = ok ? sph[lane][k] + sph[lane][3] : -big;
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+      l = fmin(l, __shfl_xor(l, o, WAVE));
+      h = fmax(h, __shfl_xor(h, o, WAVE));
+    }
+    lo[k] = l;
+    hi[k] = h;
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(256) void neighbor_rows_kernel(
+    StackView s, const int4* __restrict__ tiles, int64_t n_tiles,
+    const Real* __restrict__ spheres, Real reach, uint32_t* __restrict__ row_count,
+    int32_t* __restrict__ row_items, int64_t row_cap, uint32_t* __restrict__ counters,
+    int32_t* __restrict__ dense) {
+#pragma clang fp contract(off)
+  __shared__ Real sph_r[NBR_TILE][4];
+  __shared__ Real sph_c[NBR_TILE][4];
+  __shared__ int typ_r[NBR_TILE];
+  __shared__ int typ_c[NBR_TILE];
+  const int64_t tile_id = blockIdx.x;
+  if (tile_id >= n_tiles) return;
+  const int4 tl = tiles[tile_id];
+  if (tl.x != tl.y) return;
+  const int row0 = tl.x, end = tl.z;
+  const int tid = threadIdx.x;
+  const int lane = lane_id();
+  const int wave = tid / WAVE;
+  if (tid < NBR_TILE) {
+    const int g = row0 + tid;
+    const bool ok = g < end;
+    for (int k = 0; k < 4; ++k) sph_r[tid][k] = ok ? spheres[int64_t(g) * 4 + k] : Real(0);
+    typ_r[tid] = ok ? s.block_type[g] : -1;
+  }
+  __syncthreads();
+  Real rlo[3], rhi[3];
+  chunk_box(sph_r, typ_r, lane, rlo, rhi);
+  const Real pad = reach + Real(1e-3);
+  uint32_t cnt[16];
+#pragma unroll
+  for (int r = 0; r < 16; ++r) cnt[r] = 0;
+  bool overflow = false;
+  for (int col0 = row0; col0 < end; col0 += NBR_TILE) {
+    __syncthreads();  // the previous column chunk is no longer read
+    if (tid < NBR_TILE) {
+      const int g = col0 + tid;
+      const bool ok = g < end;
+      for (int k = 0; k < 4; ++k) sph_c[tid][k] = ok ? spheres[int64_t(g) * 4 + k] : Real(0);
+      typ_c[tid] = ok ? s.block_type[g] : -1;
+    }
+    __syncthreads();
+    Real clo[3], chi[3];
+    chunk_box(sph_c, typ_c, lane, clo, chi);
+    Real gap2 = Real(0);
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+      const Real gk = fmax(Real(0), fmax(clo[k] - rhi[k], rlo[k] - chi[k]));
+      gap2 += gk * gk;
+    }
+    if (gap2 > pad * pad) continue;  // uniform across the workgroup
+    const int g2 = col0 + lane;
+    const Real c0 = sph_c[lane][0], c1 = sph_c[lane][1], c2 = sph_c[lane][2], c3 = sph_c[lane][3];
+    const bool col_ok = typ_c[lane] >= 0;
+#pragma unroll
+    for (int r = 0; r < 16; ++r) {
+      const int lr = wave * 16 + r;
+      const int g1 = row0 + lr;
+      const bool ok = col_ok && typ_r[lr] >= 0 && g1 <= g2;
+      const Real dx = sph_r[lr][0] - c0;
+      const Real dy = sph_r[lr][1] - c1;
+      const Real dz = sph_r[lr][2] - c2;
+      const Real dxx = dx * dx;
+      const Real dyy = dy * dy;
+      const Real dzz = dz * dz;
+      const Real d2 = (dxx + dyy) + dzz;
+      const Real thr = (sph_r[lr][3] + c3) + reach;
+      const Real thr2 = thr * thr;
+      const bool hit = ok && (d2 < thr2);
+      const uint64_t m = __ballot(hit);
+      if (m == 0) continue;
+      if (hit) {
+        const int64_t idx = int64_t(cnt[r]) + lanes_below(m);
+        if (idx < row_cap)
+          row_items[int64_t(g1) * row_cap + idx] = g2;
+        else
+          overflow = true;
+        if (dense) {
+          const int b2 = g2 - (g2 / s.B) * s.B;
+          dense[int64_t(g1) * s.B + b2] = 1;
+        }
+      }
+      cnt[r] += __popcll(m);
+    }
+  }
+  uint32_t total = 0;
+#pragma unroll
+  for (int r = 0; r < 16; ++r) {
+    const int g1 = row0 + wave * 16 + r;
+    if (g1 < end && lane == 0) row_count[g1] = cnt[r];
+    total += cnt[r];
+  }
+  if (__ballot(overflow) != 0 && lane == 0) atomicOr(&counters[1], 1u);
+  if (total > 0 && lane == 0) atomicAdd(&counters[0], total);
+}
+
 // ------------------------------------------------------------------ small kernels
 
 template <typename T>
