@@ -105,85 +105,6 @@ __global__ void spheres_kernel(StackView s, BTView bt, const Real* __restrict__ 
 
 // ------------------------------------------------------------------ neighbors
 
-// One workgroup per 64x64 tile of a pose's block-pair upper triangle.  Wave w
-// tests rows 16w..16w+15 against the 64 columns (one lane per column); for
-// each row the ballot of hits is appended to that row of the work list with
-// one atomic on the row's counter; one more atomic per wave counts the
-// total.  Rows hold at most row_cap items; beyond that the overflow flag is
-// raised.  Optionally writes the reference's dense [P, B, B] flags.
-template <typename Real>
-__global__ __launch_bounds__(256) void neighbors_kernel(
-    StackView s, const int4* __restrict__ tiles, int64_t n_tiles,
-    const Real* __restrict__ spheres, Real reach, uint32_t* __restrict__ row_count,
-    int32_t* __restrict__ row_items, int64_t row_cap, uint32_t* __restrict__ counters,
-    int32_t* __restrict__ dense) {
-#pragma clang fp contract(off)
-  __shared__ Real sph_r[NBR_TILE][4];
-  __shared__ Real sph_c[NBR_TILE][4];
-  __shared__ int typ_r[NBR_TILE];
-  __shared__ int typ_c[NBR_TILE];
-  int64_t tile_id = blockIdx.x;
-  if (tile_id >= n_tiles) return;
-  int4 tl = tiles[tile_id];
-  int row0 = tl.x, col0 = tl.y, end = tl.z;
-  int tid = threadIdx.x;
-  if (tid < NBR_TILE) {
-    int g = row0 + tid;
-    bool ok = g < end;
-    for (int k = 0; k < 4; ++k) sph_r[tid][k] = ok ? spheres[int64_t(g) * 4 + k] : Real(0);
-    typ_r[tid] = ok ? s.block_type[g] : -1;
-  } else if (tid < 2 * NBR_TILE) {
-    int j = tid - NBR_TILE;
-    int g = col0 + j;
-    bool ok = g < end;
-    for (int k = 0; k < 4; ++k) sph_c[j][k] = ok ? spheres[int64_t(g) * 4 + k] : Real(0);
-    typ_c[j] = ok ? s.block_type[g] : -1;
-  }
-  __syncthreads();
-  const int lane = lane_id();
-  const int wave = tid / WAVE;
-  const int g2 = col0 + lane;
-  const Real c0 = sph_c[lane][0], c1 = sph_c[lane][1], c2 = sph_c[lane][2], c3 = sph_c[lane][3];
-  const bool col_ok = typ_c[lane] >= 0;
-  int total = 0;
-  bool overflow = false;
-  for (int r = 0; r < 16; ++r) {
-    const int lr = wave * 16 + r;
-    const int g1 = row0 + lr;
-    bool ok = col_ok && typ_r[lr] >= 0 && g1 <= g2;
-    Real dx = sph_r[lr][0] - c0;
-    Real dy = sph_r[lr][1] - c1;
-    Real dz = sph_r[lr][2] - c2;
-    Real dxx = dx * dx;
-    Real dyy = dy * dy;
-    Real dzz = dz * dz;
-    Real d2 = (dxx + dyy) + dzz;
-    Real thr = (sph_r[lr][3] + c3) + reach;
-    Real thr2 = thr * thr;
-    bool hit = ok && (d2 < thr2);
-    uint64_t m = __ballot(hit);
-    if (m == 0) continue;
-    int cnt = __popcll(m);
-    total += cnt;
-    uint32_t base = 0;
-    if (lane == 0) base = atomicAdd(&row_count[g1], (uint32_t)cnt);
-    base = __shfl(base, 0, WAVE);
-    if (hit) {
-      int64_t idx = int64_t(base) + lanes_below(m);
-      if (idx < row_cap)
-        row_items[int64_t(g1) * row_cap + idx] = g2;
-      else
-        overflow = true;
-      if (dense) {
-        int b2 = g2 - (g2 / s.B) * s.B;
-        dense[int64_t(g1) * s.B + b2] = 1;
-      }
-    }
-  }
-  if (__ballot(overflow) != 0 && lane == 0) atomicOr(&counters[1], 1u);
-  if (total > 0 && lane == 0) atomicAdd(&counters[0], (uint32_t)total);
-}
-
 // One workgroup per 64-block chunk of a pose: the diagonal tiles of the tile
 // list do the work, the other tiles return at once.  The chunk's rows are
 // tested against every column chunk from itself to the end of the pose, in
@@ -192,7 +113,8 @@ __global__ __launch_bounds__(256) void neighbors_kernel(
 // + 1e-3 A: then every sphere pair is, and no test can pass.  Wave w owns
 // rows 16w..16w+15 and appends to them in column order with counters in
 // registers, so the lists need no atomics and come out sorted by g2.
-// Same test, same work-list contents and dense flags as neighbors_kernel.
+// The pair test is the reference's (sphere_overlap.impl.hh:287-351):
+// d^2 < ((r1 + r2) + reach)^2 with its association, no contraction.
 template <typename Real>
 __device__ __forceinline__ void chunk_box(const Real (*sph)[4], const int* typ, int lane,
                                           Real* lo, Real* hi) {
