@@ -96,3 +96,53 @@ def test_op_schema_matches_reference_arity():
     schema = str(torch.ops.tmol_amd.ljlk_pose_forward.default._schema)
     # 27 reference arguments + require_gradient
     assert schema.count(",") + 1 == 28
+
+
+def test_rotamer_and_topology_validation():
+    """Argument checks of the rotamer-pair and topology entry points run on
+    the host, before any device work."""
+    from tmol_amd import ops
+
+    L = ops._rot_lib()
+    b = ops.CBlockTypes()
+    b.n_block_types = 3
+    b.max_n_atoms = 24
+    b.max_n_tiles = 1
+    b.max_n_conn = 2
+    b.n_atom_types = 105
+    b.dtype = ops.F32
+    r = ops.CRotamerSet()
+    r.n_poses, r.max_n_blocks, r.n_rots = 2, 10, 40
+    r.n_atoms_total = 600
+    # missing tables: the size query refuses
+    assert L.tmol_ljlk_rotamer_workspace_bytes(ctypes.byref(r), ctypes.byref(b), ops.F32) == -1
+    dummy = ctypes.c_void_p(16)
+    for f in ("rot_coord_offset", "block_ind_for_rot", "block_type_ind_for_rot", "n_rots_for_block",
+              "rot_offset_for_block", "first_rot_block_type", "min_bondsep", "inter_block_bondsep"):
+        setattr(r, f, dummy)
+    n32 = L.tmol_ljlk_rotamer_workspace_bytes(ctypes.byref(r), ctypes.byref(b), ops.F32)
+    # the dtype must match the block-type tables'
+    assert L.tmol_ljlk_rotamer_workspace_bytes(ctypes.byref(r), ctypes.byref(b), ops.F64) == -1
+    b.dtype = ops.F64
+    n64 = L.tmol_ljlk_rotamer_workspace_bytes(ctypes.byref(r), ctypes.byref(b), ops.F64)
+    b.dtype = ops.F32
+    assert n32 > 0 and n64 > n32
+    # dtype mismatch with the block-type tables, and block types over 64 atoms
+    assert L.tmol_ljlk_rotamer_neighbors(ctypes.byref(r), ctypes.byref(b), dummy, ops.F64, 6.0,
+                                         dummy, n64, None) == ops.EINVAL
+    b.max_n_atoms, b.max_n_tiles = 70, 3
+    assert L.tmol_ljlk_rotamer_workspace_bytes(ctypes.byref(r), ctypes.byref(b), ops.F32) == -1
+    assert L.tmol_ljlk_rotamer_score(ctypes.byref(r), ctypes.byref(b), dummy, ops.F32, dummy, n32,
+                                     dummy, 1, 1, 0, None, dummy, None, 0, None) == ops.ENOTSUP
+    b.max_n_atoms, b.max_n_tiles = 24, 1
+    # unsupported score flags; more pairs than int32 dispatch indices hold
+    assert L.tmol_ljlk_rotamer_score(ctypes.byref(r), ctypes.byref(b), dummy, ops.F32, dummy, n32,
+                                     dummy, 1, 1, ops.DVDX, None, dummy, None, 0, None) == ops.ENOTSUP
+    assert L.tmol_ljlk_rotamer_dispatch(ctypes.byref(r), ctypes.byref(b), ops.F32, 6.0, dummy, n32,
+                                        dummy, 1 << 31, dummy, 1, None) == ops.EOVERFLOW
+    # topology: null pointers and too many connections
+    assert ops.lib().tmol_ljlk_inter_block_bondsep(ctypes.byref(b), 2, 10, None, None, None, None,
+                                                   None, None) == ops.EINVAL
+    b.max_n_conn = 5
+    assert ops.lib().tmol_ljlk_inter_block_bondsep(ctypes.byref(b), 2, 10, dummy, dummy, dummy, None,
+                                                   None, None) == ops.ENOTSUP
