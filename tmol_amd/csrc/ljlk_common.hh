@@ -16,6 +16,7 @@ constexpr int WAVE = 64;         // CDNA wavefront
 constexpr int WAVES_PER_WG = 4;  // scoring workgroups: 4 independent waves
 constexpr int NBR_TILE = 64;     // neighbor pass: 64 x 64 block tiles
 constexpr int REC = 8;           // Reals per prepared (block type, atom) record
+constexpr int INTRA_MAX = TILE * (TILE - 1) / 2;  // intra-block atom pairs of a 32-atom type
 
 // prepared per (block type, atom) record:
 //   [0] lj_radius  [1] sqrt(lj_wdepth)  [2] K = dg / (2 pi^1.5 lambda)
@@ -67,6 +68,7 @@ struct WsLayout {
   int64_t spheres;    // Real[4 * G]
   int64_t records;    // Real[T * Amax * REC]
   int64_t conn_pd;    // uint8[T * Amax * MAX_CONN]: path distance atom -> connection atom
+  int64_t intra;      // int32[T] + uint16[T * INTRA_MAX]: weighted intra-block pairs per type
   int64_t row_count;  // uint32[G]
   int64_t row_items;  // int32[G * row_cap]
   int64_t total;
@@ -92,6 +94,8 @@ inline WsLayout ws_layout(const tmol_ljlk_stack* s, const tmol_ljlk_block_types*
   o = align256(o + rs * REC * int64_t(bt->n_block_types) * bt->max_n_atoms);
   L.conn_pd = o;
   o = align256(o + int64_t(bt->n_block_types) * MAX_CONN * bt->max_n_atoms);
+  L.intra = o;
+  o = align256(o + 4 * int64_t(bt->n_block_types) + 2 * int64_t(bt->n_block_types) * INTRA_MAX);
   L.row_count = o;
   o = align256(o + 4 * G);
   L.row_items = o;
@@ -304,6 +308,8 @@ struct ScoreArgs {
   const Real* spheres;  // [G, 4] centre, radius (from the neighbor pass)
   const Real* records;
   const uint8_t* conn_pd;
+  const int32_t* intra_count;   // [T] weighted intra-block pairs per type
+  const uint16_t* intra_pairs;  // [T, INTRA_MAX]: a1 | a2 << 5 | (separation 4) << 10
   const Real* global_params;
   const uint32_t* counters;
   uint32_t* row_cursor;        // [8 * QSTRIDE] RowQueue cursors (workspace)

@@ -85,6 +85,9 @@ static int run_neighbors(const tmol_ljlk_stack* st, const tmol_ljlk_block_types*
   int nrec = bt.T * bt.Amax;
   hipLaunchKernelGGL(prepare_records_kernel<Real>, dim3(grid_for(nrec)), dim3(256), 0, stream, bt,
                      (const Real*)btp->type_params, btp->n_atom_types, records, conn_pd);
+  hipLaunchKernelGGL(prepare_intra_kernel, dim3(bt.T), dim3(64), 0, stream, bt,
+                     (int32_t*)(w + Lw.intra),
+                     (uint16_t*)(w + Lw.intra + 4 * int64_t(bt.T)));
   hipLaunchKernelGGL(spheres_kernel<Real>, dim3(grid_for(s.G)), dim3(256), 0, stream, s, bt,
                      (const Real*)coords, spheres);
   if (dense) {
@@ -113,6 +116,8 @@ static ScoreArgs<Real> make_args(const tmol_ljlk_stack* st, const tmol_ljlk_bloc
   a.spheres = (const Real*)(w + Lw.spheres);
   a.records = (const Real*)(w + Lw.records);
   a.conn_pd = (const uint8_t*)(w + Lw.conn_pd);
+  a.intra_count = (const int32_t*)(w + Lw.intra);
+  a.intra_pairs = (const uint16_t*)(w + Lw.intra + 4 * int64_t(btp->n_block_types));
   a.global_params = (const Real*)btp->global_params;
   a.counters = (const uint32_t*)(w + Lw.counters);
   a.row_cursor = (uint32_t*)(w + Lw.cursors);

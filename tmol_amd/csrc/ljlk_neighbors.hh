@@ -56,6 +56,33 @@ __global__ void prepare_records_kernel(BTView bt, const Real* __restrict__ type_
   }
 }
 
+// Intra-block pairs of every block type with at most 32 atoms that carry
+// count-pair weight (path distance >= 4; the others score exactly zero),
+// a1 < a2 in the order (a1, a2), one wave per type (lanes over a2, ballot
+// compaction keeps the order):
+//   pairs[t * INTRA_MAX + k] = a1 | a2 << 5 | (path distance == 4) << 10
+__global__ __launch_bounds__(64) void prepare_intra_kernel(BTView bt, int32_t* __restrict__ count,
+                                                           uint16_t* __restrict__ pairs) {
+  const int t = blockIdx.x;
+  if (t >= bt.T) return;
+  const int lane = threadIdx.x;
+  const int n = bt.n_atoms[t];
+  int k = 0;
+  if (n <= TILE)
+    for (int a1 = 0; a1 < n; ++a1) {
+      const int a2 = a1 + 1 + lane;
+      int sep = 0;
+      if (a2 < n) sep = bt.path_distance[(int64_t(t) * bt.Amax + a1) * bt.Amax + a2];
+      const bool keep = a2 < n && sep >= 4;
+      const uint64_t m = __ballot(keep);
+      if (keep)
+        pairs[int64_t(t) * INTRA_MAX + k + lanes_below(m)] =
+            (uint16_t)(a1 | (a2 << 5) | ((sep == 4) << 10));
+      k += __popcll(m);
+    }
+  if (lane == 0) count[t] = k;
+}
+
 // ------------------------------------------------------------------ spheres
 
 // Bounding sphere per block, bit-identical to the reference's CPU pass

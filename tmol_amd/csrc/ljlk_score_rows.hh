@@ -449,18 +449,21 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
             if (lane < n) load_atom(pf, A, t, o, lane);
           }
           wave_sync();
-          const int total = n1 * n1;
-          const int q = small_div(WAVE, n1), rstep = WAVE - q * n1;
-          int a1 = small_div(lane, n1), a2 = lane - a1 * n1;
-          for (int k0 = 0; k0 < total; k0 += WAVE) {
+          // only the type's pairs of non-zero weight (prepared table)
+          const int n_ip = uni(A.intra_count[t1]);
+          const uint16_t* ip = A.intra_pairs + int64_t(t1) * INTRA_MAX;
+          for (int k0 = 0; k0 < n_ip; k0 += WAVE) {
             if (n_lj > LIST - WAVE || n_lk > LIST - WAVE) drain(false);
             bool in = false, both_heavy = false, w02 = false;
-            if (a1 < a2 && a1 < n1) {
-              const int sep = bt.path_distance[(int64_t(t1) * bt.Amax + a1) * bt.Amax + a2];
+            int a1 = 0, a2 = 0;
+            if (k0 + lane < n_ip) {
+              const int e = ip[k0 + lane];
+              a1 = e & 31;
+              a2 = (e >> 5) & 31;
               const V pa = *(const V*)L.pos[a1], pb = *(const V*)L.pos[a2];
               const Real dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
-              in = sep >= 4 && dx * dx + dy * dy + dz * dz < gp.max_dis2;
-              w02 = sep == 4;
+              in = dx * dx + dy * dy + dz * dz < gp.max_dis2;
+              w02 = (e >> 10) & 1;
               both_heavy = in && ((hmask1 >> a1) & (hmask1 >> a2) & 1u);
             }
             const uint64_t m = __ballot(in);
@@ -471,12 +474,6 @@ __global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
             if (both_heavy) L.lk[n_lk + lanes_below(mh)] = make_entry(a1, a2, w02);
             n_lk += __popcll(mh);
             seq_lk += __popcll(mh);
-            a2 += rstep;
-            a1 += q;
-            if (a2 >= n1) {
-              a2 -= n1;
-              a1 += 1;
-            }
             wave_sync();
           }
         } else {
