@@ -1,6 +1,7 @@
 #!/bin/bash
 # Build the reference's own CPU LJ+LK op (tmol_ljlk::ljlk_pose_scores and
-# ljlk_rotamer_scores) and its all-pairs-shortest-paths op (tmol_apsp::apsp_op)
+# ljlk_rotamer_scores), its elec op (tmol_elec::elec_pose_scores) and its
+# all-pairs-shortest-paths op (tmol_apsp::apsp_op)
 # from the sources where they lie under the reference checkout, into
 # oracle/_ref/.
 # Test infrastructure only: used to cross-check the C oracle and as the CPU
@@ -40,3 +41,14 @@ g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
   -o "$OUT2.tmp"
 mv "$OUT2.tmp" "$OUT2"
 echo "built $OUT2"
+# the reference's elec op (tmol_elec::elec_pose_scores, elec_rotamer_scores)
+OUT3="$HERE/_ref/tmol_elec_ref.so"
+g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
+  -DTORCH_EXTENSION_NAME=tmol_elec_ref \
+  -I"$R" -I"$R/tmol/extern" $TI -I"$PYI" \
+  "$R/tmol/score/elec/potentials/compiled.ops.cpp" \
+  "$R/tmol/score/elec/potentials/elec_pose_score.cpu.cpp" \
+  -L"$TL" -ltorch -ltorch_cpu -lc10 -Wl,-rpath,"$TL" \
+  -o "$OUT3.tmp"
+mv "$OUT3.tmp" "$OUT3"
+echo "built $OUT3"

@@ -23,12 +23,22 @@ typedef struct {
   const int32_t* bt_is_ligand;        /* [T] */
 } oracle_stack;
 
+/* eps() of the elec potential at a float argument (potentials.hh:43-48 with
+ * Real = float): the reference evaluates it this way for max_dis and min_dis
+ * in both precisions; the 0.5 literal lifts the product to double */
+static float elec_eps_float(float dist, float D, float D0, float S) {
+  double a = 0.5 * (D - D0);
+  float poly = 2 + 2 * dist * S + dist * dist * S * S;
+  return (float)(D - a * poly * expf(-dist * S));
+}
+
 #define REAL float
 #define RSUF _f
 #define SQRT sqrtf
 #define EXP expf
 #define FLOOR floorf
 #include "ljlk_oracle_impl.h"
+#include "elec_oracle_impl.h"
 #undef REAL
 #undef RSUF
 #undef SQRT
@@ -41,6 +51,7 @@ typedef struct {
 #define EXP exp
 #define FLOOR floor
 #include "ljlk_oracle_impl.h"
+#include "elec_oracle_impl.h"
 #undef REAL
 #undef RSUF
 #undef SQRT

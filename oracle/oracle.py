@@ -78,6 +78,15 @@ def lib():
                 ctypes.POINTER(OracleStack), PR, ctypes.c_int64, PR, PR, ctypes.c_int64, _P,
                 _P, _P, _P, PR, PR, PR,
             ]
+        for suf, ct in (("_f", ctypes.c_float), ("_d", ctypes.c_double)):
+            PR = ctypes.POINTER(ct)
+            f = getattr(L, "oracle_elec_pair" + suf)
+            f.argtypes = [ct, ct, ct, ctypes.c_int, PR, ctypes.c_int, PR]
+            f = getattr(L, "oracle_elec_scores" + suf)
+            f.argtypes = [
+                ctypes.POINTER(OracleStack), PR, ctypes.c_int64, PR, _P, PR, _P,
+                ctypes.c_int, ctypes.c_int, PR, PR,
+            ]
         L.oracle_rot_dispatch.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P,
                                           ctypes.c_int64]
         L.oracle_rot_dispatch.restype = ctypes.c_int64
@@ -371,8 +380,91 @@ def lk_pair(dist, sep, params_i, params_j, global_params, with_deriv=True,
     return o
 
 
+def elec_pair(dist, q1, q2, sep, global_params, with_deriv=True, dtype=numpy.float64):
+    """Single elec pair: returns (V, dV/dd); global_params = D, D0, S,
+    min_dis, max_dis."""
+    L = lib()
+    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+    gp = numpy.ascontiguousarray(numpy.asarray(global_params, dtype=dtype).reshape(-1))
+    o = numpy.zeros(2, dtype=dtype)
+    getattr(L, "oracle_elec_pair" + suf)(
+        ct(dist), ct(q1), ct(q2), int(sep), _ptr(gp, ct), int(with_deriv), _ptr(o, ct))
+    return o
+
+
+def oracle_elec_scores(*args, require_grad=False, dtype=None):
+    """Run the elec oracle on the 25 arguments of ``elec_pose_scores``
+    (tmol/score/elec/potentials/compiled.ops.cpp:526-586).
+
+    Returns (scores, block_neighbors, dV_dx, spheres): scores is (1, P) or
+    (1, P, B, B), block_neighbors (P, B, B) int32, dV_dx (1, NA, 3) or None."""
+    (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+     first_rot_block_type, block_ind_for_rot, pose_ind_for_rot,
+     block_type_ind_for_rot, n_rots_for_pose, rot_offset_for_pose,
+     n_rots_for_block, rot_offset_for_block, max_n_rots_per_pose,
+     min_bondsep, inter_block_bondsep, bt_n_atoms, bt_charge, bt_n_conn,
+     bt_conn_atoms, bt_inter_rpd, bt_intra_rpd, bt_is_ligand, global_params,
+     max_dis, block_pair) = args
+    P, B = first_rot_for_block.shape
+    T, Am = bt_charge.shape
+    if dtype is None:
+        dtype = numpy.float64 if coords.dtype == torch.float64 else numpy.float32
+    L = lib()
+    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+    arrays = {
+        "rot_coord_offset": _np_i32(rot_coord_offset),
+        "block_type": _np_i32(block_type_ind_for_rot),
+        "min_bondsep": _np_i32(min_bondsep),
+        "inter_block_bondsep": _np_i32(inter_block_bondsep),
+        "bt_n_atoms": _np_i32(bt_n_atoms),
+        "bt_n_conn": _np_i32(bt_n_conn),
+        "bt_conn_atoms": _np_i32(bt_conn_atoms),
+        "bt_path_distance": _np_i32(bt_inter_rpd),
+        "bt_is_ligand": _np_i32(bt_is_ligand),
+    }
+    st = OracleStack()
+    st.n_poses, st.max_n_blocks = P, B
+    st.n_block_types, st.max_n_block_atoms = T, Am
+    st.max_n_tiles = (Am - 1) // 32 + 1
+    st.max_n_conn = bt_conn_atoms.shape[1]
+    for k, v in arrays.items():
+        setattr(st, k, _ptr(v, ctypes.c_int32))
+    intra = _np_i32(bt_intra_rpd)
+    x = numpy.ascontiguousarray(coords.detach().cpu().numpy().astype(dtype))
+    q = numpy.ascontiguousarray(bt_charge.detach().cpu().numpy().astype(dtype))
+    gp = numpy.ascontiguousarray(global_params.detach().cpu().numpy().astype(dtype).reshape(-1))
+    NA = x.shape[0]
+    spheres = numpy.zeros((P * B, 4), dtype=dtype)
+    getattr(L, "oracle_block_spheres" + suf)(
+        _ptr(x, ct), P * B, arrays["rot_coord_offset"].ctypes.data_as(_P),
+        arrays["block_type"].ctypes.data_as(_P), arrays["bt_n_atoms"].ctypes.data_as(_P),
+        _ptr(spheres, ct))
+    nbr = numpy.zeros((P, B, B), dtype=numpy.int32)
+    getattr(L, "oracle_block_neighbors" + suf)(
+        P, B, arrays["block_type"].ctypes.data_as(_P), _ptr(spheres, ct),
+        ct(dtype(max_dis)), nbr.ctypes.data_as(_P))
+    out = numpy.zeros((1, P, B, B) if block_pair else (1, P), dtype=dtype)
+    dvdx = numpy.zeros((1, NA, 3), dtype=dtype) if require_grad else None
+    getattr(L, "oracle_elec_scores" + suf)(
+        ctypes.byref(st), _ptr(x, ct), NA, _ptr(q, ct), intra.ctypes.data_as(_P), _ptr(gp, ct),
+        nbr.ctypes.data_as(_P), int(bool(block_pair)), int(require_grad), _ptr(out, ct),
+        _ptr(dvdx, ct) if dvdx is not None else None)
+    return out, nbr, dvdx, spheres
+
+
 def reference_op_available():
     return os.path.exists(REF_LIB_PATH)
+
+
+def load_reference_elec_op():
+    """The reference's own CPU elec op built by build_ref.sh
+    (``torch.ops.tmol_elec.elec_pose_scores``)."""
+    path = os.path.join(HERE, "_ref", "tmol_elec_ref.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    if not hasattr(torch.ops, "tmol_elec") or not hasattr(torch.ops.tmol_elec, "elec_pose_scores"):
+        torch.ops.load_library(path)
+    return torch.ops.tmol_elec.elec_pose_scores
 
 
 def load_reference_op():

@@ -28,12 +28,13 @@ def ljlk_term():
     return LJLKEnergyTerm()
 
 
-def load_term_baseline(name):
-    """tmol's YAML term baselines -> numpy array (term, pose[, res, res])."""
+def load_term_baseline(name, term="ljlk"):
+    """tmol's YAML term baselines (tmol/tests/data/term_baselines/
+    LJLKEnergyTerm, ElecEnergyTerm) -> numpy array (term, pose[, res, res])."""
     import numpy
     import yaml
 
-    with open(os.path.join(GOLDEN, "ljlk_term_baselines", name + ".yaml")) as f:
+    with open(os.path.join(GOLDEN, term + "_term_baselines", name + ".yaml")) as f:
         data = yaml.safe_load(f)
 
     def unpack(d):

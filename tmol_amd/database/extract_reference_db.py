@@ -4,6 +4,8 @@ Reads (only when run by hand, in a checkout that has the reference tree):
   tmol/database/default/chemical/chemical.yaml  (atom types, canonical residues,
                                                  the alpha-amino-acid termini variants)
   tmol/database/default/scoring/ljlk.yaml       (LJ/LK global + per-atom-type params)
+  tmol/database/default/scoring/elec.yaml       (elec global params, partial charges
+                                                 and count-pair representatives)
 and writes ``default_db.json`` next to this file.  The JSON is data only: the
 same numbers, names and connectivity as the YAML, restricted to what the
 20 canonical amino acids with termini need.  Nothing at run time reads the
@@ -136,12 +138,36 @@ def extract(ref_root):
             for p in ljlk["atom_type_parameters"]
         ],
     }
+    elec = yaml.safe_load(
+        open(os.path.join(ref_root, "tmol/database/default/scoring/elec.yaml"))
+    )
+    keep = set(CANONICAL)
+
+    def _res(tag):
+        # "ALA" / "ALA:cterm" -> ("ALA", "" / "cterm")
+        parts = tag.split(":")
+        return parts[0], parts[1] if len(parts) > 1 else ""
+
+    elec_out = {
+        "global": {k: float(v) for k, v in elec["global_parameters"].items()},
+        "atom_charge_parameters": [
+            [*_res(x["res"]), x["atom"], float(x["charge"])]
+            for x in elec["atom_charge_parameters"]
+            if _res(x["res"])[0] in keep
+        ],
+        "atom_cp_reps_parameters": [
+            [*_res(x["res"]), x["atm_inner"], x["atm_outer"]]
+            for x in elec["atom_cp_reps_parameters"]
+            if _res(x["res"])[0] in keep
+        ],
+    }
     return {
-        "source": "tmol/database/default (chemical.yaml, scoring/ljlk.yaml)",
+        "source": "tmol/database/default (chemical.yaml, scoring/ljlk.yaml, scoring/elec.yaml)",
         "atom_types": atom_types,
         "residues": residues,
         "variants": variants,
         "ljlk": ljlk_out,
+        "elec": elec_out,
     }
 
 

@@ -1,7 +1,9 @@
-"""Scoring API slice for the LJ+LK term: ScoreFunction, score types, term."""
+"""Scoring API slice for the LJ+LK and elec terms: ScoreFunction, score types,
+terms."""
 
 from .score_types import ScoreType  # noqa: F401
 from .ljlk_energy_term import LJLKEnergyTerm  # noqa: F401
+from .elec_energy_term import ElecEnergyTerm  # noqa: F401
 from .score_function import ScoreFunction, WholePoseScoringModule  # noqa: F401
 
 

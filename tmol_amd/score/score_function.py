@@ -1,30 +1,42 @@
-"""ScoreFunction / WholePoseScoringModule slice for the LJ+LK term
-(tmol/score/score_function.py:26-165, 321-355)."""
+"""ScoreFunction / WholePoseScoringModule slice for the two-body terms built
+here, LJ+LK and elec (tmol/score/score_function.py:26-165, 321-355).  A term
+is created the first time one of its score types gets a weight; terms keep
+their creation order, and each contributes its score types in its own order.
+"""
 
 import torch
 
+from .elec_energy_term import ElecEnergyTerm
 from .ljlk_energy_term import LJLKEnergyTerm
 from .score_types import ScoreType
+
+_TERM_FOR_SCORE_TYPE = {
+    ScoreType.fa_ljatr: LJLKEnergyTerm,
+    ScoreType.fa_ljrep: LJLKEnergyTerm,
+    ScoreType.fa_lk: LJLKEnergyTerm,
+    ScoreType.fa_elec: ElecEnergyTerm,
+}
 
 
 class ScoreFunction:
     def __init__(self, param_db=None, device="cpu"):
         self._device = torch.device(device)
         self._weights = torch.zeros(len(ScoreType), device=self._device)
-        self._term = None
+        self._terms = {}
         self._param_db = param_db
         self.term_options = {}
 
     def set_weight(self, st: ScoreType, weight: float):
-        if self._term is None:
-            self._term = LJLKEnergyTerm(device=self._device, chemical_db=self._param_db)
+        cls = _TERM_FOR_SCORE_TYPE[ScoreType(st)]
+        if cls not in self._terms:
+            self._terms[cls] = cls(device=self._device, chemical_db=self._param_db)
         self._weights[int(st)] = weight
 
     def get_weight(self, st: ScoreType):
         return self._weights[int(st)]
 
     def all_terms(self):
-        return [] if self._term is None else [self._term]
+        return list(self._terms.values())
 
     def set_option(self, key, value):
         self.term_options[key] = value

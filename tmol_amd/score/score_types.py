@@ -1,5 +1,6 @@
-"""Score types produced by the LJ+LK term, in output order
-(tmol/score/score_types.py; tmol/score/terms/ljlk_creator.py:7-19)."""
+"""Score types of the two-body terms built here, in output order within each
+term (tmol/score/score_types.py; tmol/score/terms/ljlk_creator.py:7-19,
+elec_creator.py)."""
 
 import enum
 
@@ -8,3 +9,4 @@ class ScoreType(enum.IntEnum):
     fa_ljatr = 0
     fa_ljrep = 1
     fa_lk = 2
+    fa_elec = 3
