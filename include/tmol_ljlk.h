@@ -359,6 +359,45 @@ int tmol_ljlk_inter_block_bondsep(const tmol_ljlk_block_types* bt, int32_t n_pos
                                   int32_t* inter_block_bondsep, int32_t* min_bondsep,
                                   uint32_t* overflow, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Elec, a sibling two-body term on the same scheduler.  Replaces the
+ * reference's op
+ *   tmol_elec::elec_pose_scores   tmol/score/elec/potentials/compiled.ops.cpp:526-586
+ * and its dispatch
+ *   ElecPoseScoreDispatch::forward    elec_pose_score.impl.hh:383-856
+ *   ElecPoseScoreOp::backward (whole pose)  compiled.ops.cpp:150-168
+ *   ElecPoseScoreDispatch::backward (block pair)  elec_pose_score.impl.hh:858-1125
+ * The stack and block-type structs are the LJ+LK ones; elec reads n_atoms,
+ * n_conn, conn_atoms and is_ligand (elec_is_ligand_fragment) of the block
+ * types, which must be at most 32 atoms (ENOTSUP beyond).  The workspace
+ * has tmol_ljlk_workspace_bytes() bytes and holds elec's own work list. */
+typedef struct {
+  const void* partial_charge;              /* [T, Amax] (block-type dtype) */
+  const int32_t* inter_repr_path_distance; /* [T, Amax, Amax]: [i][j] = path_dist[i][rep(j)] */
+  const int32_t* intra_repr_path_distance; /* [T, Amax, Amax]: path_dist[rep(i)][rep(j)] */
+  float global_params[5];                  /* host values: D, D0, S, min_dis, max_dis */
+} tmol_elec_tables;
+
+/* Neighbor pass with elec's reach (max_dis, the reference's host scalar):
+ * spheres, the row-major work list, the packed count-pair tables and the
+ * intra-block pair table; dense_neighbors as for tmol_ljlk_neighbors. */
+int tmol_elec_neighbors(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                        const tmol_elec_tables* elec, const void* coords, int32_t dtype,
+                        double max_dis, const int32_t* tiles, int64_t n_tiles,
+                        void* workspace, int64_t ws_bytes, int64_t row_capacity,
+                        int32_t* dense_neighbors, void* stream);
+
+/* Score the work list.  flags as for tmol_ljlk_score with one term:
+ *   energies: [P] (or [P, B, B] / ragged [G, row_capacity] with BLOCK_PAIR)
+ *   dvdx:     [NA, 3] with DVDX (the reference's saved dV/dx)
+ *   weights:  [P] (or block-pair shaped) with WGRAD; grad [NA, 3]
+ * Outputs are overwritten unless ACCUMULATE. */
+int tmol_elec_score(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                    const tmol_elec_tables* elec, const void* coords, int32_t dtype,
+                    void* workspace, int64_t ws_bytes, int64_t row_capacity, int32_t flags,
+                    const void* weights, void* energies, void* dvdx, void* grad,
+                    int32_t n_wave_groups, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

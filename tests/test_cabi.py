@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "tmol_ljlk.h")
 
 def _declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(tmol_ljlk_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(tmol_(?:ljlk|elec)_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_header_symbols():
@@ -146,3 +146,51 @@ def test_rotamer_and_topology_validation():
     b.max_n_conn = 5
     assert ops.lib().tmol_ljlk_inter_block_bondsep(ctypes.byref(b), 2, 10, dummy, dummy, dummy, None,
                                                    None, None) == ops.ENOTSUP
+
+
+def test_elec_validation():
+    """Argument checks of the elec entry points run on the host before any
+    device work: missing tables, block types over 32 atoms, unordered
+    potential regions, non-finite parameters, bad flags."""
+    from tmol_amd import ops
+
+    L = ops._elec_lib()
+    dummy = ctypes.c_void_p(16)
+    s = ops.CStack()
+    s.layout = ops.PADDED
+    s.n_poses, s.max_n_blocks, s.n_blocks_total, s.n_atoms_total = 2, 10, 20, 300
+    s.block_atom_offset = s.block_type = s.min_bondsep = s.inter_block_bondsep = dummy
+    b = ops.CBlockTypes()
+    b.n_block_types, b.max_n_atoms, b.max_n_tiles, b.max_n_conn = 3, 24, 1, 2
+    b.dtype = ops.F32
+    b.n_atoms = b.n_conn = b.conn_atoms = b.is_ligand = dummy
+    e = ops.CElecTables()
+    for i, v in enumerate((79.931, 6.648, 0.441546, 1.6, 5.5)):
+        e.global_params[i] = v
+    nbytes = L.tmol_ljlk_workspace_bytes(ctypes.byref(s), ctypes.byref(b), 10, ops.F32)
+
+    def score(flags=0, energies=dummy):
+        return L.tmol_elec_score(ctypes.byref(s), ctypes.byref(b), ctypes.byref(e), dummy,
+                                 ops.F32, dummy, nbytes, 10, flags, dummy, energies, dummy, dummy,
+                                 0, None)
+
+    assert score() == ops.EINVAL  # tables missing
+    e.partial_charge = dummy
+    e.inter_repr_path_distance = ctypes.cast(dummy, ops._P32)
+    e.intra_repr_path_distance = ctypes.cast(dummy, ops._P32)
+    assert score(flags=ops.DVDX | ops.WGRAD) == ops.EINVAL
+    assert score(flags=ops.BLOCK_PAIR | ops.DVDX) == ops.ENOTSUP
+    assert score(flags=ops.GENERAL_KERNEL) == ops.EINVAL
+    assert score(energies=None) == ops.EINVAL
+    b.max_n_atoms, b.max_n_tiles = 40, 2
+    assert score() == ops.ENOTSUP
+    b.max_n_atoms, b.max_n_tiles = 24, 1
+    e.global_params[3] = 5.0  # min_dis + 0.25 beyond max_dis - 1
+    assert score() == ops.EINVAL
+    e.global_params[3] = float("nan")
+    assert score() == ops.EINVAL
+    e.global_params[3] = 1.6
+    assert L.tmol_elec_neighbors(ctypes.byref(s), ctypes.byref(b), ctypes.byref(e), dummy, ops.F32,
+                                 float("inf"), dummy, 1, dummy, nbytes, 10, None, None) == ops.EINVAL
+    assert L.tmol_elec_neighbors(ctypes.byref(s), ctypes.byref(b), ctypes.byref(e), dummy, ops.F32,
+                                 5.5, dummy, 1, dummy, nbytes - 1, 10, None, None) == ops.EINVAL

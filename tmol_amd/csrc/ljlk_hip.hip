@@ -25,6 +25,9 @@
 #include "ljlk_prune.hh"
 #include "ljlk_rotamer.hh"
 #include "ljlk_topology.hh"
+#include "ljlk_elec.hh"
+
+#include <cmath>
 
 namespace tmol_ljlk {
 
@@ -363,6 +366,176 @@ static void launch_rot_score(const RotArgs<Real>& a, int n_wg, hipStream_t strea
 }  // namespace tmol_ljlk
 
 using namespace tmol_ljlk;
+
+// ------------------------------------------------------------------ elec
+
+// eps() of the elec potential with the reference's argument types
+// (potentials.hh:43-53): Real dist, float D, D0, S; the 0.5 literal is
+// double, so part of the product is formed in double whatever Real is
+template <typename Real>
+static Real elec_eps_host(Real dist, float D, float D0, float S) {
+  const double a = 0.5 * (D - D0);
+  const Real poly = 2 + 2 * dist * S + dist * dist * S * S;
+  return (Real)(D - a * poly * std::exp(-dist * S));
+}
+
+template <typename Real>
+static Real elec_deps_host(Real dist, float D, float D0, float S) {
+  return (Real)(0.5 * (D - D0) * dist * dist * S * S * S * std::exp(-dist * S));
+}
+
+// per-launch constants (potentials.hh:55-138): region bounds, the flat
+// value, and the Coulomb value and slope at the two fade joins, per unit
+// charge product; eps at max_dis and min_dis in float as in the reference
+template <typename Real>
+static ElecConst<Real> elec_constants(const float* g) {
+  const float D = g[0], D0 = g[1], S = g[2], mn = g[3], mx = g[4];
+  ElecConst<Real> k;
+  k.D = D;
+  k.S = S;
+  k.hdd = (Real)(0.5 * (D - D0));
+  k.C1 = (Real)322.0637;
+  k.C2 = k.C1 / (mx * elec_eps_host<float>(mx, D, D0, S));
+  k.lps = (Real)(mn - 0.25);
+  k.lpe = (Real)(mn + 0.25);
+  k.hps = (Real)(mx - 1.0);
+  k.hpe = mx;
+  k.mds = k.C1 / (mn * elec_eps_host<float>(mn, D, D0, S)) - k.C2;
+  const Real el = elec_eps_host<Real>(k.lpe, D, D0, S), dl = elec_deps_host<Real>(k.lpe, D, D0, S);
+  k.e_lo = k.C1 / (k.lpe * el) - k.C2;
+  k.de_lo = -k.C1 * (el + k.lpe * dl) / (k.lpe * k.lpe * el * el);
+  const Real eh = elec_eps_host<Real>(k.hps, D, D0, S), dh = elec_deps_host<Real>(k.hps, D, D0, S);
+  k.e_hi = k.C1 / (k.hps * eh) - k.C2;
+  k.de_hi = -k.C1 * (eh + k.hps * dh) / (k.hps * k.hps * eh * eh);
+  k.lw = k.lpe - k.lps;
+  k.hw = k.hpe - k.hps;
+  k.inv_lw = Real(1) / k.lw;
+  k.inv_hw = Real(1) / k.hw;
+  const Real reach = (Real)mx + Real(1e-3);
+  k.cull2 = reach * reach;
+  return k;
+}
+
+static int validate_elec(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                         const tmol_elec_tables* e, int dtype) {
+  int v = validate(s, bt, dtype);
+  if (v) return v;
+  if (!e || !e->partial_charge || !e->inter_repr_path_distance || !e->intra_repr_path_distance)
+    return TMOL_LJLK_EINVAL;
+  if (!bt->n_atoms || !bt->n_conn || !bt->is_ligand || (bt->max_n_conn > 0 && !bt->conn_atoms))
+    return TMOL_LJLK_EINVAL;
+  if (bt->max_n_atoms > TILE) return TMOL_LJLK_ENOTSUP;
+  const float* g = e->global_params;
+  for (int i = 0; i < 5; ++i)
+    if (!std::isfinite(g[i])) return TMOL_LJLK_EINVAL;
+  // region bounds must be ordered: min_dis - 0.25 < min_dis + 0.25 <= max_dis - 1 < max_dis
+  if (!(g[3] > 0.25f && double(g[3]) + 0.25 <= double(g[4]) - 1.0)) return TMOL_LJLK_EINVAL;
+  return TMOL_LJLK_OK;
+}
+
+template <typename Real>
+static int run_elec_neighbors(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
+                              const tmol_elec_tables* e, const void* coords, double max_dis,
+                              const int32_t* tiles, int64_t n_tiles, void* ws, int64_t row_cap,
+                              int32_t* dense, hipStream_t stream, const WsLayout& Lw) {
+  StackView s = make_stack_view(st, btp->max_n_conn);
+  BTView bt = make_bt_view(btp);
+  char* w = (char*)ws;
+  uint32_t* counters = (uint32_t*)(w + Lw.counters);
+  Real* spheres = (Real*)(w + Lw.spheres);
+  uint32_t* row_count = (uint32_t*)(w + Lw.row_count);
+  int32_t* row_items = (int32_t*)(w + Lw.row_items);
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, stream, counters, 16,
+                     (uint32_t)0);
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(grid_for(s.G)), dim3(256), 0, stream,
+                     row_count, int64_t(s.G), (uint32_t)0);
+  hipLaunchKernelGGL(elec_prepare_kernel<Real>, dim3(bt.T), dim3(64), 0, stream, bt,
+                     (const Real*)e->partial_charge, e->inter_repr_path_distance,
+                     e->intra_repr_path_distance, (uint32_t*)(w + Lw.conn_pd),
+                     (int32_t*)(w + Lw.intra), (uint16_t*)(w + Lw.intra + 4 * int64_t(bt.T)));
+  hipLaunchKernelGGL(spheres_kernel<Real>, dim3(grid_for(s.G)), dim3(256), 0, stream, s, bt,
+                     (const Real*)coords, spheres);
+  if (dense) {
+    int64_t n = int64_t(s.P) * s.B * s.B;
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream, dense, n, 0);
+  }
+  if (n_tiles > 0)
+    hipLaunchKernelGGL(neighbor_rows_kernel<Real>, dim3((unsigned)n_tiles), dim3(256), 0, stream, s,
+                       (const int4*)tiles, n_tiles, spheres, (Real)max_dis, row_count, row_items,
+                       row_cap, counters, dense);
+  return check(hipGetLastError());
+}
+
+template <typename Real, int MODE>
+static void launch_elec(const ElecArgs<Real>& a, int n_wg, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
+                     8 * QSTRIDE, 0u);
+  auto k = elec_rows_kernel<Real, MODE>;
+  if (n_wg <= 0) n_wg = resident_workgroups(k);
+  hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+}
+
+template <typename Real>
+static int run_elec_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
+                          const tmol_elec_tables* e, const void* coords, void* ws,
+                          int64_t row_cap, int flags, const void* weights, void* energies,
+                          void* dvdx, void* grad, int n_wg, hipStream_t stream,
+                          const WsLayout& Lw) {
+  ElecArgs<Real> a{};
+  a.s = make_stack_view(st, btp->max_n_conn);
+  a.bt = make_bt_view(btp);
+  char* w = (char*)ws;
+  a.w.row_count = (const uint32_t*)(w + Lw.row_count);
+  a.w.row_items = (const int32_t*)(w + Lw.row_items);
+  a.w.row_cap = row_cap;
+  a.coords = (const Real*)coords;
+  a.spheres = (const Real*)(w + Lw.spheres);
+  a.charge = (const Real*)e->partial_charge;
+  a.epd = (const uint32_t*)(w + Lw.conn_pd);
+  a.intra_count = (const int32_t*)(w + Lw.intra);
+  a.intra_pairs = (const uint16_t*)(w + Lw.intra + 4 * int64_t(btp->n_block_types));
+  a.k = elec_constants<Real>(e->global_params);
+  a.row_cursor = (uint32_t*)(w + Lw.cursors);
+  a.weights = (const Real*)weights;
+  a.energies64 = (double*)(w + Lw.energies);
+  a.energies_bp = (Real*)energies;
+  a.dvdx = (Real*)dvdx;
+  a.grad = (Real*)grad;
+  a.n_atoms = st->n_atoms_total;
+  const bool bp = flags & TMOL_LJLK_BLOCK_PAIR;
+  const bool noe = flags & TMOL_LJLK_NO_ENERGY;
+  const bool acc = flags & TMOL_LJLK_ACCUMULATE;
+  const int P = st->n_poses;
+  const int64_t NA3 = st->n_atoms_total * 3;
+  if (!noe) {
+    if (bp)
+      fill<Real>((Real*)energies,
+                 st->layout == TMOL_LJLK_PADDED
+                     ? int64_t(P) * st->max_n_blocks * st->max_n_blocks
+                     : int64_t(st->n_blocks_total) * row_cap,
+                 stream);
+    else
+      fill<double>(a.energies64, P, stream);
+  }
+  if ((flags & TMOL_LJLK_DVDX) && !acc) fill<Real>((Real*)dvdx, NA3, stream);
+  if ((flags & TMOL_LJLK_WGRAD) && !acc) fill<Real>((Real*)grad, NA3, stream);
+  const int mode = flags & (TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR |
+                            TMOL_LJLK_NO_ENERGY);
+  switch (mode) {
+    case 0: launch_elec<Real, 0>(a, n_wg, stream); break;
+    case M_DVDX: launch_elec<Real, M_DVDX>(a, n_wg, stream); break;
+    case M_WGRAD: launch_elec<Real, M_WGRAD>(a, n_wg, stream); break;
+    case M_WGRAD | M_NOE: launch_elec<Real, M_WGRAD | M_NOE>(a, n_wg, stream); break;
+    case M_BP: launch_elec<Real, M_BP>(a, n_wg, stream); break;
+    case M_BP | M_WGRAD | M_NOE: launch_elec<Real, M_BP | M_WGRAD | M_NOE>(a, n_wg, stream); break;
+    case M_BP | M_WGRAD: launch_elec<Real, M_BP | M_WGRAD>(a, n_wg, stream); break;
+    default: return TMOL_LJLK_ENOTSUP;
+  }
+  if (!noe && !bp)
+    hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(P)), dim3(256), 0, stream,
+                       (const double*)a.energies64, (Real*)energies, P);
+  return check(hipGetLastError());
+}
 
 extern "C" {
 
@@ -869,6 +1042,54 @@ int tmol_ljlk_inter_block_bondsep(const tmol_ljlk_block_types* bt, int32_t n_pos
     hipLaunchKernelGGL(bondsep_min_kernel, dim3(grid_for(PB * max_n_blocks)), dim3(256), 0, st, v,
                        (const int32_t*)inter_block_bondsep, min_bondsep);
   return check(hipGetLastError());
+}
+
+int tmol_elec_neighbors(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                        const tmol_elec_tables* e, const void* coords, int32_t dtype,
+                        double max_dis, const int32_t* tiles, int64_t n_tiles, void* workspace,
+                        int64_t ws_bytes, int64_t row_capacity, int32_t* dense, void* stream) {
+  int v = validate_elec(s, bt, e, dtype);
+  if (v) return v;
+  if (!coords || !workspace || (n_tiles > 0 && !tiles) || row_capacity <= 0)
+    return TMOL_LJLK_EINVAL;
+  if (!std::isfinite(max_dis) || max_dis < 0) return TMOL_LJLK_EINVAL;
+  if (n_tiles >= (int64_t(1) << 31)) return TMOL_LJLK_ENOTSUP;
+  if (dense && s->layout != TMOL_LJLK_PADDED) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_elec_neighbors<float>(s, bt, e, coords, max_dis, tiles, n_tiles, workspace,
+                                     row_capacity, dense, st, L);
+  return run_elec_neighbors<double>(s, bt, e, coords, max_dis, tiles, n_tiles, workspace,
+                                    row_capacity, dense, st, L);
+}
+
+int tmol_elec_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                    const tmol_elec_tables* e, const void* coords, int32_t dtype,
+                    void* workspace, int64_t ws_bytes, int64_t row_capacity, int32_t flags,
+                    const void* weights, void* energies, void* dvdx, void* grad,
+                    int32_t n_wave_groups, void* stream) {
+  int v = validate_elec(s, bt, e, dtype);
+  if (v) return v;
+  if (!coords || !workspace || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  if (flags & ~(TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY |
+                TMOL_LJLK_ACCUMULATE))
+    return TMOL_LJLK_EINVAL;
+  if (!(flags & TMOL_LJLK_NO_ENERGY) && !energies) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_DVDX) && !dvdx) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_WGRAD) && (!grad || !weights)) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_DVDX) && (flags & TMOL_LJLK_WGRAD)) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_BLOCK_PAIR) && (flags & TMOL_LJLK_DVDX)) return TMOL_LJLK_ENOTSUP;
+  if (n_wave_groups < 0 || n_wave_groups > (1 << 20)) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_elec_score<float>(s, bt, e, coords, workspace, row_capacity, flags, weights,
+                                 energies, dvdx, grad, n_wave_groups, st, L);
+  return run_elec_score<double>(s, bt, e, coords, workspace, row_capacity, flags, weights,
+                                energies, dvdx, grad, n_wave_groups, st, L);
 }
 
 }  // extern "C"

@@ -783,3 +783,222 @@ def ljlk_rotamer_scores(*args):
         raise TypeError("ljlk_rotamer_scores takes 27 arguments (%d given)" % len(args))
     score, idx = LJLKRotamerScoreFunction.apply(*args)
     return [score, idx]
+
+
+# ---------------------------------------------------------------------------
+# elec: a sibling two-body term on the same scheduler
+# (tmol_elec::elec_pose_scores, tmol/score/elec/potentials/compiled.ops.cpp:
+# 22-216, 526-586)
+
+
+class CElecTables(ctypes.Structure):
+    _fields_ = [
+        ("partial_charge", _VP),
+        ("inter_repr_path_distance", _P32),
+        ("intra_repr_path_distance", _P32),
+        ("global_params", ctypes.c_float * 5),
+    ]
+
+
+_elec_bound = False
+
+
+def _elec_lib():
+    global _elec_bound
+    L = lib()
+    with _lib_lock:
+        if not _elec_bound:
+            SP, BP, EP = (ctypes.POINTER(CStack), ctypes.POINTER(CBlockTypes),
+                          ctypes.POINTER(CElecTables))
+            L.tmol_elec_neighbors.argtypes = [
+                SP, BP, EP, _VP, ctypes.c_int32, ctypes.c_double, _VP, ctypes.c_int64, _VP,
+                ctypes.c_int64, ctypes.c_int64, _VP, _VP,
+            ]
+            L.tmol_elec_score.argtypes = [
+                SP, BP, EP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64,
+                ctypes.c_int32, _VP, _VP, _VP, _VP, ctypes.c_int32, _VP,
+            ]
+            _elec_bound = True
+    return L
+
+
+_host_param_cache = {}
+
+
+def _host_floats(t: torch.Tensor, n: int):
+    """Host float32 values of a small parameter tensor; device tensors are
+    read once per (storage, version) and cached, so repeated calls with the
+    scoring module's fixed parameters do not synchronise."""
+    if not t.is_cuda:
+        return [float(v) for v in t.detach().reshape(-1).to(torch.float32).tolist()]
+    key = (t.data_ptr(), t._version, t.dtype, t.numel())
+    v = _host_param_cache.get(key)
+    if v is None:
+        v = [float(x) for x in t.detach().reshape(-1).to("cpu", torch.float32).tolist()]
+        if len(_host_param_cache) > 64:
+            _host_param_cache.clear()
+        _host_param_cache[key] = v
+    if len(v) != n:
+        raise LJLKLibraryError("global_params must hold %d values" % n)
+    return v
+
+
+class ElecPaddedCall:
+    """Device state of one padded-layout elec op call."""
+
+    def __init__(self, args, dtype):
+        (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+         first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+         n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+         max_n_rots_per_pose, min_bondsep, inter_block_bondsep, bt_n_atoms, bt_partial_charge,
+         bt_n_conn, bt_conn_atoms, bt_inter_rpd, bt_intra_rpd, bt_is_ligand, global_params,
+         max_dis, block_pair) = args
+        _require_device(coords)
+        dev = coords.device
+        self.device = dev
+        P, B = first_rot_for_block.shape
+        self.P, self.B = int(P), int(B)
+        NA = coords.shape[0]
+        if coords.dim() != 2 or coords.shape[1] != 3:
+            raise LJLKLibraryError("rot_coords must be [n_atoms, 3]")
+        if rot_coord_offset.numel() != P * B or block_type_ind_for_rot.numel() != P * B:
+            raise LJLKLibraryError("rot index tensors must have P * B entries")
+        if tuple(min_bondsep.shape) != (P, B, B):
+            raise LJLKLibraryError("pose_stack_min_bond_separation must be [P, B, B]")
+        if inter_block_bondsep.dim() != 5 or tuple(inter_block_bondsep.shape[:3]) != (P, B, B):
+            raise LJLKLibraryError("pose_stack_inter_block_bondsep must be [P, B, B, C, C]")
+        T, Amax = bt_partial_charge.shape
+        C = bt_conn_atoms.shape[1] if bt_conn_atoms.dim() == 2 else 0
+        if inter_block_bondsep.shape[3] != C:
+            raise LJLKLibraryError("inter_block_bondsep connection dims must match conn table")
+        for name, t in (("block_type_inter_repr_path_distance", bt_inter_rpd),
+                        ("block_type_intra_repr_path_distance", bt_intra_rpd)):
+            if tuple(t.shape) != (T, Amax, Amax):
+                raise LJLKLibraryError("%s must be [T, Amax, Amax]" % name)
+        if bt_n_atoms.numel() != T or bt_n_conn.numel() != T or bt_is_ligand.numel() != T:
+            raise LJLKLibraryError("block-type tables must have one entry per block type")
+        self.keep = dict(
+            block_atom_offset=_i32(rot_coord_offset, dev),
+            block_type=_i32(block_type_ind_for_rot, dev),
+            atom_pose=_i32(pose_ind_for_atom, dev),
+            min_bondsep=_i32(min_bondsep, dev),
+            inter_block_bondsep=_i32(inter_block_bondsep, dev),
+        )
+        s = CStack()
+        s.layout = PADDED
+        s.n_poses, s.max_n_blocks = self.P, self.B
+        s.n_blocks_total = self.P * self.B
+        s.n_atoms_total = NA
+        for k, v in self.keep.items():
+            setattr(s, k, _p(v))
+        self.cstack = s
+        self.bt_keep = dict(
+            n_atoms=_i32(bt_n_atoms, dev),
+            n_conn=_i32(bt_n_conn, dev),
+            conn_atoms=_i32(bt_conn_atoms, dev),
+            is_ligand=_i32(bt_is_ligand, dev),
+            partial_charge=bt_partial_charge.to(device=dev, dtype=dtype).contiguous(),
+            inter=_i32(bt_inter_rpd, dev),
+            intra=_i32(bt_intra_rpd, dev),
+        )
+        b = CBlockTypes()
+        b.n_block_types, b.max_n_atoms = int(T), int(Amax)
+        b.max_n_tiles = (int(Amax) - 1) // 32 + 1
+        b.max_n_conn = int(C)
+        b.n_atom_types = 0
+        b.dtype = _dtype_code(dtype)
+        for k in ("n_atoms", "n_conn", "conn_atoms", "is_ligand"):
+            setattr(b, k, _p(self.bt_keep[k]))
+        self.cbt = b
+        e = CElecTables()
+        e.partial_charge = _p(self.bt_keep["partial_charge"])
+        e.inter_repr_path_distance = ctypes.cast(self.bt_keep["inter"].data_ptr(), _P32)
+        e.intra_repr_path_distance = ctypes.cast(self.bt_keep["intra"].data_ptr(), _P32)
+        for i, v in enumerate(_host_floats(global_params, 5)):
+            e.global_params[i] = v
+        self.celec = e
+        self.coords = coords.contiguous()
+        self.dtype = dtype
+        self.dcode = _dtype_code(dtype)
+        self.max_dis = float(max_dis)
+        self.block_pair = bool(block_pair)
+        pbs = torch.arange(0, P + 1, dtype=torch.int32) * B
+        self.tiles = neighbor_tiles(pbs, dev)
+        self.ws = Workspace(self.cstack, self.cbt, padded_row_capacity(self.P, self.B),
+                            self.dcode, dev)
+
+    def neighbors(self, dense: Optional[torch.Tensor]):
+        _check(_elec_lib().tmol_elec_neighbors(
+            ctypes.byref(self.cstack), ctypes.byref(self.cbt), ctypes.byref(self.celec),
+            _p(self.coords), self.dcode, self.max_dis, _p(self.tiles), self.tiles.shape[0],
+            _p(self.ws.buf), self.ws.nbytes, self.ws.capacity, _p(dense),
+            _stream(self.device)), "elec neighbors")
+
+    def score(self, flags, weights=None, energies=None, dvdx=None, grad=None, n_wave_groups=0):
+        _check(_elec_lib().tmol_elec_score(
+            ctypes.byref(self.cstack), ctypes.byref(self.cbt), ctypes.byref(self.celec),
+            _p(self.coords), self.dcode, _p(self.ws.buf), self.ws.nbytes, self.ws.capacity,
+            flags, _p(weights), _p(energies), _p(dvdx), _p(grad), n_wave_groups,
+            _stream(self.device)), "elec score")
+
+
+class ElecPoseScoreFunction(torch.autograd.Function):
+    """Forward/backward of the elec pose score (ElecPoseScoreOp,
+    compiled.ops.cpp:22-216): whole-pose mode saves dV/dx [1, NA, 3] and
+    scales it by the incoming (1, P) gradient; block-pair mode re-walks the
+    neighbor list with the (1, P, B, B) gradient as per-block-pair weights
+    (elec_pose_score.impl.hh:858-1125)."""
+
+    @staticmethod
+    def forward(ctx, *args):
+        coords = args[0]
+        dtype = coords.dtype
+        call = ElecPaddedCall(args, dtype)
+        P, B = call.P, call.B
+        dev = coords.device
+        nbr = torch.empty((P, B, B), dtype=torch.int32, device=dev)
+        call.neighbors(nbr)
+        ctx.block_pair = call.block_pair
+        ctx.n_args = len(args)
+        ctx.mark_non_differentiable(nbr)
+        if call.block_pair:
+            score = torch.empty((1, P, B, B), dtype=dtype, device=dev)
+            call.score(BLOCK_PAIR, energies=score.view(-1))
+            ctx.call = call
+            return score, nbr
+        score = torch.empty((1, P), dtype=dtype, device=dev)
+        if coords.requires_grad:
+            dvdx = torch.empty((1, coords.shape[0], 3), dtype=dtype, device=dev)
+            call.score(DVDX, energies=score, dvdx=dvdx)
+            ctx.save_for_backward(dvdx, args[2])
+        else:
+            call.score(0, energies=score)
+            ctx.save_for_backward(None, args[2])
+        return score, nbr
+
+    @staticmethod
+    def backward(ctx, grad_score, grad_nbr):
+        if grad_score is None:
+            return (None,) * ctx.n_args
+        if not ctx.block_pair:
+            dvdx, atom_pose = ctx.saved_tensors
+            if dvdx is None:
+                return (None,) * ctx.n_args
+            # dE/dx[a] = dE/dV[pose(a)] * dV/dx[a] (compiled.ops.cpp:150-168)
+            gs = grad_score.to(dvdx.dtype)[0].index_select(0, atom_pose.to(dvdx.device).long())
+            grad = dvdx[0] * gs.unsqueeze(-1)
+        else:
+            call = ctx.call
+            grad = torch.empty_like(call.coords)
+            w = grad_score.to(call.dtype).contiguous()
+            call.score(BLOCK_PAIR | WGRAD | NO_ENERGY, weights=w, grad=grad)
+        return (grad,) + (None,) * (ctx.n_args - 1)
+
+
+def elec_pose_scores(*args):
+    """Drop-in for ``torch.ops.tmol_elec.elec_pose_scores`` (25 arguments);
+    returns ``[score, block_neighbors]``: score (1, P) or (1, P, B, B)."""
+    if len(args) != 25:
+        raise TypeError("elec_pose_scores takes 25 arguments (%d given)" % len(args))
+    score, nbr = ElecPoseScoreFunction.apply(*args)
+    return [score, nbr]
