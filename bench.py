@@ -131,6 +131,8 @@ def main():
                     help="skip the learned-pruner leg (1024 x 300-residue batch)")
     ap.add_argument("--no-rotamer", action="store_true",
                     help="skip the rotamer-pair scoring leg (16 x 300-residue repacking)")
+    ap.add_argument("--no-elec", action="store_true",
+                    help="skip the elec leg (same 512-pose batch, elec energy + gradient)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported. Poses are "
@@ -283,6 +285,12 @@ def main():
         log("rotamer pairs ...")
         result["rotamer"] = rotamer_bench.run(dev, steps=10, warmup=2, cpu=not args.no_cpu, log=log)
         result["rotamer"].pop("census", None)
+    if not args.no_elec:
+        from tmol_amd import elec_bench
+
+        log("elec ...")
+        result["elec"] = elec_bench.run(dev, poses=poses, cpu=not args.no_cpu, log=log)
+        result["elec"].pop("census", None)
     from tmol_amd import topology_bench
 
     log("topology ...")

@@ -398,6 +398,16 @@ int tmol_elec_score(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* b
                     const void* weights, void* energies, void* dvdx, void* grad,
                     int32_t n_wave_groups, void* stream);
 
+/* Census of the elec work list (float32 only), census[16] (device uint64):
+ * block-pair items, intra-block items, enumerated atom pairs, pairs of two
+ * charged atoms, charged pairs of count-pair weight 0, and evaluated pairs by
+ * region: flat (d < min_dis - 0.25), low fade, Coulomb, high fade, beyond
+ * max_dis.  For algorithmic FLOP accounting. */
+int tmol_elec_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                     const tmol_elec_tables* elec, const void* coords, int32_t dtype,
+                     void* workspace, int64_t ws_bytes, int64_t row_capacity, uint64_t* census,
+                     void* stream);
+
 #ifdef __cplusplus
 }
 #endif

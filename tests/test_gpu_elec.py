@@ -243,3 +243,107 @@ def test_score_function_with_ljlk_and_elec(ubq_pdb):
     numpy.testing.assert_allclose(tot.detach().cpu().numpy(), want, rtol=1e-5)
     tot.sum().backward()
     assert torch.isfinite(coords.grad).all()
+
+
+@pytest.mark.parametrize("dtype", [torch.float32, torch.float64])
+def test_ragged_scorer_matches_oracle(ubq_pdb, dtype):
+    """The ragged batch path (count pair from chain positions, one work
+    list over all poses): per-pose energies and the weighted gradient against
+    the oracle on the padded stack of the same poses."""
+    from tmol_amd.generator import generate_poses
+    from tmol_amd.pose import RaggedPoseStack
+    from tmol_amd.scorer import ElecRaggedScorer
+
+    _say("elec ragged %s" % dtype)
+    poses = [pose_from_pdb(ubq_pdb), pose_from_pdb(ubq_pdb, (20, 50))] + generate_poses(
+        6, (10, 150), seed=5, log_uniform=True)
+    stack = RaggedPoseStack(poses, dtype=dtype)
+    sc = ElecRaggedScorer(stack, TERM, device=DEV, dtype=dtype)
+    x = stack.coords.to(DEV)
+    w = torch.linspace(0.5, 1.5, len(poses), dtype=torch.float64)
+    e, g = sc.energy_and_grad(x, w)
+    e = e.double().cpu().numpy()
+    g = g.double().cpu().numpy()
+    ps = PoseStack(poses, dtype=torch.float64)
+    E64, _, dv64, _ = orc.oracle_elec_scores(*_args(ps, torch.float64), require_grad=True)
+    A = ps.max_n_atoms
+    keep = numpy.concatenate([p * A + numpy.arange(pose.n_atoms) for p, pose in enumerate(poses)])
+    pose_of_atom = numpy.concatenate([numpy.full(pose.n_atoms, p) for p, pose in enumerate(poses)])
+    og = dv64[0, keep] * w.numpy()[pose_of_atom][:, None]
+    tol = 1e-10 if dtype == torch.float64 else 1e-4
+    assert (numpy.abs(e - E64[0]) / numpy.maximum(1.0, numpy.abs(E64[0]))).max() < tol
+    assert numpy.abs(g - og).max() / max(1.0, numpy.abs(og).max()) < tol
+    # energies-only pass agrees with the fused one
+    e2 = sc.energies_only(x).double().cpu().numpy()
+    assert numpy.abs(e2 - e).max() <= 1e-12 * max(1.0, numpy.abs(e).max()) or dtype == torch.float32
+
+
+def test_elec_census_counts(ubq_pdb):
+    """Census of the ragged work list: pair counts from the block sizes,
+    charged-pair counts from the charge tables, and the evaluated pairs
+    inside max_dis against a brute-force count."""
+    from tmol_amd.flops import elec_far_pairs
+    from tmol_amd.pose import RaggedPoseStack
+    from tmol_amd.scorer import ElecRaggedScorer
+
+    _say("elec census")
+    poses = [pose_from_pdb(ubq_pdb)]
+    stack = RaggedPoseStack(poses)
+    sc = ElecRaggedScorer(stack, TERM, device=DEV)
+    x = stack.coords.to(DEV)
+    c = sc.census(x)
+    pose = poses[0]
+    nb = pose.n_blocks
+    q = [TERM.setup_block_type(rt)[0] for rt in pose.residue_types]
+    n = [len(qq) for qq in q]
+    nq = [int((qq != 0).sum()) for qq in q]
+    cnt, items = sc.ws.rows(sc.cstack, sc.tables.cbt)
+    cnt = cnt.cpu().numpy()
+    items = items.cpu().numpy()
+    want_pairs = want_charged = n_inter = 0
+    for g1 in range(nb):
+        for k in range(min(cnt[g1], sc.capacity)):
+            g2 = items[g1, k]
+            if g2 == g1:
+                want_pairs += n[g1] * (n[g1] - 1) // 2
+                want_charged += nq[g1] * (nq[g1] - 1) // 2
+            else:
+                n_inter += 1
+                want_pairs += n[g1] * n[g2]
+                want_charged += nq[g1] * nq[g2]
+    assert c["items"] == n_inter and c["intra_items"] == nb
+    assert c["pairs"] == want_pairs and c["charged"] == want_charged
+    # brute force: charged pairs of count-pair weight > 0 closer than max_dis,
+    # separations from the representative path distances and the pose's
+    # connection-atom separations (elec_pose_score.impl.hh:47-66)
+    ps = PoseStack(poses, dtype=torch.float64)
+    xyz = ps.coords[0].numpy()
+    tabs = [TERM.setup_block_type(rt) for rt in pose.residue_types]
+    conn = [rt.conn_atom for rt in pose.residue_types]
+    ibs = ps.inter_block_bondsep[0].numpy()
+    mins = ps.min_block_bondsep[0].numpy()
+    off = numpy.concatenate([[0], numpy.cumsum(n)])
+    weighted = 0
+    for b1 in range(nb):
+        for b2 in range(b1, nb):
+            d = numpy.linalg.norm(xyz[off[b1]:off[b1 + 1], None] - xyz[None, off[b2]:off[b2 + 1]],
+                                  axis=-1)
+            m = (q[b1][:, None] != 0) & (q[b2][None, :] != 0) & (d < 5.5)
+            if b1 == b2:
+                sep = tabs[b1][2]
+                m &= numpy.triu(numpy.ones_like(m, dtype=bool), 1)
+            elif mins[b1, b2] > 4:
+                sep = numpy.full(m.shape, mins[b1, b2])
+            else:
+                sep = numpy.full(m.shape, 5)
+                for c1, a1 in enumerate(conn[b1]):
+                    for c2, a2 in enumerate(conn[b2]):
+                        v = (tabs[b1][1][a1][:, None] + ibs[b1, b2, c1, c2]
+                             + tabs[b2][1][a2][None, :])
+                        sep = numpy.minimum(sep, v)
+            weighted += int((m & (sep >= 4)).sum())
+    evaluated = c["flat"] + c["low"] + c["coulomb"] + c["high"]
+    _say("census %s; brute force %d" % (c, weighted))
+    # float32 distances may move a pair across max_dis: allow a handful
+    assert abs(evaluated - weighted) <= 3
+    assert elec_far_pairs(c) >= 0

@@ -91,6 +91,12 @@ struct ElecAtom {
   uint32_t pd;  // byte c: representative path distance from connection atom c
 };
 
+// elec census slots (counts of atom pairs; FLOP accounting)
+enum : int {
+  EC_ITEMS = 0, EC_INTRA_ITEMS, EC_PAIRS, EC_CHARGED, EC_CP0,
+  EC_FLAT, EC_LOW, EC_COULOMB, EC_HIGH, EC_BEYOND, EC_N
+};
+
 template <typename Real>
 struct ElecArgs {
   StackView s;
@@ -110,6 +116,7 @@ struct ElecArgs {
   Real* dvdx;            // [NA, 3]
   Real* grad;            // [NA, 3]
   int64_t n_atoms;
+  unsigned long long* census;  // [16] with M_CENSUS
 };
 
 // Per block type: packed inter-block path distances and the intra-block
@@ -173,10 +180,15 @@ struct ElecLDS {
   Real wt[3];
 };
 
+// per-lane census counters (M_CENSUS): evaluated pairs by potential region
+struct ElecCensus {
+  uint32_t flat = 0, low = 0, coulomb = 0, high = 0, beyond = 0;
+};
+
 template <typename Real, int MODE>
 __device__ __forceinline__ void eval_elec(ElecLDS<Real, MODE>& L, int consume,
                                           const ElecConst<Real>& k, Real wrow, LaneSum& e_sum,
-                                          int lane) {
+                                          ElecCensus& cen, int lane) {
   using C = RowCtx<Real, MODE>;
   using Acc = LdsAcc<Real>;
   using V = typename V4<Real>::T;
@@ -191,6 +203,19 @@ __device__ __forceinline__ void eval_elec(ElecLDS<Real, MODE>& L, int consume,
       const Real d2 = dx * dx + dy * dy + dz * dz;
       const Real inv_d = fast_rsqrt(d2);
       const Real dist = d2 > Real(0) ? d2 * inv_d : Real(0);
+      if (MODE & M_CENSUS) {
+        if (dist >= k.hpe)
+          ++cen.beyond;
+        else if (dist < k.lps)
+          ++cen.flat;
+        else if (dist < k.lpe)
+          ++cen.low;
+        else if (dist < k.hps)
+          ++cen.coulomb;
+        else
+          ++cen.high;
+        continue;
+      }
       Real v, dv;
       elec_unit<Real, C::DERIV>(dist, k, v, dv);
       const Real qq = p1.w * p2.w;
@@ -273,6 +298,9 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
   const BTView& bt = A.bt;
   RowQueue queue(A.row_cursor, s.G);
   const Real cull_pad = k.hpe + Real(1e-3);
+  constexpr bool CEN = MODE & M_CENSUS;
+  ElecCensus cen;
+  unsigned long long c_items = 0, c_intra = 0, c_pairs = 0, c_charged = 0, c_cp0 = 0;
 
   if (C::DERIV)
     for (int i = lane; i < 3 * RSLOTS; i += WAVE) (&L.grad[0][0])[i] = 0;
@@ -290,11 +318,14 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
     const bool lig1 = bt.is_ligand[t1] != 0;
     const V sph1 = *(const V*)(A.spheres + 4 * int64_t(g1));
     const Real wrow = (C::WGRAD && !C::BP) ? A.weights[p] : Real(0);
+    bool q1 = false;
     if (lane < n1) {
       ElecAtom<Real> R;
       load_elec_atom(R, A, t1, o1, lane);
       stage_elec_atom(L, lane, 0, lane, R);
+      q1 = R.q != Real(0);
     }
+    const int nq1 = __popcll(__ballot(q1));
     if (C::BP && lane < 3) L.ebp[lane] = 0;
     LaneSum e_sum;
     int n_f = 0, seq = 0, done = 0, end0 = 0, end1 = 0;
@@ -306,7 +337,7 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
 
     auto drain = [&](bool all) {
       const int c = all ? n_f : (n_f & ~(WAVE - 1));
-      if (c > 0) eval_elec<Real, MODE>(L, c, k, wrow, e_sum, lane);
+      if (c > 0) eval_elec<Real, MODE>(L, c, k, wrow, e_sum, cen, lane);
       wave_sync();
       fifo_shift(L.fifo, c, n_f, lane);
       n_f -= c;
@@ -363,6 +394,12 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
           wave_sync();
           const int n_ip = uni(A.intra_count[t1]);
           const uint16_t* ip = A.intra_pairs + int64_t(t1) * INTRA_MAX;
+          if (CEN) {
+            c_intra += 1;
+            c_pairs += uint64_t(n1) * (n1 - 1) / 2;
+            c_charged += uint64_t(nq1) * (nq1 - 1) / 2;
+            c_cp0 += uint64_t(nq1) * (nq1 - 1) / 2 - n_ip;
+          }
           for (int q0 = 0; q0 < n_ip; q0 += WAVE) {
             if (n_f > LIST - WAVE) drain(false);
             bool in = false, w02 = false;
@@ -421,6 +458,12 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
           const uint64_t mnear = __ballot(near);
           const uint32_t m2 = (uint32_t)mnear, m1 = (uint32_t)(mnear >> 32);
           const int n1c = __popc(m1), n2c = __popc(m2);
+          if (CEN) {
+            const int nq2 = __popcll(__ballot(lane < n2 && cur.q != Real(0)));
+            c_items += 1;
+            c_pairs += uint64_t(n1) * n2;
+            c_charged += uint64_t(nq1) * nq2;
+          }
           if (n1c == 0 || n2c == 0) {
             ring = r;
             if (n_f >= WAVE) drain(false);
@@ -452,6 +495,7 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
           auto test_pair = [&](int t, int u, bool& in, bool& w02, int& a1, int& a2) {
             in = w02 = false;
             a1 = a2 = 0;
+            bool cp0 = false;
             if (t < n1c) {
               const V P1 = *(const V*)L.c1[t], P2 = *(const V*)L.c2[u];
               a1 = int(P1.w);
@@ -462,7 +506,9 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
                                       : min_sep;
               in = sep >= 4 && dx * dx + dy * dy + dz * dz < k.cull2;
               w02 = sep == 4;
+              cp0 = sep < 4;
             }
+            if (CEN) c_cp0 += __popcll(__ballot(cp0));
           };
           auto append = [&](bool in, bool w02, int a1, int a2) {
             const uint64_t m = __ballot(in);
@@ -510,11 +556,30 @@ __global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
       A.energies_bp[bp_intra] = LdsAcc<Real>::dec(L.ebp[2]);
       L.ebp[2] = 0;
     }
-    if (!C::NOE && !C::BP) {
+    if (!C::NOE && !C::BP && !CEN) {
       const double e = wave_sum(e_sum.value());
       if (lane == 0) unsafeAtomicAdd(&A.energies64[p], e);
     }
     wave_sync();
+  }
+  if (CEN) {
+    const unsigned long long f = wave_sum((unsigned long long)cen.flat);
+    const unsigned long long lo = wave_sum((unsigned long long)cen.low);
+    const unsigned long long co = wave_sum((unsigned long long)cen.coulomb);
+    const unsigned long long hi = wave_sum((unsigned long long)cen.high);
+    const unsigned long long be = wave_sum((unsigned long long)cen.beyond);
+    if (lane == 0) {
+      atomicAdd(&A.census[EC_ITEMS], c_items);
+      atomicAdd(&A.census[EC_INTRA_ITEMS], c_intra);
+      atomicAdd(&A.census[EC_PAIRS], c_pairs);
+      atomicAdd(&A.census[EC_CHARGED], c_charged);
+      atomicAdd(&A.census[EC_CP0], c_cp0);
+      atomicAdd(&A.census[EC_FLAT], f);
+      atomicAdd(&A.census[EC_LOW], lo);
+      atomicAdd(&A.census[EC_COULOMB], co);
+      atomicAdd(&A.census[EC_HIGH], hi);
+      atomicAdd(&A.census[EC_BEYOND], be);
+    }
   }
 }
 

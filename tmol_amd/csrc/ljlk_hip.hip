@@ -480,7 +480,7 @@ static int run_elec_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types
                           const tmol_elec_tables* e, const void* coords, void* ws,
                           int64_t row_cap, int flags, const void* weights, void* energies,
                           void* dvdx, void* grad, int n_wg, hipStream_t stream,
-                          const WsLayout& Lw) {
+                          const WsLayout& Lw, unsigned long long* census = nullptr) {
   ElecArgs<Real> a{};
   a.s = make_stack_view(st, btp->max_n_conn);
   a.bt = make_bt_view(btp);
@@ -502,6 +502,13 @@ static int run_elec_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types
   a.dvdx = (Real*)dvdx;
   a.grad = (Real*)grad;
   a.n_atoms = st->n_atoms_total;
+  a.census = census;
+  if (census) {
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, stream, census, 16,
+                       0ull);
+    launch_elec<Real, M_CENSUS>(a, n_wg, stream);
+    return check(hipGetLastError());
+  }
   const bool bp = flags & TMOL_LJLK_BLOCK_PAIR;
   const bool noe = flags & TMOL_LJLK_NO_ENERGY;
   const bool acc = flags & TMOL_LJLK_ACCUMULATE;
@@ -1090,6 +1097,21 @@ int tmol_elec_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                                  energies, dvdx, grad, n_wave_groups, st, L);
   return run_elec_score<double>(s, bt, e, coords, workspace, row_capacity, flags, weights,
                                 energies, dvdx, grad, n_wave_groups, st, L);
+}
+
+int tmol_elec_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                     const tmol_elec_tables* e, const void* coords, int32_t dtype,
+                     void* workspace, int64_t ws_bytes, int64_t row_capacity, uint64_t* census,
+                     void* stream) {
+  int v = validate_elec(s, bt, e, dtype);
+  if (v) return v;
+  if (!coords || !workspace || !census || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32) return TMOL_LJLK_ENOTSUP;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  return run_elec_score<float>(s, bt, e, coords, workspace, row_capacity, 0, nullptr, nullptr,
+                               nullptr, nullptr, 0, (hipStream_t)stream, L,
+                               (unsigned long long*)census);
 }
 
 }  // extern "C"

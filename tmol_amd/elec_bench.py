@@ -1,0 +1,174 @@
+"""Elec benchmark: energy + gradient of the elec term on the 512-pose ragged
+batch (the same poses as the LJ+LK headline), on the same scheduler.
+
+One step = elec's neighbor pass + the elec row kernel in weighted-gradient
+mode (per-pose energies and sum_p w[p] dE_p/dx), inputs resident in HBM.
+Reported: poses/s of the step, the kernel alone (HIP events on its stream),
+its roofline from the atom-pair census x elec FLOP table
+(tmol_amd/flops.py), and the reference's own CPU elec op (oracle/_ref, one
+thread, forward + backward) on a bounded sample of the poses.
+
+    python -m tmol_amd.elec_bench [--steps 20] [--no-cpu]
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+FP32_PEAK_TFLOPS = 157.3
+
+
+def cpu_baseline(poses, stride=8, time_budget_s=12.0):
+    """The reference CPU elec op (one thread) on every ``stride``-th pose,
+    each scored alone, forward + backward; the C oracle if oracle/_ref is
+    absent."""
+    import numpy
+    import torch
+
+    from oracle import oracle as orc
+
+    from .pose import PoseStack
+    from .score.elec_energy_term import ElecEnergyTerm, elec_op_args
+
+    torch.set_num_threads(1)
+    term = ElecEnergyTerm()
+    try:
+        ref = orc.load_reference_elec_op()
+        kind = "reference"
+    except FileNotFoundError:
+        ref, kind = None, "port"
+    t_total, n_done, n_res = 0.0, 0, 0
+    for p in poses[::stride]:
+        args = elec_op_args(PoseStack([p]), term)
+        if ref is not None:
+            c = args[0].clone().requires_grad_(True)
+            args[0] = c
+            t0 = time.perf_counter()
+            s, _ = ref(*args)
+            s.sum().backward()
+        else:
+            t0 = time.perf_counter()
+            orc.oracle_elec_scores(*args, require_grad=True, dtype=numpy.float32)
+        t_total += time.perf_counter() - t0
+        n_done += 1
+        n_res += p.n_blocks
+        if t_total > time_budget_s:
+            break
+    return {
+        "value": n_done / t_total,
+        "unit": "poses/s",
+        "cores": 1,
+        "kind": kind,
+        "sample": "%d poses (every %dth of the batch, %d residues), each scored alone, "
+                  "forward+backward, torch threads = 1, %.1f s" % (n_done, stride, n_res, t_total),
+    }
+
+
+def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=True, log=print):
+    import numpy
+    import torch
+
+    from .flops import elec_algorithmic_flops, elec_far_pairs
+    from .generator import workload_poses
+    from .pose import RaggedPoseStack
+    from .score.elec_energy_term import ElecEnergyTerm
+    from .scorer import ElecRaggedScorer
+
+    if poses is None:
+        poses = workload_poses("ragged_512x50_1200")
+    stack = RaggedPoseStack(poses)
+    term = ElecEnergyTerm()
+    scorer = ElecRaggedScorer(stack, term, device=device)
+    coords = stack.coords.to(device)
+    w = scorer._weights(1.0)
+    scorer.neighbors(coords)
+    n_items = scorer.check_capacity()
+    census = scorer.census(coords)
+    flops = elec_algorithmic_flops(census)
+    stream = torch.cuda.current_stream(device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    scorer.neighbors(coords)
+    for _ in range(3):
+        scorer.score(coords, 4, weights=w, energies=scorer.energies, grad=scorer.grad)
+    torch.cuda.synchronize(device)
+    ev0.record(stream)
+    for _ in range(kernel_reps):
+        scorer.score(coords, 4, weights=w, energies=scorer.energies, grad=scorer.grad)
+    ev1.record(stream)
+    torch.cuda.synchronize(device)
+    score_ms = ev0.elapsed_time(ev1) / kernel_reps  # + 2 fill and 1 finalize kernels
+    ev0.record(stream)
+    for _ in range(kernel_reps):
+        scorer.neighbors(coords)
+    ev1.record(stream)
+    torch.cuda.synchronize(device)
+    nbr_ms = ev0.elapsed_time(ev1) / kernel_reps
+
+    for _ in range(warmup):
+        scorer.energy_and_grad(coords, w)
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        scorer.energy_and_grad(coords, w)
+    torch.cuda.synchronize(device)
+    ms = (time.perf_counter() - t0) * 1000.0 / steps
+    e = scorer.energies.double().cpu()
+    achieved = flops / (score_ms * 1e-3) / 1e12
+    traffic = None
+    prof = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
+                        "elec_kernel_ragged512_traffic.json")
+    if os.path.exists(prof):
+        try:
+            traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    res = {
+        "workload": "ragged_512x50_1200",
+        "poses": stack.n_poses,
+        "value": stack.n_poses / (ms * 1e-3),
+        "unit": "poses/s",
+        "ms_per_step": ms,
+        "neighbors_ms": nbr_ms,
+        "score_ms": score_ms,
+        "block_pairs": n_items,
+        "atom_pairs": census["pairs"],
+        "charged_pairs_within_max_dis": census["flat"] + census["low"] + census["coulomb"]
+        + census["high"],
+        "charged_pairs_beyond": elec_far_pairs(census),
+        "energies_finite": bool(torch.isfinite(e).all()),
+        "dtype": "f32",
+        "roofline": {
+            "bound": "mfma",
+            "achieved": achieved,
+            "peak": FP32_PEAK_TFLOPS,
+            "unit": "TFLOP/s",
+            "frac": achieved / FP32_PEAK_TFLOPS,
+            "traffic": traffic,
+            "kernel": "elec_rows_kernel<float, WGRAD> (FP32 vector ALU)",
+            "algorithmic_flops_per_launch": flops,
+        },
+        "census": census,
+    }
+    if cpu:
+        log("elec cpu baseline ...")
+        res["cpu_baseline"] = cpu_baseline(poses)
+    return res
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    args = ap.parse_args()
+    import torch
+
+    res = run(torch.device("cuda:0"), steps=args.steps, cpu=not args.no_cpu,
+              log=lambda m: print("[elec] " + m, file=sys.stderr, flush=True))
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,6 +65,45 @@ FLOPS_ENERGY = {
 }
 
 
+# Elec (potentials.hh:55-201), per enumerated atom pair, in the same
+# convention (energy + gradient):
+#   pairs with a zero charge or count-pair weight 0       0
+#   beyond max_dis                                          8 (distance)
+#   flat (d < min_dis - 0.25)   d, q1 q2 scale, w, accumulate          DIST+SQRT+3
+#   low / high Hermite fade     t, Horner V and dV, q1 q2 (2), w (2),
+#                               gradient, accumulate                   DIST+SQRT+13+4+GRAD+1
+#   Coulomb                     eps: exp, x S, polynomial, product,
+#                               D - (8); V: C1 / (d eps) - C2 (3);
+#                               dV: eps' (4), eps + d eps' (2),
+#                               -C1 (..) / (d^2 eps^2) (5); q1 q2 (2),
+#                               w (2), gradient, accumulate             DIST+SQRT+26+GRAD+1
+ELEC_CENSUS_FIELDS = [
+    "items", "intra_items", "pairs", "charged", "cp0",
+    "flat", "low", "coulomb", "high", "beyond",
+]
+
+ELEC_FLOPS_ENERGY_GRAD = {
+    "far": DIST,
+    "flat": DIST + SQRT + 3,
+    "low": DIST + SQRT + 13 + 4 + GRAD + 1,
+    "coulomb": DIST + SQRT + 26 + GRAD + 1,
+    "high": DIST + SQRT + 13 + 4 + GRAD + 1,
+}
+
+
+def elec_far_pairs(census: dict) -> int:
+    """Charged pairs of non-zero weight beyond max_dis (culled before the
+    evaluation, or evaluated to zero)."""
+    inside = census["flat"] + census["low"] + census["coulomb"] + census["high"]
+    return census["charged"] - census["cp0"] - inside
+
+
+def elec_algorithmic_flops(census: dict) -> int:
+    n = dict(census)
+    n["far"] = elec_far_pairs(census)
+    return sum(n[k] * f for k, f in ELEC_FLOPS_ENERGY_GRAD.items())
+
+
 def census_dict(values):
     return {k: int(v) for k, v in zip(CENSUS_FIELDS, values)}
 

@@ -818,6 +818,9 @@ def _elec_lib():
                 SP, BP, EP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64,
                 ctypes.c_int32, _VP, _VP, _VP, _VP, ctypes.c_int32, _VP,
             ]
+            L.tmol_elec_census.argtypes = [
+                SP, BP, EP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64, _VP, _VP,
+            ]
             _elec_bound = True
     return L
 
@@ -841,6 +844,49 @@ def _host_floats(t: torch.Tensor, n: int):
     if len(v) != n:
         raise LJLKLibraryError("global_params must hold %d values" % n)
     return v
+
+
+class ElecTypeTables:
+    """Device copies of elec's block-type tables and the C structs that
+    point at them (block types: n_atoms, n_conn, conn_atoms and elec's
+    ligand flags; elec: charges, representative path distances, global
+    parameters as host floats)."""
+
+    def __init__(self, n_atoms, n_conn, conn_atoms, is_ligand, partial_charge, inter_rpd,
+                 intra_rpd, global_params, dtype, device):
+        T, Amax = partial_charge.shape
+        C = conn_atoms.shape[1] if conn_atoms.dim() == 2 else 0
+        for name, t in (("block_type_inter_repr_path_distance", inter_rpd),
+                        ("block_type_intra_repr_path_distance", intra_rpd)):
+            if tuple(t.shape) != (T, Amax, Amax):
+                raise LJLKLibraryError("%s must be [T, Amax, Amax]" % name)
+        if n_atoms.numel() != T or n_conn.numel() != T or is_ligand.numel() != T:
+            raise LJLKLibraryError("block-type tables must have one entry per block type")
+        self.t = dict(
+            n_atoms=_i32(n_atoms, device),
+            n_conn=_i32(n_conn, device),
+            conn_atoms=_i32(conn_atoms, device),
+            is_ligand=_i32(is_ligand, device),
+            partial_charge=partial_charge.to(device=device, dtype=dtype).contiguous(),
+            inter=_i32(inter_rpd, device),
+            intra=_i32(intra_rpd, device),
+        )
+        b = CBlockTypes()
+        b.n_block_types, b.max_n_atoms = int(T), int(Amax)
+        b.max_n_tiles = (int(Amax) - 1) // 32 + 1
+        b.max_n_conn = int(C)
+        b.n_atom_types = 0
+        b.dtype = _dtype_code(dtype)
+        for k in ("n_atoms", "n_conn", "conn_atoms", "is_ligand"):
+            setattr(b, k, _p(self.t[k]))
+        self.cbt = b
+        e = CElecTables()
+        e.partial_charge = _p(self.t["partial_charge"])
+        e.inter_repr_path_distance = ctypes.cast(self.t["inter"].data_ptr(), _P32)
+        e.intra_repr_path_distance = ctypes.cast(self.t["intra"].data_ptr(), _P32)
+        for i, v in enumerate(_host_floats(global_params, 5)):
+            e.global_params[i] = v
+        self.celec = e
 
 
 class ElecPaddedCall:
@@ -867,16 +913,9 @@ class ElecPaddedCall:
             raise LJLKLibraryError("pose_stack_min_bond_separation must be [P, B, B]")
         if inter_block_bondsep.dim() != 5 or tuple(inter_block_bondsep.shape[:3]) != (P, B, B):
             raise LJLKLibraryError("pose_stack_inter_block_bondsep must be [P, B, B, C, C]")
-        T, Amax = bt_partial_charge.shape
         C = bt_conn_atoms.shape[1] if bt_conn_atoms.dim() == 2 else 0
         if inter_block_bondsep.shape[3] != C:
             raise LJLKLibraryError("inter_block_bondsep connection dims must match conn table")
-        for name, t in (("block_type_inter_repr_path_distance", bt_inter_rpd),
-                        ("block_type_intra_repr_path_distance", bt_intra_rpd)):
-            if tuple(t.shape) != (T, Amax, Amax):
-                raise LJLKLibraryError("%s must be [T, Amax, Amax]" % name)
-        if bt_n_atoms.numel() != T or bt_n_conn.numel() != T or bt_is_ligand.numel() != T:
-            raise LJLKLibraryError("block-type tables must have one entry per block type")
         self.keep = dict(
             block_atom_offset=_i32(rot_coord_offset, dev),
             block_type=_i32(block_type_ind_for_rot, dev),
@@ -892,31 +931,11 @@ class ElecPaddedCall:
         for k, v in self.keep.items():
             setattr(s, k, _p(v))
         self.cstack = s
-        self.bt_keep = dict(
-            n_atoms=_i32(bt_n_atoms, dev),
-            n_conn=_i32(bt_n_conn, dev),
-            conn_atoms=_i32(bt_conn_atoms, dev),
-            is_ligand=_i32(bt_is_ligand, dev),
-            partial_charge=bt_partial_charge.to(device=dev, dtype=dtype).contiguous(),
-            inter=_i32(bt_inter_rpd, dev),
-            intra=_i32(bt_intra_rpd, dev),
-        )
-        b = CBlockTypes()
-        b.n_block_types, b.max_n_atoms = int(T), int(Amax)
-        b.max_n_tiles = (int(Amax) - 1) // 32 + 1
-        b.max_n_conn = int(C)
-        b.n_atom_types = 0
-        b.dtype = _dtype_code(dtype)
-        for k in ("n_atoms", "n_conn", "conn_atoms", "is_ligand"):
-            setattr(b, k, _p(self.bt_keep[k]))
-        self.cbt = b
-        e = CElecTables()
-        e.partial_charge = _p(self.bt_keep["partial_charge"])
-        e.inter_repr_path_distance = ctypes.cast(self.bt_keep["inter"].data_ptr(), _P32)
-        e.intra_repr_path_distance = ctypes.cast(self.bt_keep["intra"].data_ptr(), _P32)
-        for i, v in enumerate(_host_floats(global_params, 5)):
-            e.global_params[i] = v
-        self.celec = e
+        self.tables = ElecTypeTables(bt_n_atoms, bt_n_conn, bt_conn_atoms, bt_is_ligand,
+                                     bt_partial_charge, bt_inter_rpd, bt_intra_rpd,
+                                     global_params, dtype, dev)
+        self.cbt = self.tables.cbt
+        self.celec = self.tables.celec
         self.coords = coords.contiguous()
         self.dtype = dtype
         self.dcode = _dtype_code(dtype)

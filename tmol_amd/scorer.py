@@ -22,6 +22,30 @@ from .pose import RaggedPoseStack
 _TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
 
 
+def ragged_cstack(stack: RaggedPoseStack, dev):
+    """The C stack struct of a ragged pose stack and the device tensors its
+    pointers refer to."""
+    i32 = lambda t: ops._i32(t, dev)  # noqa: E731
+    keep = dict(
+        block_atom_offset=i32(stack.block_atom_offset),
+        block_type=i32(stack.block_type),
+        block_pose=i32(stack.block_pose),
+        pose_block_start=i32(stack.pose_block_start),
+        atom_pose=i32(stack.atom_pose),
+        block_conn_pos=i32(stack.block_conn_pos),
+        block_segment=i32(stack.block_segment),
+    )
+    s = ops.CStack()
+    s.layout = ops.RAGGED
+    s.n_poses = stack.n_poses
+    s.max_n_blocks = stack.max_n_blocks
+    s.n_blocks_total = stack.n_blocks_total
+    s.n_atoms_total = stack.n_atoms_total
+    for k, v in keep.items():
+        setattr(s, k, ops._p(v))
+    return s, keep
+
+
 class LJLKRaggedScorer:
     """Scores a ``RaggedPoseStack`` with the HIP library.
 
@@ -40,25 +64,7 @@ class LJLKRaggedScorer:
         self.P = stack.n_poses
         self.NA = stack.n_atoms_total
         self.G = stack.n_blocks_total
-        i32 = lambda t: ops._i32(t, dev)  # noqa: E731
-        self.keep = dict(
-            block_atom_offset=i32(stack.block_atom_offset),
-            block_type=i32(stack.block_type),
-            block_pose=i32(stack.block_pose),
-            pose_block_start=i32(stack.pose_block_start),
-            atom_pose=i32(stack.atom_pose),
-            block_conn_pos=i32(stack.block_conn_pos),
-            block_segment=i32(stack.block_segment),
-        )
-        s = ops.CStack()
-        s.layout = ops.RAGGED
-        s.n_poses = self.P
-        s.max_n_blocks = stack.max_n_blocks
-        s.n_blocks_total = self.G
-        s.n_atoms_total = self.NA
-        for k, v in self.keep.items():
-            setattr(s, k, ops._p(v))
-        self.cstack = s
+        self.cstack, self.keep = ragged_cstack(stack, dev)
         self.bt = ops.BlockTypeTables(
             pbt.n_atoms, pbt.ljlk_n_heavy_atoms_in_tile, pbt.ljlk_heavy_atoms_in_tile,
             pbt.atom_types, pbt.n_conn, pbt.conn_atom, pbt.ljlk_bond_separation,
@@ -234,6 +240,96 @@ class LJLKRaggedScorer:
             except OSError:
                 pass
         return entry
+
+
+class ElecRaggedScorer:
+    """Elec over a ``RaggedPoseStack`` on the same scheduler: its own
+    neighbor pass (elec's max_dis) into its own work list, then the
+    persistent elec row kernel.  ``energy_and_grad(coords, weights)`` gives
+    per-pose energies [P] and the weighted gradient
+    sum_p weights[p] dE_p/dx [NA, 3] in one pass."""
+
+    def __init__(self, stack: RaggedPoseStack, term, device="cuda", dtype=torch.float32,
+                 capacity: Optional[int] = None, wave_groups: Optional[int] = None):
+        dev = torch.device(device)
+        self.device = dev
+        self.dtype = dtype
+        self.dcode = ops._dtype_code(dtype)
+        self.stack = stack
+        pbt = stack.packed_block_types
+        self.P = stack.n_poses
+        self.NA = stack.n_atoms_total
+        self.G = stack.n_blocks_total
+        self.cstack, self.keep = ragged_cstack(stack, dev)
+        t = term.packed_tables(pbt)
+        self.tables = ops.ElecTypeTables(
+            pbt.n_atoms, pbt.n_conn, pbt.conn_atom, t["is_ligand_fragment"], t["partial_charge"],
+            t["inter_repr_path_distance"], t["intra_repr_path_distance"], term.global_params,
+            dtype, dev)
+        self.max_dis = term.max_dis
+        self.tiles = ops.neighbor_tiles(stack.pose_block_start, dev)
+        max_nb = int(stack.n_blocks_per_pose.max())
+        self.capacity = int(capacity or max_nb)
+        self.ws = ops.Workspace(self.cstack, self.tables.cbt, self.capacity, self.dcode, dev)
+        self.energies = torch.zeros((self.P,), dtype=dtype, device=dev)
+        self.grad = torch.zeros((self.NA, 3), dtype=dtype, device=dev)
+        self.wave_groups = wave_groups or 0
+        self._checked_capacity = self.capacity >= max_nb
+
+    def _refs(self):
+        return ctypes.byref(self.cstack), ctypes.byref(self.tables.cbt), ctypes.byref(self.tables.celec)
+
+    def neighbors(self, coords: torch.Tensor):
+        ops._check(ops._elec_lib().tmol_elec_neighbors(
+            *self._refs(), ops._p(coords), self.dcode, self.max_dis, ops._p(self.tiles),
+            self.tiles.shape[0], ops._p(self.ws.buf), self.ws.nbytes, self.ws.capacity, None,
+            ops._stream(self.device)), "elec neighbors")
+
+    def score(self, coords, flags, weights=None, energies=None, dvdx=None, grad=None):
+        ops._check(ops._elec_lib().tmol_elec_score(
+            *self._refs(), ops._p(coords), self.dcode, ops._p(self.ws.buf), self.ws.nbytes,
+            self.ws.capacity, flags, ops._p(weights), ops._p(energies), ops._p(dvdx),
+            ops._p(grad), int(self.wave_groups), ops._stream(self.device)), "elec score")
+
+    def check_capacity(self):
+        n, overflow = self.ws.counters(self.device)
+        if overflow:
+            raise ops.LJLKLibraryError(
+                "work list overflow: %d block pairs > capacity %d" % (n, self.capacity))
+        return n
+
+    def _weights(self, weights):
+        if weights is None:
+            weights = 1.0
+        w = torch.as_tensor(weights, dtype=self.dtype, device=self.device)
+        if w.dim() == 0:
+            w = w.expand(self.P)
+        return w.contiguous()
+
+    def energy_and_grad(self, coords: torch.Tensor, weights=None):
+        w = self._weights(weights)
+        self.neighbors(coords)
+        self.score(coords, ops.WGRAD, weights=w, energies=self.energies, grad=self.grad)
+        if not self._checked_capacity:
+            self.check_capacity()
+            self._checked_capacity = True
+        return self.energies, self.grad
+
+    def energies_only(self, coords: torch.Tensor):
+        self.neighbors(coords)
+        self.score(coords, 0, energies=self.energies)
+        return self.energies
+
+    def census(self, coords: torch.Tensor) -> dict:
+        """Atom-pair census of the current work list (float32 scorer)."""
+        from .flops import ELEC_CENSUS_FIELDS
+
+        out = torch.zeros(16, dtype=torch.int64, device=self.device)
+        self.neighbors(coords)
+        ops._check(ops._elec_lib().tmol_elec_census(
+            *self._refs(), ops._p(coords), self.dcode, ops._p(self.ws.buf), self.ws.nbytes,
+            self.ws.capacity, ops._p(out), ops._stream(self.device)), "elec census")
+        return {k: int(v) for k, v in zip(ELEC_CENSUS_FIELDS, out.cpu().tolist())}
 
 
 class RaggedScoreFunction(torch.autograd.Function):
