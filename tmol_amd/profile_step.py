@@ -3,7 +3,10 @@ the scoring kernel a fixed number of times (no autotuning, no CPU work), so
 that ``rocprofv3`` sees a short, repeatable kernel sequence.
 
     python -m tmol_amd.profile_step --workload ragged_512x50_1200 --reps 5 \
-        [--mode wgrad|dvdx|energy] [--general] [--wave-groups N]
+        [--mode wgrad|dvdx|energy] [--general] [--wave-groups N] [--term ljlk|elec]
+
+``--term elec`` runs the elec term's neighbor pass and row kernel instead
+(weights 1 per pose in wgrad mode).
 
 ``--copy-bytes N`` instead copies an N-byte device buffer ``reps`` times with
 torch's copy kernel: the reference point that scales the HBM counters
@@ -28,6 +31,7 @@ def main(argv=None):
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--copy-bytes", type=int, default=0)
+    ap.add_argument("--term", default="ljlk", choices=["ljlk", "elec"])
     args = ap.parse_args(argv)
 
     if args.copy_bytes > 0:
@@ -54,14 +58,25 @@ def main(argv=None):
     dtype = torch.float32 if args.dtype == "f32" else torch.float64
     poses = workload_poses(args.workload, args.n_poses)
     stack = RaggedPoseStack(poses, dtype=dtype)
-    sc = LJLKRaggedScorer(stack, LJLKEnergyTerm(device=dev), device=dev, dtype=dtype,
-                          wave_groups=args.wave_groups)
+    if args.term == "elec":
+        from .score.elec_energy_term import ElecEnergyTerm
+        from .scorer import ElecRaggedScorer
+
+        sc = ElecRaggedScorer(stack, ElecEnergyTerm(), device=dev, dtype=dtype,
+                              wave_groups=args.wave_groups)
+        w = sc._weights(1.0)
+        n_planes = 1
+    else:
+        sc = LJLKRaggedScorer(stack, LJLKEnergyTerm(device=dev), device=dev, dtype=dtype,
+                              wave_groups=args.wave_groups)
+        w = sc._weights(torch.tensor([1.0, 0.55, 1.0]))
+        n_planes = 3
     x = stack.coords.to(dev)
-    w = sc._weights(torch.tensor([1.0, 0.55, 1.0]))
     flags = {"wgrad": ops.WGRAD, "dvdx": ops.DVDX, "energy": 0}[args.mode]
     if args.general:
         flags |= ops.GENERAL_KERNEL
-    dvdx = torch.empty((3, sc.NA, 3), dtype=dtype, device=dev) if args.mode == "dvdx" else None
+    dvdx = (torch.empty((n_planes, sc.NA, 3), dtype=dtype, device=dev)
+            if args.mode == "dvdx" else None)
     sc.neighbors(x)
     n_items = sc.check_capacity()
     torch.cuda.synchronize()
@@ -72,7 +87,8 @@ def main(argv=None):
         sc.score(x, flags, weights=w, energies=sc.energies, dvdx=dvdx, grad=sc.grad)
         torch.cuda.synchronize()
         times.append(time.perf_counter() - t0)
-    print(json.dumps({"workload": args.workload, "mode": args.mode, "general": args.general,
+    print(json.dumps({"workload": args.workload, "term": args.term, "mode": args.mode,
+                      "general": args.general,
                       "block_pairs": n_items, "step_ms": [1e3 * t for t in times]}), flush=True)
 
 
