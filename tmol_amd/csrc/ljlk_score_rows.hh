@@ -136,10 +136,10 @@ struct RowLDS {
   using C = RowCtx<Real, MODE>;
   using T = typename LdsAcc<Real>::T;
   T grad[C::R][C::NG > 0 ? C::NG : 1][3][C::DERIV ? RSLOTS : 1];
-  T ebp[3][3];          // block-pair energies: ring 0, ring 1, intra
+  T ebp[C::BP ? 3 : 1][3];  // block-pair energies: ring 0, ring 1, intra
   alignas(4 * sizeof(Real)) Real pos[RSLOTS][4];  // x, y, z, lj_radius
   alignas(4 * sizeof(Real)) Real prm[RSLOTS][4];  // sqrt(lj_wdepth), K, 1/lambda^2, lk_volume
-  int flg[RSLOTS];
+  uint8_t flg[RSLOTS];  // F_* bits and F_HEAVY
   uint16_t lj[LIST];
   uint16_t lk[LIST];
   // culled atoms of the current block pair, compacted: x, y, z, atom index
@@ -147,7 +147,7 @@ struct RowLDS {
   alignas(4 * sizeof(Real)) Real c2[TILE][4];  // of g2
   uint32_t pdw[3][TILE];               // packed path distances: [0] g1, [1 + r] ring slot r
   uint8_t cs[2][MAX_CONN * MAX_CONN];  // per ring slot
-  Real wt[3][3];                       // block-pair weights:  ring 0, ring 1, intra
+  Real wt[C::BP ? 3 : 1][3];           // block-pair weights:  ring 0, ring 1, intra
 };
 
 // entry: a1 slot (5 bits) | slot2 (7 bits) | weight-0.2 flag (bit 12)
@@ -323,12 +323,16 @@ __device__ __forceinline__ void stage_atom(RowLDS<Real, MODE>& L, int slot, int 
   q.w = R.vol;
   *(V*)L.pos[slot] = p;
   *(V*)L.prm[slot] = q;
-  L.flg[slot] = R.flags;
+  L.flg[slot] = (uint8_t)R.flags;
   L.pdw[pd_row][a] = R.pd;
 }
 
+// Occupancy: 5 waves per SIMD without gradients (the trimmed LDS layout fits
+// 5 workgroups per CU); 4 with them, where a 96-register cap would spill and
+// the LDS pipe, busy with the gradient atomics, gains nothing from more waves.
 template <typename Real, int MODE>
-__global__ __launch_bounds__(256, 4) void score_rows_kernel(ScoreArgs<Real> A) {
+__global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void score_rows_kernel(
+    ScoreArgs<Real> A) {
   using C = RowCtx<Real, MODE>;
   using V = typename V4<Real>::T;
   __shared__ RowLDS<Real, MODE> lds_all[WAVES_PER_WG];
