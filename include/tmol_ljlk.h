@@ -244,13 +244,15 @@ int tmol_ljlk_prune_flatten(int32_t n_blocks, const uint32_t* row_count,
 
 /* Fused inference on the matrix cores: features, three BF16 layers, head.
  * keep[i] = 0 for a pair predicted negligible (logit > logit_cut) unless it
- * is a block with itself; logits [n] may be NULL.  n is read from *n_pairs
- * (device).  n_workgroups <= 0: one per resident slot. */
+ * is a block with itself or its sphere gap |c1 - c2| - r1 - r2 (a lower
+ * bound on its atom-pair distances) is below min_gap; logits [n] may be NULL.
+ * n is read from *n_pairs (device).  n_workgroups <= 0: one per resident
+ * slot. */
 int tmol_ljlk_prune_infer(const int64_t* n_pairs, const int32_t* pairs,
                           const tmol_ljlk_prune_blocks* blocks, const uint16_t* W1,
                           const uint16_t* W2, const uint16_t* W3, const float* b1,
                           const float* b2, const float* b3, const float* w4, const float* b4,
-                          float logit_cut, uint8_t* keep, float* logits,
+                          float logit_cut, float min_gap, uint8_t* keep, float* logits,
                           int32_t n_workgroups, void* stream);
 
 /* Remove the pairs with keep == 0 from the work list, in place, keeping the

@@ -141,11 +141,40 @@ def test_fused_inference_matches_training_forward():
     assert n == pairs.shape[0]
     torch.testing.assert_close(pr.logits[:n], ref_logits, rtol=0, atol=2e-2)
     # compaction: rows now hold exactly the kept pairs, in order
-    keep = (pr.logits[:n] <= pr.logit_cut) | (pairs[:, 0] == pairs[:, 1])
+    gap = _sphere_gap(pr, pairs)
+    keep = (pr.logits[:n] <= pr.logit_cut) | (pairs[:, 0] == pairs[:, 1]) | (gap < pr.min_gap)
     want = pairs[keep].cpu()
     got = prune.work_list_pairs(sc).cpu()
     assert torch.equal(got, want)
     assert 0 < got.shape[0] < n
+
+
+def _sphere_gap(pr, pairs):
+    """|c1 - c2| - r1 - r2 of each pair from the neighbor pass's spheres."""
+    sph = pr.inputs.spheres()
+    a, b = sph[pairs[:, 0].long()], sph[pairs[:, 1].long()]
+    d = a[:, :3] - b[:, :3]
+    return torch.sqrt(d[:, 0] * d[:, 0] + d[:, 1] * d[:, 1] + d[:, 2] * d[:, 2]) - a[:, 3] - b[:, 3]
+
+
+def test_gap_guard_keeps_pairs_that_could_clash():
+    """With a threshold that lets the network remove every pair, the kept
+    work list is exactly the self pairs and the pairs whose sphere gap (a
+    lower bound on their atom-pair distances) is below the guard."""
+    prune, sc, x = _small_setup(seed=13)
+    _say("gap guard")
+    params = prune.PrunerParams(DEV, seed=4)
+    pr = prune.Pruner(sc, params, threshold=0.0)
+    assert pr.min_gap == prune.MIN_GAP == 3.0
+    sc.neighbors(x)
+    pairs = prune.work_list_pairs(sc)
+    pr.prune(x)
+    gap = _sphere_gap(pr, pairs)
+    assert not bool(((gap - pr.min_gap).abs() < 1e-4).any())  # nothing on the boundary
+    want = pairs[(pairs[:, 0] == pairs[:, 1]) | (gap < pr.min_gap)].cpu()
+    got = prune.work_list_pairs(sc).cpu()
+    assert torch.equal(got, want)
+    assert want.shape[0] > int((pairs[:, 0] == pairs[:, 1]).sum())
 
 
 def test_pruning_nothing_reproduces_exact_scores():

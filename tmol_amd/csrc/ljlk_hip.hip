@@ -848,14 +848,15 @@ int tmol_ljlk_prune_infer(const int64_t* n_pairs, const int32_t* pairs,
                           const tmol_ljlk_prune_blocks* blocks, const uint16_t* W1,
                           const uint16_t* W2, const uint16_t* W3, const float* b1, const float* b2,
                           const float* b3, const float* w4, const float* b4, float logit_cut,
-                          uint8_t* keep, float* logits, int32_t n_workgroups, void* stream) {
+                          float min_gap, uint8_t* keep, float* logits, int32_t n_workgroups,
+                          void* stream) {
   if (!n_pairs || !pairs || !blocks || !W1 || !W2 || !W3 || !b1 || !b2 || !b3 || !w4 || !b4 || !keep)
     return TMOL_LJLK_EINVAL;
   PruneNet net{W1, W2, W3, b1, b2, b3, w4, b4};
   int nwg = n_workgroups;
   if (nwg <= 0) nwg = resident_workgroups(prune_infer_kernel);
   hipLaunchKernelGGL(prune_infer_kernel, dim3(nwg), dim3(PI_WAVES * 64), 0, (hipStream_t)stream,
-                     n_pairs, pairs, prune_blocks(blocks), net, logit_cut, keep, logits);
+                     n_pairs, pairs, prune_blocks(blocks), net, logit_cut, min_gap, keep, logits);
   return check(hipGetLastError());
 }
 

@@ -405,10 +405,14 @@ __device__ __forceinline__ void prune_layer(uint16_t (*act)[PSH], const uint16_t
 // Scores every pair of a flat list [n, 2] (n read from device memory, so the
 // launch shape does not depend on it) with the MLP; keep[i] = 0 when the pair
 // is predicted negligible with probability above the threshold
-// (logit > logit_cut) and is not a block with itself.  logits may be null.
+// (logit > logit_cut), is not a block with itself, and its sphere gap
+// |c1 - c2| - r1 - r2 -- a lower bound on every atom-pair distance of the two
+// blocks -- is at least min_gap (pairs whose atoms may come closer are always
+// scored: a clash there is worth far more than the bound).  logits may be null.
 __global__ __launch_bounds__(PI_WAVES * 64) void prune_infer_kernel(
     const int64_t* __restrict__ n_pairs, const int32_t* __restrict__ pairs, PruneBlocks B,
-    PruneNet net, float logit_cut, uint8_t* __restrict__ keep, float* __restrict__ logits) {
+    PruneNet net, float logit_cut, float min_gap, uint8_t* __restrict__ keep,
+    float* __restrict__ logits) {
   __shared__ PruneLDS L;
   const int64_t n = *n_pairs;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -432,6 +436,7 @@ __global__ __launch_bounds__(PI_WAVES * 64) void prune_infer_kernel(
        t += int64_t(gridDim.x) * PI_WAVES) {
     const int64_t i = t * 32 + r;
     const bool valid = i < n;
+    float gap = 0.f;  // sphere gap of row r (half-0 lanes)
     int g1 = 0, g2 = 0;
     if (valid) {
       g1 = pairs[2 * i];
@@ -456,8 +461,10 @@ __global__ __launch_bounds__(PI_WAVES * 64) void prune_infer_kernel(
       const int sp = B.seqpos[gb];
       const float ox = __shfl_xor(sph.x, 32, WAVE), oy = __shfl_xor(sph.y, 32, WAVE);
       const float oz = __shfl_xor(sph.z, 32, WAVE);
+      const float ow = __shfl_xor(sph.w, 32, WAVE);
       const int osp = __shfl_xor(sp, 32, WAVE);
       const float dx = sph.x - ox, dy = sph.y - oy, dz = sph.z - oz;
+      gap = sqrtf(dx * dx + dy * dy + dz * dz) - sph.w - ow;
       const float ev[PRUNE_EMB] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
       const uint16_t z = 0;
       // columns: 0 distance, 1 + half radius, 3 + 8 half embedding,
@@ -490,7 +497,7 @@ __global__ __launch_bounds__(PI_WAVES * 64) void prune_infer_kernel(
     s += __shfl_xor(s, 32, WAVE);
     if (half == 0 && valid) {
       const float logit = s + L.b4;
-      keep[i] = (g1 == g2 || !(logit > logit_cut)) ? 1 : 0;
+      keep[i] = (g1 == g2 || !(logit > logit_cut) || gap < min_gap) ? 1 : 0;
       if (logits) logits[i] = logit;
     }
     wave_sync();

@@ -81,12 +81,13 @@ def pose_errors(pruner, sc, x, n_res):
 
 
 def calibrate(params, dev, candidates=(0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.99, 0.995, 0.999),
+              min_gap=prune.MIN_GAP,
               log=print):
     poses = workload_poses("pruner_calib")
     st, sc = _scorer(poses, dev)
     x = st.coords.to(dev)
     n_res = torch.tensor([p.n_blocks for p in poses], dtype=torch.float64)
-    pr = prune.Pruner(sc, params)
+    pr = prune.Pruner(sc, params, min_gap=min_gap)
     table = []
     best = 1.0
     for tau in sorted(candidates, reverse=True):
@@ -125,13 +126,13 @@ def _time(g, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def measure(params, threshold, dev, steps=20, warmup=3, log=print):
+def measure(params, threshold, dev, steps=20, warmup=3, log=print, min_gap=prune.MIN_GAP):
     poses = workload_poses("pruner_1024x300")
     st, sc = _scorer(poses, dev)
     x = st.coords.to(dev).contiguous()
     n_res = torch.tensor([p.n_blocks for p in poses], dtype=torch.float64)
     w = sc._weights(None)
-    pr = prune.Pruner(sc, params, threshold=threshold)
+    pr = prune.Pruner(sc, params, threshold=threshold, min_gap=min_gap)
     sc.energy_and_grad(x)
     sc.check_capacity()
 
@@ -164,7 +165,7 @@ def measure(params, threshold, dev, steps=20, warmup=3, log=print):
     P = len(poses)
     out = {
         "workload": "pruner_1024x300", "poses": P, "atoms": st.n_atoms_total,
-        "threshold": threshold,
+        "threshold": threshold, "min_gap": min_gap,
         "exact_poses_per_s": P / t_exact, "pruned_poses_per_s": P / t_pruned,
         "speedup": t_exact / t_pruned,
         "ms_exact": 1e3 * t_exact, "ms_pruned": 1e3 * t_pruned,
