@@ -7,7 +7,7 @@
  * Follows tmol/score/elec/potentials/potentials.hh:31-201 (the potential,
  * including its float/double promotions: eps() of max_dis and min_dis runs in
  * float whatever Real is, and the 0.5 literal lifts part of eps to double)
- * and elec_pose_score.impl.hh:400-880 with elec.hh:406-548 (serial CPU
+ * and elec_pose_score.impl.hh:383-856 with elec.hh:406-548 (serial CPU
  * order: tiles of 32 atoms, pairs row-major, intra-block pairs a1 < a2;
  * count pair through the representative-atom path distances).
  *
