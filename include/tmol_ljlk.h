@@ -400,6 +400,25 @@ int tmol_elec_score(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* b
                     const void* weights, void* energies, void* dvdx, void* grad,
                     int32_t n_wave_groups, void* stream);
 
+/* LJ+LK and elec in one pass over the LJ+LK work list (tmol_ljlk_neighbors;
+ * elec's 5.5 A reach lies inside LJ+LK's 6 A, so the list holds every elec
+ * block pair).  No reference counterpart: the reference scores each term in
+ * its own launch.  Blocks share staging, cull and the pair tests; elec
+ * entries go to a third FIFO.
+ *   tmol_ljlk_elec_prepare: elec's packed path distances and intra-block
+ *     table into the LJ+LK workspace (once per block-type set).
+ *   tmol_ljlk_elec_score: flags 0, WGRAD, WGRAD | NO_ENERGY (ACCUMULATE
+ *     allowed); energies [4, P] = atr, rep, lk, elec; weights [4, P]; grad
+ *     [NA, 3] = sum_t w[t, p] dE_t/dx.  Block types of at most 32 atoms. */
+int tmol_ljlk_elec_prepare(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                           const tmol_elec_tables* elec, void* workspace, int64_t ws_bytes,
+                           int64_t row_capacity, int32_t dtype, void* stream);
+int tmol_ljlk_elec_score(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                         const tmol_elec_tables* elec, const void* coords, int32_t dtype,
+                         void* workspace, int64_t ws_bytes, int64_t row_capacity, int32_t flags,
+                         const void* weights, void* energies, void* grad, int32_t n_wave_groups,
+                         void* stream);
+
 /* Census of the elec work list (float32 only), census[16] (device uint64):
  * block-pair items, intra-block items, enumerated atom pairs, pairs of two
  * charged atoms, charged pairs of count-pair weight 0, and evaluated pairs by

@@ -30,6 +30,7 @@ constexpr int M_WGRAD = TMOL_LJLK_WGRAD;
 constexpr int M_BP = TMOL_LJLK_BLOCK_PAIR;
 constexpr int M_NOE = TMOL_LJLK_NO_ENERGY;
 constexpr int M_CENSUS = 64;  // branch census instead of energies (FLOP accounting)
+constexpr int M_ELEC = 256;   // elec scored in the same pass (fourth term)
 
 // census slots (counts of enumerated atom pairs)
 enum : int {
@@ -64,11 +65,13 @@ __device__ __forceinline__ int lanes_below(uint64_t mask) {
 struct WsLayout {
   int64_t counters;   // uint32[16]: [0] block pairs found, [1] row overflow
   int64_t cursors;    // uint32[8 * 64]: row-queue cursors, one per 256-byte line
-  int64_t energies;   // double[3 * P]
+  int64_t energies;   // double[4 * P] (atr, rep, lk, elec)
   int64_t spheres;    // Real[4 * G]
   int64_t records;    // Real[T * Amax * REC]
   int64_t conn_pd;    // uint8[T * Amax * MAX_CONN]: path distance atom -> connection atom
   int64_t intra;      // int32[T] + uint16[T * INTRA_MAX]: weighted intra-block pairs per type
+  int64_t e_pd;       // uint32[T * Amax]: elec representative path distances, packed
+  int64_t e_intra;    // int32[T] + uint16[T * INTRA_MAX]: elec's intra-block pairs per type
   int64_t row_count;  // uint32[G]
   int64_t row_items;  // int32[G * row_cap]
   int64_t total;
@@ -87,7 +90,7 @@ inline WsLayout ws_layout(const tmol_ljlk_stack* s, const tmol_ljlk_block_types*
   L.cursors = o;
   o = align256(o + 8 * 256);
   L.energies = o;
-  o = align256(o + 8 * 3 * int64_t(s->n_poses));
+  o = align256(o + 8 * 4 * int64_t(s->n_poses));
   L.spheres = o;
   o = align256(o + rs * 4 * G);
   L.records = o;
@@ -95,6 +98,10 @@ inline WsLayout ws_layout(const tmol_ljlk_stack* s, const tmol_ljlk_block_types*
   L.conn_pd = o;
   o = align256(o + int64_t(bt->n_block_types) * MAX_CONN * bt->max_n_atoms);
   L.intra = o;
+  o = align256(o + 4 * int64_t(bt->n_block_types) + 2 * int64_t(bt->n_block_types) * INTRA_MAX);
+  L.e_pd = o;
+  o = align256(o + 4 * int64_t(bt->n_block_types) * bt->max_n_atoms);
+  L.e_intra = o;
   o = align256(o + 4 * int64_t(bt->n_block_types) + 2 * int64_t(bt->n_block_types) * INTRA_MAX);
   L.row_count = o;
   o = align256(o + 4 * G);
@@ -320,6 +327,13 @@ struct ScoreArgs {
   Real* grad;                  // [NA, 3]
   int64_t n_atoms;
   unsigned long long* census;  // [16] with M_CENSUS
+  // M_ELEC: elec's tables (charges [T, Amax]; packed representative path
+  // distances and intra-block pairs prepared in the workspace) and constants
+  const Real* e_charge;
+  const uint32_t* e_pd;
+  const int32_t* e_intra_count;
+  const uint16_t* e_intra_pairs;
+  ElecConst<Real> ek;
 };
 
 template <typename Real>

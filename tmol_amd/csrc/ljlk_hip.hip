@@ -176,12 +176,33 @@ static void launch_score(const ScoreArgs<Real>& a, bool general, int n_wg, hipSt
   }
 }
 
+// rows kernel only (modes the general kernel does not implement)
+template <typename Real, int MODE>
+static void launch_rows(const ScoreArgs<Real>& a, int n_wg, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
+                     8 * QSTRIDE, 0u);
+  auto k = score_rows_kernel<Real, MODE>;
+  if (n_wg <= 0) n_wg = resident_workgroups(k);
+  hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+}
+
+// ek != nullptr: elec is scored in the same pass (M_ELEC; tables prepared in
+// the workspace by tmol_ljlk_elec_prepare), energies and weights [4, P]
 template <typename Real>
 static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
                      const void* coords, void* ws, int64_t row_cap, int flags,
                      const void* weights, void* energies, void* dvdx, void* grad,
-                     int n_wave_groups, hipStream_t stream, const WsLayout& Lw) {
+                     int n_wave_groups, hipStream_t stream, const WsLayout& Lw,
+                     const tmol_elec_tables* e = nullptr, const ElecConst<Real>* ek = nullptr) {
   ScoreArgs<Real> a = make_args<Real>(st, btp, coords, ws, row_cap, Lw);
+  const int n_terms = ek ? 4 : 3;
+  if (ek) {
+    a.e_charge = (const Real*)e->partial_charge;
+    a.e_pd = (const uint32_t*)((char*)ws + Lw.e_pd);
+    a.e_intra_count = (const int32_t*)((char*)ws + Lw.e_intra);
+    a.e_intra_pairs = (const uint16_t*)((char*)ws + Lw.e_intra + 4 * int64_t(btp->n_block_types));
+    a.ek = *ek;
+  }
   a.weights = (const Real*)weights;
   a.energies_bp = (Real*)energies;
   a.dvdx = (Real*)dvdx;
@@ -200,13 +221,19 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
                           : int64_t(st->n_blocks_total) * row_cap),
                  stream);
     else
-      fill<double>(a.energies64, int64_t(3) * P, stream);
+      fill<double>(a.energies64, int64_t(n_terms) * P, stream);
   }
   if ((flags & TMOL_LJLK_DVDX) && !acc) fill<Real>((Real*)dvdx, 3 * NA3, stream);
   if ((flags & TMOL_LJLK_WGRAD) && !acc) fill<Real>((Real*)grad, NA3, stream);
   const int n_wg = n_wave_groups;
   int mode = flags & (TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY);
+  if (ek) mode |= M_ELEC;
   switch (mode) {
+    case M_ELEC: launch_rows<Real, M_ELEC>(a, n_wave_groups, stream); break;
+    case M_ELEC | M_WGRAD: launch_rows<Real, M_ELEC | M_WGRAD>(a, n_wave_groups, stream); break;
+    case M_ELEC | M_WGRAD | M_NOE:
+      launch_rows<Real, M_ELEC | M_WGRAD | M_NOE>(a, n_wave_groups, stream);
+      break;
     case 0: launch_score<Real, 0>(a, general, n_wg, stream); break;
     case M_DVDX: launch_score<Real, M_DVDX>(a, general, n_wg, stream); break;
     case M_WGRAD: launch_score<Real, M_WGRAD>(a, general, n_wg, stream); break;
@@ -219,8 +246,8 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
     default: return TMOL_LJLK_ENOTSUP;
   }
   if (!noe && !bp)
-    hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(3 * P)), dim3(256), 0, stream,
-                       (const double*)a.energies64, (Real*)energies, 3 * P);
+    hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(n_terms * P)), dim3(256), 0, stream,
+                       (const double*)a.energies64, (Real*)energies, n_terms * P);
   return check(hipGetLastError());
 }
 
@@ -1098,6 +1125,58 @@ int tmol_elec_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                                  energies, dvdx, grad, n_wave_groups, st, L);
   return run_elec_score<double>(s, bt, e, coords, workspace, row_capacity, flags, weights,
                                 energies, dvdx, grad, n_wave_groups, st, L);
+}
+
+int tmol_ljlk_elec_prepare(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                           const tmol_elec_tables* e, void* workspace, int64_t ws_bytes,
+                           int64_t row_capacity, int32_t dtype, void* stream) {
+  int v = validate_elec(s, bt, e, dtype);
+  if (v) return v;
+  if (!workspace || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  BTView b = make_bt_view(bt);
+  char* w = (char*)workspace;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    hipLaunchKernelGGL(elec_prepare_kernel<float>, dim3(b.T), dim3(64), 0, st, b,
+                       (const float*)e->partial_charge, e->inter_repr_path_distance,
+                       e->intra_repr_path_distance, (uint32_t*)(w + L.e_pd),
+                       (int32_t*)(w + L.e_intra), (uint16_t*)(w + L.e_intra + 4 * int64_t(b.T)));
+  else
+    hipLaunchKernelGGL(elec_prepare_kernel<double>, dim3(b.T), dim3(64), 0, st, b,
+                       (const double*)e->partial_charge, e->inter_repr_path_distance,
+                       e->intra_repr_path_distance, (uint32_t*)(w + L.e_pd),
+                       (int32_t*)(w + L.e_intra), (uint16_t*)(w + L.e_intra + 4 * int64_t(b.T)));
+  return check(hipGetLastError());
+}
+
+int tmol_ljlk_elec_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                         const tmol_elec_tables* e, const void* coords, int32_t dtype,
+                         void* workspace, int64_t ws_bytes, int64_t row_capacity, int32_t flags,
+                         const void* weights, void* energies, void* grad, int32_t n_wave_groups,
+                         void* stream) {
+  int v = validate_elec(s, bt, e, dtype);
+  if (v) return v;
+  if (!coords || !workspace || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  if (flags & ~(TMOL_LJLK_WGRAD | TMOL_LJLK_NO_ENERGY | TMOL_LJLK_ACCUMULATE))
+    return TMOL_LJLK_ENOTSUP;
+  if (!(flags & TMOL_LJLK_NO_ENERGY) && !energies) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_NO_ENERGY) && !(flags & TMOL_LJLK_WGRAD)) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_WGRAD) && (!grad || !weights)) return TMOL_LJLK_EINVAL;
+  if (!bt->type_params || !bt->global_params || !bt->path_distance) return TMOL_LJLK_EINVAL;
+  if (n_wave_groups < 0 || n_wave_groups > (1 << 20)) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32) {
+    const ElecConst<float> k = elec_constants<float>(e->global_params);
+    return run_score<float>(s, bt, coords, workspace, row_capacity, flags, weights, energies,
+                            nullptr, grad, n_wave_groups, st, L, e, &k);
+  }
+  const ElecConst<double> k = elec_constants<double>(e->global_params);
+  return run_score<double>(s, bt, coords, workspace, row_capacity, flags, weights, energies,
+                           nullptr, grad, n_wave_groups, st, L, e, &k);
 }
 
 int tmol_elec_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,

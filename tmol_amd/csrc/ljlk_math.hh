@@ -229,4 +229,65 @@ __device__ __forceinline__ void lk_dir(Real dist, Real d_min, Real cdmin,
   if (!DERIV) dV = Real(0);
 }
 
+// ---- elec (tmol/score/elec/potentials/potentials.hh)
+
+// Per-launch constants of the potential (computed on the host in the
+// reference's precision: eps() at max_dis and min_dis in float, the rest in
+// Real).  Hermite pieces take the charge product out as a common factor.
+template <typename Real>
+struct ElecConst {
+  Real D, S, hdd;       // hdd = 0.5 (D - D0)
+  Real C1, C2;          // 322.0637, C1 / (max_dis eps(max_dis))
+  Real lps, lpe;        // min_dis -+ 0.25
+  Real hps, hpe;        // max_dis - 1, max_dis
+  Real mds;             // C1 / (min_dis eps(min_dis)) - C2 (flat part)
+  Real e_lo, de_lo;     // Coulomb value and slope at lpe (per unit q1 q2)
+  Real e_hi, de_hi;     // ... at hps
+  Real lw, hw;          // lpe - lps, hpe - hps
+  Real inv_lw, inv_hw;
+  Real cull2;           // (max_dis + 1e-3)^2: phase A keeps pairs below it
+};
+
+// V and dV/dd per unit charge product (potentials.hh:55-138).  Branch-free
+// over the wave: the Coulomb form is evaluated at clamp(d, lpe, hps), and one
+// Hermite piece, its ends chosen by region, covers both fades (below lps it
+// sits at t = 0: the flat part, slope 0).
+template <typename Real, bool DERIV>
+__device__ __forceinline__ void elec_unit(Real dist, const ElecConst<Real>& k, Real& V,
+                                          Real& dV) {
+  const Real x = dist < k.lpe ? k.lpe : (dist < k.hps ? dist : k.hps);
+  const Real ex = fast_exp(-x * k.S);
+  const Real xs = x * k.S;
+  const Real eps = k.D - k.hdd * (Real(2) + Real(2) * xs + xs * xs) * ex;
+  const Real r = fast_rcp(eps);
+  const Real ix = fast_rcp(x);
+  const Real vc = k.C1 * ix * r - k.C2;
+  Real dc = Real(0);
+  if (DERIV) {
+    const Real deps = k.hdd * xs * xs * k.S * ex;  // 0.5 (D - D0) x^2 S^3 e^{-xS}
+    dc = -k.C1 * (eps + x * deps) * ix * ix * r * r;
+  }
+  const bool lo = dist < k.lpe;
+  const bool hi = dist >= k.hps;
+  const Real x0 = lo ? k.lps : k.hps;
+  const Real w = lo ? k.lw : k.hw;
+  const Real iw = lo ? k.inv_lw : k.inv_hw;
+  const Real p0 = lo ? k.mds : k.e_hi;
+  const Real p1 = lo ? k.e_lo : Real(0);
+  const Real dp0 = lo ? Real(0) : k.de_hi * w;
+  const Real dp1 = lo ? k.de_lo * w : Real(0);
+  Real t = (dist - x0) * iw;
+  t = t > Real(0) ? t : Real(0);
+  const Real c2 = Real(-2) * dp0 - dp1 - Real(3) * p0 + Real(3) * p1;
+  const Real c3 = dp0 + dp1 + Real(2) * p0 - Real(2) * p1;
+  const Real vh = p0 + t * (dp0 + t * (c2 + t * c3));
+  const Real dh = (dp0 + t * (Real(2) * c2 + Real(3) * t * c3)) * iw;
+  const bool fade = lo || hi;
+  V = fade ? vh : vc;
+  dV = DERIV ? (fade ? dh : dc) : Real(0);
+  const bool out = dist >= k.hpe;
+  V = out ? Real(0) : V;
+  dV = out ? Real(0) : dV;
+}
+
 }  // namespace tmol_ljlk

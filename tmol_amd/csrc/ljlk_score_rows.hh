@@ -79,9 +79,11 @@ struct AtomRegs {
   Real vol;
   int flags;
   uint32_t pd;  // byte c: path distance to connection atom c
+  Real q;       // elec: partial charge (M_ELEC)
+  uint32_t epd; // elec: representative path distances, packed (M_ELEC)
 };
 
-template <typename Real>
+template <typename Real, bool ELEC = false>
 __device__ __forceinline__ void load_atom(AtomRegs<Real>& R, const ScoreArgs<Real>& A, int t,
                                           int o, int a) {
   using V = typename V4<Real>::T;
@@ -95,6 +97,10 @@ __device__ __forceinline__ void load_atom(AtomRegs<Real>& R, const ScoreArgs<Rea
   R.vol = r1.x;
   R.flags = int(r1.y);
   R.pd = *(const uint32_t*)(A.conn_pd + (int64_t(t) * A.bt.Amax + a) * MAX_CONN);
+  if (ELEC) {
+    R.q = A.e_charge[int64_t(t) * A.bt.Amax + a];
+    R.epd = A.e_pd[int64_t(t) * A.bt.Amax + a];
+  }
 }
 
 // Count-pair separation of an inter-block atom pair from packed path
@@ -126,6 +132,7 @@ struct RowCtx {
   static constexpr bool BP = MODE & M_BP;
   static constexpr bool NOE = MODE & M_NOE;
   static constexpr bool DERIV = DVDX || WGRAD;
+  static constexpr bool ELEC = MODE & M_ELEC;  // whole-pose energies / WGRAD only
   static constexpr int NG = DVDX ? 3 : (WGRAD ? 1 : 0);
   // gradient accumulator copies: fewer when the planes are many or wide
   static constexpr int R = 1;
@@ -148,6 +155,10 @@ struct RowLDS {
   uint32_t pdw[3][TILE];               // packed path distances: [0] g1, [1 + r] ring slot r
   uint8_t cs[2][MAX_CONN * MAX_CONN];  // per ring slot
   Real wt[C::BP ? 3 : 1][3];           // block-pair weights:  ring 0, ring 1, intra
+  // M_ELEC: charge per slot, elec's packed path distances, elec FIFO
+  Real qch[C::ELEC ? RSLOTS : 1];
+  uint32_t epdw[C::ELEC ? 3 : 1][C::ELEC ? TILE : 1];
+  uint16_t el[C::ELEC ? LIST : 1];
 };
 
 // entry: a1 slot (5 bits) | slot2 (7 bits) | weight-0.2 flag (bit 12)
@@ -265,6 +276,42 @@ __device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
   }
 }
 
+// Elec entries [0, consume) of the FIFO (M_ELEC), one per lane: the
+// potential per unit charge product (elec_unit), times q1 q2 and the
+// count-pair weight; the gradient, times the pose's elec weight, joins the
+// same LDS plane as LJ and LK.
+template <typename Real, int MODE>
+__device__ __forceinline__ void eval_el(RowLDS<Real, MODE>& L, int consume,
+                                        const ElecConst<Real>& k, Real wel, LaneSum& e_el,
+                                        int lane) {
+  using C = RowCtx<Real, MODE>;
+  using V = typename V4<Real>::T;
+  for (int i0 = 0; i0 < consume; i0 += WAVE) {
+    const int i = i0 + lane;
+    if (i < consume) {
+      const int e = L.el[i];
+      const int s1 = e & 31, s2 = (e >> 5) & 127;
+      const Real wcp = (e >> 12) & 1 ? Real(0.2) : Real(1);
+      const V p1 = *(const V*)L.pos[s1], p2 = *(const V*)L.pos[s2];
+      const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
+      const Real d2 = dx * dx + dy * dy + dz * dz;
+      const Real inv_d = fast_rsqrt(d2);
+      const Real dist = d2 > Real(0) ? d2 * inv_d : Real(0);
+      Real v, dv;
+      elec_unit<Real, C::DERIV>(dist, k, v, dv);
+      const Real qq = L.qch[s1] * L.qch[s2];
+      if (!C::NOE) e_el.add(wcp * (qq * v));
+      if (C::DERIV && dist > Real(0)) {
+        const Real dE = wcp * (qq * dv);
+        if (dE != Real(0)) {
+          const Real gs = dE * inv_d * wel;
+          grad_scatter(L, 0, s1, s2, gs * dx, gs * dy, gs * dz, lane);
+        }
+      }
+    }
+  }
+}
+
 // move the unconsumed tail [consumed, n) of a FIFO to its front (rem < 64)
 __device__ __forceinline__ void fifo_shift(uint16_t* list, int consumed, int n, int lane) {
   const int rem = n - consumed;
@@ -325,6 +372,10 @@ __device__ __forceinline__ void stage_atom(RowLDS<Real, MODE>& L, int slot, int 
   *(V*)L.prm[slot] = q;
   L.flg[slot] = (uint8_t)R.flags;
   L.pdw[pd_row][a] = R.pd;
+  if (RowCtx<Real, MODE>::ELEC) {
+    L.qch[slot] = R.q;
+    L.epdw[pd_row][a] = R.epd;
+  }
 }
 
 // Occupancy: 5 waves per SIMD without gradients (the trimmed LDS layout fits
@@ -343,6 +394,7 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
   const BTView& bt = A.bt;
   RowQueue queue(A.row_cursor, s.G);
   const Real cull_pad = gp.max_dis + Real(1e-3);
+  const ElecConst<Real> ek = A.ek;
 
   if (C::DERIV) {
     auto* g = &L.grad[0][0][0][0];
@@ -367,22 +419,24 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
       wrow[1] = A.weights[s.P + p];
       wrow[2] = A.weights[2 * s.P + p];
     }
+    const Real wel = (C::ELEC && C::WGRAD) ? A.weights[3 * s.P + p] : Real(0);
     // ---- stage g1 into slots 0..31
     bool heavy1 = false;
     if (lane < n1) {
       AtomRegs<Real> R;
-      load_atom(R, A, t1, o1, lane);
+      load_atom<Real, C::ELEC>(R, A, t1, o1, lane);
       stage_atom(L, lane, 0, lane, R);
       heavy1 = (R.flags & F_HEAVY) != 0;
     }
     const uint32_t hmask1 = (uint32_t)__ballot(heavy1);
     if (C::BP && lane < 9) (&L.ebp[0][0])[lane] = 0;
-    LaneSum e_atr, e_rep, e_lk;
+    LaneSum e_atr, e_rep, e_lk, e_el;
     // FIFO state (wave-uniform)
     int n_lj = 0, n_lk = 0;      // pending entries
     int seq_lj = 0, seq_lk = 0;  // entries appended in this row
     int done_lj = 0, done_lk = 0;
     int end_lj0 = 0, end_lj1 = 0, end_lk0 = 0, end_lk1 = 0;
+    int n_el = 0, seq_el = 0, done_el = 0, end_el0 = 0, end_el1 = 0;  // M_ELEC
     bool used0 = false, used1 = false;
     int o2s0 = 0, o2s1 = 0;
     int n2s0 = 0, n2s1 = 0;
@@ -393,15 +447,25 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
     auto drain = [&](bool all) {
       const int c1 = all ? n_lj : (n_lj & ~(WAVE - 1));
       const int c2 = all ? n_lk : (n_lk & ~(WAVE - 1));
+      const int c3 = C::ELEC ? (all ? n_el : (n_el & ~(WAVE - 1))) : 0;
       if (c1 > 0) eval_lj<Real, MODE>(L, c1, gp, wrow, e_atr, e_rep, lane);
       if (c2 > 0) eval_lk<Real, MODE>(L, c2, gp, wrow, e_lk, lane);
+      if (C::ELEC && c3 > 0) eval_el<Real, MODE>(L, c3, ek, wel, e_el, lane);
       wave_sync();
       fifo_shift(L.lj, c1, n_lj, lane);
       fifo_shift(L.lk, c2, n_lk, lane);
+      if (C::ELEC) fifo_shift(L.el, c3, n_el, lane);
       n_lj -= c1;
       done_lj += c1;
       n_lk -= c2;
       done_lk += c2;
+      n_el -= c3;
+      done_el += c3;
+    };
+    // some FIFO holds a full wave / could overflow with `room` more entries
+    auto full = [&]() { return n_lj >= WAVE || n_lk >= WAVE || (C::ELEC && n_el >= WAVE); };
+    auto near_full = [&](int room) {
+      return n_lj > LIST - room || n_lk > LIST - room || (C::ELEC && n_el > LIST - room);
     };
     auto retire = [&](int r) {
       // all entries of ring slot r are consumed: flush its g2 atoms
@@ -436,7 +500,7 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
       AtomRegs<Real> pf;
       {
         const int t = lane_val(d_t2, 0), o = lane_val(d_o2, 0), n = lane_val(d_pk, 0) & 255;
-        if (lane < n) load_atom(pf, A, t, o, lane);
+        if (lane < n) load_atom<Real, C::ELEC>(pf, A, t, o, lane);
       }
       for (int j = 0; j < n_chunk; ++j) {
         const int g2 = lane_val(d_g2, j);
@@ -450,14 +514,14 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
           if (j + 1 < n_chunk) {
             const int t = lane_val(d_t2, j + 1), o = lane_val(d_o2, j + 1);
             const int n = lane_val(d_pk, j + 1) & 255;
-            if (lane < n) load_atom(pf, A, t, o, lane);
+            if (lane < n) load_atom<Real, C::ELEC>(pf, A, t, o, lane);
           }
           wave_sync();
           // only the type's pairs of non-zero weight (prepared table)
           const int n_ip = uni(A.intra_count[t1]);
           const uint16_t* ip = A.intra_pairs + int64_t(t1) * INTRA_MAX;
           for (int k0 = 0; k0 < n_ip; k0 += WAVE) {
-            if (n_lj > LIST - WAVE || n_lk > LIST - WAVE) drain(false);
+            if (near_full(WAVE)) drain(false);
             bool in = false, both_heavy = false, w02 = false;
             int a1 = 0, a2 = 0;
             if (k0 + lane < n_ip) {
@@ -479,6 +543,31 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
             n_lk += __popcll(mh);
             seq_lk += __popcll(mh);
             wave_sync();
+          }
+          if (C::ELEC) {
+            // elec's own intra-block table (representative separations,
+            // charged atoms only)
+            const int n_ie = uni(A.e_intra_count[t1]);
+            const uint16_t* ie = A.e_intra_pairs + int64_t(t1) * INTRA_MAX;
+            for (int k0 = 0; k0 < n_ie; k0 += WAVE) {
+              if (near_full(WAVE)) drain(false);
+              bool in = false, w02 = false;
+              int a1 = 0, a2 = 0;
+              if (k0 + lane < n_ie) {
+                const int e = ie[k0 + lane];
+                a1 = e & 31;
+                a2 = (e >> 5) & 31;
+                const V pa = *(const V*)L.pos[a1], pb = *(const V*)L.pos[a2];
+                const Real dx = pa.x - pb.x, dy = pa.y - pb.y, dz = pa.z - pb.z;
+                in = dx * dx + dy * dy + dz * dz < ek.cull2;
+                w02 = (e >> 10) & 1;
+              }
+              const uint64_t m = __ballot(in);
+              if (in) L.el[n_el + lanes_below(m)] = make_entry(a1, a2, w02);
+              n_el += __popcll(m);
+              seq_el += __popcll(m);
+              wave_sync();
+            }
           }
         } else {
           // ---- inter-block pairs through ring slot `ring`
@@ -522,16 +611,17 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
             if (j + 1 < n_chunk) {
               const int t = lane_val(d_t2, j + 1), o = lane_val(d_o2, j + 1);
               const int n = lane_val(d_pk, j + 1) & 255;
-              if (lane < n) load_atom(pf, A, t, o, lane);
+              if (lane < n) load_atom<Real, C::ELEC>(pf, A, t, o, lane);
             }
             ring = r;
-            if (n_lj >= WAVE || n_lk >= WAVE) drain(false);
+            if (full()) drain(false);
             continue;
           }
           if (r ? used1 : used0) {
             const int end_lj = r ? end_lj1 : end_lj0;
             const int end_lk = r ? end_lk1 : end_lk0;
-            if (end_lj > done_lj || end_lk > done_lk) drain(true);
+            const int end_el = r ? end_el1 : end_el0;
+            if (end_lj > done_lj || end_lk > done_lk || (C::ELEC && end_el > done_el)) drain(true);
             retire(r);
           }
           // global reads this item needs now, before the next prefetch
@@ -551,7 +641,7 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
           if (j + 1 < n_chunk) {
             const int t = lane_val(d_t2, j + 1), o = lane_val(d_o2, j + 1);
             const int n = lane_val(d_pk, j + 1) & 255;
-            if (lane < n) load_atom(pf, A, t, o, lane);
+            if (lane < n) load_atom<Real, C::ELEC>(pf, A, t, o, lane);
           }
           if (r) {
             used1 = true; o2s1 = o2; n2s1 = n2; bps1 = bpi;
@@ -561,23 +651,35 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
           wave_sync();
           // phase A over the culled atoms, two pair indices per lane and
           // iteration: k = lane + 128 it (+ 64), t = k / n2c, u = k % n2c
-          auto test_pair = [&](int t, int u, bool& in, bool& hv, bool& w02, int& a1, int& a2) {
-            in = hv = w02 = false;
+          // ie / we: the elec entry (M_ELEC) -- charged atoms within elec's
+          // max_dis, separation from the representative path distances
+          auto test_pair = [&](int t, int u, bool& in, bool& hv, bool& w02, int& a1, int& a2,
+                               bool& ie, bool& we) {
+            in = hv = w02 = ie = we = false;
             a1 = a2 = 0;
             if (t < n1c) {
               const V P1 = *(const V*)L.c1[t], P2 = *(const V*)L.c2[u];
               a1 = int(P1.w);
               a2 = int(P2.w);
               const Real dx = P1.x - P2.x, dy = P1.y - P2.y, dz = P1.z - P2.z;
+              const Real d2 = dx * dx + dy * dy + dz * dz;
               const int sep = need_cp ? packed_pair_sep(L.pdw[0][a1], L.pdw[1 + r][a2], L.cs[r],
                                                         min_sep, nc1, nc2, lig)
                                       : min_sep;
-              in = sep >= 4 && dx * dx + dy * dy + dz * dz < gp.max_dis2;
+              in = sep >= 4 && d2 < gp.max_dis2;
               w02 = sep == 4;
               hv = in && ((hmask1 >> a1) & (hmask2 >> a2) & 1u);
+              if (C::ELEC) {
+                const int sep_e = need_cp ? packed_pair_sep(L.epdw[0][a1], L.epdw[1 + r][a2],
+                                                            L.cs[r], min_sep, nc1, nc2, lig)
+                                          : min_sep;
+                ie = sep_e >= 4 && d2 < ek.cull2 && L.qch[a1] != Real(0) &&
+                     L.qch[sb + a2] != Real(0);
+                we = sep_e == 4;
+              }
             }
           };
-          auto append = [&](bool in, bool hv, bool w02, int a1, int a2) {
+          auto append = [&](bool in, bool hv, bool w02, int a1, int a2, bool ie, bool we) {
             const uint64_t m = __ballot(in);
             if (in) L.lj[n_lj + lanes_below(m)] = make_entry(a1, sb + a2, w02);
             n_lj += __popcll(m);
@@ -586,23 +688,29 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
             if (hv) L.lk[n_lk + lanes_below(mh)] = make_entry(a1, sb + a2, w02);
             n_lk += __popcll(mh);
             seq_lk += __popcll(mh);
+            if (C::ELEC) {
+              const uint64_t me = __ballot(ie);
+              if (ie) L.el[n_el + lanes_below(me)] = make_entry(a1, sb + a2, we);
+              n_el += __popcll(me);
+              seq_el += __popcll(me);
+            }
           };
           const int total = n1c * n2c;
           const int q = small_div(WAVE, n2c), rstep = WAVE - q * n2c;
           int t = small_div(lane, n2c), u = lane - t * n2c;
           for (int k0 = 0; k0 < total; k0 += 2 * WAVE) {
-            if (n_lj > LIST - 2 * WAVE || n_lk > LIST - 2 * WAVE) drain(false);
+            if (near_full(2 * WAVE)) drain(false);
             int uB = u + rstep, tB = t + q;
             if (uB >= n2c) {
               uB -= n2c;
               tB += 1;
             }
-            bool inA, hvA, wA, inB, hvB, wB;
+            bool inA, hvA, wA, inB, hvB, wB, ieA, weA, ieB, weB;
             int a1A, a2A, a1B, a2B;
-            test_pair(t, u, inA, hvA, wA, a1A, a2A);
-            test_pair(tB, uB, inB, hvB, wB, a1B, a2B);
-            append(inA, hvA, wA, a1A, a2A);
-            append(inB, hvB, wB, a1B, a2B);
+            test_pair(t, u, inA, hvA, wA, a1A, a2A, ieA, weA);
+            test_pair(tB, uB, inB, hvB, wB, a1B, a2B, ieB, weB);
+            append(inA, hvA, wA, a1A, a2A, ieA, weA);
+            append(inB, hvB, wB, a1B, a2B, ieB, weB);
             u = uB + rstep;
             t = tB + q;
             if (u >= n2c) {
@@ -614,13 +722,15 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
           if (r) {
             end_lj1 = seq_lj;
             end_lk1 = seq_lk;
+            end_el1 = seq_el;
           } else {
             end_lj0 = seq_lj;
             end_lk0 = seq_lk;
+            end_el0 = seq_el;
           }
         }
         // evaluate whatever fills whole waves
-        if (n_lj >= WAVE || n_lk >= WAVE) drain(false);
+        if (full()) drain(false);
       }
     }
     // ---- row end: evaluate the rest, flush everything
@@ -640,6 +750,10 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
         unsafeAtomicAdd(&A.energies64[p], ea);
         unsafeAtomicAdd(&A.energies64[s.P + p], er);
         unsafeAtomicAdd(&A.energies64[2 * s.P + p], el);
+      }
+      if (C::ELEC) {
+        const double ee = wave_sum(e_el.value());
+        if (lane == 0) unsafeAtomicAdd(&A.energies64[3 * s.P + p], ee);
       }
     }
     wave_sync();

@@ -6,7 +6,9 @@ mode (per-pose energies and sum_p w[p] dE_p/dx), inputs resident in HBM.
 Reported: poses/s of the step, the kernel alone (HIP events on its stream),
 its roofline from the atom-pair census x elec FLOP table
 (tmol_amd/flops.py), and the reference's own CPU elec op (oracle/_ref, one
-thread, forward + backward) on a bounded sample of the poses.
+thread, forward + backward) on a bounded sample of the poses.  The ``fused``
+entry times LJ+LK and elec scored in one pass against the two passes run
+one after the other.
 
     python -m tmol_amd.elec_bench [--steps 20] [--no-cpu]
 """
@@ -152,10 +154,88 @@ def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=True, log=pr
         },
         "census": census,
     }
+    res["fused"] = fused_leg(stack, coords, device, steps, warmup, kernel_reps, log)
     if cpu:
         log("elec cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline(poses)
     return res
+
+
+def fused_leg(stack, coords, device, steps=20, warmup=3, kernel_reps=10, log=print):
+    """LJ+LK and elec energy + gradient (beta2016 weights) as one neighbor pass
+    and one scoring launch, against the two terms as separate passes; every
+    step captured in a hipGraph."""
+    import torch
+
+    from .score.elec_energy_term import ElecEnergyTerm
+    from .score.ljlk_energy_term import LJLKEnergyTerm
+    from .scorer import ElecRaggedScorer, LJLKElecRaggedScorer, LJLKRaggedScorer
+
+    lj_term = LJLKEnergyTerm(device=device)
+    el_term = ElecEnergyTerm()
+    fused = LJLKElecRaggedScorer(stack, lj_term, el_term, device=device)
+    lj = LJLKRaggedScorer(stack, lj_term, device=device)
+    el = ElecRaggedScorer(stack, el_term, device=device)
+    w4 = fused._weights(None)
+    w3 = lj._weights(None)
+    we = el._weights(1.0)
+
+    def both():
+        lj.neighbors(coords)
+        lj.score(coords, 4, weights=w3, energies=lj.energies, grad=lj.grad)
+        el.neighbors(coords)
+        el.score(coords, 4, weights=we, energies=el.energies, grad=el.grad)
+
+    def one():
+        fused.neighbors(coords)
+        fused.score(coords, 4, weights=w4, energies=fused.energies, grad=fused.grad)
+
+    def timed(fn):
+        fn()
+        torch.cuda.synchronize(device)
+        g = torch.cuda.CUDAGraph()
+        s = torch.cuda.Stream(device)
+        s.wait_stream(torch.cuda.current_stream(device))
+        with torch.cuda.stream(s):
+            with torch.cuda.graph(g, stream=s):
+                fn()
+        torch.cuda.current_stream(device).wait_stream(s)
+        for _ in range(warmup):
+            g.replay()
+        torch.cuda.synchronize(device)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.replay()
+        torch.cuda.synchronize(device)
+        return (time.perf_counter() - t0) * 1000.0 / steps
+
+    ms_sep = timed(both)
+    ms_fused = timed(one)
+    stream = torch.cuda.current_stream(device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    fused.neighbors(coords)
+    fused.score(coords, 4, weights=w4, energies=fused.energies, grad=fused.grad)
+    torch.cuda.synchronize(device)
+    ev0.record(stream)
+    for _ in range(kernel_reps):
+        fused.score(coords, 4, weights=w4, energies=fused.energies, grad=fused.grad)
+    ev1.record(stream)
+    torch.cuda.synchronize(device)
+    score_ms = ev0.elapsed_time(ev1) / kernel_reps
+    e = fused.energies.double()
+    log("fused LJ+LK+elec %.3f ms, separate %.3f ms" % (ms_fused, ms_sep))
+    return {
+        "workload": "ragged_512x50_1200",
+        "terms": "fa_ljatr, fa_ljrep, fa_lk, fa_elec (weights 1, 0.55, 1, 1)",
+        "value": stack.n_poses / (ms_fused * 1e-3),
+        "unit": "poses/s",
+        "ms_per_step": ms_fused,
+        "score_ms": score_ms,
+        "separate_ms_per_step": ms_sep,
+        "separate_poses_per_s": stack.n_poses / (ms_sep * 1e-3),
+        "speedup_vs_separate": ms_sep / ms_fused,
+        "energies_finite": bool(torch.isfinite(e).all()),
+    }
 
 
 def main():

@@ -818,6 +818,13 @@ def _elec_lib():
                 SP, BP, EP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64,
                 ctypes.c_int32, _VP, _VP, _VP, _VP, ctypes.c_int32, _VP,
             ]
+            L.tmol_ljlk_elec_prepare.argtypes = [
+                SP, BP, EP, _VP, ctypes.c_int64, ctypes.c_int64, ctypes.c_int32, _VP,
+            ]
+            L.tmol_ljlk_elec_score.argtypes = [
+                SP, BP, EP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64,
+                ctypes.c_int32, _VP, _VP, _VP, ctypes.c_int32, _VP,
+            ]
             L.tmol_elec_census.argtypes = [
                 SP, BP, EP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64, _VP, _VP,
             ]

@@ -242,6 +242,49 @@ class LJLKRaggedScorer:
         return entry
 
 
+class LJLKElecRaggedScorer(LJLKRaggedScorer):
+    """LJ+LK and elec of a ``RaggedPoseStack`` in one pass: one neighbor pass
+    (LJ+LK's 6 A reach covers elec's 5.5 A) and one launch of the scoring
+    kernel with elec as a fourth term sharing staging, cull and pair tests.
+    ``energy_and_grad(coords, weights)`` returns energies [4, P] (atr, rep,
+    lk, elec) and sum_t weights[t, p] dE_t/dx; default weights are beta2016's
+    1.0, 0.55, 1.0, 1.0."""
+
+    def __init__(self, stack: RaggedPoseStack, ljlk_term, elec_term, device="cuda",
+                 dtype=torch.float32, capacity: Optional[int] = None,
+                 wave_groups: Optional[int] = None):
+        super().__init__(stack, ljlk_term, device=device, dtype=dtype, capacity=capacity,
+                         wave_groups=wave_groups)
+        pbt = stack.packed_block_types
+        t = elec_term.packed_tables(pbt)
+        self.elec = ops.ElecTypeTables(
+            pbt.n_atoms, pbt.n_conn, pbt.conn_atom, t["is_ligand_fragment"], t["partial_charge"],
+            t["inter_repr_path_distance"], t["intra_repr_path_distance"],
+            elec_term.global_params, dtype, self.device)
+        self.energies = torch.zeros((4, self.P), dtype=dtype, device=self.device)
+        ops._check(ops._elec_lib().tmol_ljlk_elec_prepare(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ctypes.byref(self.elec.celec),
+            ops._p(self.ws.buf), self.ws.nbytes, self.ws.capacity, self.dcode,
+            ops._stream(self.device)), "elec prepare")
+
+    def score(self, coords, flags, weights=None, energies=None, dvdx=None, grad=None):
+        if dvdx is not None or flags & ~(ops.WGRAD | ops.NO_ENERGY | ops.ACCUMULATE):
+            raise ops.LJLKLibraryError("the fused LJ+LK+elec pass supports energies and WGRAD only")
+        ops._check(ops._elec_lib().tmol_ljlk_elec_score(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ctypes.byref(self.elec.celec),
+            ops._p(coords), self.dcode, ops._p(self.ws.buf), self.ws.nbytes, self.ws.capacity,
+            flags, ops._p(weights), ops._p(energies), ops._p(grad), int(self.wave_groups),
+            ops._stream(self.device)), "ljlk+elec score")
+
+    def _weights(self, weights):
+        if weights is None:
+            weights = torch.tensor([1.0, 0.55, 1.0, 1.0])
+        w = torch.as_tensor(weights, dtype=self.dtype, device=self.device)
+        if w.dim() == 1:
+            w = w[:, None].expand(4, self.P)
+        return w.contiguous()
+
+
 class ElecRaggedScorer:
     """Elec over a ``RaggedPoseStack`` on the same scheduler: its own
     neighbor pass (elec's max_dis) into its own work list, then the
