@@ -99,10 +99,12 @@ void FN(oracle_elec_pair)(REAL dist, REAL q1, REAL q2, int sep, const REAL* gp,
 }
 
 /* one atom pair into the block-pair partial (elec.hh:406-548): energy, and
- * dV/dx of both atoms into plane 0 of dvdx, zero components skipped */
+ * w * dV/dx of both atoms into dvdx (w = 1 for the pose op, dTdV of the
+ * rotamer pair for the rotamer op, elec.hh:466, 478), zero components
+ * skipped */
 static REAL FN(elec_eval_pair)(const REAL* coords, int64_t i1, int64_t i2, int sep,
                                REAL q1, REAL q2, const REAL* gp, int require_grad,
-                               REAL* dvdx) {
+                               REAL w, REAL* dvdx) {
   const REAL* x1 = coords + 3 * i1;
   const REAL* x2 = coords + 3 * i2;
   REAL d[3] = {x1[0] - x2[0], x1[1] - x2[1], x1[2] - x2[2]};
@@ -113,18 +115,59 @@ static REAL FN(elec_eval_pair)(const REAL* coords, int64_t i1, int64_t i2, int s
   REAL dV = 0;
   REAL V = FN(elec_pair)(dist, q1, q2, (REAL)sep, D, D0, S, mn, mx, &dV);
   if (dvdx) {
+    REAL wdV = w * dV;
     for (int k = 0; k < 3; ++k) {
       REAL u = dist != 0 ? d[k] / dist : 0;
-      REAL v = dV * u;
+      REAL v = wdV * u;
       if (v != 0) dvdx[i1 * 3 + k] += v;
     }
     for (int k = 0; k < 3; ++k) {
       REAL u = dist != 0 ? -d[k] / dist : 0;
-      REAL v = dV * u;
+      REAL v = wdV * u;
       if (v != 0) dvdx[i2 * 3 + k] += v;
     }
   }
   return V;
+}
+
+/* Elec of one block pair (b1 <= b2 of pose p; types bt1, bt2; first atoms
+ * o1, o2), reference CPU order: tiles of 32 atoms, pairs row-major inside a
+ * tile pair, one partial per tile pair; b1 == b2 takes the pairs a1 < a2
+ * with the intra-block representative separations. */
+static REAL FN(elec_block_pair)(const oracle_stack* st, const REAL* coords, const REAL* charges,
+                                const int32_t* intra_pd, const REAL* gp, int p, int b1, int b2,
+                                int bt1, int bt2, int64_t o1, int64_t o2, int require_grad,
+                                REAL w, REAL* dvdx) {
+  const int Am = st->max_n_block_atoms;
+  const int TILE = 32;
+  int n1 = st->bt_n_atoms[bt1], n2 = st->bt_n_atoms[bt2];
+  const REAL* q1 = charges + (int64_t)bt1 * Am;
+  const REAL* q2 = charges + (int64_t)bt2 * Am;
+  REAL tot = 0;
+  int nti1 = (n1 - 1) / TILE + 1, nti2 = (n2 - 1) / TILE + 1;
+  for (int ti = 0; ti < nti1; ++ti) {
+    int s1 = TILE * ti;
+    int m1 = n1 - s1 < TILE ? n1 - s1 : TILE;
+    for (int tj = (b1 == b2 ? ti : 0); tj < nti2; ++tj) {
+      int s2 = TILE * tj;
+      int m2 = n2 - s2 < TILE ? n2 - s2 : TILE;
+      REAL part = 0;
+      for (int k = 0; k < m1 * m2; ++k) {
+        int a1 = s1 + k / m2, a2 = s2 + k % m2;
+        int sep;
+        if (b1 == b2) {
+          if (a1 >= a2) continue;
+          sep = intra_pd[((int64_t)bt1 * Am + a1) * Am + a2];
+        } else {
+          sep = FN(inter_sep)(st, p, b1, b2, bt1, bt2, a1, a2);
+        }
+        part += FN(elec_eval_pair)(coords, o1 + a1, o2 + a2, sep, q1[a1], q2[a2], gp,
+                                   require_grad, w, dvdx);
+      }
+      tot += part;
+    }
+  }
+  return tot;
 }
 
 /* Elec over every neighbor block pair of a padded stack, reference CPU
@@ -139,8 +182,7 @@ void FN(oracle_elec_scores)(const oracle_stack* st, const REAL* coords, int64_t 
                             const REAL* charges, const int32_t* intra_pd, const REAL* gp,
                             const int32_t* neighbors, int block_pair, int require_grad,
                             REAL* out, REAL* dvdx) {
-  const int P = st->n_poses, B = st->max_n_blocks, Am = st->max_n_block_atoms;
-  const int TILE = 32;
+  const int P = st->n_poses, B = st->max_n_blocks;
   if (block_pair)
     for (int64_t k = 0; k < (int64_t)P * B * B; ++k) out[k] = 0;
   else
@@ -155,38 +197,50 @@ void FN(oracle_elec_scores)(const oracle_stack* st, const REAL* coords, int64_t 
         if (bt1 < 0 || bt2 < 0) continue;
         int64_t o1 = st->rot_coord_offset[p * B + b1];
         int64_t o2 = st->rot_coord_offset[p * B + b2];
-        int n1 = st->bt_n_atoms[bt1], n2 = st->bt_n_atoms[bt2];
-        const REAL* q1 = charges + (int64_t)bt1 * Am;
-        const REAL* q2 = charges + (int64_t)bt2 * Am;
-        REAL tot = 0;
-        int nti1 = (n1 - 1) / TILE + 1, nti2 = (n2 - 1) / TILE + 1;
-        for (int ti = 0; ti < nti1; ++ti) {
-          int s1 = TILE * ti;
-          int m1 = n1 - s1 < TILE ? n1 - s1 : TILE;
-          for (int tj = (b1 == b2 ? ti : 0); tj < nti2; ++tj) {
-            int s2 = TILE * tj;
-            int m2 = n2 - s2 < TILE ? n2 - s2 : TILE;
-            REAL part = 0;
-            for (int k = 0; k < m1 * m2; ++k) {
-              int a1 = s1 + k / m2, a2 = s2 + k % m2;
-              int sep;
-              if (b1 == b2) {
-                if (a1 >= a2) continue;
-                sep = intra_pd[((int64_t)bt1 * Am + a1) * Am + a2];
-              } else {
-                sep = FN(inter_sep)(st, p, b1, b2, bt1, bt2, a1, a2);
-              }
-              part += FN(elec_eval_pair)(coords, o1 + a1, o2 + a2, sep, q1[a1], q2[a2], gp,
-                                         require_grad, dvdx);
-            }
-            tot += part;
-          }
-        }
+        REAL tot = FN(elec_block_pair)(st, coords, charges, intra_pd, gp, p, b1, b2, bt1, bt2,
+                                       o1, o2, require_grad, (REAL)1, dvdx);
         if (block_pair)
           out[((int64_t)p * B + b1) * B + b2] = tot;
         else
           out[p] += tot;
       }
+}
+
+/* Elec of every rotamer pair of a dispatch list indices [3, n] = (pose,
+ * rot1, rot2) (ElecRotamerScoreDispatch, elec_pose_score.impl.hh:1132-1436
+ * forward, :1437-1700 backward; tile_evaluate_rot_pair,
+ * tile_atom_pair_evaluation.hh:377-500): dTdV == NULL gives the energies
+ * out[n]; dTdV [n] gives grad [NA, 3] = sum_i dTdV[i] dV(pair i)/dx.  The
+ * two rotamers of a pair take their own block types and first atoms; the
+ * count pair of different blocks uses the pose's inter-block separations,
+ * and a pair of rotamers of one block (a rotamer with itself) is that
+ * rotamer's intra-block energy.  Pairs with a rotamer of no block type score
+ * zero. */
+void FN(oracle_elec_rotamer_scores)(const oracle_stack* st, const REAL* coords, int64_t n_atoms,
+                                    const REAL* charges, const int32_t* intra_pd, const REAL* gp,
+                                    int64_t n_pairs, const int32_t* indices,
+                                    const int32_t* rot_coord_offset,
+                                    const int32_t* block_ind_for_rot,
+                                    const int32_t* block_type_ind_for_rot, const REAL* dTdV,
+                                    REAL* out, REAL* grad) {
+  if (out)
+    for (int64_t k = 0; k < n_pairs; ++k) out[k] = 0;
+  if (grad)
+    for (int64_t k = 0; k < 3 * n_atoms; ++k) grad[k] = 0;
+  for (int64_t i = 0; i < n_pairs; ++i) {
+    int p = indices[i], r1 = indices[n_pairs + i], r2 = indices[2 * n_pairs + i];
+    if (p < 0 || r1 < 0 || r2 < 0) continue;
+    int bt1 = block_type_ind_for_rot[r1], bt2 = block_type_ind_for_rot[r2];
+    if (bt1 < 0 || bt2 < 0) continue;
+    int b1 = block_ind_for_rot[r1], b2 = block_ind_for_rot[r2];
+    if (dTdV) {
+      FN(elec_block_pair)(st, coords, charges, intra_pd, gp, p, b1, b2, bt1, bt2,
+                          rot_coord_offset[r1], rot_coord_offset[r2], 1, dTdV[i], grad);
+    } else {
+      out[i] = FN(elec_block_pair)(st, coords, charges, intra_pd, gp, p, b1, b2, bt1, bt2,
+                                   rot_coord_offset[r1], rot_coord_offset[r2], 0, (REAL)1, 0);
+    }
+  }
 }
 
 #undef FN

@@ -87,6 +87,11 @@ def lib():
                 ctypes.POINTER(OracleStack), PR, ctypes.c_int64, PR, _P, PR, _P,
                 ctypes.c_int, ctypes.c_int, PR, PR,
             ]
+            f = getattr(L, "oracle_elec_rotamer_scores" + suf)
+            f.argtypes = [
+                ctypes.POINTER(OracleStack), PR, ctypes.c_int64, PR, _P, PR, ctypes.c_int64, _P,
+                _P, _P, _P, PR, PR, PR,
+            ]
         L.oracle_rot_dispatch.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P,
                                           ctypes.c_int64]
         L.oracle_rot_dispatch.restype = ctypes.c_int64
@@ -392,49 +397,68 @@ def elec_pair(dist, q1, q2, sep, global_params, with_deriv=True, dtype=numpy.flo
     return o
 
 
+class _ElecStack:
+    """Host arrays of the elec op's pose-stack and block-type arguments
+    (positions 13-22 of ``elec_pose_scores`` / ``elec_rotamer_scores``) as an
+    oracle stack: bt_path_distance holds the inter-block representative path
+    distances."""
+
+    def __init__(self, args, dtype):
+        (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+         first_rot_block_type, block_ind_for_rot, pose_ind_for_rot,
+         block_type_ind_for_rot, n_rots_for_pose, rot_offset_for_pose,
+         n_rots_for_block, rot_offset_for_block, max_n_rots_per_pose,
+         min_bondsep, inter_block_bondsep, bt_n_atoms, bt_charge, bt_n_conn,
+         bt_conn_atoms, bt_inter_rpd, bt_intra_rpd, bt_is_ligand, global_params,
+         max_dis, block_pair) = args
+        P, B = first_rot_for_block.shape
+        T, Am = bt_charge.shape
+        if dtype is None:
+            dtype = numpy.float64 if coords.dtype == torch.float64 else numpy.float32
+        self.dtype = dtype
+        self.suf, self.ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+        self.arrays = arrays = {
+            "rot_coord_offset": _np_i32(rot_coord_offset),
+            "block_type": _np_i32(block_type_ind_for_rot),
+            "min_bondsep": _np_i32(min_bondsep),
+            "inter_block_bondsep": _np_i32(inter_block_bondsep),
+            "bt_n_atoms": _np_i32(bt_n_atoms),
+            "bt_n_conn": _np_i32(bt_n_conn),
+            "bt_conn_atoms": _np_i32(bt_conn_atoms),
+            "bt_path_distance": _np_i32(bt_inter_rpd),
+            "bt_is_ligand": _np_i32(bt_is_ligand),
+        }
+        st = OracleStack()
+        st.n_poses, st.max_n_blocks = P, B
+        st.n_block_types, st.max_n_block_atoms = T, Am
+        st.max_n_tiles = (Am - 1) // 32 + 1
+        st.max_n_conn = bt_conn_atoms.shape[1]
+        for k, v in arrays.items():
+            setattr(st, k, _ptr(v, ctypes.c_int32))
+        self.struct = st
+        self.P, self.B = P, B
+        self.intra = _np_i32(bt_intra_rpd)
+        self.x = numpy.ascontiguousarray(coords.detach().cpu().numpy().astype(dtype))
+        self.q = numpy.ascontiguousarray(bt_charge.detach().cpu().numpy().astype(dtype))
+        self.gp = numpy.ascontiguousarray(
+            global_params.detach().cpu().numpy().astype(dtype).reshape(-1))
+        self.max_dis = float(max_dis)
+        self.block_pair = bool(block_pair)
+        self.rest = (first_rot_block_type, block_ind_for_rot, n_rots_for_block,
+                     rot_offset_for_block)
+
+
 def oracle_elec_scores(*args, require_grad=False, dtype=None):
     """Run the elec oracle on the 25 arguments of ``elec_pose_scores``
     (tmol/score/elec/potentials/compiled.ops.cpp:526-586).
 
     Returns (scores, block_neighbors, dV_dx, spheres): scores is (1, P) or
     (1, P, B, B), block_neighbors (P, B, B) int32, dV_dx (1, NA, 3) or None."""
-    (coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
-     first_rot_block_type, block_ind_for_rot, pose_ind_for_rot,
-     block_type_ind_for_rot, n_rots_for_pose, rot_offset_for_pose,
-     n_rots_for_block, rot_offset_for_block, max_n_rots_per_pose,
-     min_bondsep, inter_block_bondsep, bt_n_atoms, bt_charge, bt_n_conn,
-     bt_conn_atoms, bt_inter_rpd, bt_intra_rpd, bt_is_ligand, global_params,
-     max_dis, block_pair) = args
-    P, B = first_rot_for_block.shape
-    T, Am = bt_charge.shape
-    if dtype is None:
-        dtype = numpy.float64 if coords.dtype == torch.float64 else numpy.float32
-    L = lib()
-    suf, ct = ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
-    arrays = {
-        "rot_coord_offset": _np_i32(rot_coord_offset),
-        "block_type": _np_i32(block_type_ind_for_rot),
-        "min_bondsep": _np_i32(min_bondsep),
-        "inter_block_bondsep": _np_i32(inter_block_bondsep),
-        "bt_n_atoms": _np_i32(bt_n_atoms),
-        "bt_n_conn": _np_i32(bt_n_conn),
-        "bt_conn_atoms": _np_i32(bt_conn_atoms),
-        "bt_path_distance": _np_i32(bt_inter_rpd),
-        "bt_is_ligand": _np_i32(bt_is_ligand),
-    }
-    st = OracleStack()
-    st.n_poses, st.max_n_blocks = P, B
-    st.n_block_types, st.max_n_block_atoms = T, Am
-    st.max_n_tiles = (Am - 1) // 32 + 1
-    st.max_n_conn = bt_conn_atoms.shape[1]
-    for k, v in arrays.items():
-        setattr(st, k, _ptr(v, ctypes.c_int32))
-    intra = _np_i32(bt_intra_rpd)
-    x = numpy.ascontiguousarray(coords.detach().cpu().numpy().astype(dtype))
-    q = numpy.ascontiguousarray(bt_charge.detach().cpu().numpy().astype(dtype))
-    gp = numpy.ascontiguousarray(global_params.detach().cpu().numpy().astype(dtype).reshape(-1))
+    E = _ElecStack(args, dtype)
+    L, ct, suf, P, B, arrays = lib(), E.ct, E.suf, E.P, E.B, E.arrays
+    x = E.x
     NA = x.shape[0]
-    spheres = numpy.zeros((P * B, 4), dtype=dtype)
+    spheres = numpy.zeros((P * B, 4), dtype=E.dtype)
     getattr(L, "oracle_block_spheres" + suf)(
         _ptr(x, ct), P * B, arrays["rot_coord_offset"].ctypes.data_as(_P),
         arrays["block_type"].ctypes.data_as(_P), arrays["bt_n_atoms"].ctypes.data_as(_P),
@@ -442,29 +466,94 @@ def oracle_elec_scores(*args, require_grad=False, dtype=None):
     nbr = numpy.zeros((P, B, B), dtype=numpy.int32)
     getattr(L, "oracle_block_neighbors" + suf)(
         P, B, arrays["block_type"].ctypes.data_as(_P), _ptr(spheres, ct),
-        ct(dtype(max_dis)), nbr.ctypes.data_as(_P))
-    out = numpy.zeros((1, P, B, B) if block_pair else (1, P), dtype=dtype)
-    dvdx = numpy.zeros((1, NA, 3), dtype=dtype) if require_grad else None
+        ct(E.dtype(E.max_dis)), nbr.ctypes.data_as(_P))
+    out = numpy.zeros((1, P, B, B) if E.block_pair else (1, P), dtype=E.dtype)
+    dvdx = numpy.zeros((1, NA, 3), dtype=E.dtype) if require_grad else None
     getattr(L, "oracle_elec_scores" + suf)(
-        ctypes.byref(st), _ptr(x, ct), NA, _ptr(q, ct), intra.ctypes.data_as(_P), _ptr(gp, ct),
-        nbr.ctypes.data_as(_P), int(bool(block_pair)), int(require_grad), _ptr(out, ct),
-        _ptr(dvdx, ct) if dvdx is not None else None)
+        ctypes.byref(E.struct), _ptr(x, ct), NA, _ptr(E.q, ct), E.intra.ctypes.data_as(_P),
+        _ptr(E.gp, ct), nbr.ctypes.data_as(_P), int(E.block_pair), int(require_grad),
+        _ptr(out, ct), _ptr(dvdx, ct) if dvdx is not None else None)
     return out, nbr, dvdx, spheres
+
+
+def oracle_elec_rotamer_scores(*args, dTdV=None, dtype=None, indices=None):
+    """Run the elec oracle on the 25 arguments of ``elec_rotamer_scores``
+    (compiled.ops.cpp:588-650; ElecRotamerScoreDispatch,
+    elec_pose_score.impl.hh:1132-1700), the rotamer set in place of the pose
+    stack's rotamer tensors.  The dispatch list is the rotamer sphere /
+    block-neighbor / dispatch chain of the LJ+LK rotamer oracle with elec's
+    max_dis.
+
+    Returns (scores [1, n], dispatch_indices [3, n] int32, block_neighbors
+    [P, B, B]); with dTdV [1, n] also the weighted gradient [NA, 3] as a
+    fourth item.  ``indices`` [3, n] scores that list instead of the
+    dispatch (block_neighbors is then None)."""
+    E = _ElecStack(args, dtype)
+    L, ct, suf, arrays = lib(), E.ct, E.suf, E.arrays
+    frt, bir, nrb, rob = (_np_i32(t) for t in E.rest)
+    I = lambda a: _ptr(a, ctypes.c_int32)
+    rco, btr = arrays["rot_coord_offset"], arrays["block_type"]
+    if indices is not None:
+        idx = numpy.ascontiguousarray(numpy.asarray(indices, dtype=numpy.int32))
+        n, nbr = idx.shape[1], None
+    else:
+        idx, nbr = _elec_rot_dispatch(E, L, frt, nrb, rob)
+        n = idx.shape[1]
+    NA = E.x.shape[0]
+
+    def run(w):
+        out = numpy.zeros((1, n), dtype=E.dtype)
+        grad = numpy.zeros((NA, 3), dtype=E.dtype) if w is not None else None
+        getattr(L, "oracle_elec_rotamer_scores" + suf)(
+            ctypes.byref(E.struct), _ptr(E.x, ct), NA, _ptr(E.q, ct), I(E.intra),
+            _ptr(E.gp, ct), n, I(idx) if n else None, I(rco), I(bir), I(btr),
+            _ptr(w, ct) if w is not None else None,
+            _ptr(out, ct) if w is None else None,
+            _ptr(grad, ct) if grad is not None else None)
+        return grad if w is not None else out
+
+    res = (run(None), idx, nbr)
+    if dTdV is not None:
+        w = numpy.ascontiguousarray(numpy.asarray(dTdV, dtype=E.dtype).reshape(n))
+        res = res + (run(w),)
+    return res
+
+
+def _elec_rot_dispatch(E, L, frt, nrb, rob):
+    """Rotamer spheres, block spheres, block neighbors at elec's max_dis and
+    the dispatch list (the LJ+LK rotamer oracle's chain)."""
+    ct, suf, P, B, arrays = E.ct, E.suf, E.P, E.B, E.arrays
+    I = lambda a: _ptr(a, ctypes.c_int32)
+    rco, btr = arrays["rot_coord_offset"], arrays["block_type"]
+    R = rco.shape[0]
+    rot_sph = numpy.zeros((R, 4), dtype=E.dtype)
+    getattr(L, "oracle_block_spheres" + suf)(
+        _ptr(E.x, ct), R, I(rco), I(btr), I(arrays["bt_n_atoms"]), _ptr(rot_sph, ct))
+    blk_sph = numpy.zeros((P, B, 4), dtype=E.dtype)
+    getattr(L, "oracle_rot_block_spheres" + suf)(P, B, _ptr(rot_sph, ct), I(nrb), I(rob),
+                                                 _ptr(blk_sph, ct))
+    nbr = numpy.zeros((P, B, B), dtype=numpy.int32)
+    getattr(L, "oracle_block_neighbors" + suf)(P, B, I(frt), _ptr(blk_sph, ct),
+                                               ct(E.dtype(E.max_dis)), I(nbr))
+    n = int(L.oracle_rot_dispatch(P, B, I(nbr), I(nrb), I(rob), None, 0))
+    idx = numpy.full((3, max(n, 1)), -1, dtype=numpy.int32)
+    L.oracle_rot_dispatch(P, B, I(nbr), I(nrb), I(rob), I(idx), n)
+    return numpy.ascontiguousarray(idx[:, :n]), nbr
 
 
 def reference_op_available():
     return os.path.exists(REF_LIB_PATH)
 
 
-def load_reference_elec_op():
+def load_reference_elec_op(name="elec_pose_scores"):
     """The reference's own CPU elec op built by build_ref.sh
-    (``torch.ops.tmol_elec.elec_pose_scores``)."""
+    (``torch.ops.tmol_elec.elec_pose_scores``, or ``elec_rotamer_scores``)."""
     path = os.path.join(HERE, "_ref", "tmol_elec_ref.so")
     if not os.path.exists(path):
         raise FileNotFoundError(path)
     if not hasattr(torch.ops, "tmol_elec") or not hasattr(torch.ops.tmol_elec, "elec_pose_scores"):
         torch.ops.load_library(path)
-    return torch.ops.tmol_elec.elec_pose_scores
+    return getattr(torch.ops.tmol_elec, name)
 
 
 def load_reference_op():

@@ -1,0 +1,174 @@
+"""CPU tests of the elec rotamer-pair oracle: the C restatement of
+ElecRotamerScoreDispatch (tmol/score/elec/potentials/elec_pose_score.impl.hh:
+1132-1700) on the rotamer dispatch chain of sphere_overlap.impl.hh.
+
+Pinned by:
+* the reference's own CPU op, tmol_elec::elec_rotamer_scores (oracle/_ref,
+  where it builds) -- identical dispatch lists, energies and gradients;
+* golden vectors written by that op
+  (tests/golden/elec_rotamer_ref_single100.json, make_elec_rotamer_golden.py);
+* the elec block-pair oracle, itself pinned by tmol's 1ubq elec baselines: a
+  set of one rotamer per block (the pose's own residues) scores exactly the
+  block-pair energies;
+* central differences of the weighted energy for the backward.
+"""
+
+import json
+import os
+
+import numpy
+import pytest
+import torch
+
+from oracle import oracle as orc
+from tmol_amd.rotamers import current_rotamers, dispatch_checksum, repack_workload, rotamer_op_args
+from tmol_amd.score.elec_energy_term import ElecEnergyTerm, elec_op_args
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def _args(ps, rs, dt):
+    a = rotamer_op_args(ps, rs, ElecEnergyTerm())
+    a[16], a[22] = a[16].to(dt), a[22].to(dt)
+    return a
+
+
+@pytest.fixture(scope="module")
+def small():
+    """The first 24 residues of the 100-residue workload pose, with
+    repacking rotamers (float64)."""
+    from tmol_amd.chemical import default_chemical_database
+    from tmol_amd.generator import build_chains_fast, workload_specs
+    from tmol_amd.pose import Pose, PoseStack
+    from tmol_amd.rotamers import build_repack_rotamers
+
+    rts, phis, psis, omegas, chis = workload_specs("cpu_single_100")[0]
+    n = 24
+    db_rts = list(rts[:n])
+    db = default_chemical_database()
+    db_rts[-1] = db.residue_type(db_rts[-1].base_name, cterm=True)
+    sp = [(db_rts, phis[:n], psis[:n], omegas[:n], chis[:n])]
+    coords = build_chains_fast(sp)
+    ps = PoseStack([Pose(sp[0][0], coords[0])], dtype=torch.float64)
+    return ps, build_repack_rotamers(sp, ps, seed=3)
+
+
+def _golden():
+    return json.load(open(os.path.join(HERE, "golden", "elec_rotamer_ref_single100.json")))
+
+
+@pytest.mark.parametrize("name,dt", [("float32", torch.float32), ("float64", torch.float64)])
+def test_oracle_matches_reference_golden(name, dt):
+    g = _golden()[name]
+    ps, rs = repack_workload("cpu_single_100", seed=1, dtype=dt)
+    args = _args(ps, rs, dt)
+    assert abs(float(args[0].double().sum()) - g["coords_sum"]) < 1e-9 * abs(g["coords_sum"])
+    E, idx, _ = orc.oracle_elec_rotamer_scores(*args)
+    assert idx.shape[1] == g["n_pairs"]
+    assert dispatch_checksum(idx) == int(g["dispatch_checksum"])
+    pick = numpy.array(g["sample_pairs"])
+    numpy.testing.assert_array_equal(idx[:, pick], numpy.array(g["sample_indices"]))
+    tol = 1e-12 if dt == torch.float64 else 2e-5
+    numpy.testing.assert_allclose(E[0, pick], numpy.array(g["sample_energies"]), rtol=tol,
+                                  atol=tol)
+    assert abs(E.astype(numpy.float64).sum() - g["total"]) <= (
+        1e-12 if dt == torch.float64 else 1e-6) * g["abs_total"]
+    gen = torch.Generator().manual_seed(20261204)
+    w = torch.rand(E.shape, generator=gen, dtype=torch.float64).to(dt).numpy()
+    *_, grad = orc.oracle_elec_rotamer_scores(*args, dTdV=w, indices=idx)
+    atoms = numpy.array(g["sample_atoms"])
+    gr = numpy.array(g["sample_grad"])
+    err = numpy.abs(grad[atoms] - gr).max() / max(1.0, numpy.abs(gr).max())
+    assert err < (1e-12 if dt == torch.float64 else 1e-5)
+
+
+@pytest.mark.skipif(not orc.reference_op_available(), reason="oracle/_ref not built")
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_oracle_matches_reference_op(small, dt):
+    ps, rs = small
+    args = _args(ps, rs, dt)
+    args[0] = args[0].to(dt)
+    ref = orc.load_reference_elec_op("elec_rotamer_scores")
+    c = args[0].clone().requires_grad_(True)
+    a2 = list(args)
+    a2[0] = c
+    s, ridx = ref(*a2)
+    E, idx, _ = orc.oracle_elec_rotamer_scores(*args)
+    numpy.testing.assert_array_equal(ridx.numpy(), idx)
+    tol = 0.0 if dt == torch.float64 else 1e-5
+    d = numpy.abs(s.detach().numpy() - E) / numpy.maximum(1.0, numpy.abs(E))
+    assert d.max() <= tol
+    w = torch.rand(s.shape, generator=torch.Generator().manual_seed(2), dtype=dt)
+    (s * w).sum().backward()
+    *_, grad = orc.oracle_elec_rotamer_scores(*args, dTdV=w.numpy(), indices=idx)
+    gr = c.grad.numpy()
+    assert numpy.linalg.norm(grad - gr) <= (1e-14 if dt == torch.float64 else 1e-6) * \
+        numpy.linalg.norm(gr)
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_single_rotamer_set_equals_block_pair_energies(dt):
+    """One rotamer per block, the pose's own residues: every dispatched pair
+    scores exactly the elec block-pair oracle's (b1, b2) entry (the upper
+    triangle holds each block pair once) and the dispatch covers every
+    neighbor cell."""
+    from tmol_amd.generator import workload_poses
+    from tmol_amd.pose import PoseStack
+
+    poses = workload_poses("cpu_single_100") + workload_poses("padded_64x150_600", 1)
+    ps = PoseStack(poses, dtype=dt)
+    rs = current_rotamers(ps)
+    E, idx, nbr = orc.oracle_elec_rotamer_scores(*_args(ps, rs, dt))
+    pa = elec_op_args(ps, ElecEnergyTerm(), block_pair=True)
+    pa[16], pa[22] = pa[16].to(dt), pa[22].to(dt)
+    bp, pnbr, _, _ = orc.oracle_elec_scores(*pa)
+    numpy.testing.assert_array_equal(nbr, pnbr)
+    p, r1, r2 = idx
+    blk = rs.block_ind_for_rot.numpy()
+    b1, b2 = blk[r1], blk[r2]
+    lo, hi = numpy.minimum(b1, b2), numpy.maximum(b1, b2)
+    numpy.testing.assert_array_equal(E[0], bp[0, p, lo, hi])
+    assert idx.shape[1] == int(nbr.sum())
+    # each off-diagonal cell is listed in both orders
+    up = b1 <= b2
+    assert numpy.abs(bp.astype(numpy.float64)).sum() == pytest.approx(
+        numpy.abs(E[0, up].astype(numpy.float64)).sum(), rel=1e-12)
+
+
+def test_backward_central_differences(small):
+    """Central differences (h = 1e-5 A) of sum_i w_i E_i against the
+    dTdV-weighted backward; only the pairs of the perturbed atom's rotamer
+    change, so only those are re-scored."""
+    ps, rs = small
+    args = _args(ps, rs, torch.float64)
+    E, idx, _ = orc.oracle_elec_rotamer_scores(*args)
+    w = numpy.random.default_rng(5).random(idx.shape[1])
+    *_, grad = orc.oracle_elec_rotamer_scores(*args, dTdV=w, indices=idx)
+    rng = numpy.random.default_rng(9)
+    x0 = args[0].clone()
+    NA = x0.shape[0]
+    q = args[16].numpy()
+    bt = rs.block_type_ind_for_rot.numpy()
+    off = rs.coord_offset_for_rot.numpy()
+    rot_of_atom = numpy.searchsorted(off, numpy.arange(NA), side="right") - 1
+    charged = numpy.array([q[bt[rot_of_atom[a]], a - off[rot_of_atom[a]]] != 0
+                           for a in range(NA)])
+    atoms = rng.choice(numpy.nonzero(charged)[0], 10, replace=False)
+    h = 1e-5
+    n_nonzero = 0
+    for a in atoms:
+        r = rot_of_atom[a]
+        sel = numpy.nonzero((idx[1] == r) | (idx[2] == r))[0]
+        for k in range(3):
+            vals = []
+            for s in (1, -1):
+                x = x0.clone()
+                x[a, k] += s * h
+                args[0] = x
+                e, _, _ = orc.oracle_elec_rotamer_scores(*args, indices=idx[:, sel])
+                vals.append(float((w[sel] * e[0]).sum()))
+            fd = (vals[0] - vals[1]) / (2 * h)
+            assert abs(fd - grad[a, k]) <= 1e-5 * max(1.0, abs(fd)), (a, k, fd, grad[a, k])
+            n_nonzero += abs(fd) > 1e-3
+    args[0] = x0
+    assert n_nonzero > 5
