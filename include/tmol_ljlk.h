@@ -429,6 +429,32 @@ int tmol_elec_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* 
                      void* workspace, int64_t ws_bytes, int64_t row_capacity, uint64_t* census,
                      void* stream);
 
+/* Elec rotamer-pair scoring for packing.  Replaces the reference's op
+ *   tmol_elec::elec_rotamer_scores   tmol/score/elec/potentials/compiled.ops.cpp:588-650
+ *   ElecRotamerScoreDispatch::forward / ::backward
+ *                                    elec_pose_score.impl.hh:1132-1700
+ * on the LJ+LK rotamer dispatch: size the workspace with
+ * tmol_ljlk_rotamer_workspace_bytes, run tmol_elec_rotamer_neighbors (in
+ * place of tmol_ljlk_rotamer_neighbors: elec's tables instead of the LJ+LK
+ * atom records, max_dis = elec's), then tmol_ljlk_rotamer_totals and
+ * tmol_ljlk_rotamer_dispatch with the same max_dis, then
+ * tmol_elec_rotamer_score.  The block-type struct is elec's (as for
+ * tmol_elec_neighbors); block types of at most 32 atoms (ENOTSUP beyond). */
+int tmol_elec_rotamer_neighbors(const tmol_ljlk_rotamer_set* rs,
+                                const tmol_ljlk_block_types* bt, const tmol_elec_tables* elec,
+                                const void* coords, int32_t dtype, double max_dis,
+                                void* workspace, int64_t ws_bytes, void* stream);
+
+/* flags 0: energies [n_pairs] (the elec energy of each rotamer pair; the
+ * reference's [1, n] output).  flags WGRAD | NO_ENERGY: the backward,
+ * grad [NA, 3] = sum_i dTdV[i] dV(pair i)/dx (zeroed first unless
+ * ACCUMULATE).  n_workgroups <= 0: one per resident slot. */
+int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                            const tmol_elec_tables* elec, const void* coords, int32_t dtype,
+                            void* workspace, int64_t ws_bytes, const void* units,
+                            int64_t n_units, int64_t n_pairs, int32_t flags, const void* dTdV,
+                            void* energies, void* grad, int32_t n_workgroups, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

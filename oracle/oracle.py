@@ -519,6 +519,14 @@ def oracle_elec_rotamer_scores(*args, dTdV=None, dtype=None, indices=None):
     return res
 
 
+def oracle_elec_rotamer_dispatch(*args, dtype=None):
+    """(dispatch_indices [3, n] int32, block_neighbors [P, B, B]) of the 25
+    arguments of ``elec_rotamer_scores``, without scoring."""
+    E = _ElecStack(args, dtype)
+    frt, _, nrb, rob = (_np_i32(t) for t in E.rest)
+    return _elec_rot_dispatch(E, lib(), frt, nrb, rob)
+
+
 def _elec_rot_dispatch(E, L, frt, nrb, rob):
     """Rotamer spheres, block spheres, block neighbors at elec's max_dis and
     the dispatch list (the LJ+LK rotamer oracle's chain)."""

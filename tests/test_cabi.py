@@ -194,3 +194,52 @@ def test_elec_validation():
                                  float("inf"), dummy, 1, dummy, nbytes, 10, None, None) == ops.EINVAL
     assert L.tmol_elec_neighbors(ctypes.byref(s), ctypes.byref(b), ctypes.byref(e), dummy, ops.F32,
                                  5.5, dummy, 1, dummy, nbytes - 1, 10, None, None) == ops.EINVAL
+
+
+def test_elec_rotamer_validation():
+    """Argument checks of the elec rotamer entry points (host side, before
+    any device work): missing elec tables, block types over 32 atoms, a
+    short workspace, unsupported flags, missing outputs."""
+    from tmol_amd import ops
+
+    L = ops._elec_lib()
+    ops._rot_lib()
+    dummy = ctypes.c_void_p(16)
+    r = ops.CRotamerSet()
+    r.n_poses, r.max_n_blocks, r.n_rots, r.reserved, r.n_atoms_total = 2, 10, 50, 0, 900
+    for k in ("rot_coord_offset", "block_ind_for_rot", "block_type_ind_for_rot",
+              "n_rots_for_block", "rot_offset_for_block", "first_rot_block_type",
+              "min_bondsep", "inter_block_bondsep"):
+        setattr(r, k, dummy)
+    b = ops.CBlockTypes()
+    b.n_block_types, b.max_n_atoms, b.max_n_tiles, b.max_n_conn = 3, 24, 1, 2
+    b.dtype = ops.F32
+    b.n_atoms = b.n_conn = b.conn_atoms = b.is_ligand = dummy
+    e = ops.CElecTables()
+    for i, v in enumerate((79.931, 6.648, 0.441546, 1.6, 5.5)):
+        e.global_params[i] = v
+    nbytes = L.tmol_ljlk_rotamer_workspace_bytes(ctypes.byref(r), ctypes.byref(b), ops.F32)
+    assert nbytes > 0
+
+    def nbr(max_dis=5.5, ws=nbytes):
+        return L.tmol_elec_rotamer_neighbors(ctypes.byref(r), ctypes.byref(b), ctypes.byref(e),
+                                             dummy, ops.F32, max_dis, dummy, ws, None)
+
+    def score(flags=0, energies=dummy, grad=dummy, dTdV=dummy):
+        return L.tmol_elec_rotamer_score(ctypes.byref(r), ctypes.byref(b), ctypes.byref(e), dummy,
+                                         ops.F32, dummy, nbytes, dummy, 4, 10, flags, dTdV,
+                                         energies, grad, 0, None)
+
+    assert nbr() == ops.EINVAL and score() == ops.EINVAL  # elec tables missing
+    e.partial_charge = dummy
+    e.inter_repr_path_distance = ctypes.cast(dummy, ops._P32)
+    e.intra_repr_path_distance = ctypes.cast(dummy, ops._P32)
+    assert nbr(ws=nbytes - 1) == ops.EINVAL
+    assert nbr(max_dis=float("nan")) == ops.EINVAL
+    assert score(flags=ops.WGRAD) == ops.ENOTSUP
+    assert score(flags=ops.DVDX) == ops.ENOTSUP
+    assert score(energies=None) == ops.EINVAL
+    assert score(flags=ops.WGRAD | ops.NO_ENERGY, grad=None) == ops.EINVAL
+    assert score(flags=ops.WGRAD | ops.NO_ENERGY, dTdV=None) == ops.EINVAL
+    b.max_n_atoms, b.max_n_tiles = 40, 2
+    assert nbr() == ops.ENOTSUP and score() == ops.ENOTSUP

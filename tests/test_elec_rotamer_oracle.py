@@ -109,9 +109,8 @@ def test_oracle_matches_reference_op(small, dt):
 @pytest.mark.parametrize("dt", [torch.float32, torch.float64])
 def test_single_rotamer_set_equals_block_pair_energies(dt):
     """One rotamer per block, the pose's own residues: every dispatched pair
-    scores exactly the elec block-pair oracle's (b1, b2) entry (the upper
-    triangle holds each block pair once) and the dispatch covers every
-    neighbor cell."""
+    scores exactly the elec block-pair oracle's (b1, b2) entry and the
+    dispatch covers every neighbor cell."""
     from tmol_amd.generator import workload_poses
     from tmol_amd.pose import PoseStack
 
@@ -126,13 +125,12 @@ def test_single_rotamer_set_equals_block_pair_energies(dt):
     p, r1, r2 = idx
     blk = rs.block_ind_for_rot.numpy()
     b1, b2 = blk[r1], blk[r2]
-    lo, hi = numpy.minimum(b1, b2), numpy.maximum(b1, b2)
-    numpy.testing.assert_array_equal(E[0], bp[0, p, lo, hi])
+    numpy.testing.assert_array_equal(E[0], bp[0, p, b1, b2])
     assert idx.shape[1] == int(nbr.sum())
-    # each off-diagonal cell is listed in both orders
-    up = b1 <= b2
+    # each neighbor cell (b1 <= b2) is listed once, and nothing lies outside them
+    assert (b1 <= b2).all()
     assert numpy.abs(bp.astype(numpy.float64)).sum() == pytest.approx(
-        numpy.abs(E[0, up].astype(numpy.float64)).sum(), rel=1e-12)
+        numpy.abs(E[0].astype(numpy.float64)).sum(), rel=1e-12)
 
 
 def test_backward_central_differences(small):

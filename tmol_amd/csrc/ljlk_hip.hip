@@ -24,6 +24,7 @@
 #include "ljlk_score_rows.hh"
 #include "ljlk_prune.hh"
 #include "ljlk_rotamer.hh"
+#include "ljlk_elec_rotamer.hh"
 #include "ljlk_topology.hh"
 #include "ljlk_elec.hh"
 
@@ -255,7 +256,9 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
 
 struct RotWsLayout {
   int64_t cursors, rot_spheres, block_spheres, records, conn_pd;
-  int64_t row_pairs, row_units, pair_off, unit_off, total;
+  int64_t row_pairs, row_units, pair_off, unit_off;
+  int64_t e_pd, e_intra;  // elec: packed representative path distances, intra pair tables
+  int64_t total;
 };
 
 static RotWsLayout rot_ws_layout(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_block_types* bt,
@@ -282,6 +285,10 @@ static RotWsLayout rot_ws_layout(const tmol_ljlk_rotamer_set* r, const tmol_ljlk
   o = align256(o + 8 * (G + 1));
   L.unit_off = o;
   o = align256(o + 8 * (G + 1));
+  L.e_pd = o;
+  o = align256(o + 4 * int64_t(bt->n_block_types) * bt->max_n_atoms);
+  L.e_intra = o;
+  o = align256(o + 4 * int64_t(bt->n_block_types) + 2 * int64_t(bt->n_block_types) * INTRA_MAX);
   L.total = o;
   return L;
 }
@@ -322,10 +329,13 @@ static RotView make_rot_view(const tmol_ljlk_rotamer_set* r, int C) {
   return v;
 }
 
+// Rotamer spheres, block spheres, neighbor rows and their scans, after the
+// term's per-block-type tables: LJ+LK atom records (elec == nullptr) or
+// elec's packed path distances and intra pair tables.
 template <typename Real>
 static int run_rot_neighbors(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_block_types* btp,
                              const void* coords, double max_dis, void* ws, hipStream_t stream,
-                             const RotWsLayout& Lw) {
+                             const RotWsLayout& Lw, const tmol_elec_tables* elec = nullptr) {
   RotView v = make_rot_view(r, btp->max_n_conn);
   BTView bt = make_bt_view(btp);
   char* w = (char*)ws;
@@ -333,9 +343,16 @@ static int run_rot_neighbors(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_blo
   Real* bsph = (Real*)(w + Lw.block_spheres);
   const int64_t G = int64_t(v.P) * v.B;
   const int nrec = bt.T * bt.Amax;
-  hipLaunchKernelGGL(prepare_records_kernel<Real>, dim3(grid_for(nrec)), dim3(256), 0, stream, bt,
-                     (const Real*)btp->type_params, btp->n_atom_types, (Real*)(w + Lw.records),
-                     (uint8_t*)(w + Lw.conn_pd));
+  if (elec)
+    hipLaunchKernelGGL(elec_prepare_kernel<Real>, dim3(bt.T), dim3(64), 0, stream, bt,
+                       (const Real*)elec->partial_charge, elec->inter_repr_path_distance,
+                       elec->intra_repr_path_distance, (uint32_t*)(w + Lw.e_pd),
+                       (int32_t*)(w + Lw.e_intra),
+                       (uint16_t*)(w + Lw.e_intra + 4 * int64_t(bt.T)));
+  else
+    hipLaunchKernelGGL(prepare_records_kernel<Real>, dim3(grid_for(nrec)), dim3(256), 0, stream,
+                       bt, (const Real*)btp->type_params, btp->n_atom_types,
+                       (Real*)(w + Lw.records), (uint8_t*)(w + Lw.conn_pd));
   if (v.R > 0) {
     // rotamer spheres: the block-sphere pass with rotamers as blocks
     StackView s{};
@@ -443,10 +460,9 @@ static ElecConst<Real> elec_constants(const float* g) {
   return k;
 }
 
-static int validate_elec(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
-                         const tmol_elec_tables* e, int dtype) {
-  int v = validate(s, bt, dtype);
-  if (v) return v;
+// elec's block-type tables and global parameters (shared by the pose and
+// rotamer entry points)
+static int validate_elec_tables(const tmol_ljlk_block_types* bt, const tmol_elec_tables* e) {
   if (!e || !e->partial_charge || !e->inter_repr_path_distance || !e->intra_repr_path_distance)
     return TMOL_LJLK_EINVAL;
   if (!bt->n_atoms || !bt->n_conn || !bt->is_ligand || (bt->max_n_conn > 0 && !bt->conn_atoms))
@@ -458,6 +474,54 @@ static int validate_elec(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* 
   // region bounds must be ordered: min_dis - 0.25 < min_dis + 0.25 <= max_dis - 1 < max_dis
   if (!(g[3] > 0.25f && double(g[3]) + 0.25 <= double(g[4]) - 1.0)) return TMOL_LJLK_EINVAL;
   return TMOL_LJLK_OK;
+}
+
+static int validate_elec(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                         const tmol_elec_tables* e, int dtype) {
+  int v = validate(s, bt, dtype);
+  if (v) return v;
+  return validate_elec_tables(bt, e);
+}
+
+static int validate_elec_rot(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_block_types* bt,
+                             const tmol_elec_tables* e, int dtype) {
+  int v = validate_rot(r, bt, dtype);
+  if (v) return v;
+  return validate_elec_tables(bt, e);
+}
+
+template <typename Real, bool BWD>
+static int run_elec_rot_score(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_block_types* btp,
+                              const tmol_elec_tables* e, const void* coords, void* ws,
+                              const RotWsLayout& Lw, const void* units, int64_t n_units,
+                              int64_t n_pairs, const void* dTdV, void* energies, void* grad,
+                              bool accumulate, int n_wg, hipStream_t stream) {
+  ElecRotArgs<Real> a{};
+  a.v = make_rot_view(r, btp->max_n_conn);
+  a.bt = make_bt_view(btp);
+  char* w = (char*)ws;
+  a.coords = (const Real*)coords;
+  a.charge = (const Real*)e->partial_charge;
+  a.epd = (const uint32_t*)(w + Lw.e_pd);
+  a.intra_count = (const int32_t*)(w + Lw.e_intra);
+  a.intra_pairs = (const uint16_t*)(w + Lw.e_intra + 4 * int64_t(btp->n_block_types));
+  a.k = elec_constants<Real>(e->global_params);
+  a.units = (const RotUnit*)units;
+  a.n_units = n_units;
+  a.n_pairs = n_pairs;
+  a.cursor = (uint32_t*)(w + Lw.cursors);
+  a.energies = (Real*)energies;
+  a.dTdV = (const Real*)dTdV;
+  a.grad = (Real*)grad;
+  if (BWD && !accumulate) fill<Real>((Real*)grad, r->n_atoms_total * 3, stream);
+  if (n_units > 0) {
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.cursor,
+                       int64_t(8 * QSTRIDE), 0u);
+    auto k = elec_rotamer_kernel<Real, BWD>;
+    if (n_wg <= 0) n_wg = resident_workgroups(k);
+    hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  }
+  return check(hipGetLastError());
 }
 
 template <typename Real>
@@ -1192,6 +1256,57 @@ int tmol_elec_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
   return run_elec_score<float>(s, bt, e, coords, workspace, row_capacity, 0, nullptr, nullptr,
                                nullptr, nullptr, 0, (hipStream_t)stream, L,
                                (unsigned long long*)census);
+}
+
+int tmol_elec_rotamer_neighbors(const tmol_ljlk_rotamer_set* rs,
+                                const tmol_ljlk_block_types* bt, const tmol_elec_tables* e,
+                                const void* coords, int32_t dtype, double max_dis,
+                                void* workspace, int64_t ws_bytes, void* stream) {
+  int v = validate_elec_rot(rs, bt, e, dtype);
+  if (v) return v;
+  if (!coords || !workspace) return TMOL_LJLK_EINVAL;
+  if (!std::isfinite(max_dis) || max_dis < 0) return TMOL_LJLK_EINVAL;
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_rot_neighbors<float>(rs, bt, coords, max_dis, workspace, st, L, e);
+  return run_rot_neighbors<double>(rs, bt, coords, max_dis, workspace, st, L, e);
+}
+
+int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_block_types* bt,
+                            const tmol_elec_tables* e, const void* coords, int32_t dtype,
+                            void* workspace, int64_t ws_bytes, const void* units,
+                            int64_t n_units, int64_t n_pairs, int32_t flags, const void* dTdV,
+                            void* energies, void* grad, int32_t n_workgroups, void* stream) {
+  int v = validate_elec_rot(rs, bt, e, dtype);
+  if (v) return v;
+  if (!coords || !workspace || n_units < 0 || n_pairs < 0 || (n_units > 0 && !units))
+    return TMOL_LJLK_EINVAL;
+  if (n_workgroups < 0 || n_workgroups > (1 << 20)) return TMOL_LJLK_EINVAL;
+  RotWsLayout L = rot_ws_layout(rs, bt, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  const int mode = flags & ~TMOL_LJLK_ACCUMULATE;
+  const bool bwd = mode == (TMOL_LJLK_WGRAD | TMOL_LJLK_NO_ENERGY);
+  if (mode != 0 && !bwd) return TMOL_LJLK_ENOTSUP;
+  if (n_pairs >= (int64_t(1) << 31)) return TMOL_LJLK_EOVERFLOW;
+  if (!bwd && n_pairs > 0 && !energies) return TMOL_LJLK_EINVAL;
+  if (bwd && (!grad || (n_pairs > 0 && !dTdV))) return TMOL_LJLK_EINVAL;
+  const bool acc = (flags & TMOL_LJLK_ACCUMULATE) != 0;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32) {
+    if (bwd)
+      return run_elec_rot_score<float, true>(rs, bt, e, coords, workspace, L, units, n_units,
+                                             n_pairs, dTdV, energies, grad, acc, n_workgroups,
+                                             st);
+    return run_elec_rot_score<float, false>(rs, bt, e, coords, workspace, L, units, n_units,
+                                            n_pairs, dTdV, energies, grad, acc, n_workgroups, st);
+  }
+  if (bwd)
+    return run_elec_rot_score<double, true>(rs, bt, e, coords, workspace, L, units, n_units,
+                                            n_pairs, dTdV, energies, grad, acc, n_workgroups, st);
+  return run_elec_rot_score<double, false>(rs, bt, e, coords, workspace, L, units, n_units,
+                                           n_pairs, dTdV, energies, grad, acc, n_workgroups, st);
 }
 
 }  // extern "C"
