@@ -6,12 +6,14 @@
 // block spheres, neighbor cells, scans, fill) built with elec's max_dis;
 // only the scoring kernel differs:
 //   inter     the lanes of a wave hold the atoms of a pack of consecutive b2
-//             rotamers (as many as fit in 64 lanes); each b1 rotamer of the
-//             unit is staged in LDS (x, y, z, charge and the packed
-//             representative path distances to its connection atoms) and
-//             read as wave-uniform broadcasts.  Uncharged b1 atoms are
-//             skipped (wave-uniform), uncharged lanes never evaluate.  Each
-//             rotamer pair's energy is a segmented lane reduction.
+//             rotamers (as many as fit in 64 lanes); for each b1 rotamer of
+//             the unit, lanes whose atom lies beyond the b1 rotamer's sphere
+//             radius + max_dis drop out (a pack with none left skips the
+//             rotamer), and the rotamer is staged in LDS (x, y, z, charge and
+//             the packed representative path distances to its connection
+//             atoms) and read as wave-uniform broadcasts.  Uncharged b1 atoms
+//             are skipped (wave-uniform), uncharged lanes never evaluate.
+//             Each rotamer pair's energy is a segmented lane reduction.
 //   intra     a rotamer with itself: the block type's prepared table of
 //             atom pairs a1 < a2 of representative separation >= 4 with both
 //             charges non-zero, one pair per lane.
@@ -28,6 +30,7 @@ struct ElecRotArgs {
   RotView v;
   BTView bt;  // n_atoms, n_conn, conn_atoms, is_ligand (elec's table)
   const Real* coords;
+  const Real* rot_spheres;      // [R, 4] centre, radius (from the neighbor pass)
   const Real* charge;           // [T, Amax]
   const uint32_t* epd;          // [T, Amax] packed representative path distances
   const int32_t* intra_count;   // [T]
@@ -198,13 +201,20 @@ __device__ void elec_rot_inter(const ElecRotArgs<Real>& A, ElecRotLDS<Real>& L, 
       const int64_t e = entry + int64_t(i) * nr2 + ps + slot_c;
       const Real w = (BWD && active) ? A.dTdV[e] : Real(0);
       Real ee = 0;
-      stage_elec_rot(L, A, t1, o1, n1, lane);
-      for (int j = 0; j < n1; ++j) {
+      // cull: a lane's atom farther than r1's radius + max_dis from r1's
+      // centre reaches no atom of r1 (1e-3 A margin for rounding)
+      const V sph = *(const V*)(A.rot_spheres + 4 * int64_t(r1));
+      const Real cx = R2.x - sph.x, cy = R2.y - sph.y, cz = R2.z - sph.z;
+      const Real reach = sph.w + k.hpe + Real(1e-3);
+      const bool near = charged2 && cx * cx + cy * cy + cz * cz < reach * reach;
+      const int n1n = __ballot(near) != 0 ? n1 : 0;  // wave-uniform
+      if (n1n > 0) stage_elec_rot(L, A, t1, o1, n1, lane);
+      for (int j = 0; j < n1n; ++j) {
         const V p1 = *(const V*)L.pos[j];
         if (p1.w == Real(0)) continue;  // uncharged b1 atom: every pair is 0
         const Real dx = p1.x - R2.x, dy = p1.y - R2.y, dz = p1.z - R2.z;
         const Real d2 = dx * dx + dy * dy + dz * dz;
-        bool in = charged2 && d2 < k.cull2;
+        bool in = near && d2 < k.cull2;
         if (__ballot(in) == 0) continue;
         Real wcp = Real(1);
         if (cp) {

@@ -501,6 +501,7 @@ static int run_elec_rot_score(const tmol_ljlk_rotamer_set* r, const tmol_ljlk_bl
   a.bt = make_bt_view(btp);
   char* w = (char*)ws;
   a.coords = (const Real*)coords;
+  a.rot_spheres = (const Real*)(w + Lw.rot_spheres);
   a.charge = (const Real*)e->partial_charge;
   a.epd = (const uint32_t*)(w + Lw.e_pd);
   a.intra_count = (const int32_t*)(w + Lw.e_intra);

@@ -11,6 +11,8 @@ Reported: rotamer pairs/s of the step; the scoring kernel alone (HIP events
 on its stream around repeated launches over a fixed dispatch list) with its
 roofline from the branch census x energy-only FLOP table; the reference's
 own CPU op (oracle/_ref, one thread) on the first pose as the CPU baseline.
+The ``elec`` entry does the same for ``elec_rotamer_scores`` on the same
+rotamers (plus its dTdV-weighted backward kernel).
 
     python -m tmol_amd.rotamer_bench [--steps 10] [--no-cpu]
 """
@@ -158,6 +160,101 @@ def run(device, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
     if cpu:
         log("rotamer cpu baseline ...")
         res["cpu_baseline"] = cpu_baseline()
+    log("elec rotamer pairs ...")
+    res["elec"] = elec_leg(device, ps, rs, steps, warmup, kernel_reps, cpu, log)
+    return res
+
+
+def _elec_args(ps, rs, device):
+    import torch
+
+    from .rotamers import rotamer_op_args
+    from .score.elec_energy_term import ElecEnergyTerm
+
+    a = rotamer_op_args(ps, rs, ElecEnergyTerm())
+    return [x.to(device) if isinstance(x, torch.Tensor) else x for x in a]
+
+
+def elec_leg(device, ps, rs, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
+    """``elec_rotamer_scores`` on the same rotamers: the whole forward (its
+    own dispatch at elec's 5.5 A reach), the scoring kernel alone, the
+    dTdV-weighted backward kernel, and the reference's CPU elec rotamer op
+    (one thread) on pose 0."""
+    import torch
+
+    from .ops import ElecRotamerCall, elec_rotamer_scores
+
+    a = _elec_args(ps, rs, device)
+    for _ in range(warmup):
+        s, idx = elec_rotamer_scores(*a)
+    torch.cuda.synchronize(device)
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        s, idx = elec_rotamer_scores(*a)
+    torch.cuda.synchronize(device)
+    op_ms = (time.perf_counter() - t0) * 1000.0 / steps
+    n_pairs = int(idx.shape[1])
+    finite = bool(torch.isfinite(s).all())
+    call = ElecRotamerCall(a, torch.float32)
+    call.dispatch()
+    out = torch.empty((1, call.n_pairs), dtype=torch.float32, device=device)
+    w = torch.ones((1, call.n_pairs), dtype=torch.float32, device=device)
+    grad = torch.empty_like(call.coords)
+    stream = torch.cuda.current_stream(device)
+    ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+
+    def timed(fn):
+        for _ in range(2):
+            fn()
+        torch.cuda.synchronize(device)
+        ev0.record(stream)
+        for _ in range(kernel_reps):
+            fn()
+        ev1.record(stream)
+        torch.cuda.synchronize(device)
+        return ev0.elapsed_time(ev1) / kernel_reps
+
+    score_ms = timed(lambda: call.energies(out))
+    bwd_ms = timed(lambda: call.backward(w, out=grad))
+    res = {
+        "workload": "repack_16x300",
+        "rotamer_pairs": n_pairs,
+        "value": n_pairs / (op_ms * 1e-3),
+        "unit": "rotamer pairs/s",
+        "ms_per_step": op_ms,
+        "score_ms": score_ms,
+        "backward_ms": bwd_ms,
+        "energies_finite": finite,
+        "dtype": "f32",
+    }
+    if cpu:
+        from oracle import oracle as orc
+
+        from .rotamers import repack_workload
+
+        log("elec rotamer cpu baseline ...")
+        torch.set_num_threads(1)
+        ps1, rs1 = repack_workload("repack_16x300", n_poses=1, seed=1)
+        a1 = _elec_args(ps1, rs1, "cpu")
+        try:
+            kind, f = "reference", orc.load_reference_elec_op("elec_rotamer_scores")
+        except FileNotFoundError:
+            kind = "port"
+
+            def f(*x):
+                return orc.oracle_elec_rotamer_scores(*x)[:2]
+        t0 = time.perf_counter()
+        _, i1 = f(*a1)
+        dt = time.perf_counter() - t0
+        n1 = int(i1.shape[1])
+        res["cpu_baseline"] = {
+            "value": n1 / dt,
+            "unit": "rotamer pairs/s",
+            "cores": 1,
+            "kind": kind,
+            "sample": "pose 0 of the batch (300 residues, %d rotamers, %d rotamer pairs), "
+                      "forward, torch threads = 1, %.1f s" % (rs1.n_rotamers_total, n1, dt),
+        }
     return res
 
 
