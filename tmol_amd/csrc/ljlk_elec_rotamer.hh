@@ -14,11 +14,16 @@
 //             atoms) and read as wave-uniform broadcasts.  Uncharged b1 atoms
 //             are skipped (wave-uniform), uncharged lanes never evaluate.
 //             Each rotamer pair's energy is a segmented lane reduction.
+//   energies  the forward's inter-block units are culled per b1 rotamer over
+//             every b2 atom and compacted into full chunks first (below,
+//             "energies, culled"); the backward keeps the pack layout above
 //   intra     a rotamer with itself: the block type's prepared table of
 //             atom pairs a1 < a2 of representative separation >= 4 with both
 //             charges non-zero, one pair per lane.
 // Block types of at most 32 atoms (the intra table's encoding).
 #pragma once
+
+#include <type_traits>
 
 #include "ljlk_elec.hh"
 #include "ljlk_rotamer.hh"
@@ -311,15 +316,248 @@ __device__ void elec_rot_intra(const ElecRotArgs<Real>& A, ElecRotLDS<Real>& L, 
   }
 }
 
+// ------------------------------------------------------------------ energies, culled
+
+// Energy forward with culling and compaction (the structure of the LJ+LK
+// rotamer energy kernel).  Per b1 rotamer r1 of a unit:
+//   cull      every atom of b2's rotamers (the candidates, 64 at a time in
+//             rotamer order, over windows of up to 64 b2 rotamers) is tested
+//             against r1's sphere; charged survivors are compacted, in order,
+//             into a 64-lane chunk
+//   phase A   for a full (or the last) chunk: r1's atoms come from the
+//             registers of the lanes that staged them (readlane), each lane
+//             tests its chunk atom; pairs within max_dis of non-zero
+//             count-pair weight are ballot-compacted into a FIFO
+//   phase B   64 FIFO entries at a time, one per lane, into the chunk lane's
+//             64-bit fixed-point LDS accumulator (exact, order-independent)
+//   fold      at the end of a chunk, a segmented scan adds the lanes' sums
+//             per rotamer into the window's per-rotamer sums
+//   write     after the window, one store per rotamer pair (zero for pairs
+//             with no surviving atoms)
+
+template <typename Real>
+struct ElecRotLDS2 {
+  using T = typename LdsAcc<Real>::T;
+  ElecRotLDS<Real> r1;  // staged atoms of the b1 rotamer, cell separations, marks
+  alignas(4 * sizeof(Real)) Real pos2[WAVE][4];  // chunk: x, y, z, charge
+  int meta2[WAVE];       // n_conn | ligand << 3 | slot << 16
+  uint32_t pd2[WAVE];    // packed representative path distances
+  uint16_t fifo[2 * WAVE];  // j | chunk lane << 5 | (separation 4) << 11
+  T lacc[WAVE];          // per chunk lane
+  Real sum[WAVE];        // per rotamer of the window
+};
+
+template <typename Real>
+__device__ __forceinline__ void elec_chunk_eval(ElecRotLDS2<Real>& L, int consume,
+                                                const ElecConst<Real>& k, int lane) {
+  using V = typename V4<Real>::T;
+  using Acc = LdsAcc<Real>;
+  if (lane < consume) {
+    const int en = L.fifo[lane];
+    const int j = en & 31, l2 = (en >> 5) & 63;
+    const Real wcp = (en >> 11) & 1 ? Real(0.2) : Real(1);
+    const V p1 = *(const V*)L.r1.pos[j], p2 = *(const V*)L.pos2[l2];
+    const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
+    Real gs;
+    const Real E =
+        elec_rot_pair<Real, false>(dx * dx + dy * dy + dz * dz, p1.w * p2.w, wcp, Real(0), k, gs);
+    atomicAdd(&L.lacc[l2], Acc::enc(E));
+  }
+}
+
+// phase A + B over the chunk's nb lanes against r1 (n1 atoms held in the
+// registers R1 of lanes 0..n1-1), then the fold into the window's sums
+template <typename Real>
+__device__ void elec_chunk(ElecRotLDS2<Real>& L, const ElecConst<Real>& k, int nb, int n1,
+                           bool cp, int min_sep, int nc1, bool lig1, const ElecAtom<Real>& R1,
+                           int lane) {
+  using V = typename V4<Real>::T;
+  using Acc = LdsAcc<Real>;
+  wave_sync();
+  const bool valid = lane < nb;
+  V p2 = {};
+  int meta = 0;
+  uint32_t pd2 = 0;
+  if (valid) {
+    p2 = *(const V*)L.pos2[lane];
+    meta = L.meta2[lane];
+    pd2 = L.pd2[lane];
+  }
+  const int nc2 = meta & 7;
+  const bool lig = lig1 && ((meta >> 3) & 1);
+  L.lacc[lane] = 0;
+  int n_f = 0;
+  for (int j = 0; j < n1; ++j) {
+    if (lane_val(R1.q, j) == Real(0)) continue;  // uncharged b1 atom
+    const Real dx = lane_val(R1.x, j) - p2.x, dy = lane_val(R1.y, j) - p2.y,
+               dz = lane_val(R1.z, j) - p2.z;
+    bool in = valid && dx * dx + dy * dy + dz * dz < k.cull2;
+    if (__ballot(in) == 0) continue;
+    bool w02 = false;
+    if (cp) {
+      const int sep = packed_pair_sep((uint32_t)lane_val((int)R1.pd, j), pd2, L.r1.cs, min_sep,
+                                      nc1, nc2, lig);
+      in = in && sep >= 4;
+      w02 = sep == 4;
+    }
+    const uint64_t m = __ballot(in);
+    if (in) L.fifo[n_f + lanes_below(m)] = (uint16_t)(j | (lane << 5) | (w02 ? (1 << 11) : 0));
+    n_f += __popcll(m);
+    if (n_f >= WAVE) {
+      wave_sync();
+      elec_chunk_eval(L, WAVE, k, lane);
+      fifo_shift(L.fifo, WAVE, n_f, lane);
+      n_f -= WAVE;
+    }
+  }
+  wave_sync();
+  if (n_f > 0) elec_chunk_eval(L, n_f, k, lane);
+  wave_sync();
+  // fold: lanes of one rotamer (slot) are contiguous
+  const int slot = meta >> 16;
+  const int prev = lane > 0 ? L.meta2[lane - 1] >> 16 : -1;
+  const uint64_t starts = __ballot(valid && (lane == 0 || slot != prev));
+  const uint64_t below = starts & ((2ull << lane) - 1ull);
+  const int seg_start = below ? 63 - __builtin_clzll(below) : 0;
+  const bool last =
+      valid && (lane == nb - 1 || (L.meta2[lane + 1 < WAVE ? lane + 1 : lane] >> 16) != slot);
+  Real e = valid ? Acc::dec(L.lacc[lane]) : Real(0);
+  e = segment_scan(e, seg_start, lane);
+  if (last) L.sum[slot] += e;
+  wave_sync();
+}
+
+// inter-block unit, energies only
+template <typename Real>
+__device__ void elec_rot_inter_energy(const ElecRotArgs<Real>& A, ElecRotLDS2<Real>& L, int g1,
+                                      int g2, int b2, int r1_first, int r1_count, int64_t entry,
+                                      int lane) {
+  using V = typename V4<Real>::T;
+  const RotView& v = A.v;
+  const ElecConst<Real>& k = A.k;
+  const int nr2 = uni(v.n_rots_for_block[g2]), off2 = uni(v.rot_offset_for_block[g2]);
+  const int min_sep = uni(v.min_bondsep[int64_t(g1) * v.B + b2]);
+  const bool cp = min_sep <= 4;
+  if (cp) {
+    wave_sync();
+    if (lane < MAX_CONN * MAX_CONN) {
+      const int c1 = lane / MAX_CONN, c2 = lane - c1 * MAX_CONN;
+      L.r1.cs[lane] = (c1 < v.C && c2 < v.C)
+                          ? (uint8_t)v.inter_block_bondsep[((int64_t(g1) * v.B + b2) * v.C + c1) *
+                                                               v.C + c2]
+                          : (uint8_t)6;
+    }
+    wave_sync();
+  }
+  for (int w0 = 0; w0 < nr2; w0 += WAVE) {
+    const int wn = min(WAVE, nr2 - w0);
+    // the window's rotamers (lane = rotamer): type, first atom, atom count,
+    // first candidate; candidates fetch theirs by lane shuffle
+    int nat = 0, t2l = -1, o2l = 0;
+    if (lane < wn) {
+      t2l = v.block_type_for_rot[off2 + w0 + lane];
+      o2l = v.rot_coord_offset[off2 + w0 + lane];
+      nat = t2l >= 0 ? A.bt.n_atoms[t2l] : 0;
+    }
+    int incl = nat;
+#pragma unroll
+    for (int o = 1; o < WAVE; o <<= 1) {
+      const int t = __shfl_up(incl, o, WAVE);
+      if (lane >= o) incl += t;
+    }
+    const int excl = incl - nat;
+    const int total = lane_val(incl, wn - 1);
+    for (int i = 0; i < r1_count; ++i) {
+      const int r1 = r1_first + i;
+      const int t1 = uni(v.block_type_for_rot[r1]);
+      if (t1 < 0) {
+        elec_rot_zero(A, entry + int64_t(i) * nr2 + w0, wn, lane);
+        continue;
+      }
+      const int o1 = uni(v.rot_coord_offset[r1]);
+      const int n1 = uni(A.bt.n_atoms[t1]);
+      const int nc1 = uni(A.bt.n_conn[t1]);
+      const bool lig1 = A.bt.is_ligand[t1];
+      const V sph = *(const V*)(A.rot_spheres + 4 * int64_t(r1));
+      const Real reach = sph.w + k.hpe + Real(1e-3);
+      const Real reach2 = reach * reach;
+      ElecAtom<Real> R1;
+      R1.x = R1.y = R1.z = R1.q = Real(0);
+      R1.pd = 0;
+      wave_sync();
+      if (lane < n1) {
+        load_elec_rot_atom(R1, A, t1, o1, lane);
+        V p;
+        p.x = R1.x;
+        p.y = R1.y;
+        p.z = R1.z;
+        p.w = R1.q;
+        *(V*)L.r1.pos[lane] = p;
+      }
+      L.sum[lane] = 0;
+      int nb = 0;  // surviving candidates in the chunk (wave-uniform)
+      for (int c0 = 0; c0 < total; c0 += WAVE) {
+        // candidate c = c0 + lane: its rotamer (slot) and atom
+        const int c = c0 + lane;
+        const uint64_t before = __ballot(lane < wn && excl <= c0 && nat > 0);
+        const int carry = before ? 63 - __builtin_clzll(before) : 0;
+        wave_sync();
+        L.r1.mark[lane] = 0;
+        wave_sync();
+        if (lane < wn && nat > 0 && excl > c0 && excl < c0 + WAVE)
+          L.r1.mark[excl - c0] = (uint8_t)(lane + 1);
+        wave_sync();
+        const uint64_t st = __ballot(L.r1.mark[lane] != 0);
+        const uint64_t bl = st & ((2ull << lane) - 1ull);
+        const int slot = bl ? int(L.r1.mark[63 - __builtin_clzll(bl)]) - 1 : carry;
+        const bool cand = c < total;
+        const int a2 = c - __shfl(excl, slot, WAVE);
+        const int t2 = __shfl(t2l, slot, WAVE);
+        const int o2 = __shfl(o2l, slot, WAVE);
+        ElecAtom<Real> C2;
+        bool keep = false;
+        if (cand) {
+          load_elec_rot_atom(C2, A, t2, o2, a2);
+          const Real ex = C2.x - sph.x, ey = C2.y - sph.y, ez = C2.z - sph.z;
+          keep = C2.q != Real(0) && ex * ex + ey * ey + ez * ez < reach2;
+        }
+        const uint64_t mk = __ballot(keep);
+        const int nk = __popcll(mk);
+        if (nk == 0) continue;
+        if (nb + nk > WAVE) {
+          elec_chunk(L, k, nb, n1, cp, min_sep, nc1, lig1, R1, lane);
+          nb = 0;
+        }
+        if (keep) {
+          const int pos = nb + lanes_below(mk);
+          V p;
+          p.x = C2.x;
+          p.y = C2.y;
+          p.z = C2.z;
+          p.w = C2.q;
+          *(V*)L.pos2[pos] = p;
+          L.meta2[pos] = A.bt.n_conn[t2] | ((A.bt.is_ligand[t2] ? 1 : 0) << 3) | (slot << 16);
+          L.pd2[pos] = C2.pd;
+        }
+        nb += nk;
+      }
+      if (nb > 0) elec_chunk(L, k, nb, n1, cp, min_sep, nc1, lig1, R1, lane);
+      wave_sync();
+      if (lane < wn) A.energies[entry + int64_t(i) * nr2 + w0 + lane] = L.sum[lane];
+    }
+  }
+}
+
 // Persistent elec rotamer-pair kernel: 4 independent waves per workgroup,
 // units from the per-XCD queues.  BWD false: energies [n_pairs] (every entry
-// written, zero for pairs that are not scored); BWD true:
-// grad [NA, 3] += sum_i dTdV[i] dV(pair i)/dx.
+// written, zero for pairs that are not scored), inter-block units culled and
+// compacted; BWD true: grad [NA, 3] += sum_i dTdV[i] dV(pair i)/dx.
 template <typename Real, bool BWD>
 __global__ __launch_bounds__(256) void elec_rotamer_kernel(ElecRotArgs<Real> A) {
-  __shared__ ElecRotLDS<Real> lds[WAVES_PER_WG];
+  using LDS = typename std::conditional<BWD, ElecRotLDS<Real>, ElecRotLDS2<Real>>::type;
+  __shared__ LDS lds[WAVES_PER_WG];
   const int lane = lane_id();
-  ElecRotLDS<Real>& L = lds[threadIdx.x / WAVE];
+  LDS& L = lds[threadIdx.x / WAVE];
   RowQueue queue(A.cursor, A.n_units);
   for (int64_t u = queue.next(); u >= 0; u = queue.next()) {
     const RotUnit* up = A.units + u;
@@ -335,10 +573,18 @@ __global__ __launch_bounds__(256) void elec_rotamer_kernel(ElecRotArgs<Real> A) 
                       lane);
       continue;
     }
-    if (b1 == b2)
-      elec_rot_intra<Real, BWD>(A, L, r1_first, r1_count, entry, lane);
-    else
-      elec_rot_inter<Real, BWD>(A, L, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry, lane);
+    if constexpr (BWD) {
+      if (b1 == b2)
+        elec_rot_intra<Real, true>(A, L, r1_first, r1_count, entry, lane);
+      else
+        elec_rot_inter<Real, true>(A, L, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry, lane);
+    } else {
+      if (b1 == b2)
+        elec_rot_intra<Real, false>(A, L.r1, r1_first, r1_count, entry, lane);
+      else
+        elec_rot_inter_energy<Real>(A, L, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry,
+                                    lane);
+    }
   }
 }
 
