@@ -1,0 +1,164 @@
+"""GPU edge cases of the drop-in ops against the oracle:
+
+* tiny poses (one and two residues) stacked with a larger one;
+* blocks without a block type (-1) inside a pose, and a pose whose every
+  block is -1 (energies and gradient exactly zero there);
+* the ligand-fragment count-pair rule (both blocks ligand: separations 3 and
+  4 count as 5), switched on through the op's is_ligand table: separations
+  bit-exact, energies and gradients against the oracle;
+* the same three cases for elec.
+Tolerances: f64 within 1e-9 of max(1, |E|), f32 within 1e-4.
+"""
+
+import sys
+
+import numpy
+import pytest
+import torch
+
+from oracle import oracle as orc
+from tmol_amd.generator import generate_poses
+from tmol_amd.pose import PoseStack
+from tmol_amd.score.elec_energy_term import ElecEnergyTerm, elec_op_args
+from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+TOL = {torch.float64: 1e-9, torch.float32: 1e-4}
+
+
+def _say(msg):
+    print("[gpu-case] " + msg, flush=True)
+    sys.stdout.flush()
+
+
+def _poses():
+    return (generate_poses(1, (1, 1), seed=5) + generate_poses(1, (2, 2), seed=6)
+            + generate_poses(1, (40, 40), seed=7) + generate_poses(1, (25, 25), seed=8))
+
+
+def _ljlk_args(poses, dt):
+    a = pose_op_args(PoseStack(poses, dtype=dt), LJLKEnergyTerm())
+    a[23], a[24] = a[23].to(dt), a[24].to(dt)
+    return a
+
+
+def _elec_args(poses, dt):
+    a = elec_op_args(PoseStack(poses, dtype=dt), ElecEnergyTerm())
+    a[16], a[22] = a[16].to(dt), a[22].to(dt)
+    return a
+
+
+def _drop_blocks(a, B):
+    """Blocks without a type: every third block of pose 2, every block of
+    pose 3 (rot_coord_offset stays as it was)."""
+    a = list(a)
+    bt = a[7].clone().view(-1, B)
+    bt[2, ::3] = -1
+    bt[3, :] = -1
+    a[7] = bt.view(-1)
+    a[4] = bt.clone()
+    return a
+
+
+def _to(a, dev):
+    return [x.to(dev) if isinstance(x, torch.Tensor) else x for x in a]
+
+
+def _check_ljlk(a, dt, what):
+    from tmol_amd.ops import ljlk_pose_scores
+
+    _say("ljlk %s %s" % (what, dt))
+    o, onbr, odv, _ = orc.oracle_pose_scores(*a, require_grad=True, dtype=numpy.float64)
+    g = _to(a, DEV)
+    c = g[0].clone().requires_grad_(True)
+    g[0] = c
+    s, nbr = ljlk_pose_scores(*g)
+    numpy.testing.assert_array_equal(nbr.cpu().numpy(), onbr)
+    e = s.detach().double().cpu().numpy()
+    err = numpy.abs(e - o) / numpy.maximum(1.0, numpy.abs(o))
+    _say("max rel err %.3g" % err.max())
+    assert err.max() < TOL[dt]
+    s.sum().backward()
+    gg = c.grad.double().cpu().numpy()
+    og = odv.sum(0)
+    assert numpy.abs(gg - og).max() <= TOL[dt] * max(1.0, numpy.abs(og).max())
+    return e, gg
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_tiny_poses(dt):
+    e, _ = _check_ljlk(_ljlk_args(_poses(), dt), dt, "tiny poses")
+    assert numpy.abs(e[:, 0]).sum() > 0  # a lone residue still has intra-block energy
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_blocks_without_type(dt):
+    poses = _poses()
+    B = max(p.n_blocks for p in poses)
+    a = _drop_blocks(_ljlk_args(poses, dt), B)
+    e, gg = _check_ljlk(a, dt, "blocks without type")
+    assert (e[:, 3] == 0).all()
+    pose_of = a[2].numpy()
+    assert (gg[pose_of == 3] == 0).all()
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_ligand_count_pair(dt):
+    """Every block type flagged as a ligand fragment: inter-block
+    separations 3 and 4 become 5 (count_pair.hh; impl.hh:47-67)."""
+    from tmol_amd.ops import debug_pair_separations, padded_work_list
+
+    poses = _poses()
+    a = _ljlk_args(poses, dt)
+    a[22] = torch.ones_like(a[22])
+    _check_ljlk(a, dt, "ligand fragments")
+    g = _to(a, DEV)
+    items, _, _ = padded_work_list(g)
+    seps = debug_pair_separations(g, items).cpu().numpy()
+    B = max(p.n_blocks for p in poses)
+    trip = [(int(g1) // B, int(g1) % B, int(g2) % B) for g1, g2 in items.numpy()]
+    ref = orc.oracle_pair_separations(a, trip)
+    n_inter_close = 0
+    for k, m in enumerate(ref):
+        n1, n2 = m.shape
+        assert (seps[k, :n1, :n2] == m).all(), trip[k]
+        if trip[k][1] != trip[k][2]:
+            assert not ((m == 3) | (m == 4)).any()
+            n_inter_close += (m == 5).sum()
+    assert n_inter_close > 0
+
+
+def _check_elec(a, dt, what):
+    from tmol_amd.ops import elec_pose_scores
+
+    _say("elec %s %s" % (what, dt))
+    o, onbr, odv, _ = orc.oracle_elec_scores(*a, require_grad=True, dtype=numpy.float64)
+    g = _to(a, DEV)
+    c = g[0].clone().requires_grad_(True)
+    g[0] = c
+    s, nbr = elec_pose_scores(*g)
+    numpy.testing.assert_array_equal(nbr.cpu().numpy(), onbr)
+    e = s.detach().double().cpu().numpy()
+    err = numpy.abs(e - o) / numpy.maximum(1.0, numpy.abs(o))
+    _say("max rel err %.3g" % err.max())
+    assert err.max() < TOL[dt]
+    s.sum().backward()
+    gg = c.grad.double().cpu().numpy()
+    og = odv[0]
+    assert numpy.abs(gg - og).max() <= TOL[dt] * max(1.0, numpy.abs(og).max())
+    return e, gg
+
+
+@pytest.mark.parametrize("dt", [torch.float32, torch.float64])
+def test_elec_edge_cases(dt):
+    poses = _poses()
+    B = max(p.n_blocks for p in poses)
+    _check_elec(_elec_args(poses, dt), dt, "tiny poses")
+    a = _drop_blocks(_elec_args(poses, dt), B)
+    e, gg = _check_elec(a, dt, "blocks without type")
+    assert (e[:, 3] == 0).all()
+    assert (gg[a[2].numpy() == 3] == 0).all()
+    a = _elec_args(poses, dt)
+    a[21] = torch.ones_like(a[21])
+    _check_elec(a, dt, "ligand fragments")
