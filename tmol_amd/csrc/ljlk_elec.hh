@@ -227,8 +227,12 @@ __device__ __forceinline__ void stage_elec_atom(ElecLDS<Real, MODE>& L, int slot
   L.pdw[pd_row][a] = R.pd;
 }
 
+// Float WGRAD (the weighted energy + gradient step) runs at 5 waves per
+// SIMD with a few spilled registers: 2.15 against 2.33 ms at 4 on the
+// 512-pose batch; the other modes keep 4.
 template <typename Real, int MODE>
-__global__ __launch_bounds__(256, 4) void elec_rows_kernel(ElecArgs<Real> A) {
+__global__ __launch_bounds__(256, (sizeof(Real) == 4 && MODE == M_WGRAD) ? 5 : 4) void
+elec_rows_kernel(ElecArgs<Real> A) {
   using C = RowCtx<Real, MODE>;
   using V = typename V4<Real>::T;
   __shared__ ElecLDS<Real, MODE> lds_all[WAVES_PER_WG];
