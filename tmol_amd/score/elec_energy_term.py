@@ -198,6 +198,24 @@ class ElecEnergyTerm:
             self.get_pose_score_term_function(),
         )
 
+    def get_rotamer_score_term_function(self):
+        from ..ops import elec_rotamer_scores
+
+        return elec_rotamer_scores
+
+    def render_rotamer_scoring_module(self, pose_stack, rotamer_set):
+        """Rotamer-pair scoring for packing (the term's
+        render_rotamer_scoring_module): the pose stack supplies the
+        bond-separation tables, the rotamer set the coordinates' layout."""
+        from .scoring_module import TermRotamerScoringModule
+
+        return TermRotamerScoringModule(
+            self.class_name(),
+            rotamer_set,
+            self.get_score_term_attributes(pose_stack),
+            self.get_rotamer_score_term_function(),
+        )
+
 
 def elec_op_args(pose_stack, term: ElecEnergyTerm, coords=None, block_pair=False):
     """Full 25-argument list of ``elec_pose_scores`` for a padded stack

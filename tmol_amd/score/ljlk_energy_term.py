@@ -136,6 +136,24 @@ class LJLKEnergyTerm:
             self.get_pose_score_term_function(),
         )
 
+    def get_rotamer_score_term_function(self):
+        from ..ops import ljlk_rotamer_scores
+
+        return ljlk_rotamer_scores
+
+    def render_rotamer_scoring_module(self, pose_stack, rotamer_set):
+        """Rotamer-pair scoring for packing (the term's
+        render_rotamer_scoring_module): the pose stack supplies the
+        bond-separation tables, the rotamer set the coordinates' layout."""
+        from .scoring_module import TermRotamerScoringModule
+
+        return TermRotamerScoringModule(
+            self.class_name(),
+            rotamer_set,
+            self.get_score_term_attributes(pose_stack),
+            self.get_rotamer_score_term_function(),
+        )
+
 
 def pose_op_args(pose_stack, term: LJLKEnergyTerm, coords=None, block_pair=False):
     """Full 27-argument list of ``ljlk_pose_scores`` for a padded stack
