@@ -58,7 +58,7 @@ def test_workload_parity(workload):
     o32, onbr, _, _ = orc.oracle_pose_scores(*ca32, require_grad=False)
     assert (nbr32.cpu().numpy() == onbr).all()
     # energies: f32 within 1e-4 of the f64 kernels
-    rel = ((s32.double() - s64).abs() / s64.abs().clamp(min=1.0)).max()
+    rel = ((s32.detach().double() - s64.detach()).abs() / s64.detach().abs().clamp(min=1.0)).max()
     _say("max rel f32 vs f64 %.3g" % float(rel))
     assert float(rel) < 1e-4
     # f64 kernels against the f64 oracle, pose by pose (threads over poses)
