@@ -5,18 +5,13 @@
 // The dispatch list and work units are the LJ+LK rotamer ones (rotamer and
 // block spheres, neighbor cells, scans, fill) built with elec's max_dis;
 // only the scoring kernel differs:
-//   inter     the lanes of a wave hold the atoms of a pack of consecutive b2
-//             rotamers (as many as fit in 64 lanes); for each b1 rotamer of
-//             the unit, lanes whose atom lies beyond the b1 rotamer's sphere
-//             radius + max_dis drop out (a pack with none left skips the
-//             rotamer), and the rotamer is staged in LDS (x, y, z, charge and
-//             the packed representative path distances to its connection
-//             atoms) and read as wave-uniform broadcasts.  Uncharged b1 atoms
-//             are skipped (wave-uniform), uncharged lanes never evaluate.
-//             Each rotamer pair's energy is a segmented lane reduction.
-//   energies  the forward's inter-block units are culled per b1 rotamer over
-//             every b2 atom and compacted into full chunks first (below,
-//             "energies, culled"); the backward keeps the pack layout above
+//   inter     culled and compacted ("energies and gradients, culled" below):
+//             per b1 rotamer, every b2 atom is tested against the b1
+//             rotamer's sphere and charged survivors fill 64-lane chunks;
+//             pair tests read the b1 atoms from registers and compact the
+//             pairs within max_dis into a FIFO evaluated 64 at a time into
+//             fixed-point LDS accumulators (energy per chunk lane, or the
+//             gradient of both atoms)
 //   intra     a rotamer with itself: the block type's prepared table of
 //             atom pairs a1 < a2 of representative separation >= 4 with both
 //             charges non-zero, one pair per lane.
@@ -56,7 +51,7 @@ struct ElecRotLDS {
   alignas(4 * sizeof(Real)) Real pos[TILE][4];  // x, y, z, charge
   uint32_t pdw[TILE];                 // packed representative path distances
   uint8_t cs[MAX_CONN * MAX_CONN];    // connection-pair bond separations of the cell
-  uint8_t mark[WAVE];                 // lane starts a rotamer of the pack
+  uint8_t mark[WAVE];                 // candidate lane that starts a b2 rotamer
 };
 
 template <typename Real>
@@ -106,153 +101,10 @@ __device__ __forceinline__ Real elec_rot_pair(Real d2, Real qq, Real wcp, Real w
   return wcp * (qq * v);
 }
 
-// a1's gradient: sum over the wave, one atomic per component
-template <typename Real>
-__device__ __forceinline__ void elec_rot_flush_a1(Real* grad, int64_t atom, Real gx, Real gy,
-                                                  Real gz, int lane) {
-  gx = wave_sum(gx);
-  gy = wave_sum(gy);
-  gz = wave_sum(gz);
-  if (lane < 3) {
-    const Real v = lane == 0 ? gx : (lane == 1 ? gy : gz);
-    if (v != Real(0)) unsafeAtomicAdd(&grad[atom * 3 + lane], v);
-  }
-}
-
 template <typename Real>
 __device__ __forceinline__ void elec_rot_zero(const ElecRotArgs<Real>& A, int64_t e0, int64_t n,
                                               int lane) {
   for (int64_t i = lane; i < n; i += WAVE) A.energies[e0 + i] = Real(0);
-}
-
-// inter-block unit: rotamers [r1_first, +r1_count) of g1 x every rotamer of g2
-template <typename Real, bool BWD>
-__device__ void elec_rot_inter(const ElecRotArgs<Real>& A, ElecRotLDS<Real>& L, int g1, int g2,
-                               int b2, int r1_first, int r1_count, int64_t entry, int lane) {
-  using V = typename V4<Real>::T;
-  const RotView& v = A.v;
-  const ElecConst<Real>& k = A.k;
-  const int nr2 = uni(v.n_rots_for_block[g2]), off2 = uni(v.rot_offset_for_block[g2]);
-  const int min_sep = uni(v.min_bondsep[int64_t(g1) * v.B + b2]);
-  const bool cp = min_sep <= 4;
-  if (cp) {
-    wave_sync();
-    if (lane < MAX_CONN * MAX_CONN) {
-      const int c1 = lane / MAX_CONN, c2 = lane - c1 * MAX_CONN;
-      L.cs[lane] = (c1 < v.C && c2 < v.C)
-                       ? (uint8_t)v.inter_block_bondsep[((int64_t(g1) * v.B + b2) * v.C + c1) *
-                                                            v.C + c2]
-                       : (uint8_t)6;
-    }
-    wave_sync();
-  }
-  for (int ps = 0; ps < nr2;) {
-    // pack: consecutive rotamers ps.. whose atoms fit in 64 lanes (every
-    // block type has at most 32 atoms, so a pack holds at least two)
-    const int rl = ps + lane;
-    int nat = 0;
-    if (rl < nr2) {
-      const int t = v.block_type_for_rot[off2 + rl];
-      nat = t >= 0 ? A.bt.n_atoms[t] : 0;
-    }
-    int incl = nat;
-#pragma unroll
-    for (int o = 1; o < WAVE; o <<= 1) {
-      const int t = __shfl_up(incl, o, WAVE);
-      if (lane >= o) incl += t;
-    }
-    const int kp = __popcll(__ballot(rl < nr2 && incl <= WAVE));
-    const int excl = incl - nat;
-    wave_sync();
-    L.mark[lane] = 0;
-    wave_sync();
-    if (lane < kp && nat > 0) L.mark[excl] = (uint8_t)(lane + 1);
-    wave_sync();
-    const uint64_t starts = __ballot(L.mark[lane] != 0);
-    const int total = lane_val(incl, kp - 1);
-    const uint64_t below = starts & ((2ull << lane) - 1ull);
-    const int start_pos = below ? 63 - __builtin_clzll(below) : -1;
-    const int slot = start_pos >= 0 ? int(L.mark[start_pos]) - 1 : -1;
-    const int slot_c = slot < 0 ? 0 : slot;
-    const int seg_start = __shfl(excl, slot_c, WAVE);
-    const int seg_nat = __shfl(nat, slot_c, WAVE);
-    const int a2 = lane - seg_start;
-    const bool last = lane == seg_start + seg_nat - 1;
-    bool active = lane < total && slot >= 0;
-    const int r2 = off2 + ps + slot_c;
-    const int t2 = active ? v.block_type_for_rot[r2] : 0;
-    const int o2 = active ? v.rot_coord_offset[r2] : 0;
-    active = active && t2 >= 0 && a2 < seg_nat;
-    const int nc2 = active ? A.bt.n_conn[t2] : 0;
-    const bool lig2 = active && A.bt.is_ligand[t2];
-    ElecAtom<Real> R2;
-    R2.x = R2.y = R2.z = R2.q = Real(0);
-    R2.pd = 0;
-    if (active) load_elec_rot_atom(R2, A, t2, o2, a2);
-    const bool charged2 = active && R2.q != Real(0);
-    Real g2x = 0, g2y = 0, g2z = 0;
-    for (int i = 0; i < r1_count; ++i) {
-      const int r1 = r1_first + i;
-      const int t1 = uni(v.block_type_for_rot[r1]);
-      if (t1 < 0) {
-        if (!BWD) elec_rot_zero(A, entry + int64_t(i) * nr2 + ps, kp, lane);
-        continue;
-      }
-      if (!BWD && lane < kp && nat == 0) A.energies[entry + int64_t(i) * nr2 + ps + lane] = Real(0);
-      const int o1 = uni(v.rot_coord_offset[r1]);
-      const int n1 = uni(A.bt.n_atoms[t1]);
-      const int nc1 = uni(A.bt.n_conn[t1]);
-      const bool lig = lig2 && A.bt.is_ligand[t1];
-      const int64_t e = entry + int64_t(i) * nr2 + ps + slot_c;
-      const Real w = (BWD && active) ? A.dTdV[e] : Real(0);
-      Real ee = 0;
-      // cull: a lane's atom farther than r1's radius + max_dis from r1's
-      // centre reaches no atom of r1 (1e-3 A margin for rounding)
-      const V sph = *(const V*)(A.rot_spheres + 4 * int64_t(r1));
-      const Real cx = R2.x - sph.x, cy = R2.y - sph.y, cz = R2.z - sph.z;
-      const Real reach = sph.w + k.hpe + Real(1e-3);
-      const bool near = charged2 && cx * cx + cy * cy + cz * cz < reach * reach;
-      const int n1n = __ballot(near) != 0 ? n1 : 0;  // wave-uniform
-      if (n1n > 0) stage_elec_rot(L, A, t1, o1, n1, lane);
-      for (int j = 0; j < n1n; ++j) {
-        const V p1 = *(const V*)L.pos[j];
-        if (p1.w == Real(0)) continue;  // uncharged b1 atom: every pair is 0
-        const Real dx = p1.x - R2.x, dy = p1.y - R2.y, dz = p1.z - R2.z;
-        const Real d2 = dx * dx + dy * dy + dz * dz;
-        bool in = near && d2 < k.cull2;
-        if (__ballot(in) == 0) continue;
-        Real wcp = Real(1);
-        if (cp) {
-          const int sep = packed_pair_sep(L.pdw[j], R2.pd, L.cs, min_sep, nc1, nc2, lig);
-          wcp = sep > 4 ? Real(1) : (sep == 4 ? Real(0.2) : Real(0));
-          in = in && sep >= 4;
-        }
-        Real gs = Real(0);
-        if (in) {
-          const Real E = elec_rot_pair<Real, BWD>(d2, p1.w * R2.q, wcp, w, k, gs);
-          if (!BWD) ee += E;
-        }
-        if (BWD) {
-          g2x -= gs * dx;
-          g2y -= gs * dy;
-          g2z -= gs * dz;
-          if (__ballot(gs != Real(0)) != 0)
-            elec_rot_flush_a1(A.grad, int64_t(o1) + j, gs * dx, gs * dy, gs * dz, lane);
-        }
-      }
-      if (!BWD) {
-        ee = segment_scan(ee, seg_start, lane);
-        if (active && last) A.energies[e] = ee;
-      }
-    }
-    if (BWD && active) {
-      const int64_t at = (int64_t(o2) + a2) * 3;
-      if (g2x != Real(0)) unsafeAtomicAdd(&A.grad[at + 0], g2x);
-      if (g2y != Real(0)) unsafeAtomicAdd(&A.grad[at + 1], g2y);
-      if (g2z != Real(0)) unsafeAtomicAdd(&A.grad[at + 2], g2z);
-    }
-    ps += kp;
-  }
 }
 
 // intra-block unit: each rotamer r of [r1_first, +r1_count) with itself,
@@ -316,10 +168,11 @@ __device__ void elec_rot_intra(const ElecRotArgs<Real>& A, ElecRotLDS<Real>& L, 
   }
 }
 
-// ------------------------------------------------------------------ energies, culled
+// ------------------------------------------------------------------ energies and gradients, culled
 
-// Energy forward with culling and compaction (the structure of the LJ+LK
-// rotamer energy kernel).  Per b1 rotamer r1 of a unit:
+// Culling and compaction (the structure of the LJ+LK rotamer energy kernel),
+// for the energies and for the dTdV-weighted backward.  Per b1 rotamer r1 of
+// a unit:
 //   cull      every atom of b2's rotamers (the candidates, 64 at a time in
 //             rotamer order, over windows of up to 64 b2 rotamers) is tested
 //             against r1's sphere; charged survivors are compacted, in order,
@@ -328,14 +181,19 @@ __device__ void elec_rot_intra(const ElecRotArgs<Real>& A, ElecRotLDS<Real>& L, 
 //             registers of the lanes that staged them (readlane), each lane
 //             tests its chunk atom; pairs within max_dis of non-zero
 //             count-pair weight are ballot-compacted into a FIFO
-//   phase B   64 FIFO entries at a time, one per lane, into the chunk lane's
-//             64-bit fixed-point LDS accumulator (exact, order-independent)
-//   fold      at the end of a chunk, a segmented scan adds the lanes' sums
-//             per rotamer into the window's per-rotamer sums
-//   write     after the window, one store per rotamer pair (zero for pairs
-//             with no surviving atoms)
+//   phase B   64 FIFO entries at a time, one per lane, into 64-bit
+//             fixed-point LDS accumulators (exact, order-independent): the
+//             chunk lane's energy, or (backward) the gradient of the b1 atom
+//             and of the chunk atom, the pair weighted by its dTdV
+//   fold      forward: at the end of a chunk, a segmented scan adds the
+//             lanes' sums per rotamer into the window's per-rotamer sums;
+//             backward: the chunk atoms' gradients are flushed (one atomic
+//             per non-zero component)
+//   write     forward: after the window, one store per rotamer pair (zero
+//             for pairs with no surviving atoms); backward: the b1 atoms'
+//             gradients are flushed
 
-template <typename Real>
+template <typename Real, bool BWD>
 struct ElecRotLDS2 {
   using T = typename LdsAcc<Real>::T;
   ElecRotLDS<Real> r1;  // staged atoms of the b1 rotamer, cell separations, marks
@@ -343,15 +201,20 @@ struct ElecRotLDS2 {
   int meta2[WAVE];       // n_conn | ligand << 3 | slot << 16
   uint32_t pd2[WAVE];    // packed representative path distances
   uint16_t fifo[2 * WAVE];  // j | chunk lane << 5 | (separation 4) << 11
-  T lacc[WAVE];          // per chunk lane
-  Real sum[WAVE];        // per rotamer of the window
+  T lacc[BWD ? 1 : WAVE];   // forward: energy per chunk lane
+  Real sum[BWD ? 1 : WAVE];  // forward: per rotamer of the window
+  T acc1[BWD ? 3 : 1][BWD ? TILE : 1];  // backward: gradient of the b1 rotamer's atoms
+  T acc2[BWD ? 3 : 1][BWD ? WAVE : 1];  // backward: gradient of the chunk's atoms
+  int idx2[BWD ? WAVE : 1];             // backward: chunk lane's atom
+  Real wt2[BWD ? WAVE : 1];             // backward: dTdV of the chunk lane's rotamer pair
 };
 
-template <typename Real>
-__device__ __forceinline__ void elec_chunk_eval(ElecRotLDS2<Real>& L, int consume,
+template <typename Real, bool BWD>
+__device__ __forceinline__ void elec_chunk_eval(ElecRotLDS2<Real, BWD>& L, int consume,
                                                 const ElecConst<Real>& k, int lane) {
   using V = typename V4<Real>::T;
   using Acc = LdsAcc<Real>;
+  using T = typename Acc::T;
   if (lane < consume) {
     const int en = L.fifo[lane];
     const int j = en & 31, l2 = (en >> 5) & 63;
@@ -359,18 +222,32 @@ __device__ __forceinline__ void elec_chunk_eval(ElecRotLDS2<Real>& L, int consum
     const V p1 = *(const V*)L.r1.pos[j], p2 = *(const V*)L.pos2[l2];
     const Real dx = p1.x - p2.x, dy = p1.y - p2.y, dz = p1.z - p2.z;
     Real gs;
-    const Real E =
-        elec_rot_pair<Real, false>(dx * dx + dy * dy + dz * dz, p1.w * p2.w, wcp, Real(0), k, gs);
-    atomicAdd(&L.lacc[l2], Acc::enc(E));
+    if constexpr (!BWD) {
+      const Real E = elec_rot_pair<Real, false>(dx * dx + dy * dy + dz * dz, p1.w * p2.w, wcp,
+                                                Real(0), k, gs);
+      atomicAdd(&L.lacc[l2], Acc::enc(E));
+    } else {
+      elec_rot_pair<Real, true>(dx * dx + dy * dy + dz * dz, p1.w * p2.w, wcp, L.wt2[l2], k, gs);
+      if (gs != Real(0)) {
+        const T ex = Acc::enc(gs * dx), ey = Acc::enc(gs * dy), ez = Acc::enc(gs * dz);
+        atomicAdd(&L.acc1[0][j], ex);
+        atomicAdd(&L.acc1[1][j], ey);
+        atomicAdd(&L.acc1[2][j], ez);
+        atomicAdd(&L.acc2[0][l2], T(0) - ex);
+        atomicAdd(&L.acc2[1][l2], T(0) - ey);
+        atomicAdd(&L.acc2[2][l2], T(0) - ez);
+      }
+    }
   }
 }
 
 // phase A + B over the chunk's nb lanes against r1 (n1 atoms held in the
-// registers R1 of lanes 0..n1-1), then the fold into the window's sums
-template <typename Real>
-__device__ void elec_chunk(ElecRotLDS2<Real>& L, const ElecConst<Real>& k, int nb, int n1,
+// registers R1 of lanes 0..n1-1), then (forward) the fold into the window's
+// sums or (backward) the flush of the chunk atoms' gradient
+template <typename Real, bool BWD>
+__device__ void elec_chunk(ElecRotLDS2<Real, BWD>& L, const ElecConst<Real>& k, int nb, int n1,
                            bool cp, int min_sep, int nc1, bool lig1, const ElecAtom<Real>& R1,
-                           int lane) {
+                           Real* grad, int lane) {
   using V = typename V4<Real>::T;
   using Acc = LdsAcc<Real>;
   wave_sync();
@@ -385,7 +262,10 @@ __device__ void elec_chunk(ElecRotLDS2<Real>& L, const ElecConst<Real>& k, int n
   }
   const int nc2 = meta & 7;
   const bool lig = lig1 && ((meta >> 3) & 1);
-  L.lacc[lane] = 0;
+  if constexpr (!BWD)
+    L.lacc[lane] = 0;
+  else
+    L.acc2[0][lane] = L.acc2[1][lane] = L.acc2[2][lane] = 0;
   int n_f = 0;
   for (int j = 0; j < n1; ++j) {
     if (lane_val(R1.q, j) == Real(0)) continue;  // uncharged b1 atom
@@ -413,6 +293,18 @@ __device__ void elec_chunk(ElecRotLDS2<Real>& L, const ElecConst<Real>& k, int n
   wave_sync();
   if (n_f > 0) elec_chunk_eval(L, n_f, k, lane);
   wave_sync();
+  if constexpr (BWD) {
+    if (valid) {
+      const int64_t at = int64_t(L.idx2[lane]) * 3;
+      const Real gx = Acc::dec(L.acc2[0][lane]), gy = Acc::dec(L.acc2[1][lane]),
+                 gz = Acc::dec(L.acc2[2][lane]);
+      if (gx != Real(0)) unsafeAtomicAdd(&grad[at + 0], gx);
+      if (gy != Real(0)) unsafeAtomicAdd(&grad[at + 1], gy);
+      if (gz != Real(0)) unsafeAtomicAdd(&grad[at + 2], gz);
+    }
+    wave_sync();
+    return;
+  }
   // fold: lanes of one rotamer (slot) are contiguous
   const int slot = meta >> 16;
   const int prev = lane > 0 ? L.meta2[lane - 1] >> 16 : -1;
@@ -421,18 +313,22 @@ __device__ void elec_chunk(ElecRotLDS2<Real>& L, const ElecConst<Real>& k, int n
   const int seg_start = below ? 63 - __builtin_clzll(below) : 0;
   const bool last =
       valid && (lane == nb - 1 || (L.meta2[lane + 1 < WAVE ? lane + 1 : lane] >> 16) != slot);
-  Real e = valid ? Acc::dec(L.lacc[lane]) : Real(0);
-  e = segment_scan(e, seg_start, lane);
-  if (last) L.sum[slot] += e;
+  if constexpr (!BWD) {
+    Real e = valid ? Acc::dec(L.lacc[lane]) : Real(0);
+    e = segment_scan(e, seg_start, lane);
+    if (last) L.sum[slot] += e;
+  }
   wave_sync();
 }
 
-// inter-block unit, energies only
-template <typename Real>
-__device__ void elec_rot_inter_energy(const ElecRotArgs<Real>& A, ElecRotLDS2<Real>& L, int g1,
-                                      int g2, int b2, int r1_first, int r1_count, int64_t entry,
-                                      int lane) {
+// inter-block unit, culled and compacted: energies (forward) or the
+// dTdV-weighted gradient (backward)
+template <typename Real, bool BWD>
+__device__ void elec_rot_inter_chunked(const ElecRotArgs<Real>& A, ElecRotLDS2<Real, BWD>& L,
+                                       int g1, int g2, int b2, int r1_first, int r1_count,
+                                       int64_t entry, int lane) {
   using V = typename V4<Real>::T;
+  using Acc = LdsAcc<Real>;
   const RotView& v = A.v;
   const ElecConst<Real>& k = A.k;
   const int nr2 = uni(v.n_rots_for_block[g2]), off2 = uni(v.rot_offset_for_block[g2]);
@@ -471,7 +367,7 @@ __device__ void elec_rot_inter_energy(const ElecRotArgs<Real>& A, ElecRotLDS2<Re
       const int r1 = r1_first + i;
       const int t1 = uni(v.block_type_for_rot[r1]);
       if (t1 < 0) {
-        elec_rot_zero(A, entry + int64_t(i) * nr2 + w0, wn, lane);
+        if (!BWD) elec_rot_zero(A, entry + int64_t(i) * nr2 + w0, wn, lane);
         continue;
       }
       const int o1 = uni(v.rot_coord_offset[r1]);
@@ -494,7 +390,10 @@ __device__ void elec_rot_inter_energy(const ElecRotArgs<Real>& A, ElecRotLDS2<Re
         p.w = R1.q;
         *(V*)L.r1.pos[lane] = p;
       }
-      L.sum[lane] = 0;
+      if constexpr (!BWD)
+        L.sum[lane] = 0;
+      else if (lane < TILE)
+        L.acc1[0][lane] = L.acc1[1][lane] = L.acc1[2][lane] = 0;
       int nb = 0;  // surviving candidates in the chunk (wave-uniform)
       for (int c0 = 0; c0 < total; c0 += WAVE) {
         // candidate c = c0 + lane: its rotamer (slot) and atom
@@ -525,7 +424,7 @@ __device__ void elec_rot_inter_energy(const ElecRotArgs<Real>& A, ElecRotLDS2<Re
         const int nk = __popcll(mk);
         if (nk == 0) continue;
         if (nb + nk > WAVE) {
-          elec_chunk(L, k, nb, n1, cp, min_sep, nc1, lig1, R1, lane);
+          elec_chunk(L, k, nb, n1, cp, min_sep, nc1, lig1, R1, A.grad, lane);
           nb = 0;
         }
         if (keep) {
@@ -538,28 +437,39 @@ __device__ void elec_rot_inter_energy(const ElecRotArgs<Real>& A, ElecRotLDS2<Re
           *(V*)L.pos2[pos] = p;
           L.meta2[pos] = A.bt.n_conn[t2] | ((A.bt.is_ligand[t2] ? 1 : 0) << 3) | (slot << 16);
           L.pd2[pos] = C2.pd;
+          if constexpr (BWD) {
+            L.idx2[pos] = o2 + a2;
+            L.wt2[pos] = A.dTdV[entry + int64_t(i) * nr2 + w0 + slot];
+          }
         }
         nb += nk;
       }
-      if (nb > 0) elec_chunk(L, k, nb, n1, cp, min_sep, nc1, lig1, R1, lane);
+      if (nb > 0) elec_chunk(L, k, nb, n1, cp, min_sep, nc1, lig1, R1, A.grad, lane);
       wave_sync();
-      if (lane < wn) A.energies[entry + int64_t(i) * nr2 + w0 + lane] = L.sum[lane];
+      if constexpr (!BWD) {
+        if (lane < wn) A.energies[entry + int64_t(i) * nr2 + w0 + lane] = L.sum[lane];
+      } else {
+        for (int q = lane; q < 3 * n1; q += WAVE) {
+          const int a = q / 3, c = q - 3 * a;
+          const Real gv = Acc::dec(L.acc1[c][a]);
+          if (gv != Real(0)) unsafeAtomicAdd(&A.grad[(int64_t(o1) + a) * 3 + c], gv);
+        }
+      }
     }
   }
 }
 
 // Persistent elec rotamer-pair kernel: 4 independent waves per workgroup,
 // units from the per-XCD queues.  BWD false: energies [n_pairs] (every entry
-// written, zero for pairs that are not scored), inter-block units culled and
-// compacted; BWD true: grad [NA, 3] += sum_i dTdV[i] dV(pair i)/dx.
-// Float builds are held to 7 (forward) and 8 (backward) waves per SIMD: the
-// kernel is latency-bound, and the few spilled registers cost less than the
-// waves gained (forward 5.76 -> 4.77 ms, backward 11.79 -> 10.45 ms on the
-// 16 x 300-residue repacking set).
+// written, zero for pairs that are not scored); BWD true: grad [NA, 3] +=
+// sum_i dTdV[i] dV(pair i)/dx.  Float builds are held to 7 waves per SIMD:
+// the kernel is latency-bound, and the few spilled registers cost less than
+// the waves gained (forward 5.76 -> 4.77 ms on the 16 x 300-residue
+// repacking set).
 template <typename Real, bool BWD>
-__global__ __launch_bounds__(256, sizeof(Real) == 4 ? (BWD ? 8 : 7) : 2) void elec_rotamer_kernel(
+__global__ __launch_bounds__(256, sizeof(Real) == 4 ? 7 : 2) void elec_rotamer_kernel(
     ElecRotArgs<Real> A) {
-  using LDS = typename std::conditional<BWD, ElecRotLDS<Real>, ElecRotLDS2<Real>>::type;
+  using LDS = ElecRotLDS2<Real, BWD>;
   __shared__ LDS lds[WAVES_PER_WG];
   const int lane = lane_id();
   LDS& L = lds[threadIdx.x / WAVE];
@@ -578,18 +488,11 @@ __global__ __launch_bounds__(256, sizeof(Real) == 4 ? (BWD ? 8 : 7) : 2) void el
                       lane);
       continue;
     }
-    if constexpr (BWD) {
-      if (b1 == b2)
-        elec_rot_intra<Real, true>(A, L, r1_first, r1_count, entry, lane);
-      else
-        elec_rot_inter<Real, true>(A, L, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry, lane);
-    } else {
-      if (b1 == b2)
-        elec_rot_intra<Real, false>(A, L.r1, r1_first, r1_count, entry, lane);
-      else
-        elec_rot_inter_energy<Real>(A, L, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry,
-                                    lane);
-    }
+    if (b1 == b2)
+      elec_rot_intra<Real, BWD>(A, L.r1, r1_first, r1_count, entry, lane);
+    else
+      elec_rot_inter_chunked<Real, BWD>(A, L, g1, p * A.v.B + b2, b2, r1_first, r1_count, entry,
+                                        lane);
   }
 }
 
