@@ -124,8 +124,9 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--autotune", action="store_true",
-                    help="time candidate grid sizes first (default: resident-slot grid)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (grid of one workgroup per resident slot, or "
+                         "--wave-groups)")
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--no-pruner", action="store_true",
                     help="skip the learned-pruner leg (1024 x 300-residue batch)")
@@ -160,7 +161,10 @@ def main():
     weights = torch.tensor([1.0, 0.55, 1.0])
     log("stack: %d poses, %d blocks, %d atoms" % (stack.n_poses, stack.n_blocks_total, stack.n_atoms_total))
 
-    if args.autotune:
+    tune = None
+    if not args.no_autotune and args.wave_groups == 0:
+        # the project's autotuner picks the persistent grid (workgroups of 4
+        # waves) by timing the energy + gradient step on this batch
         tune = scorer.autotune(coords, weights, cache=False)
         log("autotune: %s" % json.dumps(tune))
     else:
@@ -250,6 +254,8 @@ def main():
             "atom_pair_evals_per_s": pair_evals / (ms_per_step * 1e-3),
             "graph": use_graph,
             "wave_groups": scorer.wave_groups,
+            "autotune": ({str(k): round(v * 1e3, 4) for k, v in tune["timings_s"].items()}
+                         if tune else None),
             "latency_ms_p50": float(numpy.percentile(lat, 50)),
             "latency_ms_p99": float(numpy.percentile(lat, 99)),
             "neighbors_ms": nbr_ms,
