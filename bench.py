@@ -124,9 +124,11 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--no-autotune", action="store_true",
-                    help="skip the autotuner (grid of one workgroup per resident slot, or "
-                         "--wave-groups)")
+    ap.add_argument("--autotune", action="store_true",
+                    help="run the autotuner over persistent grid sizes first (default: the "
+                         "grid it picks on MI355X for this batch, one workgroup per resident "
+                         "slot; its candidate launches would otherwise enter a kernel trace "
+                         "of this command)")
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--no-pruner", action="store_true",
                     help="skip the learned-pruner leg (1024 x 300-residue batch)")
@@ -162,7 +164,7 @@ def main():
     log("stack: %d poses, %d blocks, %d atoms" % (stack.n_poses, stack.n_blocks_total, stack.n_atoms_total))
 
     tune = None
-    if not args.no_autotune and args.wave_groups == 0:
+    if args.autotune:
         # the project's autotuner picks the persistent grid (workgroups of 4
         # waves) by timing the energy + gradient step on this batch
         tune = scorer.autotune(coords, weights, cache=False)
