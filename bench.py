@@ -31,89 +31,6 @@ def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(poses, sample_stride, time_budget_s=25.0):
-    """Reference CPU op (single thread) on every ``sample_stride``-th pose,
-    each pose scored alone (forward + backward), until the budget is used."""
-    import torch
-
-    from oracle import oracle as orc
-    from tmol_amd.pose import PoseStack
-    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
-
-    torch.set_num_threads(1)
-    term = LJLKEnergyTerm()
-    kind = "reference" if orc.reference_op_available() else "port"
-    ref = orc.load_reference_op() if kind == "reference" else None
-    sample = poses[::sample_stride]
-    t_total = 0.0
-    n_done = 0
-    n_res = 0
-    for p in sample:
-        ps = PoseStack([p])
-        args = pose_op_args(ps, term)
-        if kind == "reference":
-            c = args[0].clone().requires_grad_(True)
-            args[0] = c
-            t0 = time.perf_counter()
-            s, _ = ref(*args)
-            (s * torch.tensor([[1.0], [0.55], [1.0]])).sum().backward()
-            t_total += time.perf_counter() - t0
-        else:
-            t0 = time.perf_counter()
-            orc.oracle_pose_scores(*args, require_grad=True)
-            t_total += time.perf_counter() - t0
-        n_done += 1
-        n_res += p.n_blocks
-        if t_total > time_budget_s:
-            break
-    return {
-        "value": n_done / t_total,
-        "unit": "poses/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": "%d poses (every %dth of the batch, %d residues), each scored alone, "
-                  "forward+backward, torch threads = 1, %.1f s" % (n_done, sample_stride, n_res, t_total),
-    }
-
-
-def cpu_baseline_all_cores(poses, sample_stride, time_budget_s=15.0):
-    """The C restatement (oracle) on every host core at once: one pose per
-    task, tasks spread over a thread pool (the C code runs without the GIL)."""
-    import os
-    from concurrent.futures import ThreadPoolExecutor
-
-    import numpy
-    import torch
-
-    from oracle import oracle as orc
-    from tmol_amd.pose import PoseStack
-    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
-
-    cores = os.cpu_count() or 1
-    term = LJLKEnergyTerm()
-    sample = poses[::sample_stride]
-    prepared = []
-    for p in sample:
-        a = pose_op_args(PoseStack([p]), term)
-        prepared.append(a)
-
-    def one(a):
-        orc.oracle_pose_scores(*a, require_grad=True, dtype=numpy.float32)
-        return 1
-
-    done = 0
-    t0 = time.perf_counter()
-    with ThreadPoolExecutor(max_workers=cores) as ex:
-        for k in range(0, len(prepared), cores):
-            done += sum(ex.map(one, prepared[k:k + cores]))
-            if time.perf_counter() - t0 > time_budget_s:
-                break
-    dt = time.perf_counter() - t0
-    return {"value": done / dt, "unit": "poses/s", "cores": cores, "kind": "port",
-            "sample": "%d poses (every %dth), f32 C restatement, forward+backward, %d threads, %.1f s"
-                      % (done, sample_stride, cores, dt)}
-
-
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -146,6 +63,7 @@ def main():
     import numpy
     import torch
 
+    import bench_cpu
     from tmol_amd.flops import algorithmic_bytes, algorithmic_flops, pair_evaluations
     from tmol_amd.generator import workload_poses
     from tmol_amd.pose import RaggedPoseStack
@@ -291,22 +209,26 @@ def main():
         from tmol_amd import rotamer_bench
 
         log("rotamer pairs ...")
-        result["rotamer"] = rotamer_bench.run(dev, steps=10, warmup=2, cpu=not args.no_cpu, log=log)
+        result["rotamer"] = rotamer_bench.run(
+            dev, steps=10, warmup=2, cpu=None if args.no_cpu else bench_cpu.ljlk_rotamers,
+            elec_cpu=None if args.no_cpu else bench_cpu.elec_rotamers, log=log)
         result["rotamer"].pop("census", None)
     if not args.no_elec:
         from tmol_amd import elec_bench
 
         log("elec ...")
-        result["elec"] = elec_bench.run(dev, poses=poses, cpu=not args.no_cpu, log=log)
+        result["elec"] = elec_bench.run(dev, poses=poses,
+                                        cpu=None if args.no_cpu else bench_cpu.elec_poses, log=log)
         result["elec"].pop("census", None)
     from tmol_amd import topology_bench
 
     log("topology ...")
-    result["topology"] = topology_bench.run(dev, cpu=not args.no_cpu, log=log)
+    result["topology"] = topology_bench.run(dev, cpu=None if args.no_cpu else bench_cpu.topology,
+                                            log=log)
     if not args.no_cpu:
         log("cpu baseline ...")
-        result["cpu_baseline"] = cpu_baseline(poses, args.cpu_stride)
-        result["cpu_baseline"]["all_cores"] = cpu_baseline_all_cores(poses, args.cpu_stride)
+        result["cpu_baseline"] = bench_cpu.ljlk_poses(args.workload, poses, args.cpu_stride)
+        result["cpu_baseline"]["all_cores"] = bench_cpu.ljlk_poses_all_cores(poses, args.cpu_stride)
     print(json.dumps(result), flush=True)
 
 

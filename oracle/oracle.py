@@ -327,13 +327,6 @@ def load_reference_apsp_op():
     return torch.ops.tmol_apsp.apsp_op
 
 
-def load_reference_rotamer_op():
-    """The reference's own CPU ``tmol_ljlk::ljlk_rotamer_scores`` built by
-    build_ref.sh."""
-    load_reference_op()
-    return torch.ops.tmol_ljlk.ljlk_rotamer_scores
-
-
 def oracle_pair_separations(args, items):
     """Count-pair separations for every atom pair of each (p, b1, b2) item.
     Returns a list of int arrays [n1, n2] (-1 where not real)."""
@@ -550,6 +543,8 @@ def _elec_rot_dispatch(E, L, frt, nrb, rob):
 
 
 def reference_op_available():
+    """True when oracle/_ref holds the reference's own LJ+LK op.  It runs in a
+    child process (oracle/refop.py): tmol_amd registers ops of the same names."""
     return os.path.exists(REF_LIB_PATH)
 
 
@@ -562,16 +557,6 @@ def load_reference_elec_op(name="elec_pose_scores"):
     if not hasattr(torch.ops, "tmol_elec") or not hasattr(torch.ops.tmol_elec, "elec_pose_scores"):
         torch.ops.load_library(path)
     return getattr(torch.ops.tmol_elec, name)
-
-
-def load_reference_op():
-    """Loads the reference's own CPU op built by build_ref.sh; returns
-    ``torch.ops.tmol_ljlk.ljlk_pose_scores``."""
-    if not reference_op_available():
-        raise FileNotFoundError(REF_LIB_PATH)
-    if not hasattr(torch.ops, "tmol_ljlk") or not hasattr(torch.ops.tmol_ljlk, "ljlk_pose_scores"):
-        torch.ops.load_library(REF_LIB_PATH)
-    return torch.ops.tmol_ljlk.ljlk_pose_scores
 
 
 def pose_oracle_evaluator(poses, weights=(1.0, 0.55, 1.0), threads=8, dtype=numpy.float64):

@@ -1,6 +1,7 @@
 """Golden vectors for rotamer-pair scoring from the reference's own CPU op,
 tmol_ljlk::ljlk_rotamer_scores, compiled from the reference sources by
-oracle/build_ref.sh (oracle/_ref/tmol_ljlk_ref.so).
+oracle/build_ref.sh (oracle/_ref/tmol_ljlk_ref.so), run in a child process
+(oracle/refop.py).
 
 Inputs: the 100-residue workload pose (generator) and its repacking rotamers
 (tmol_amd.rotamers, seed 1).  Writes rotamer_ref_single100.json with, for
@@ -23,7 +24,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ROOT = os.path.dirname(os.path.dirname(HERE))
 sys.path.insert(0, ROOT)
 
-from oracle import oracle as orc  # noqa: E402
+from oracle import refop  # noqa: E402
 from tmol_amd.rotamers import dispatch_checksum, repack_workload, rotamer_op_args  # noqa: E402
 from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm  # noqa: E402
 
@@ -37,19 +38,17 @@ def dTdV_for(shape, dtype):
 
 
 def main():
-    ref = orc.load_reference_rotamer_op()
     out = {"workload": "cpu_single_100", "rotamer_seed": 1}
     for name, dt in (("float32", torch.float32), ("float64", torch.float64)):
         ps, rs = repack_workload("cpu_single_100", seed=1, dtype=dt)
         args = rotamer_op_args(ps, rs, LJLKEnergyTerm())
         if dt == torch.float64:
             args[23], args[24] = args[23].double(), args[24].double()
-        c = args[0].clone().requires_grad_(True)
-        args[0] = c
-        s, idx = ref(*args)
+        s, idx, _ = refop.rotamer_scores(args)
         n = idx.shape[1]
         w = dTdV_for(s.shape, dt)
-        (s * w).sum().backward()
+        _, _, (grad,) = refop.rotamer_scores(args, [w])
+        c = args[0]
         rng = numpy.random.default_rng(7)
         pick = numpy.sort(rng.choice(n, N_PAIR_SAMPLES, replace=False))
         atoms = numpy.sort(rng.choice(c.shape[0], N_ATOM_SAMPLES, replace=False))
@@ -65,7 +64,7 @@ def main():
             "sample_indices": idx.numpy()[:, pick].tolist(),
             "sample_energies": sd[:, pick].tolist(),
             "sample_atoms": atoms.tolist(),
-            "sample_grad": c.grad.double().numpy()[atoms].tolist(),
+            "sample_grad": grad.double().numpy()[atoms].tolist(),
         }
     with open(os.path.join(HERE, "rotamer_ref_single100.json"), "w") as f:
         json.dump(out, f)

@@ -90,12 +90,61 @@ def test_cpu_tensor_is_rejected(ubq_pdb):
         ljlk_pose_scores(*pose_op_args(ps, LJLKEnergyTerm()))
 
 
-def test_op_schema_matches_reference_arity():
+# The signature of ljlk_pose_scores_op and ljlk_rotamer_scores_op,
+# tmol/score/ljlk/potentials/compiled.ops.cpp:533-561 and 597-625, written out
+# here as (type, name) pairs in their order; TORCH_LIBRARY(tmol_ljlk, m)
+# (compiled.ops.cpp:667-670) returns Tensor[].
+REFERENCE_SIGNATURE = [
+    ("Tensor", "rot_coords"), ("Tensor", "rot_coord_offset"), ("Tensor", "pose_ind_for_atom"),
+    ("Tensor", "first_rot_for_block"), ("Tensor", "first_rot_block_type"),
+    ("Tensor", "block_ind_for_rot"), ("Tensor", "pose_ind_for_rot"),
+    ("Tensor", "block_type_ind_for_rot"), ("Tensor", "n_rots_for_pose"),
+    ("Tensor", "rot_offset_for_pose"), ("Tensor", "n_rots_for_block"),
+    ("Tensor", "rot_offset_for_block"), ("int", "max_n_rots_per_pose"),
+    ("Tensor", "pose_stack_min_bond_separation"), ("Tensor", "pose_stack_inter_block_bondsep"),
+    ("Tensor", "block_type_n_atoms"), ("Tensor", "block_type_n_heavy_atoms_in_tile"),
+    ("Tensor", "block_type_heavy_atoms_in_tile"), ("Tensor", "block_type_atom_types"),
+    ("Tensor", "block_type_n_interblock_bonds"),
+    ("Tensor", "block_type_atoms_forming_chemical_bonds"),
+    ("Tensor", "block_type_path_distance"), ("Tensor", "block_type_is_ligand_fragment"),
+    ("Tensor", "ljlk_type_params"), ("Tensor", "global_params"), ("float", "max_dis"),
+    ("bool", "output_block_pair_energies"),
+]
+
+
+def _signature(schema):
+    return [(str(a.type), a.name) for a in schema.arguments], [str(r.type) for r in schema.returns]
+
+
+@pytest.mark.parametrize("name", ["ljlk_pose_scores", "ljlk_rotamer_scores"])
+def test_tmol_ljlk_op_schema_matches_reference(name):
+    """torch.ops.tmol_ljlk.<name> exists with the reference's argument names,
+    types and order and a Tensor[] return."""
     import tmol_amd.ops  # noqa: F401
 
-    schema = str(torch.ops.tmol_amd.ljlk_pose_forward.default._schema)
-    # 27 reference arguments + require_gradient
-    assert schema.count(",") + 1 == 28
+    schema = getattr(torch.ops.tmol_ljlk, name).default._schema
+    args, rets = _signature(schema)
+    assert schema.name == "tmol_ljlk::" + name
+    assert args == REFERENCE_SIGNATURE
+    assert rets == ["List[Tensor]"]
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(ROOT, "oracle", "_ref", "tmol_ljlk_ref.so")),
+                    reason="oracle/_ref not built (needs the reference tree)")
+def test_tmol_ljlk_op_types_match_reference_build():
+    """The argument and return types equal those of the reference's own
+    registration, read from the library built from its sources (the
+    reference's schema is inferred from the C++ function, so its argument
+    names are positional)."""
+    import tmol_amd.ops  # noqa: F401
+    from oracle import refop
+
+    ref = refop.schemas()
+    for name in ("ljlk_pose_scores", "ljlk_rotamer_scores"):
+        ours = str(getattr(torch.ops.tmol_ljlk, name).default._schema)
+        strip = lambda sch: [a.strip().rsplit(" ", 1)[0] for a in sch[sch.index("(") + 1: sch.index(")")].split(",")]  # noqa: E731
+        assert strip(ours) == strip(ref[name])
+        assert ours.split("->")[1].split()[0] == ref[name].split("->")[1].split()[0] == "Tensor[]"
 
 
 def test_rotamer_and_topology_validation():

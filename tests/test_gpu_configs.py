@@ -39,7 +39,9 @@ def _args(poses, dtype, device):
 
 @pytest.mark.parametrize("workload", ["padded_64x150_600", "cpu_single_100"])
 def test_workload_parity(workload):
-    from tmol_amd.ops import debug_pair_separations, ljlk_pose_scores, padded_work_list
+    from tmol_amd.ops import debug_pair_separations, padded_work_list
+
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     poses = workload_poses(workload)
     _say("%s: %d poses" % (workload, len(poses)))

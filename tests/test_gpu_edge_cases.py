@@ -66,7 +66,8 @@ def _to(a, dev):
 
 
 def _check_ljlk(a, dt, what):
-    from tmol_amd.ops import ljlk_pose_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     _say("ljlk %s %s" % (what, dt))
     o, onbr, odv, _ = orc.oracle_pose_scores(*a, require_grad=True, dtype=numpy.float64)

@@ -57,7 +57,8 @@ def _cpu_args(poses, dtype=torch.float32, block_pair=False):
 
 
 def test_goldens_on_gpu(ubq_pdb):
-    from tmol_amd.ops import ljlk_pose_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     _say("goldens whole_pose_10")
     _, args = _gpu_args([pose_from_pdb(ubq_pdb)] * 10)
@@ -122,7 +123,8 @@ def test_count_pair_bit_exact(ubq_pdb):
 
 
 def test_energies_vs_fp64_and_oracle(ubq_pdb):
-    from tmol_amd.ops import ljlk_pose_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     poses = _stacks(ubq_pdb)
     _say("energies f32/f64")
@@ -144,7 +146,8 @@ def test_energies_vs_fp64_and_oracle(ubq_pdb):
 
 
 def test_whole_pose_gradients(ubq_pdb):
-    from tmol_amd.ops import ljlk_pose_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     poses = _stacks(ubq_pdb)
     _say("whole-pose gradients")
@@ -169,7 +172,8 @@ def test_whole_pose_gradients(ubq_pdb):
 
 
 def test_block_pair_backward(ubq_pdb):
-    from tmol_amd.ops import ljlk_pose_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     poses = [pose_from_pdb(ubq_pdb, (0, 12)), pose_from_pdb(ubq_pdb, (40, 48))]
     _say("block pair backward gradcheck")
@@ -190,7 +194,8 @@ def test_block_pair_backward(ubq_pdb):
 
 
 def test_whole_pose_gradcheck(ubq_pdb):
-    from tmol_amd.ops import ljlk_pose_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     _say("whole pose gradcheck")
     _, a = _gpu_args([pose_from_pdb(ubq_pdb, (0, 4))], torch.float64)
@@ -207,7 +212,9 @@ def test_whole_pose_gradcheck(ubq_pdb):
 
 
 def test_cpu_tensor_raises(ubq_pdb):
-    from tmol_amd.ops import LJLKLibraryError, ljlk_pose_scores
+    from tmol_amd.ops import LJLKLibraryError
+
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
 
     _, cargs = _cpu_args([pose_from_pdb(ubq_pdb, (0, 4))])
     with pytest.raises(LJLKLibraryError):

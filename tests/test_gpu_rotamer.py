@@ -101,7 +101,8 @@ def test_energies_match_oracle(small, dt):
     _say("rotamer energies %s" % dt)
     ps64, rs64 = small[torch.float64]
     E64, idx, *_ = orc.oracle_rotamer_scores(*_args(ps64, rs64, torch.float64))
-    from tmol_amd.ops import ljlk_rotamer_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_rotamer_scores = torch.ops.tmol_ljlk.ljlk_rotamer_scores
 
     s, gidx = ljlk_rotamer_scores(*_args(ps, rs, dt, DEV))
     numpy.testing.assert_array_equal(gidx.cpu().numpy(), idx)
@@ -116,7 +117,8 @@ def test_energies_match_oracle(small, dt):
 def test_backward_matches_oracle(small, dt):
     ps, rs = small[dt]
     _say("rotamer backward %s" % dt)
-    from tmol_amd.ops import ljlk_rotamer_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_rotamer_scores = torch.ops.tmol_ljlk.ljlk_rotamer_scores
 
     a = _args(ps, rs, dt, DEV)
     c = a[0].clone().requires_grad_(True)
@@ -139,7 +141,8 @@ def test_reference_golden(name, dt):
     g = json.load(open(os.path.join(HERE, "golden", "rotamer_ref_single100.json")))[name]
     ps, rs = repack_workload("cpu_single_100", seed=1, dtype=dt)
     _say("rotamer golden %s" % name)
-    from tmol_amd.ops import ljlk_rotamer_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_rotamer_scores = torch.ops.tmol_ljlk.ljlk_rotamer_scores
 
     a = _args(ps, rs, dt, DEV)
     c = a[0].clone().requires_grad_(True)
@@ -167,7 +170,9 @@ def test_reference_golden(name, dt):
 def test_single_rotamer_set_equals_block_pair_op(dt):
     """One rotamer per block: the rotamer op reproduces the block-pair op."""
     from tmol_amd.generator import workload_poses
-    from tmol_amd.ops import ljlk_pose_scores, ljlk_rotamer_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_pose_scores = torch.ops.tmol_ljlk.ljlk_pose_scores
+    ljlk_rotamer_scores = torch.ops.tmol_ljlk.ljlk_rotamer_scores
     from tmol_amd.pose import PoseStack
 
     _say("rotamer vs block pair %s" % dt)
@@ -203,7 +208,8 @@ def test_blocks_without_rotamers(small, dt):
     args = _args(ps, sub, dt)
     o, idx, *_ = orc.oracle_rotamer_dispatch(*args)
     E = o.score(idx)
-    from tmol_amd.ops import ljlk_rotamer_scores
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    ljlk_rotamer_scores = torch.ops.tmol_ljlk.ljlk_rotamer_scores
 
     s, gidx = ljlk_rotamer_scores(*_args(ps, sub, dt, DEV))
     numpy.testing.assert_array_equal(gidx.cpu().numpy(), idx)
@@ -219,7 +225,8 @@ def test_repack_workload_16x300():
     out = {}
     for dt in (torch.float64, torch.float32):
         ps, rs = repack_workload("repack_16x300", seed=1, dtype=dt)
-        from tmol_amd.ops import ljlk_rotamer_scores
+        import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+        ljlk_rotamer_scores = torch.ops.tmol_ljlk.ljlk_rotamer_scores
 
         s, idx = ljlk_rotamer_scores(*_args(ps, rs, dt, DEV))
         out[dt] = (ps, rs, s.double().cpu().numpy(), idx.cpu().numpy())

@@ -5,12 +5,13 @@ One step = elec's neighbor pass + the elec row kernel in weighted-gradient
 mode (per-pose energies and sum_p w[p] dE_p/dx), inputs resident in HBM.
 Reported: poses/s of the step, the kernel alone (HIP events on its stream),
 its roofline from the atom-pair census x elec FLOP table
-(tmol_amd/flops.py), and the reference's own CPU elec op (oracle/_ref, one
-thread, forward + backward) on a bounded sample of the poses.  The ``fused``
+(tmol_amd/flops.py), and, when ``cpu`` is given (bench.py passes the
+reference's own CPU elec op, one thread, forward + backward), a CPU baseline
+on a bounded sample of the poses.  The ``fused``
 entry times LJ+LK and elec scored in one pass against the two passes run
 one after the other.
 
-    python -m tmol_amd.elec_bench [--steps 20] [--no-cpu]
+    python -m tmol_amd.elec_bench [--steps 20]      (GPU legs only)
 """
 
 import argparse
@@ -22,53 +23,7 @@ import time
 FP32_PEAK_TFLOPS = 157.3
 
 
-def cpu_baseline(poses, stride=8, time_budget_s=12.0):
-    """The reference CPU elec op (one thread) on every ``stride``-th pose,
-    each scored alone, forward + backward; the C oracle if oracle/_ref is
-    absent."""
-    import numpy
-    import torch
-
-    from oracle import oracle as orc
-
-    from .pose import PoseStack
-    from .score.elec_energy_term import ElecEnergyTerm, elec_op_args
-
-    torch.set_num_threads(1)
-    term = ElecEnergyTerm()
-    try:
-        ref = orc.load_reference_elec_op()
-        kind = "reference"
-    except FileNotFoundError:
-        ref, kind = None, "port"
-    t_total, n_done, n_res = 0.0, 0, 0
-    for p in poses[::stride]:
-        args = elec_op_args(PoseStack([p]), term)
-        if ref is not None:
-            c = args[0].clone().requires_grad_(True)
-            args[0] = c
-            t0 = time.perf_counter()
-            s, _ = ref(*args)
-            s.sum().backward()
-        else:
-            t0 = time.perf_counter()
-            orc.oracle_elec_scores(*args, require_grad=True, dtype=numpy.float32)
-        t_total += time.perf_counter() - t0
-        n_done += 1
-        n_res += p.n_blocks
-        if t_total > time_budget_s:
-            break
-    return {
-        "value": n_done / t_total,
-        "unit": "poses/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": "%d poses (every %dth of the batch, %d residues), each scored alone, "
-                  "forward+backward, torch threads = 1, %.1f s" % (n_done, stride, n_res, t_total),
-    }
-
-
-def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=True, log=print):
+def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=None, log=print):
     import numpy
     import torch
 
@@ -155,9 +110,9 @@ def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=True, log=pr
         "census": census,
     }
     res["fused"] = fused_leg(stack, coords, device, steps, warmup, kernel_reps, log)
-    if cpu:
+    if cpu is not None:
         log("elec cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline(poses)
+        res["cpu_baseline"] = cpu(poses)
     return res
 
 
@@ -241,11 +196,11 @@ def fused_leg(stack, coords, device, steps=20, warmup=3, kernel_reps=10, log=pri
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="accepted; the CPU baseline runs from bench.py")
     args = ap.parse_args()
     import torch
 
-    res = run(torch.device("cuda:0"), steps=args.steps, cpu=not args.no_cpu,
+    res = run(torch.device("cuda:0"), steps=args.steps, cpu=None,
               log=lambda m: print("[elec] " + m, file=sys.stderr, flush=True))
     print(json.dumps(res), flush=True)
 

@@ -486,50 +486,96 @@ def _(grad_score, rot_coords, *rest):
 
 
 # ---------------------------------------------------------------------------
-# autograd
+# autograd of the primitive forward (LJLKPoseScoreOp, compiled.ops.cpp:29-279)
 
 
-class LJLKPoseScoreFunction(torch.autograd.Function):
-    """Forward/backward through the registered ops (LJLKPoseScoreOp,
-    compiled.ops.cpp:29-279)."""
+def _pose_forward_setup(ctx, inputs, output):
+    block_pair = bool(inputs[-2])
+    ctx.block_pair = block_pair
+    ctx.n_inputs = len(inputs)
+    if block_pair:
+        # the block-pair backward re-walks the neighbor list (impl.hh:1160-1477)
+        ctx.args = inputs[1:-2]
+        ctx.save_for_backward(inputs[0])
+    else:
+        # whole pose: dV/dx [3, NA, 3] and pose_ind_for_atom (compiled.ops.cpp:109-115)
+        ctx.save_for_backward(output[2], inputs[2])
 
-    @staticmethod
-    def forward(ctx, *args):
-        rot_coords = args[0]
-        block_pair = bool(args[-1])
-        require = rot_coords.requires_grad
-        score, nbr, dvdx = torch.ops.tmol_amd.ljlk_pose_forward(*args, require and not block_pair)
-        ctx.block_pair = block_pair
-        ctx.n_args = len(args)
-        if block_pair:
-            ctx.args = args[1:-1]
-            ctx.save_for_backward(rot_coords)
-        else:
-            ctx.save_for_backward(dvdx, args[2])
-        ctx.mark_non_differentiable(nbr)
-        return score, nbr
 
-    @staticmethod
-    def backward(ctx, grad_score, grad_nbr):
-        if grad_score is None:
-            return (None,) * ctx.n_args
-        if not ctx.block_pair:
-            dvdx, atom_pose = ctx.saved_tensors
-            grad = torch.ops.tmol_amd.ljlk_backward_combine(dvdx, grad_score.contiguous(), atom_pose)
-        else:
-            (coords,) = ctx.saved_tensors
-            grad = torch.ops.tmol_amd.ljlk_block_pair_backward(
-                grad_score.contiguous(), coords, *ctx.args)
-        return (grad,) + (None,) * (ctx.n_args - 1)
+def _pose_forward_backward(ctx, grads):
+    grad_score = grads[0]
+    none = (None,) * ctx.n_inputs
+    if grad_score is None:
+        return none
+    if not ctx.block_pair:
+        dvdx, atom_pose = ctx.saved_tensors
+        if dvdx.numel() == 0:
+            return none
+        grad = torch.ops.tmol_amd.ljlk_backward_combine(dvdx, grad_score.contiguous(), atom_pose)
+    else:
+        (coords,) = ctx.saved_tensors
+        grad = torch.ops.tmol_amd.ljlk_block_pair_backward(grad_score.contiguous(), coords, *ctx.args)
+    return (grad,) + none[1:]
+
+
+torch.library.register_autograd("tmol_amd::ljlk_pose_forward", _pose_forward_backward,
+                                setup_context=_pose_forward_setup)
+
+
+# ---------------------------------------------------------------------------
+# The reference's op names: torch.ops.tmol_ljlk.ljlk_pose_scores and
+# ljlk_rotamer_scores (TORCH_LIBRARY(tmol_ljlk, m), compiled.ops.cpp:667-670),
+# with the argument names and types of ljlk_pose_scores_op /
+# ljlk_rotamer_scores_op (compiled.ops.cpp:533-561, 597-625) and the Tensor[]
+# return.  Like the reference's registration (a catch-all kernel around an
+# autograd function), each is a composite kernel over this module's custom
+# ops, whose autograd is registered with torch.library.register_autograd.
+# A process holds one library per op name: this one replaces the reference's.
+
+TMOL_LJLK_ARGS = (
+    "Tensor rot_coords, Tensor rot_coord_offset, Tensor pose_ind_for_atom, "
+    "Tensor first_rot_for_block, Tensor first_rot_block_type, Tensor block_ind_for_rot, "
+    "Tensor pose_ind_for_rot, Tensor block_type_ind_for_rot, Tensor n_rots_for_pose, "
+    "Tensor rot_offset_for_pose, Tensor n_rots_for_block, Tensor rot_offset_for_block, "
+    "int max_n_rots_per_pose, Tensor pose_stack_min_bond_separation, "
+    "Tensor pose_stack_inter_block_bondsep, Tensor block_type_n_atoms, "
+    "Tensor block_type_n_heavy_atoms_in_tile, Tensor block_type_heavy_atoms_in_tile, "
+    "Tensor block_type_atom_types, Tensor block_type_n_interblock_bonds, "
+    "Tensor block_type_atoms_forming_chemical_bonds, Tensor block_type_path_distance, "
+    "Tensor block_type_is_ligand_fragment, Tensor ljlk_type_params, Tensor global_params, "
+    "float max_dis, bool output_block_pair_energies"
+)
+TMOL_LJLK_SCHEMAS = {
+    "ljlk_pose_scores": "ljlk_pose_scores(%s) -> Tensor[]" % TMOL_LJLK_ARGS,
+    "ljlk_rotamer_scores": "ljlk_rotamer_scores(%s) -> Tensor[]" % TMOL_LJLK_ARGS,
+}
+
+
+def _tmol_ljlk_pose_scores(*args):
+    rot_coords, block_pair = args[0], bool(args[-1])
+    require = rot_coords.requires_grad and not block_pair
+    score, nbr, _ = torch.ops.tmol_amd.ljlk_pose_forward(*args, require)
+    return [score, nbr]
+
+
+def _tmol_ljlk_rotamer_scores(*args):
+    score, idx = torch.ops.tmol_amd.ljlk_rotamer_forward(*args)
+    return [score, idx]
+
+
+_TMOL_LJLK_LIB = torch.library.Library("tmol_ljlk", "DEF")
+for _name, _schema in TMOL_LJLK_SCHEMAS.items():
+    _TMOL_LJLK_LIB.define(_schema)
+_TMOL_LJLK_LIB.impl("ljlk_pose_scores", _tmol_ljlk_pose_scores, "CompositeImplicitAutograd")
+_TMOL_LJLK_LIB.impl("ljlk_rotamer_scores", _tmol_ljlk_rotamer_scores, "CompositeImplicitAutograd")
 
 
 def ljlk_pose_scores(*args):
-    """Drop-in for ``torch.ops.tmol_ljlk.ljlk_pose_scores`` (27 arguments);
-    returns ``[score, block_neighbors]``."""
+    """``torch.ops.tmol_ljlk.ljlk_pose_scores`` (27 arguments); returns
+    ``[score, block_neighbors]``."""
     if len(args) != 27:
         raise TypeError("ljlk_pose_scores takes 27 arguments (%d given)" % len(args))
-    score, nbr = LJLKPoseScoreFunction.apply(*args)
-    return [score, nbr]
+    return torch.ops.tmol_ljlk.ljlk_pose_scores(*args)
 
 
 def debug_pair_separations(args, items: torch.Tensor) -> torch.Tensor:
@@ -763,39 +809,116 @@ class RotamerCall:
         return rs, bs
 
 
-class LJLKRotamerScoreFunction(torch.autograd.Function):
-    """Forward/backward of the rotamer-pair op (LJLKRotamerScoreOp,
-    compiled.ops.cpp:282-530): the backward weights every rotamer pair's
-    atom-pair derivatives by the incoming [3, n] gradient
-    (LJLKRotamerScoreDispatch::backward, impl.hh:1868-2126)."""
+@torch.library.custom_op("tmol_amd::ljlk_rotamer_forward", mutates_args=())
+def ljlk_rotamer_forward(
+    rot_coords: torch.Tensor, rot_coord_offset: torch.Tensor, pose_ind_for_atom: torch.Tensor,
+    first_rot_for_block: torch.Tensor, first_rot_block_type: torch.Tensor,
+    block_ind_for_rot: torch.Tensor, pose_ind_for_rot: torch.Tensor,
+    block_type_ind_for_rot: torch.Tensor, n_rots_for_pose: torch.Tensor,
+    rot_offset_for_pose: torch.Tensor, n_rots_for_block: torch.Tensor,
+    rot_offset_for_block: torch.Tensor, max_n_rots_per_pose: int,
+    pose_stack_min_bond_separation: torch.Tensor, pose_stack_inter_block_bondsep: torch.Tensor,
+    block_type_n_atoms: torch.Tensor, block_type_n_heavy_atoms_in_tile: torch.Tensor,
+    block_type_heavy_atoms_in_tile: torch.Tensor, block_type_atom_types: torch.Tensor,
+    block_type_n_interblock_bonds: torch.Tensor,
+    block_type_atoms_forming_chemical_bonds: torch.Tensor,
+    block_type_path_distance: torch.Tensor, block_type_is_ligand_fragment: torch.Tensor,
+    ljlk_type_params: torch.Tensor, global_params: torch.Tensor, max_dis: float,
+    output_block_pair_energies: bool,
+) -> List[torch.Tensor]:
+    """Forward of the rotamer-pair op (LJLKRotamerScoreOp, compiled.ops.cpp:
+    282-530): [scores [3, n], dispatch indices [3, n] int32]."""
+    args = (rot_coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+            first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+            n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+            max_n_rots_per_pose, pose_stack_min_bond_separation, pose_stack_inter_block_bondsep,
+            block_type_n_atoms, block_type_n_heavy_atoms_in_tile, block_type_heavy_atoms_in_tile,
+            block_type_atom_types, block_type_n_interblock_bonds,
+            block_type_atoms_forming_chemical_bonds, block_type_path_distance,
+            block_type_is_ligand_fragment, ljlk_type_params, global_params, max_dis,
+            output_block_pair_energies)
+    call = RotamerCall(args, rot_coords.dtype)
+    idx = call.dispatch()
+    return [call.energies(), idx]
 
-    @staticmethod
-    def forward(ctx, *args):
-        call = RotamerCall(args, args[0].dtype)
-        idx = call.dispatch()
-        score = call.energies()
-        ctx.call = call
-        ctx.n_args = len(args)
-        ctx.mark_non_differentiable(idx)
-        return score, idx
 
-    @staticmethod
-    def backward(ctx, grad_score, grad_idx):
-        if grad_score is None:
-            return (None,) * ctx.n_args
-        grad = ctx.call.backward(grad_score)
-        return (grad,) + (None,) * (ctx.n_args - 1)
+@ljlk_rotamer_forward.register_fake
+def _(rot_coords, *rest):
+    ctx = torch.library.get_ctx()
+    n = ctx.new_dynamic_size()
+    return [rot_coords.new_empty((3, n)), rot_coords.new_empty((3, n), dtype=torch.int32)]
+
+
+@torch.library.custom_op("tmol_amd::ljlk_rotamer_backward", mutates_args=())
+def ljlk_rotamer_backward(
+    grad_score: torch.Tensor, rot_coords: torch.Tensor, rot_coord_offset: torch.Tensor,
+    pose_ind_for_atom: torch.Tensor, first_rot_for_block: torch.Tensor,
+    first_rot_block_type: torch.Tensor, block_ind_for_rot: torch.Tensor,
+    pose_ind_for_rot: torch.Tensor, block_type_ind_for_rot: torch.Tensor,
+    n_rots_for_pose: torch.Tensor, rot_offset_for_pose: torch.Tensor,
+    n_rots_for_block: torch.Tensor, rot_offset_for_block: torch.Tensor,
+    max_n_rots_per_pose: int, pose_stack_min_bond_separation: torch.Tensor,
+    pose_stack_inter_block_bondsep: torch.Tensor, block_type_n_atoms: torch.Tensor,
+    block_type_n_heavy_atoms_in_tile: torch.Tensor,
+    block_type_heavy_atoms_in_tile: torch.Tensor, block_type_atom_types: torch.Tensor,
+    block_type_n_interblock_bonds: torch.Tensor,
+    block_type_atoms_forming_chemical_bonds: torch.Tensor,
+    block_type_path_distance: torch.Tensor, block_type_is_ligand_fragment: torch.Tensor,
+    ljlk_type_params: torch.Tensor, global_params: torch.Tensor, max_dis: float,
+    output_block_pair_energies: bool,
+) -> torch.Tensor:
+    """Backward of the rotamer-pair op (LJLKRotamerScoreDispatch::backward,
+    impl.hh:1868-2126): every rotamer pair's atom-pair derivatives weighted by
+    the incoming [3, n] gradient.  The dispatch list is rebuilt (it is a
+    deterministic function of the inputs, in the reference's order)."""
+    args = (rot_coords, rot_coord_offset, pose_ind_for_atom, first_rot_for_block,
+            first_rot_block_type, block_ind_for_rot, pose_ind_for_rot, block_type_ind_for_rot,
+            n_rots_for_pose, rot_offset_for_pose, n_rots_for_block, rot_offset_for_block,
+            max_n_rots_per_pose, pose_stack_min_bond_separation, pose_stack_inter_block_bondsep,
+            block_type_n_atoms, block_type_n_heavy_atoms_in_tile, block_type_heavy_atoms_in_tile,
+            block_type_atom_types, block_type_n_interblock_bonds,
+            block_type_atoms_forming_chemical_bonds, block_type_path_distance,
+            block_type_is_ligand_fragment, ljlk_type_params, global_params, max_dis,
+            output_block_pair_energies)
+    call = RotamerCall(args, rot_coords.dtype)
+    call.dispatch()
+    if tuple(grad_score.shape) != (3, call.n_pairs):
+        raise LJLKLibraryError("rotamer backward: gradient must be [3, %d]" % call.n_pairs)
+    return call.backward(grad_score)
+
+
+@ljlk_rotamer_backward.register_fake
+def _(grad_score, rot_coords, *rest):
+    return torch.empty_like(rot_coords)
+
+
+def _rotamer_forward_setup(ctx, inputs, output):
+    ctx.args = inputs[1:]
+    ctx.save_for_backward(inputs[0])
+
+
+def _rotamer_forward_backward(ctx, grads):
+    grad_score = grads[0]
+    none = (None,) * (len(ctx.args) + 1)
+    if grad_score is None:
+        return none
+    (coords,) = ctx.saved_tensors
+    grad = torch.ops.tmol_amd.ljlk_rotamer_backward(grad_score.contiguous(), coords, *ctx.args)
+    return (grad,) + none[1:]
+
+
+torch.library.register_autograd("tmol_amd::ljlk_rotamer_forward", _rotamer_forward_backward,
+                                setup_context=_rotamer_forward_setup)
 
 
 def ljlk_rotamer_scores(*args):
-    """Drop-in for ``torch.ops.tmol_ljlk.ljlk_rotamer_scores`` (27 arguments,
-    the rotamer set in place of the pose stack's rotamer tensors); returns
+    """``torch.ops.tmol_ljlk.ljlk_rotamer_scores`` (27 arguments, the rotamer
+    set in place of the pose stack's rotamer tensors); returns
     ``[scores [3, n], dispatch_indices [3, n] int32]`` with one column per
     rotamer pair (pose, rot1, rot2)."""
     if len(args) != 27:
         raise TypeError("ljlk_rotamer_scores takes 27 arguments (%d given)" % len(args))
-    score, idx = LJLKRotamerScoreFunction.apply(*args)
-    return [score, idx]
+    return torch.ops.tmol_ljlk.ljlk_rotamer_scores(*args)
 
 
 # ---------------------------------------------------------------------------

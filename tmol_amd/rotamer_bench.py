@@ -9,12 +9,13 @@ resident in HBM.
 
 Reported: rotamer pairs/s of the step; the scoring kernel alone (HIP events
 on its stream around repeated launches over a fixed dispatch list) with its
-roofline from the branch census x energy-only FLOP table; the reference's
-own CPU op (oracle/_ref, one thread) on the first pose as the CPU baseline.
+roofline from the branch census x energy-only FLOP table; and, when ``cpu``
+is given (bench.py passes the reference's own CPU op, one thread, on the
+first pose), the CPU baseline.
 The ``elec`` entry does the same for ``elec_rotamer_scores`` on the same
 rotamers (plus its dTdV-weighted backward kernel).
 
-    python -m tmol_amd.rotamer_bench [--steps 10] [--no-cpu]
+    python -m tmol_amd.rotamer_bench [--steps 10]      (GPU legs only)
 """
 
 import argparse
@@ -38,41 +39,7 @@ def _args(ps, rs, dt, device):
     return [x.to(device) if isinstance(x, torch.Tensor) else x for x in a]
 
 
-def cpu_baseline():
-    """The reference CPU op (one thread) on the first pose of the workload
-    with its rotamers (the same pose and rotamers as in the batch: poses and
-    rotamer noise are drawn pose by pose)."""
-    import torch
-
-    from oracle import oracle as orc
-
-    from .rotamers import repack_workload
-
-    torch.set_num_threads(1)
-    ps, sub = repack_workload("repack_16x300", n_poses=1, seed=1)
-    a = _args(ps, sub, torch.float32, "cpu")
-    if orc.reference_op_available():
-        kind, f = "reference", orc.load_reference_rotamer_op()
-    else:
-        kind = "port"
-
-        def f(*x):
-            return orc.oracle_rotamer_scores(*x)[:2]
-    t0 = time.perf_counter()
-    s, idx = f(*a)
-    dt = time.perf_counter() - t0
-    n = int(idx.shape[1])
-    return {
-        "value": n / dt,
-        "unit": "rotamer pairs/s",
-        "cores": 1,
-        "kind": kind,
-        "sample": "pose 0 of the batch (300 residues, %d rotamers, %d rotamer pairs), "
-                  "forward, torch threads = 1, %.1f s" % (sub.n_rotamers_total, n, dt),
-    }
-
-
-def run(device, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
+def run(device, steps=10, warmup=2, kernel_reps=10, cpu=None, elec_cpu=None, log=print):
     import numpy
     import torch
 
@@ -157,11 +124,11 @@ def run(device, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
         },
         "census": census,
     }
-    if cpu:
+    if cpu is not None:
         log("rotamer cpu baseline ...")
-        res["cpu_baseline"] = cpu_baseline()
+        res["cpu_baseline"] = cpu()
     log("elec rotamer pairs ...")
-    res["elec"] = elec_leg(device, ps, rs, steps, warmup, kernel_reps, cpu, log)
+    res["elec"] = elec_leg(device, ps, rs, steps, warmup, kernel_reps, elec_cpu, log)
     return res
 
 
@@ -175,7 +142,7 @@ def _elec_args(ps, rs, device):
     return [x.to(device) if isinstance(x, torch.Tensor) else x for x in a]
 
 
-def elec_leg(device, ps, rs, steps=10, warmup=2, kernel_reps=10, cpu=True, log=print):
+def elec_leg(device, ps, rs, steps=10, warmup=2, kernel_reps=10, cpu=None, log=print):
     """``elec_rotamer_scores`` on the same rotamers: the whole forward (its
     own dispatch at elec's 5.5 A reach), the scoring kernel alone, the
     dTdV-weighted backward kernel, and the reference's CPU elec rotamer op
@@ -227,34 +194,9 @@ def elec_leg(device, ps, rs, steps=10, warmup=2, kernel_reps=10, cpu=True, log=p
         "energies_finite": finite,
         "dtype": "f32",
     }
-    if cpu:
-        from oracle import oracle as orc
-
-        from .rotamers import repack_workload
-
+    if cpu is not None:
         log("elec rotamer cpu baseline ...")
-        torch.set_num_threads(1)
-        ps1, rs1 = repack_workload("repack_16x300", n_poses=1, seed=1)
-        a1 = _elec_args(ps1, rs1, "cpu")
-        try:
-            kind, f = "reference", orc.load_reference_elec_op("elec_rotamer_scores")
-        except FileNotFoundError:
-            kind = "port"
-
-            def f(*x):
-                return orc.oracle_elec_rotamer_scores(*x)[:2]
-        t0 = time.perf_counter()
-        _, i1 = f(*a1)
-        dt = time.perf_counter() - t0
-        n1 = int(i1.shape[1])
-        res["cpu_baseline"] = {
-            "value": n1 / dt,
-            "unit": "rotamer pairs/s",
-            "cores": 1,
-            "kind": kind,
-            "sample": "pose 0 of the batch (300 residues, %d rotamers, %d rotamer pairs), "
-                      "forward, torch threads = 1, %.1f s" % (rs1.n_rotamers_total, n1, dt),
-        }
+        res["cpu_baseline"] = cpu()
     return res
 
 
@@ -263,12 +205,12 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true", help="accepted; the CPU baseline runs from bench.py")
     args = ap.parse_args()
     import torch
 
     res = run(torch.device("cuda:0"), args.steps, args.warmup, args.kernel_reps,
-              cpu=not args.no_cpu, log=lambda m: print("[rotamer] " + m, file=sys.stderr, flush=True))
+              cpu=None, log=lambda m: print("[rotamer] " + m, file=sys.stderr, flush=True))
     print(json.dumps(res), flush=True)
 
 

@@ -19,7 +19,17 @@ from . import ops
 from .flops import census_dict
 from .pose import RaggedPoseStack
 
-_TUNE_CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "autotune_cache.json")
+
+
+def _tune_cache_path() -> str:
+    """The autotuner's results file, in the user's cache directory
+    ($TMOL_AMD_CACHE, else $XDG_CACHE_HOME/tmol_amd, else ~/.cache/tmol_amd);
+    never inside the package."""
+    d = os.environ.get("TMOL_AMD_CACHE")
+    if not d:
+        base = os.environ.get("XDG_CACHE_HOME") or os.path.join(os.path.expanduser("~"), ".cache")
+        d = os.path.join(base, "tmol_amd")
+    return os.path.join(d, "autotune.json")
 
 
 def ragged_cstack(stack: RaggedPoseStack, dev):
@@ -202,9 +212,10 @@ class LJLKRaggedScorer:
         key = "%s|G%d|NA%d|P%d|%s" % (torch.cuda.get_device_name(self.device), self.G,
                                       self.NA, self.P, self.dtype)
         table = {}
-        if cache and os.path.exists(_TUNE_CACHE):
+        path = _tune_cache_path()
+        if cache and os.path.exists(path):
             try:
-                table = json.load(open(_TUNE_CACHE))
+                table = json.load(open(path))
             except (OSError, ValueError):
                 table = {}
         if key in table:
@@ -235,7 +246,8 @@ class LJLKRaggedScorer:
         if cache:
             table[key] = entry
             try:
-                with open(_TUNE_CACHE, "w") as f:
+                os.makedirs(os.path.dirname(path), exist_ok=True)
+                with open(path, "w") as f:
                     json.dump(table, f, indent=1)
             except OSError:
                 pass

@@ -1,9 +1,9 @@
 """Topology measurement: inter_block_bondsep + min_block_bondsep of the
 64-pose padded workload (residue counts uniform in 150-600, B = 600,
-[64, 600, 600, 2, 2] int32 = 92 M entries) on the device, against the
-reference's own all-pairs-shortest-paths op (oracle/_ref, one thread) on the
-same connection-point graphs (the reference's graph assembly in torch is
-not counted).
+[64, 600, 600, 2, 2] int32 = 92 M entries) on the device; bench.py passes
+the CPU baseline (the reference's own all-pairs-shortest-paths op, one
+thread, on the same connection-point graphs; the reference's graph assembly
+in torch is not counted).
 
     python -m tmol_amd.topology_bench
 """
@@ -13,7 +13,7 @@ import sys
 import time
 
 
-def run(device, reps=10, cpu=True, log=print):
+def run(device, reps=10, cpu=None, log=print):
     import numpy
     import torch
 
@@ -39,52 +39,9 @@ def run(device, reps=10, cpu=True, log=print):
            "entries": P * B * B * C * C, "ms": ms,
            "bytes_written": 4 * (P * B * B * C * C + P * B * B),
            "write_gb_per_s": 4 * (P * B * B * C * C + P * B * B) / (ms * 1e-3) / 1e9}
-    if cpu:
-        res["cpu_baseline"] = cpu_baseline(ps)
+    if cpu is not None:
+        res["cpu_baseline"] = cpu(ps)
     return res
-
-
-def cpu_baseline(ps):
-    """The reference's apsp op (cutoff 6) on each pose's connection-point
-    graph, one thread."""
-    import os
-
-    import numpy
-    import torch
-
-    from oracle import oracle as orc
-
-    if not os.path.exists(os.path.join(orc.HERE, "_ref", "tmol_apsp_ref.so")):
-        return None
-    torch.set_num_threads(1)
-    apsp = orc.load_reference_apsp_op()
-    pbt = ps.packed_block_types
-    bti = ps.block_type_ind.numpy()
-    nconn = pbt.n_conn.numpy()
-    conn = pbt.conn_atom.numpy()
-    pd = pbt.ljlk_bond_separation.numpy()
-    irc = ps.inter_residue_connections.numpy()
-    graphs = []
-    for p in range(ps.n_poses):
-        points = [(b, c) for b in range(ps.max_n_blocks) if bti[p, b] >= 0 for c in range(nconn[bti[p, b]])]
-        index = {bc: i for i, bc in enumerate(points)}
-        n = len(points)
-        W = numpy.full((n, n), 6, dtype=numpy.int32)
-        for i, (b, c) in enumerate(points):
-            t = bti[p, b]
-            for c2 in range(nconn[t]):
-                W[i, index[(b, c2)]] = pd[t, conn[t, c], conn[t, c2]]
-            b2, cc2 = irc[p, b, c]
-            if b2 >= 0:
-                W[i, index[(b2, cc2)]] = 1
-        graphs.append(torch.from_numpy(W)[None])
-    t0 = time.perf_counter()
-    for W in graphs:
-        apsp(W, 6)
-    dt = time.perf_counter() - t0
-    return {"value_ms": dt * 1000.0, "cores": 1, "kind": "reference",
-            "sample": "all %d poses: the reference apsp op (cutoff 6) on each connection-point "
-                      "graph, graph assembly not counted" % ps.n_poses}
 
 
 def main():
