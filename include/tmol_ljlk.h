@@ -35,7 +35,7 @@ extern "C" {
 
 /* stack layouts */
 #define TMOL_LJLK_PADDED 0 /* tmol PoseStack: P poses x B blocks, dense count-pair tables */
-#define TMOL_LJLK_RAGGED 1 /* CSR over poses, count pair from chain positions */
+#define TMOL_LJLK_RAGGED 1 /* CSR over poses, count pair from a sparse bonded table */
 
 /* score flags */
 #define TMOL_LJLK_DVDX 2       /* per-term dV/dx [3, NA, 3] (reference whole-pose forward) */
@@ -83,9 +83,19 @@ typedef struct {
   const int32_t* atom_pose;           /* [NA] pose_ind_for_atom */
   const int32_t* min_bondsep;         /* padded: [P, B, B] */
   const int32_t* inter_block_bondsep; /* padded: [P, B, B, C, C] */
-  const int32_t* block_conn_pos;      /* ragged: [G, C] chain position, -1 = none */
-  const int32_t* block_segment;       /* ragged: [G] bonded-segment id */
+  const int32_t* bonded_start;        /* ragged: [G + 1] CSR offsets into bonded */
+  const int32_t* bonded;              /* ragged: [n, 8] forward bonded table, see below */
 } tmol_ljlk_stack;
+
+/* Ragged count-pair connectivity, for any bond topology (the sparse form of
+ * the padded inter_block_bondsep, pose_stack_builder.py:1271-1349): rows
+ * bonded_start[g] .. bonded_start[g + 1] - 1 of `bonded` list the blocks
+ * g2 > g of g's pose whose minimum connection-point separation is at most
+ * 4, sorted by g2.  Row: {g2, min separation, 0, 0, sep[0], sep[1], sep[2],
+ * sep[3]}, where byte c2 of sep[c1] is the bond separation (capped at 6)
+ * between connection c1 of g and connection c2 of g2.  Absent pairs are 6
+ * bonds or more apart. */
+#define TMOL_LJLK_BONDED_WIDTH 8
 
 /* Neighbor-pass tiles: 64 x 64 block tiles of each pose's upper triangle.
  * Each tile is int32[4] = {row_start_g, col_start_g, pose_end_g, pose}. */

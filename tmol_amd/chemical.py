@@ -162,7 +162,8 @@ class ChemicalDatabase:
         self.ljlk = db["ljlk"]
         self._base = {r["name"]: r for r in db["residues"]}
         self._variants = {v["name"]: v for v in db["variants"]}
-        self.one_letter = {r["one_letter_code"]: r["name"] for r in db["residues"]}
+        self.one_letter = {r["one_letter_code"]: r["name"] for r in db["residues"]
+                           if r.get("one_letter_code")}
         self._cache: Dict[Tuple[str, bool, bool], ResidueType] = {}
 
     @property

@@ -120,8 +120,8 @@ struct StackView {
   const int32_t* block_pose;
   const int32_t* min_bondsep;
   const int32_t* inter_block_bondsep;
-  const int32_t* block_conn_pos;
-  const int32_t* block_segment;
+  const int32_t* bonded_start;  // ragged: [G + 1]
+  const int32_t* bonded;        // ragged: [n, BONDED_W] forward bonded table
 
   __device__ __forceinline__ int pose_of(int g) const {
     return layout == TMOL_LJLK_PADDED ? g / B : block_pose[g];
@@ -159,8 +159,8 @@ inline StackView make_stack_view(const tmol_ljlk_stack* s, int C) {
   v.block_pose = s->block_pose;
   v.min_bondsep = s->min_bondsep;
   v.inter_block_bondsep = s->inter_block_bondsep;
-  v.block_conn_pos = s->block_conn_pos;
-  v.block_segment = s->block_segment;
+  v.bonded_start = s->bonded_start;
+  v.bonded = s->bonded;
   return v;
 }
 
@@ -197,6 +197,21 @@ __device__ __forceinline__ int64_t bp_index(const StackView& s, const WorkView& 
   return int64_t(g1) * w.row_cap + k;
 }
 
+constexpr int BONDED_W = TMOL_LJLK_BONDED_WIDTH;
+
+// ragged: the forward bonded-table row of (lo, hi), lo < hi, or null when the
+// two blocks are more than 4 bonds apart (rows are sorted by g2 and short:
+// a handful per block)
+__device__ __forceinline__ const int32_t* bonded_row(const StackView& s, int lo, int hi) {
+  const int k1 = s.bonded_start[lo + 1];
+  for (int k = s.bonded_start[lo]; k < k1; ++k) {
+    const int g = s.bonded[int64_t(k) * BONDED_W];
+    if (g == hi) return s.bonded + int64_t(k) * BONDED_W;
+    if (g > hi) break;
+  }
+  return nullptr;
+}
+
 // bond separation between connection c1 of g1 and c2 of g2 (g1 != g2)
 __device__ __forceinline__ int conn_sep(const StackView& s, int g1, int g2, int c1, int c2) {
   if (s.layout == TMOL_LJLK_PADDED) {
@@ -204,11 +219,13 @@ __device__ __forceinline__ int conn_sep(const StackView& s, int g1, int g2, int 
     int64_t idx = ((int64_t(g1) * s.B + b2) * s.C + c1) * s.C + c2;
     return s.inter_block_bondsep[idx];
   }
-  int p1 = s.block_conn_pos[int64_t(g1) * s.C + c1];
-  int p2 = s.block_conn_pos[int64_t(g2) * s.C + c2];
-  if (p1 < 0 || p2 < 0 || s.block_segment[g1] != s.block_segment[g2]) return 6;
-  int d = p1 > p2 ? p1 - p2 : p2 - p1;
-  return d < 6 ? d : 6;
+  if (g2 < g1) {
+    const int t = g1; g1 = g2; g2 = t;
+    const int u = c1; c1 = c2; c2 = u;
+  }
+  const int32_t* e = bonded_row(s, g1, g2);
+  if (!e) return 6;
+  return (int)(((uint32_t)e[4 + c1] >> (8 * c2)) & 255u);
 }
 
 // minimum bond separation over connection pairs (min_block_bondsep)
@@ -218,13 +235,8 @@ __device__ __forceinline__ int min_block_sep(const StackView& s, int g1, int g2,
     int b2 = g2 - (g2 / s.B) * s.B;
     return s.min_bondsep[int64_t(g1) * s.B + b2];
   }
-  int m = 6;
-  for (int c1 = 0; c1 < nc1; ++c1)
-    for (int c2 = 0; c2 < nc2; ++c2) {
-      int v = conn_sep(s, g1, g2, c1, c2);
-      m = v < m ? v : m;
-    }
-  return m;
+  const int32_t* e = g1 < g2 ? bonded_row(s, g1, g2) : bonded_row(s, g2, g1);
+  return e ? e[1] : 6;
 }
 
 // Inter-block count-pair separation from LDS tables (impl.hh:47-67,

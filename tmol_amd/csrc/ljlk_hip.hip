@@ -48,7 +48,7 @@ static int validate(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt, i
     if (int64_t(s->n_poses) * s->max_n_blocks != s->n_blocks_total) return TMOL_LJLK_EINVAL;
     if (!s->min_bondsep || !s->inter_block_bondsep) return TMOL_LJLK_EINVAL;
   } else if (s->layout == TMOL_LJLK_RAGGED) {
-    if (!s->block_pose || !s->block_conn_pos || !s->block_segment) return TMOL_LJLK_EINVAL;
+    if (!s->block_pose || !s->bonded_start || !s->bonded) return TMOL_LJLK_EINVAL;
   } else {
     return TMOL_LJLK_EINVAL;
   }

@@ -24,6 +24,9 @@ CANONICAL = [
     "ALA", "ARG", "ASN", "ASP", "CYS", "GLN", "GLU", "GLY", "HIS", "ILE",
     "LEU", "LYS", "MET", "PHE", "PRO", "SER", "THR", "TRP", "TYR", "VAL",
 ]
+# residue types beyond the 20 sampled ones: the disulfide-bonded cysteine
+# (a third connection, "dslf" on SG), for poses with side-chain cross-links
+EXTRA = ["CYD"]
 VARIANTS = ["CarboxyTerminus", "AminoTerminus", "AminoTerminusPro"]
 
 
@@ -75,13 +78,13 @@ def extract(ref_root):
 
     residues = []
     by_name = {r["name"]: r for r in chem["residues"]}
-    for name in CANONICAL:
+    for name in CANONICAL + EXTRA:
         r = by_name[name]
         residues.append(
             {
                 "name": r["name"],
                 "name3": r["name3"],
-                "one_letter_code": r["one_letter_code"],
+                "one_letter_code": r.get("one_letter_code"),
                 "atoms": [[a["name"], a["atom_type"]] for a in r["atoms"]],
                 "atom_aliases": [[a["name"], a["alt_name"]] for a in r["atom_aliases"]],
                 "bonds": [[b[0], b[1]] for b in r["bonds"]],
@@ -141,7 +144,7 @@ def extract(ref_root):
     elec = yaml.safe_load(
         open(os.path.join(ref_root, "tmol/database/default/scoring/elec.yaml"))
     )
-    keep = set(CANONICAL)
+    keep = set(CANONICAL + EXTRA)
 
     def _res(tag):
         # "ALA" / "ALA:cterm" -> ("ALA", "" / "cterm")

@@ -65,8 +65,8 @@ class CStack(ctypes.Structure):
         ("atom_pose", _VP),
         ("min_bondsep", _VP),
         ("inter_block_bondsep", _VP),
-        ("block_conn_pos", _VP),
-        ("block_segment", _VP),
+        ("bonded_start", _VP),
+        ("bonded", _VP),
     ]
 
 

@@ -8,22 +8,27 @@ Two layouts of the same systems:
   [P, B, B, C, C]`` and the derived "rotamer" index tensors of
   ``PoseStack.__attrs_post_init__``.  This is what the drop-in op takes.
 * ``RaggedPoseStack`` -- CSR over poses: every real block once, atoms packed
-  back to back, no padding.  Count-pair connectivity is carried as each
-  connection's position along its bonded segment (``block_conn_pos``) plus
-  a segment id, so no ``[P, B, B, ...]`` table is ever materialised.
+  back to back, no padding.  Count-pair connectivity is a sparse forward
+  bonded table: for each block, the later blocks of its pose within 4 bonds
+  of it (through connection atoms) with their connection-point separations,
+  so no ``[P, B, B, ...]`` table is ever materialised, for any bond topology
+  (chains, cycles, disulfides, cross-links).
 
 Inter-block bond separation: the reference runs an all-pairs shortest path
 over the graph of inter-residue connection points (tmol/pose/
-pose_stack_builder.py:436-600, 1230-1350), capped at 6.  For polymer
-chains of alpha amino acids that graph is a path: ``... up(i-1) -1- down(i)
--d(i)- up(i) -1- down(i+1) ...`` with d(i) the intra-residue N..C path
-distance, so the separation of two connection points is the difference of
-their positions along the path, capped at 6, and 6 across chain breaks.
-``conn_graph_bondsep`` is the generic Floyd-Warshall used to check that.
+pose_stack_builder.py:436-600, 1230-1350), capped at 6.  For linear polymer
+chains that graph is a path: ``... up(i-1) -1- down(i) -d(i)- up(i) -1-
+down(i+1) ...`` with d(i) the intra-residue N..C path distance, so the
+separation of two connection points is the difference of their positions
+along the path, capped at 6, and 6 across chain breaks.  Poses with other
+bonds (``Pose.extra_bonds``: cyclisation, disulfides) use a Dijkstra search
+over the connection points (``connection_separations``).
+``conn_graph_bondsep`` is a Floyd-Warshall used to check both.
 """
 
 from dataclasses import dataclass, field
-from typing import List, Optional, Sequence
+import heapq
+from typing import Dict, List, Optional, Sequence, Tuple
 
 import numpy
 import torch
@@ -119,6 +124,10 @@ class Pose:
     residue_types: List[ResidueType]
     coords: List[numpy.ndarray]  # per residue [n_atoms, 3]
     connected_to_prev: List[bool] = field(default_factory=list)
+    # bonds beyond the polymer chain (cyclisation, disulfides, cross-links):
+    # (block1, connection1, block2, connection2), connection indices of the
+    # blocks' residue types
+    extra_bonds: List[Tuple[int, int, int, int]] = field(default_factory=list)
 
     def __post_init__(self):
         if not self.connected_to_prev:
@@ -128,6 +137,29 @@ class Pose:
             if ok:
                 assert self.residue_types[i].down_connection_ind >= 0
                 assert self.residue_types[i - 1].up_connection_ind >= 0
+        used = set()
+        for a, b in self.bonds():
+            for bc in (a, b):
+                blk, c = bc
+                assert 0 <= blk < self.n_blocks and 0 <= c < len(self.residue_types[blk].connections)
+                assert bc not in used, "connection %s bonded twice" % (bc,)
+                used.add(bc)
+
+    def bonds(self) -> List[Tuple[Tuple[int, int], Tuple[int, int]]]:
+        """Every inter-block bond as ((block, connection), (block, connection))."""
+        out = []
+        for i in range(1, self.n_blocks):
+            if self.connected_to_prev[i]:
+                out.append(((i - 1, self.residue_types[i - 1].up_connection_ind),
+                            (i, self.residue_types[i].down_connection_ind)))
+        for b1, c1, b2, c2 in self.extra_bonds:
+            out.append(((b1, c1), (b2, c2)))
+        return out
+
+    @property
+    def is_linear(self) -> bool:
+        """Only chain bonds (down/up connections of consecutive blocks)."""
+        return not self.extra_bonds
 
     @property
     def n_blocks(self):
@@ -146,13 +178,11 @@ class Pose:
         seg = numpy.zeros(nb, dtype=numpy.int64)
         cur_seg = -1
         cursor = 0
+        if not self.is_linear:
+            raise ValueError("chain positions describe linear poses only")
         for i, rt in enumerate(self.residue_types):
             di, ui = rt.down_connection_ind, rt.up_connection_ind
-            extra = [c for c in range(len(rt.connections)) if c not in (di, ui)]
-            if extra:
-                raise NotImplementedError(
-                    "only down/up polymer connections are supported: %s" % rt.name
-                )
+            # connections other than down/up are unbonded here: position -1
             if not self.connected_to_prev[i]:
                 cur_seg += 1
                 # leave a gap bigger than the cap between segments
@@ -197,11 +227,9 @@ def conn_graph_bondsep(pose: Pose) -> numpy.ndarray:
         for c1 in range(len(rt.connections)):
             for c2 in range(len(rt.connections)):
                 D[offs[i] + c1, offs[i] + c2] = rt.path_distance[rt.conn_atom[c1], rt.conn_atom[c2]]
-    for i in range(1, nb):
-        if pose.connected_to_prev[i]:
-            a = offs[i - 1] + rts[i - 1].up_connection_ind
-            b = offs[i] + rts[i].down_connection_ind
-            D[a, b] = D[b, a] = 1
+    for (b1, c1), (b2, c2) in pose.bonds():
+        a, b = offs[b1] + c1, offs[b2] + c2
+        D[a, b] = D[b, a] = 1
     for k in range(n):
         D = numpy.minimum(D, D[:, k : k + 1] + D[k : k + 1, :])
     D = numpy.minimum(D, MAX_SIG_BOND_SEPARATION)
@@ -211,6 +239,107 @@ def conn_graph_bondsep(pose: Pose) -> numpy.ndarray:
             ni, nj = len(rts[i].connections), len(rts[j].connections)
             out[i, j, :ni, :nj] = D[offs[i] : offs[i] + ni, offs[j] : offs[j] + nj]
     return out
+
+
+def connection_separations(pose: Pose, cutoff: int = MAX_SIG_BOND_SEPARATION
+                           ) -> Dict[Tuple[int, int], Dict[Tuple[int, int], int]]:
+    """Shortest bond paths between the connection points of a pose, for any
+    bond topology: {(b1, c1): {(b2, c2): d}} for every d < cutoff (absent
+    pairs are ``cutoff`` or more).  Dijkstra over the connection-point graph:
+    inside a block, its path distance between the two connection atoms;
+    across an inter-block bond, 1 (the reference's graph and all-pairs
+    shortest paths, pose_stack_builder.py:459-599, 1230-1349)."""
+    rts = pose.residue_types
+    partner = {}
+    for a, b in pose.bonds():
+        partner[a] = b
+        partner[b] = a
+    out = {}
+    for blk, rt in enumerate(rts):
+        for c in range(len(rt.connections)):
+            src = (blk, c)
+            dist = {src: 0}
+            heap = [(0, src)]
+            while heap:
+                d, u = heapq.heappop(heap)
+                if d > dist[u]:
+                    continue
+                ub, uc = u
+                urt = rts[ub]
+                steps = [((ub, c2), int(urt.path_distance[urt.conn_atom[uc], urt.conn_atom[c2]]))
+                         for c2 in range(len(urt.connections)) if c2 != uc]
+                if u in partner:
+                    steps.append((partner[u], 1))
+                for v, w in steps:
+                    nd = d + w
+                    if nd < cutoff and nd < dist.get(v, cutoff):
+                        dist[v] = nd
+                        heapq.heappush(heap, (nd, v))
+            out[src] = dist
+    return out
+
+
+def inter_block_bondsep_of(pose: Pose, C: int) -> numpy.ndarray:
+    """inter_block_bondsep [nb, nb, C, C] of one pose (capped at 6)."""
+    nb = pose.n_blocks
+    out = numpy.full((nb, nb, C, C), MAX_SIG_BOND_SEPARATION, dtype=numpy.int32)
+    if pose.is_linear:
+        pos, seg = pose.conn_positions()
+        cb = pos.shape[1]
+        out[:, :, :cb, :cb] = chain_bondsep(
+            pos[:, None, :, None], seg[:, None, None, None],
+            pos[None, :, None, :], seg[None, :, None, None],
+        )
+        return out
+    for (b1, c1), dists in connection_separations(pose).items():
+        for (b2, c2), d in dists.items():
+            out[b1, b2, c1, c2] = d
+    return out
+
+
+# Forward bonded table of a ragged stack: for block g, the blocks g2 > g of
+# its pose whose minimum connection-point separation is at most 4 (the only
+# pairs whose count pair needs the separations), sorted by g2.  One entry is
+# int32[BONDED_WIDTH]: g2, the minimum separation, 0, 0, then for c1 = 0..3
+# the separations to c2 = 0..3 as bytes (byte c2 of word 4 + c1; 6 where no
+# path is shorter).
+BONDED_WIDTH = 8
+BONDED_MAX_SEP = 4
+
+
+def forward_bonded_entries(pose: Pose, C: int):
+    """(b1, b2, seps [C, C]) of every block pair b1 < b2 of the pose whose
+    minimum connection separation is <= 4, ordered by (b1, b2)."""
+    nb = pose.n_blocks
+    entries = []
+    if pose.is_linear:
+        pos, seg = pose.conn_positions()
+        cb = pos.shape[1]
+        # chain positions grow by at least 1 per block: a pair more than 4
+        # blocks apart is more than 4 bonds apart
+        for k in range(1, 5):
+            if k >= nb:
+                break
+            sep = chain_bondsep(pos[:-k, :, None], seg[:-k, None, None], pos[k:, None, :],
+                                seg[k:, None, None])
+            full = numpy.full((nb - k, C, C), MAX_SIG_BOND_SEPARATION, dtype=numpy.int32)
+            full[:, :cb, :cb] = sep
+            for b1 in numpy.nonzero(full.reshape(nb - k, -1).min(1) <= BONDED_MAX_SEP)[0]:
+                entries.append((int(b1), int(b1) + k, full[b1]))
+        entries.sort(key=lambda e: (e[0], e[1]))
+        return entries
+    seps = {}
+    for (b1, c1), dists in connection_separations(pose).items():
+        for (b2, c2), d in dists.items():
+            if b2 > b1:
+                m = seps.setdefault((b1, b2), numpy.full((C, C), MAX_SIG_BOND_SEPARATION,
+                                                        dtype=numpy.int32))
+                m[c1, c2] = d
+    for (b1, b2) in sorted(seps):
+        m = seps[(b1, b2)]
+        if m.min() <= BONDED_MAX_SEP:
+            entries.append((b1, b2, m))
+    return entries
 
 
 class PoseStack:
@@ -247,19 +376,11 @@ class PoseStack:
                 bti[p, b] = pbt.index[rt.name]
                 coords[p, off : off + rt.n_atoms] = pose.coords[b]
                 off += rt.n_atoms
-            for b in range(1, pose.n_blocks):
-                if pose.connected_to_prev[b]:
-                    up = pose.residue_types[b - 1].up_connection_ind
-                    dn = pose.residue_types[b].down_connection_ind
-                    irc[p, b - 1, up] = (b, dn)
-                    irc[p, b, dn] = (b - 1, up)
-            pos, seg = pose.conn_positions()
-            nb, cb = pos.shape
-            sep = chain_bondsep(
-                pos[:, None, :, None], seg[:, None, None, None],
-                pos[None, :, None, :], seg[None, :, None, None],
-            )
-            ibs[p, :nb, :nb, :cb, :cb] = sep
+            for (b1, c1), (b2, c2) in pose.bonds():
+                irc[p, b1, c1] = (b2, c2)
+                irc[p, b2, c2] = (b1, c1)
+            nb = pose.n_blocks
+            ibs[p, :nb, :nb] = inter_block_bondsep_of(pose, C)
 
         self.coords = torch.tensor(coords, dtype=dtype, device=device)
         self.block_coord_offset = torch.tensor(bco, device=device)
@@ -338,26 +459,38 @@ class RaggedPoseStack:
         block_atom_offset = numpy.zeros(G, dtype=numpy.int32)
         block_type = numpy.zeros(G, dtype=numpy.int32)
         block_pose = numpy.zeros(G, dtype=numpy.int32)
-        block_conn_pos = numpy.full((G, C), -1, dtype=numpy.int32)
-        block_segment = numpy.zeros(G, dtype=numpy.int32)
+        bonded_count = numpy.zeros(G, dtype=numpy.int64)
+        bonded_rows = []
         coords = numpy.zeros((NA, 3), dtype=numpy.float64)
         atom_pose = numpy.zeros(NA, dtype=numpy.int32)
         g = 0
-        seg_base = 0
         for p, pose in enumerate(poses):
             off = int(pose_atom_start[p])
             atom_pose[off : off + pose.n_atoms] = p
-            pos, seg = pose.conn_positions()
+            g0 = g
+            for b1, b2, sep in forward_bonded_entries(pose, C):
+                row = numpy.zeros(BONDED_WIDTH, dtype=numpy.int64)
+                row[0] = g0 + b2
+                row[1] = sep.min()
+                sp = numpy.full((4, 4), MAX_SIG_BOND_SEPARATION, dtype=numpy.int64)
+                sp[:C, :C] = sep
+                row[4:8] = (sp * (256 ** numpy.arange(4))[None, :]).sum(1)
+                bonded_rows.append(row)
+                bonded_count[g0 + b1] += 1
             for b, rt in enumerate(pose.residue_types):
                 block_atom_offset[g] = off
                 block_type[g] = pbt.index[rt.name]
                 block_pose[g] = p
-                block_segment[g] = seg_base + seg[b]
-                block_conn_pos[g, : pos.shape[1]] = pos[b]
                 coords[off : off + rt.n_atoms] = pose.coords[b]
                 off += rt.n_atoms
                 g += 1
-            seg_base += int(seg.max()) + 1
+        bonded_start = numpy.concatenate([[0], numpy.cumsum(bonded_count)]).astype(numpy.int32)
+        bonded = (numpy.stack(bonded_rows) if bonded_rows
+                  else numpy.zeros((1, BONDED_WIDTH), dtype=numpy.int64))
+        # words 4..7 hold four bytes each: keep their bit patterns in int32
+        bonded = bonded.astype(numpy.uint32).view(numpy.int32) if bonded.max() < 2**32 else None
+        if bonded is None:
+            raise ValueError("bonded table overflow")
         if NA >= 2**31:
             raise ValueError("ragged stack exceeds 2^31 atoms")
 
@@ -374,8 +507,9 @@ class RaggedPoseStack:
         self.block_atom_offset = t(block_atom_offset)
         self.block_type = t(block_type)
         self.block_pose = t(block_pose)
-        self.block_conn_pos = t(block_conn_pos)
-        self.block_segment = t(block_segment)
+        # forward bonded table (count-pair connectivity, any bond topology)
+        self.bonded_start = t(bonded_start)
+        self.bonded = t(numpy.ascontiguousarray(bonded))
         self.atom_pose = t(atom_pose)
         self.n_blocks_per_pose = t(nb.astype(numpy.int32))
         self.n_atoms_per_pose = t(na.astype(numpy.int32))

@@ -42,8 +42,8 @@ def ragged_cstack(stack: RaggedPoseStack, dev):
         block_pose=i32(stack.block_pose),
         pose_block_start=i32(stack.pose_block_start),
         atom_pose=i32(stack.atom_pose),
-        block_conn_pos=i32(stack.block_conn_pos),
-        block_segment=i32(stack.block_segment),
+        bonded_start=i32(stack.bonded_start),
+        bonded=i32(stack.bonded),
     )
     s = ops.CStack()
     s.layout = ops.RAGGED
@@ -157,6 +157,19 @@ class LJLKRaggedScorer:
         self.neighbors(coords)
         self.score(coords, ops.BLOCK_PAIR, energies=out)
         return out
+
+    def debug_pair_separations(self, coords: torch.Tensor, items: torch.Tensor) -> torch.Tensor:
+        """Count-pair separations computed by the scoring kernel's own code for
+        block pairs ``items`` [n, 2] (global block indices g1 <= g2 of one
+        pose): [n, 32, 32] uint8, 255 past the blocks' atoms."""
+        self.neighbors(coords)
+        it = ops._i32(items, self.device)
+        out = torch.empty((it.shape[0], 32 * 32), dtype=torch.uint8, device=self.device)
+        ops._check(ops.lib().tmol_ljlk_debug_pair_separations(
+            ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ops._p(self.ws.buf),
+            self.ws.capacity, self.dcode, ops._p(it), it.shape[0], ops._p(out),
+            ops._stream(self.device)), "debug separations")
+        return out.view(-1, 32, 32)
 
     def per_term_gradients(self, coords: torch.Tensor):
         dvdx = torch.empty((3, self.NA, 3), dtype=self.dtype, device=self.device)
