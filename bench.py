@@ -41,11 +41,10 @@ def main():
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--cpu-stride", type=int, default=4)
     ap.add_argument("--kernel-reps", type=int, default=10)
-    ap.add_argument("--autotune", action="store_true",
-                    help="run the autotuner over persistent grid sizes first (default: the "
-                         "grid it picks on MI355X for this batch, one workgroup per resident "
-                         "slot; its candidate launches would otherwise enter a kernel trace "
-                         "of this command)")
+    ap.add_argument("--no-autotune", action="store_true",
+                    help="skip the autotuner (waves per SIMD, persistent grid, rows per queue "
+                         "grab) and use the library's default launch, e.g. so that a kernel "
+                         "trace of this command holds only the measured launches")
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--no-pruner", action="store_true",
                     help="skip the learned-pruner leg (1024 x 300-residue batch)")
@@ -82,13 +81,14 @@ def main():
     log("stack: %d poses, %d blocks, %d atoms" % (stack.n_poses, stack.n_blocks_total, stack.n_atoms_total))
 
     tune = None
-    if args.autotune:
-        # the project's autotuner picks the persistent grid (workgroups of 4
-        # waves) by timing the energy + gradient step on this batch
+    if not args.no_autotune:
+        # the project's autotuner picks the scoring kernel's waves per SIMD,
+        # the persistent grid and the rows per queue grab by timing the
+        # energy + gradient step on this batch
         tune = scorer.autotune(coords, weights, cache=False)
-        log("autotune: %s" % json.dumps(tune))
+        log("autotune: %s" % json.dumps(tune["choice"]))
     else:
-        scorer.wave_groups = args.wave_groups  # 0: one workgroup per resident slot
+        scorer.set_launch(wave_groups=args.wave_groups)  # 0: one workgroup per resident slot
     scorer.neighbors(coords)
     n_items = scorer.check_capacity()
     census = scorer.census(coords)
@@ -173,8 +173,10 @@ def main():
             "atom_pair_evals_per_step": pair_evals,
             "atom_pair_evals_per_s": pair_evals / (ms_per_step * 1e-3),
             "graph": use_graph,
-            "wave_groups": scorer.wave_groups,
-            "autotune": ({str(k): round(v * 1e3, 4) for k, v in tune["timings_s"].items()}
+            "launch": {"waves_per_simd": scorer.waves_per_simd, "wave_groups": scorer.wave_groups,
+                       "row_grab": scorer.row_grab},
+            "autotune": ({"choice": tune["choice"],
+                          "timings_ms": {k: round(v * 1e3, 4) for k, v in tune["timings_s"].items()}}
                          if tune else None),
             "latency_ms_p50": float(numpy.percentile(lat, 50)),
             "latency_ms_p99": float(numpy.percentile(lat, 99)),

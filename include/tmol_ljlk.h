@@ -45,6 +45,13 @@ extern "C" {
 #define TMOL_LJLK_NO_ENERGY 16 /* skip the energy output (backward re-walk) */
 #define TMOL_LJLK_ACCUMULATE 32 /* add into dvdx / grad instead of overwriting */
 #define TMOL_LJLK_GENERAL_KERNEL 128 /* force the general (any block size) kernel */
+/* launch variants of the scoring kernel, chosen by an autotuner (float32
+ * whole-pose energy and weighted-gradient modes; other modes keep their
+ * defaults): waves per SIMD the kernel is compiled for (3, 4 or 5; 0 = the
+ * mode's default, 5 energy-only, 4 with gradients) and rows of the work list
+ * taken per queue grab (1..8; 0 = 1) */
+#define TMOL_LJLK_WAVES_PER_SIMD(n) ((n) << 16)
+#define TMOL_LJLK_ROW_GRAB(n) ((n) << 20)
 
 /* Per block-type tables: the PackedBlockTypes annotations the reference op
  * takes (ljlk_energy_term.py:163-178).  Maximum 32 atoms per tile. */
@@ -91,10 +98,10 @@ typedef struct {
  * the padded inter_block_bondsep, pose_stack_builder.py:1271-1349): rows
  * bonded_start[g] .. bonded_start[g + 1] - 1 of `bonded` list the blocks
  * g2 > g of g's pose whose minimum connection-point separation is at most
- * 4, sorted by g2.  Row: {g2, min separation, 0, 0, sep[0], sep[1], sep[2],
+ * 5, sorted by g2.  Row: {g2, min separation, 0, 0, sep[0], sep[1], sep[2],
  * sep[3]}, where byte c2 of sep[c1] is the bond separation (capped at 6)
  * between connection c1 of g and connection c2 of g2.  Absent pairs are 6
- * bonds or more apart. */
+ * bonds or more apart (count-pair weight 1, reported separation 6). */
 #define TMOL_LJLK_BONDED_WIDTH 8
 
 /* Neighbor-pass tiles: 64 x 64 block tiles of each pose's upper triangle.

@@ -70,6 +70,17 @@ int oracle_pair_sep(const oracle_stack* st, int p, int b1, int b2, int a1,
   return inter_sep_f(st, p, b1, b2, bt1, bt2, a1, a2);
 }
 
+/* Separations of every atom pair of block pair (p, b1, b2) into out
+ * [n1, n2] (row-major; n1, n2 = the blocks' atom counts). */
+void oracle_block_pair_seps(const oracle_stack* st, int p, int b1, int b2, int32_t* out) {
+  int B = st->max_n_blocks;
+  int bt1 = st->block_type[p * B + b1], bt2 = st->block_type[p * B + b2];
+  if (bt1 < 0 || bt2 < 0) return;
+  int n1 = st->bt_n_atoms[bt1], n2 = st->bt_n_atoms[bt2];
+  for (int a1 = 0; a1 < n1; ++a1)
+    for (int a2 = 0; a2 < n2; ++a2) out[a1 * n2 + a2] = oracle_pair_sep(st, p, b1, b2, a1, a2);
+}
+
 /* Rotamer-pair dispatch list from block neighbor flags
  * (rot_neighbor_indices_from_block_neighbors, sphere_overlap.impl.hh:516-610):
  * cells (p, b1, b2) in row-major order; a flagged diagonal cell contributes

@@ -102,6 +102,9 @@ def lib():
             ctypes.c_int, ctypes.c_int,
         ]
         L.oracle_pair_sep.restype = ctypes.c_int
+        L.oracle_block_pair_seps.argtypes = [ctypes.c_void_p, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, ctypes.c_void_p]
+        L.oracle_block_pair_seps.restype = None
         _lib = L
     return _lib
 
@@ -339,9 +342,7 @@ def oracle_pair_separations(args, items):
         t1, t2 = bt[p * S.B + b1], bt[p * S.B + b2]
         n1, n2 = bt_n[t1], bt_n[t2]
         m = numpy.zeros((n1, n2), dtype=numpy.int32)
-        for a1 in range(n1):
-            for a2 in range(n2):
-                m[a1, a2] = L.oracle_pair_sep(ctypes.byref(S.struct), p, b1, b2, a1, a2)
+        L.oracle_block_pair_seps(ctypes.byref(S.struct), p, b1, b2, _ptr(m, ctypes.c_int32))
         out.append(m)
     return out
 

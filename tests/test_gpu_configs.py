@@ -2,10 +2,17 @@
 (residues uniform in [150, 600], float32) and the single 100-residue pose.
 
 Bars: neighbor flags bit-exact against the oracle; count-pair separations
-bit-exact on a sample of block pairs (all sequence-adjacent ones included);
-per-pose energies within 1e-4 relative of the FP64 run of the same kernels;
-FP64 run against the FP64 oracle at 1e-9; gradients of the drop-in op's
-backward against the FP64 run (1e-4 of the gradient norm per pose)."""
+bit-exact for every block pair within two residues (all pairs whose count
+pair is not 1) and a sample of the others; per-pose energies within 1e-4
+relative of the FP64 run of the same kernels; FP64 run against the FP64
+oracle at 1e-9; gradients of the drop-in op's backward against the FP64 run
+(1e-4 of the gradient norm per pose).
+
+The 512-pose ragged batch (the timed workload, one persistent launch): the
+whole batch in f32 against f64 at 1e-4, and 8 sampled poses (the smallest,
+the largest and 6 seeded) against the f64 oracle -- energies per term at
+1e-9, the weighted gradient at 1e-9 of its norm, and every block pair within
+two residues bit-exact in count pair."""
 
 import os
 import sys
@@ -85,10 +92,13 @@ def test_workload_parity(workload):
     assert overflow == 0
     B = ps32.max_n_blocks
     it = items.numpy()
-    near = numpy.abs((it[:, 1] % B) - (it[:, 0] % B)) <= 1
+    # every pair within two residues (min bond separation <= 4 happens only
+    # there on linear chains) and 2000 others
+    near = numpy.abs((it[:, 1] % B) - (it[:, 0] % B)) <= 2
     rng = numpy.random.default_rng(0)
     far = rng.choice(numpy.nonzero(~near)[0], size=min(2000, int((~near).sum())), replace=False)
-    sel = numpy.concatenate([numpy.nonzero(near)[0][:4000], far])
+    sel = numpy.concatenate([numpy.nonzero(near)[0], far])
+    _say("count pair: %d block pairs (%d within two residues)" % (len(sel), int(near.sum())))
     sub = torch.from_numpy(it[sel])
     seps = debug_pair_separations(a32, sub).cpu().numpy()
     trip = [(int(g1) // B, int(g1) % B, int(g2) % B) for g1, g2 in it[sel]]
@@ -96,3 +106,66 @@ def test_workload_parity(workload):
     for k, m in enumerate(ref):
         n1, n2 = m.shape
         assert (seps[k, :n1, :n2] == m).all(), trip[k]
+
+
+def test_headline_batch_sampled_poses():
+    from tmol_amd.pose import RaggedPoseStack
+    from tmol_amd.scorer import LJLKRaggedScorer
+
+    poses = workload_poses("ragged_512x50_1200")
+    P = len(poses)
+    _say("ragged_512x50_1200: %d poses" % P)
+    term = LJLKEnergyTerm(device=DEV)
+    res = {}
+    for dt in (torch.float32, torch.float64):
+        stack = RaggedPoseStack(poses, dtype=dt)
+        sc = LJLKRaggedScorer(stack, term, device=DEV, dtype=dt)
+        e, g = sc.energy_and_grad(stack.coords.to(DEV), W.to(dt))
+        res[dt] = (stack, sc, e.double().cpu(), g.double().cpu())
+    stack, sc32, e32, g32 = res[torch.float32]
+    _, _, e64, g64 = res[torch.float64]
+    rel = ((e32 - e64).abs() / e64.abs().clamp(min=1.0)).max()
+    _say("whole batch: max rel f32 vs f64 %.3g" % float(rel))
+    assert float(rel) < 1e-4
+    sizes = numpy.array([p.n_blocks for p in poses])
+    rng = numpy.random.default_rng(512)
+    ends = {int(sizes.argmin()), int(sizes.argmax())}
+    rest = [k for k in range(P) if k not in ends]
+    pick = sorted(ends | set(rng.choice(rest, 6, replace=False).tolist()))
+    assert len(pick) == 8
+    sub = [poses[k] for k in pick]
+    _say("sampled poses %s (residues %s)" % (pick, [p.n_blocks for p in sub]))
+    # per-term energies and weighted gradient of the sampled poses: oracle f64
+    ps, ca = _args(sub, torch.float64, "cpu")
+    s_o, _, dv_o, _ = orc.oracle_pose_scores(*ca, require_grad=True, dtype=numpy.float64)
+    numpy.testing.assert_allclose(e64[:, pick].numpy(), s_o, rtol=1e-9, atol=1e-9)
+    rel32 = numpy.abs(e32[:, pick].numpy() - s_o) / numpy.maximum(numpy.abs(s_o), 1.0)
+    assert rel32.max() < 1e-4
+    A = ps.max_n_atoms
+    pas = stack.pose_atom_start.numpy()
+    for j, k in enumerate(pick):
+        n = poses[k].n_atoms
+        go = (W.numpy()[:, None, None] * dv_o[:, j * A: j * A + n]).sum(0)
+        gk = g64[pas[k]: pas[k] + n].numpy()
+        assert numpy.linalg.norm(gk - go) <= 1e-9 * numpy.linalg.norm(go)
+        gk32 = g32[pas[k]: pas[k] + n].numpy()
+        assert numpy.linalg.norm(gk32 - go) <= 1e-4 * numpy.linalg.norm(go)
+    # count pair: every block pair within two residues of the sampled poses
+    pbs = stack.pose_block_start.numpy()
+    trip, items = [], []
+    for j, k in enumerate(pick):
+        nb = poses[k].n_blocks
+        for b1 in range(nb):
+            for b2 in range(b1, min(b1 + 3, nb)):
+                trip.append((j, b1, b2))
+                items.append((pbs[k] + b1, pbs[k] + b2))
+    seps = sc32.debug_pair_separations(stack.coords.to(DEV),
+                                       torch.tensor(items, dtype=torch.int32)).cpu().numpy()
+    _, ca32 = _args(sub, torch.float32, "cpu")
+    ref = orc.oracle_pair_separations(ca32, trip)
+    n_cp = 0
+    for k, m in enumerate(ref):
+        n1, n2 = m.shape
+        assert (seps[k, :n1, :n2] == m).all(), trip[k]
+        n_cp += int((m <= 4).sum())
+    _say("count pair: %d block pairs, %d atom pairs within 4 bonds" % (len(trip), n_cp))

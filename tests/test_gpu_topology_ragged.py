@@ -4,7 +4,7 @@ head-to-tail cycle, a disulfide, an inter-chain disulfide cross-link, and a
 linear control), against the oracle on the padded stack of the same poses:
 
 * count-pair separations from the kernel's own code, for every block pair
-  within 4 bonds and a sample of the others: bit-exact;
+  within 5 bonds and a sample of the others: bit-exact;
 * per-pose energies and the weighted gradient: f64 within 1e-9 (energies)
   and 1e-8 (gradient), f32 within 1e-4 relative of the f64 oracle;
 * the extra bonds matter: the same poses scored as open chains differ.
@@ -87,7 +87,7 @@ def test_ragged_general_topology_separations_bit_exact():
     trip = []
     for p, pose in enumerate(poses):
         nb = pose.n_blocks
-        near = [(p, b1, b2) for b1 in range(nb) for b2 in range(b1, nb) if mins[p, b1, b2] <= 4]
+        near = [(p, b1, b2) for b1 in range(nb) for b2 in range(b1, nb) if mins[p, b1, b2] <= 5]
         far = [(p, int(b1), int(b2)) for b1, b2 in rng.integers(0, nb, (20, 2)) if b1 <= b2]
         trip += near + far
     items = torch.tensor([[pbs[p] + b1, pbs[p] + b2] for p, b1, b2 in trip], dtype=torch.int32)

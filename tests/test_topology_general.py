@@ -6,7 +6,7 @@ disulfides and inter-chain cross-links (tests/topology_cases.py).
   pinned by breadth-first search over the full atom bond graph and by the
   reference's own APSP op (test_topology_oracle.py);
 * the ragged stack's sparse forward bonded table holds exactly the block
-  pairs within 4 bonds, with the same connection-point separations.
+  pairs within 5 bonds, with the same connection-point separations.
 """
 
 import numpy

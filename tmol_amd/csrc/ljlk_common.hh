@@ -200,7 +200,7 @@ __device__ __forceinline__ int64_t bp_index(const StackView& s, const WorkView& 
 constexpr int BONDED_W = TMOL_LJLK_BONDED_WIDTH;
 
 // ragged: the forward bonded-table row of (lo, hi), lo < hi, or null when the
-// two blocks are more than 4 bonds apart (rows are sorted by g2 and short:
+// two blocks are more than 5 bonds apart (rows are sorted by g2 and short:
 // a handful per block)
 __device__ __forceinline__ const int32_t* bonded_row(const StackView& s, int lo, int hi) {
   const int k1 = s.bonded_start[lo + 1];
@@ -332,6 +332,7 @@ struct ScoreArgs {
   const Real* global_params;
   const uint32_t* counters;
   uint32_t* row_cursor;        // [8 * QSTRIDE] RowQueue cursors (workspace)
+  int row_grab;                // rows per queue grab (0: QROWS)
   const Real* weights;         // [3, P] or [3, P, B, B]
   double* energies64;          // [3, P] whole pose
   Real* energies_bp;           // [3, P, B, B] block pair
@@ -372,6 +373,8 @@ __device__ __forceinline__ GlobalParams<Real> load_global_params(const Real* g) 
 // work, so larger grabs lengthen the tail) and each cursor sits on its own
 // 256-byte line (same-line L2 atomics serialize).  The cursors are zeroed by
 // a fill kernel before every scoring launch.
+// The rows taken per cursor grab (`grab`, default 1) is a launch parameter the
+// autotuner may raise.
 constexpr int QROWS = 1;
 constexpr int QSTRIDE = 64;  // uint32 between cursors
 
@@ -379,20 +382,21 @@ struct RowQueue {
   uint32_t* cursor;
   int64_t n;
   int64_t cur, end;
-  int xcd, left;
-  __device__ RowQueue(uint32_t* c, int64_t n_rows)
-      : cursor(c), n(n_rows), cur(0), end(0), xcd(int(blockIdx.x & 7)), left(8) {}
+  int xcd, left, grab;
+  __device__ RowQueue(uint32_t* c, int64_t n_rows, int rows_per_grab = QROWS)
+      : cursor(c), n(n_rows), cur(0), end(0), xcd(int(blockIdx.x & 7)), left(8),
+        grab(rows_per_grab > 0 ? rows_per_grab : QROWS) {}
   // next row of this wave (wave-uniform), -1 when every row is taken
   __device__ int64_t next() {
     if (cur < end) return cur++;
     while (left > 0) {
       const int64_t lo = n * xcd / 8, hi = n * (xcd + 1) / 8;
       uint32_t k = 0;
-      if (lane_id() == 0) k = atomicAdd(&cursor[xcd * QSTRIDE], uint32_t(QROWS));
+      if (lane_id() == 0) k = atomicAdd(&cursor[xcd * QSTRIDE], uint32_t(grab));
       k = __builtin_amdgcn_readfirstlane(k);
       if (lo + int64_t(k) < hi) {
         cur = lo + int64_t(k);
-        end = cur + QROWS < hi ? cur + QROWS : hi;
+        end = cur + grab < hi ? cur + grab : hi;
         return cur++;
       }
       xcd = (xcd + 1) & 7;

@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "../../include/tmol_ljlk.h"
 #include "ljlk_common.hh"
@@ -187,6 +188,27 @@ static void launch_rows(const ScoreArgs<Real>& a, int n_wg, hipStream_t stream) 
   hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
 }
 
+// the rows kernel compiled for `occ` waves per SIMD (0: the mode's default;
+// float only -- other dtypes take the default)
+template <typename Real, int MODE>
+static void launch_rows_occ(const ScoreArgs<Real>& a, int occ, int n_wg, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
+                     8 * QSTRIDE, 0u);
+  auto go = [&](auto k) {
+    if (n_wg <= 0) n_wg = resident_workgroups(k);
+    hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  };
+  if constexpr (std::is_same<Real, float>::value) {
+    switch (occ) {
+      case 3: go(score_rows_kernel<Real, MODE, 3>); return;
+      case 4: go(score_rows_kernel<Real, MODE, 4>); return;
+      case 5: go(score_rows_kernel<Real, MODE, 5>); return;
+      default: break;
+    }
+  }
+  go(score_rows_kernel<Real, MODE>);
+}
+
 // ek != nullptr: elec is scored in the same pass (M_ELEC; tables prepared in
 // the workspace by tmol_ljlk_elec_prepare), energies and weights [4, P]
 template <typename Real>
@@ -229,7 +251,18 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
   const int n_wg = n_wave_groups;
   int mode = flags & (TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY);
   if (ek) mode |= M_ELEC;
+  // launch variants (autotuner): waves per SIMD and rows per queue grab
+  const int occ = (flags >> 16) & 15;
+  a.row_grab = (flags >> 20) & 15;
+  if (occ && !general && !ek && (mode == 0 || mode == M_WGRAD)) {
+    if (mode == 0)
+      launch_rows_occ<Real, 0>(a, occ, n_wg, stream);
+    else
+      launch_rows_occ<Real, M_WGRAD>(a, occ, n_wg, stream);
+    mode = -1;  // launched
+  }
   switch (mode) {
+    case -1: break;
     case M_ELEC: launch_rows<Real, M_ELEC>(a, n_wave_groups, stream); break;
     case M_ELEC | M_WGRAD: launch_rows<Real, M_ELEC | M_WGRAD>(a, n_wave_groups, stream); break;
     case M_ELEC | M_WGRAD | M_NOE:
@@ -725,6 +758,11 @@ int tmol_ljlk_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
   if ((flags & TMOL_LJLK_DVDX) && (flags & TMOL_LJLK_WGRAD)) return TMOL_LJLK_EINVAL;
   if ((flags & TMOL_LJLK_BLOCK_PAIR) && (flags & TMOL_LJLK_DVDX)) return TMOL_LJLK_ENOTSUP;
   if (n_wave_groups < 0 || n_wave_groups > (1 << 20)) return TMOL_LJLK_EINVAL;
+  {
+    const int occ = (flags >> 16) & 15, grab = (flags >> 20) & 15;
+    if ((occ != 0 && (occ < 3 || occ > 5)) || grab > 8 || (flags >> 24) != 0)
+      return TMOL_LJLK_EINVAL;
+  }
   hipStream_t st = (hipStream_t)stream;
   if (dtype == TMOL_LJLK_F32)
     return run_score<float>(s, bt, coords, workspace, row_capacity, flags, weights, energies,

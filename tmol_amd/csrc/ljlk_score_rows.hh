@@ -378,11 +378,18 @@ __device__ __forceinline__ void stage_atom(RowLDS<Real, MODE>& L, int slot, int 
   }
 }
 
-// Occupancy: 5 waves per SIMD without gradients (the trimmed LDS layout fits
-// 5 workgroups per CU); 4 with them, where a 96-register cap would spill and
-// the LDS pipe, busy with the gradient atomics, gains nothing from more waves.
-template <typename Real, int MODE>
-__global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void score_rows_kernel(
+// Occupancy: by default 5 waves per SIMD without gradients (the trimmed LDS
+// layout fits 5 workgroups per CU); 4 with them, where a 96-register cap would
+// spill and the LDS pipe, busy with the gradient atomics, gains nothing from
+// more waves.  OCC > 0 compiles a variant for OCC waves per SIMD (the
+// register budget follows), one of the autotuner's choices.
+template <int MODE, int OCC>
+constexpr int rows_waves_per_simd() {
+  return OCC > 0 ? OCC : ((MODE & (M_DVDX | M_WGRAD)) ? 4 : 5);
+}
+
+template <typename Real, int MODE, int OCC = 0>
+__global__ __launch_bounds__(256, (rows_waves_per_simd<MODE, OCC>())) void score_rows_kernel(
     ScoreArgs<Real> A) {
   using C = RowCtx<Real, MODE>;
   using V = typename V4<Real>::T;
@@ -392,7 +399,7 @@ __global__ __launch_bounds__(256, (MODE & (M_DVDX | M_WGRAD)) ? 4 : 5) void scor
   const GlobalParams<Real> gp = load_global_params(A.global_params);
   const StackView& s = A.s;
   const BTView& bt = A.bt;
-  RowQueue queue(A.row_cursor, s.G);
+  RowQueue queue(A.row_cursor, s.G, A.row_grab);
   const Real cull_pad = gp.max_dis + Real(1e-3);
   const ElecConst<Real> ek = A.ek;
 

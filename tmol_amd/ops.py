@@ -33,6 +33,7 @@ OK, EINVAL, EOVERFLOW, EHIP, ENOTSUP = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 PADDED, RAGGED = 0, 1
 DVDX, WGRAD, BLOCK_PAIR, NO_ENERGY, ACCUMULATE, GENERAL_KERNEL = 2, 4, 8, 16, 32, 128
+WAVES_PER_SIMD, ROW_GRAB = 1 << 16, 1 << 20  # multipliers of the launch-variant fields
 NBR_TILE = 64
 
 _ERRORS = {

@@ -298,48 +298,82 @@ def inter_block_bondsep_of(pose: Pose, C: int) -> numpy.ndarray:
 
 
 # Forward bonded table of a ragged stack: for block g, the blocks g2 > g of
-# its pose whose minimum connection-point separation is at most 4 (the only
-# pairs whose count pair needs the separations), sorted by g2.  One entry is
+# its pose whose minimum connection-point separation is at most 5 (pairs at 4
+# or less need the separations for their count pair; at 5 the reference
+# reports the minimum separation itself), sorted by g2.  One entry is
 # int32[BONDED_WIDTH]: g2, the minimum separation, 0, 0, then for c1 = 0..3
 # the separations to c2 = 0..3 as bytes (byte c2 of word 4 + c1; 6 where no
 # path is shorter).
 BONDED_WIDTH = 8
-BONDED_MAX_SEP = 4
+BONDED_MAX_SEP = 5
 
 
 def forward_bonded_entries(pose: Pose, C: int):
-    """(b1, b2, seps [C, C]) of every block pair b1 < b2 of the pose whose
-    minimum connection separation is <= 4, ordered by (b1, b2)."""
+    """Block pairs b1 < b2 of the pose whose minimum connection separation is
+    <= 5, ordered by (b1, b2): arrays b1 [n], b2 [n], seps [n, C, C]."""
     nb = pose.n_blocks
-    entries = []
     if pose.is_linear:
         pos, seg = pose.conn_positions()
         cb = pos.shape[1]
-        # chain positions grow by at least 1 per block: a pair more than 4
-        # blocks apart is more than 4 bonds apart
-        for k in range(1, 5):
-            if k >= nb:
-                break
-            sep = chain_bondsep(pos[:-k, :, None], seg[:-k, None, None], pos[k:, None, :],
-                                seg[k:, None, None])
-            full = numpy.full((nb - k, C, C), MAX_SIG_BOND_SEPARATION, dtype=numpy.int32)
-            full[:, :cb, :cb] = sep
-            for b1 in numpy.nonzero(full.reshape(nb - k, -1).min(1) <= BONDED_MAX_SEP)[0]:
-                entries.append((int(b1), int(b1) + k, full[b1]))
-        entries.sort(key=lambda e: (e[0], e[1]))
-        return entries
-    seps = {}
+        b1s, b2s, seps = [], [], []
+        # chain positions grow by at least 1 per block: a pair more than 5
+        # blocks apart is more than 5 bonds apart
+        for k in range(1, min(BONDED_MAX_SEP + 1, nb)):
+            sep = numpy.full((nb - k, C, C), MAX_SIG_BOND_SEPARATION, dtype=numpy.int32)
+            sep[:, :cb, :cb] = chain_bondsep(pos[:-k, :, None], seg[:-k, None, None],
+                                             pos[k:, None, :], seg[k:, None, None])
+            keep = numpy.nonzero(sep.reshape(nb - k, -1).min(1) <= BONDED_MAX_SEP)[0]
+            b1s.append(keep)
+            b2s.append(keep + k)
+            seps.append(sep[keep])
+        if not b1s:
+            return (numpy.zeros(0, numpy.int64), numpy.zeros(0, numpy.int64),
+                    numpy.zeros((0, C, C), numpy.int32))
+        b1, b2, sep = numpy.concatenate(b1s), numpy.concatenate(b2s), numpy.concatenate(seps)
+        order = numpy.lexsort((b2, b1))
+        return b1[order], b2[order], sep[order]
+    table = {}
     for (b1, c1), dists in connection_separations(pose).items():
         for (b2, c2), d in dists.items():
             if b2 > b1:
-                m = seps.setdefault((b1, b2), numpy.full((C, C), MAX_SIG_BOND_SEPARATION,
-                                                        dtype=numpy.int32))
+                m = table.setdefault((b1, b2), numpy.full((C, C), MAX_SIG_BOND_SEPARATION,
+                                                         dtype=numpy.int32))
                 m[c1, c2] = d
-    for (b1, b2) in sorted(seps):
-        m = seps[(b1, b2)]
-        if m.min() <= BONDED_MAX_SEP:
-            entries.append((b1, b2, m))
-    return entries
+    keys = [k for k in sorted(table) if table[k].min() <= BONDED_MAX_SEP]
+    return (numpy.array([k[0] for k in keys], dtype=numpy.int64),
+            numpy.array([k[1] for k in keys], dtype=numpy.int64),
+            numpy.array([table[k] for k in keys], dtype=numpy.int32).reshape(-1, C, C))
+
+
+def bonded_table(poses: Sequence[Pose], pose_block_start: numpy.ndarray, C: int):
+    """(bonded_start [G + 1], bonded [n, BONDED_WIDTH]) int32 of a ragged
+    stack: the forward bonded table in the layout the kernels read."""
+    G = int(pose_block_start[-1])
+    g1s, rows = [], []
+    for p, pose in enumerate(poses):
+        b1, b2, sep = forward_bonded_entries(pose, C)
+        if b1.size == 0:
+            continue
+        g0 = int(pose_block_start[p])
+        n = b1.size
+        sp = numpy.full((n, 4, 4), MAX_SIG_BOND_SEPARATION, dtype=numpy.int64)
+        sp[:, :C, :C] = sep
+        row = numpy.zeros((n, BONDED_WIDTH), dtype=numpy.int64)
+        row[:, 0] = g0 + b2
+        row[:, 1] = sep.reshape(n, -1).min(1)
+        row[:, 4:8] = (sp << (8 * numpy.arange(4))[None, None, :]).sum(2)
+        g1s.append(g0 + b1)
+        rows.append(row)
+    if rows:
+        g1 = numpy.concatenate(g1s)
+        bonded = numpy.concatenate(rows)
+    else:
+        g1 = numpy.zeros(0, dtype=numpy.int64)
+        bonded = numpy.zeros((1, BONDED_WIDTH), dtype=numpy.int64)
+    counts = numpy.bincount(g1, minlength=G)
+    start = numpy.concatenate([[0], numpy.cumsum(counts)]).astype(numpy.int32)
+    # words 4..7 hold four bytes each: keep their bit patterns in int32
+    return start, bonded.astype(numpy.uint32).view(numpy.int32)
 
 
 class PoseStack:
@@ -459,24 +493,12 @@ class RaggedPoseStack:
         block_atom_offset = numpy.zeros(G, dtype=numpy.int32)
         block_type = numpy.zeros(G, dtype=numpy.int32)
         block_pose = numpy.zeros(G, dtype=numpy.int32)
-        bonded_count = numpy.zeros(G, dtype=numpy.int64)
-        bonded_rows = []
         coords = numpy.zeros((NA, 3), dtype=numpy.float64)
         atom_pose = numpy.zeros(NA, dtype=numpy.int32)
         g = 0
         for p, pose in enumerate(poses):
             off = int(pose_atom_start[p])
             atom_pose[off : off + pose.n_atoms] = p
-            g0 = g
-            for b1, b2, sep in forward_bonded_entries(pose, C):
-                row = numpy.zeros(BONDED_WIDTH, dtype=numpy.int64)
-                row[0] = g0 + b2
-                row[1] = sep.min()
-                sp = numpy.full((4, 4), MAX_SIG_BOND_SEPARATION, dtype=numpy.int64)
-                sp[:C, :C] = sep
-                row[4:8] = (sp * (256 ** numpy.arange(4))[None, :]).sum(1)
-                bonded_rows.append(row)
-                bonded_count[g0 + b1] += 1
             for b, rt in enumerate(pose.residue_types):
                 block_atom_offset[g] = off
                 block_type[g] = pbt.index[rt.name]
@@ -484,13 +506,7 @@ class RaggedPoseStack:
                 coords[off : off + rt.n_atoms] = pose.coords[b]
                 off += rt.n_atoms
                 g += 1
-        bonded_start = numpy.concatenate([[0], numpy.cumsum(bonded_count)]).astype(numpy.int32)
-        bonded = (numpy.stack(bonded_rows) if bonded_rows
-                  else numpy.zeros((1, BONDED_WIDTH), dtype=numpy.int64))
-        # words 4..7 hold four bytes each: keep their bit patterns in int32
-        bonded = bonded.astype(numpy.uint32).view(numpy.int32) if bonded.max() < 2**32 else None
-        if bonded is None:
-            raise ValueError("bonded table overflow")
+        bonded_start, bonded = bonded_table(poses, pose_block_start, C)
         if NA >= 2**31:
             raise ValueError("ragged stack exceeds 2^31 atoms")
 

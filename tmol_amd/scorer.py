@@ -94,7 +94,7 @@ class LJLKRaggedScorer:
         self.ws = ops.Workspace(self.cstack, self.bt.c, self.capacity, self.dcode, dev)
         self.energies = torch.zeros((3, self.P), dtype=dtype, device=dev)
         self.grad = torch.zeros((self.NA, 3), dtype=dtype, device=dev)
-        self.wave_groups = wave_groups or 0
+        self.set_launch(wave_groups=wave_groups or 0)
         self.graph = None
         self._graph_coords = None
         self._graph_weights = None
@@ -108,6 +108,7 @@ class LJLKRaggedScorer:
             self.ws.nbytes, self.ws.capacity, None, ops._stream(self.device)), "neighbors")
 
     def score(self, coords, flags, weights=None, energies=None, dvdx=None, grad=None):
+        flags = flags | self.launch_flags()
         ops._check(ops.lib().tmol_ljlk_score(
             ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ops._p(coords), self.dcode,
             ops._p(self.ws.buf), self.ws.nbytes, self.ws.capacity, flags, ops._p(weights),
@@ -218,12 +219,36 @@ class LJLKRaggedScorer:
         return census_dict(out.cpu().tolist())
 
     # -- autotuner -------------------------------------------------------------
-    def autotune(self, coords, weights=None, candidates=None, reps=5, cache=True):
-        """Pick the persistent grid size (workgroups of 4 waves; 0 = resident
-        slots) by timing the fused step; results cached per device and
-        workload signature."""
-        key = "%s|G%d|NA%d|P%d|%s" % (torch.cuda.get_device_name(self.device), self.G,
-                                      self.NA, self.P, self.dtype)
+    def launch_flags(self) -> int:
+        """Score-flag bits of the chosen launch variant (waves per SIMD the
+        scoring kernel is compiled for, rows per queue grab)."""
+        return (ops.WAVES_PER_SIMD * int(self.waves_per_simd)
+                | ops.ROW_GRAB * int(self.row_grab))
+
+    def _time_step(self, coords, w, reps):
+        ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
+              for _ in range(reps)]
+        stream = torch.cuda.current_stream(self.device)
+        self.energy_and_grad(coords, w)
+        torch.cuda.synchronize(self.device)
+        for a, b in ev:
+            a.record(stream)
+            self.energy_and_grad(coords, w)
+            b.record(stream)
+        torch.cuda.synchronize(self.device)
+        t = sorted(a.elapsed_time(b) for a, b in ev)
+        return 1e-3 * t[len(t) // 2]
+
+    def autotune(self, coords, weights=None, reps=5, cache=True, log=None):
+        """Choose the launch of the energy + gradient step by timing it on
+        this batch (HIP events, median of ``reps``): the scoring kernel's
+        waves per SIMD (compiled variants 3, 4, 5), the persistent grid
+        (workgroups of 4 waves; 0 = one per resident slot) and the rows taken
+        per queue grab, searched one after the other from the library's
+        defaults.  Results are cached per device and workload signature in
+        the user's cache directory."""
+        key = "%s|G%d|NA%d|P%d|%s|v2" % (torch.cuda.get_device_name(self.device), self.G,
+                                         self.NA, self.P, self.dtype)
         table = {}
         path = _tune_cache_path()
         if cache and os.path.exists(path):
@@ -232,30 +257,34 @@ class LJLKRaggedScorer:
             except (OSError, ValueError):
                 table = {}
         if key in table:
-            self.wave_groups = int(table[key]["wave_groups"])
+            self.set_launch(**table[key]["choice"])
             return table[key]
         cu = ctypes.c_int32(0)
         ops._check(ops.lib().tmol_ljlk_device_cu_count(ctypes.byref(cu)), "cu count")
         n_cu = max(int(cu.value), 1)
-        if candidates is None:
-            # 0 = one workgroup per resident slot (the library's default)
-            candidates = [0] + [n_cu * k for k in (1, 2, 3, 4, 6, 8)]
         w = self._weights(weights)
-        results = {}
-        for c in candidates:
-            self.wave_groups = int(c)
-            self.energy_and_grad(coords, w)
-            torch.cuda.synchronize(self.device)
-            ts = []
-            for _ in range(reps):
-                t0 = time.perf_counter()
-                self.energy_and_grad(coords, w)
-                torch.cuda.synchronize(self.device)
-                ts.append(time.perf_counter() - t0)
-            results[int(c)] = min(ts)
-        best = min(results, key=results.get)
-        self.wave_groups = best
-        entry = {"wave_groups": best, "cu": n_cu, "timings_s": results}
+        choice = {"waves_per_simd": 0, "wave_groups": 0, "row_grab": 0}
+        timings = {}
+
+        def run(**kw):
+            c = dict(choice, **kw)
+            self.set_launch(**c)
+            t = self._time_step(coords, w, reps)
+            timings["simd%d|wg%d|grab%d" % (c["waves_per_simd"], c["wave_groups"],
+                                            c["row_grab"])] = t
+            if log:
+                log("autotune %s: %.4f ms" % (c, 1e3 * t))
+            return t
+
+        stages = [("waves_per_simd", [0, 3, 4, 5] if self.dtype == torch.float32 else [0]),
+                  ("wave_groups", [0] + [n_cu * k for k in (2, 4, 8)]),
+                  ("row_grab", [0, 2, 4])]
+        for name, values in stages:
+            best = min(values, key=lambda v: run(**{name: v}))
+            choice[name] = int(best)
+        self.set_launch(**choice)
+        entry = {"choice": choice, "cu": n_cu, "timings_s": timings,
+                 "best_s": min(timings.values())}
         if cache:
             table[key] = entry
             try:
@@ -265,6 +294,11 @@ class LJLKRaggedScorer:
             except OSError:
                 pass
         return entry
+
+    def set_launch(self, waves_per_simd=0, wave_groups=0, row_grab=0):
+        self.waves_per_simd = int(waves_per_simd)
+        self.wave_groups = int(wave_groups)
+        self.row_grab = int(row_grab)
 
 
 class LJLKElecRaggedScorer(LJLKRaggedScorer):
