@@ -27,8 +27,105 @@ FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
 
 
+# the compute unit the scoring kernel's arithmetic runs on (no MFMA in it)
+SCORE_KERNEL_UNIT = "valu"
+
+
 def log(msg):
     print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def _load_profile(name):
+    """A committed rocprofv3 summary under profiles/ (JSON), or None."""
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def f64_parity(stack, coords, weights, e32, g32, dev):
+    """The timed batch once more through the FP64 build of the same kernels
+    (same input coordinates, exactly representable in f64): the largest
+    relative per-pose, per-term energy error of the f32 run and the largest
+    per-pose relative gradient error (norm over the pose's atoms)."""
+    import torch
+
+    from tmol_amd.scorer import LJLKRaggedScorer
+    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm
+
+    sc = LJLKRaggedScorer(stack, LJLKEnergyTerm(device=dev), device=dev, dtype=torch.float64)
+    e64, g64 = sc.energy_and_grad(coords.double(), weights.double())
+    e64, g64 = e64.double(), g64.double()
+    rel_e = ((e32 - e64).abs() / e64.abs().clamp(min=1.0)).max(dim=1).values
+    pose = sc.keep["atom_pose"].long()
+    P = stack.n_poses
+    d2 = torch.zeros(P, dtype=torch.float64, device=dev).index_add_(
+        0, pose, ((g32 - g64) ** 2).sum(1))
+    n2 = torch.zeros(P, dtype=torch.float64, device=dev).index_add_(0, pose, (g64 ** 2).sum(1))
+    rel_g = (d2.sqrt() / n2.sqrt().clamp(min=1e-30)).max()
+    del sc
+    torch.cuda.empty_cache()
+    return {"max_rel_f32_vs_f64": float(rel_e.max()),
+            "max_rel_f32_vs_f64_per_term": [float(v) for v in rel_e.tolist()],
+            "max_rel_grad_f32_vs_f64_per_pose": float(rel_g),
+            "bound": 1e-4, "within_bound": bool(float(rel_e.max()) < 1e-4 and float(rel_g) < 1e-4),
+            "reference_run": "the same kernels in float64 on the timed batch's coordinates"}
+
+
+def padded_dropin(dev, steps=10, warmup=3):
+    """The drop-in op at its own configuration: torch.ops.tmol_ljlk.
+    ljlk_pose_scores, forward + backward of the beta2016-weighted total, on
+    the 64-pose padded stack (residues uniform in [150, 600], float32), each
+    call as a caller makes it (block-type tables, workspace and work list set
+    up per call).  Reports the synchronised wall time per call, the host time
+    until the call returns (table set-up and launches), and the stream time
+    between events around the call."""
+    import torch
+
+    import tmol_amd.ops  # noqa: F401  (registers torch.ops.tmol_ljlk)
+    from tmol_amd.generator import workload_poses
+    from tmol_amd.pose import PoseStack
+    from tmol_amd.score.ljlk_energy_term import LJLKEnergyTerm, pose_op_args
+
+    poses = workload_poses("padded_64x150_600")
+    ps = PoseStack(poses, device=dev)
+    args = pose_op_args(ps, LJLKEnergyTerm(device=dev))
+    op = torch.ops.tmol_ljlk.ljlk_pose_scores
+    w = torch.tensor([1.0, 0.55, 1.0], device=dev)[:, None]
+    stream = torch.cuda.current_stream(dev)
+
+    def call():
+        c = args[0].detach().requires_grad_(True)
+        s, _ = op(c, *args[1:])
+        (w * s).sum().backward()
+        return c.grad
+
+    for _ in range(warmup):
+        call()
+    torch.cuda.synchronize(dev)
+    wall, host, gpu = [], [], []
+    for _ in range(steps):
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        t0 = time.perf_counter()
+        a.record(stream)
+        call()
+        b.record(stream)
+        t1 = time.perf_counter()
+        torch.cuda.synchronize(dev)
+        t2 = time.perf_counter()
+        wall.append(1e3 * (t2 - t0))
+        host.append(1e3 * (t1 - t0))
+        gpu.append(a.elapsed_time(b))
+    import numpy
+
+    med = lambda v: float(numpy.median(v))  # noqa: E731
+    P = ps.n_poses
+    return {"workload": "padded_64x150_600", "poses": P, "max_blocks": ps.max_n_blocks,
+            "op": "torch.ops.tmol_ljlk.ljlk_pose_scores (forward + backward)",
+            "ms_per_call_wall": med(wall), "ms_per_call_host": med(host),
+            "ms_per_call_stream": med(gpu), "poses_per_s": P / (med(wall) * 1e-3)}
 
 
 def main():
@@ -52,6 +149,8 @@ def main():
                     help="skip the rotamer-pair scoring leg (16 x 300-residue repacking)")
     ap.add_argument("--no-elec", action="store_true",
                     help="skip the elec leg (same 512-pose batch, elec energy + gradient)")
+    ap.add_argument("--no-padded", action="store_true",
+                    help="skip the padded drop-in op leg (64 poses, per-call set-up included)")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only; --gpus %d is not supported. Poses are "
@@ -141,15 +240,16 @@ def main():
     e = scorer.energies.double().cpu()
     finite = bool(torch.isfinite(e).all())
 
+    e32 = scorer.energies.double().clone()
+    g32 = scorer.grad.double().clone()
+    parity = f64_parity(stack, coords, weights, e32, g32, dev)
+    log("parity: %s" % json.dumps(parity))
+
     achieved_tflops = flops / (score_ms * 1e-3) / 1e12
     abytes = algorithmic_bytes(stack.n_atoms_total, stack.n_blocks_total, stack.n_poses, n_items)
-    traffic = None
-    prof = os.path.join(ROOT, "profiles", "score_kernel_ragged512_traffic.json")
-    if os.path.exists(prof):
-        try:
-            traffic = json.load(open(prof)).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+    traffic = _load_profile("score_kernel_ragged512_traffic.json")
+    counters = _load_profile("score_kernel_ragged512_counters.json")
+    hbm_bytes = traffic.get("hbm_bytes_per_launch") if traffic else None
 
     result = {
         "metric": METRIC,
@@ -185,16 +285,27 @@ def main():
             "energies_finite": finite,
         },
         "roofline": {
-            "bound": "mfma",
+            # the scoring kernel's arithmetic is FP32 on the vector ALU (no
+            # matrix instructions): it is priced against the FP32 vector peak
+            "bound": SCORE_KERNEL_UNIT,
             "achieved": achieved_tflops,
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
-            "traffic": traffic,
-            "kernel": "score_kernel<float, WGRAD> (FP32 vector ALU; peak = MI355X FP32 vector)",
+            "traffic": hbm_bytes,
+            "kernel": "score_rows_kernel<float, WGRAD> (FP32 vector ALU; peak = MI355X FP32 vector)",
             "algorithmic_flops_per_launch": flops,
             "algorithmic_bytes_per_launch": abytes["score"],
+            "hbm_frac": (hbm_bytes / (score_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
+                         if hbm_bytes else None),
+            "hbm_frac_algorithmic": abytes["score"] / (score_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
+            "traffic_profile": ({k: traffic.get(k) for k in (
+                "commit", "hbm_read_bytes_per_launch", "hbm_write_bytes_per_launch")}
+                if traffic else None),
+            "counters": counters.get("derived") if counters else None,
+            "counters_commit": counters.get("commit") if counters else None,
         },
+        "parity": parity,
     }
     if not args.no_pruner:
         from tmol_amd import prune_bench
@@ -222,6 +333,9 @@ def main():
         result["elec"] = elec_bench.run(dev, poses=poses,
                                         cpu=None if args.no_cpu else bench_cpu.elec_poses, log=log)
         result["elec"].pop("census", None)
+    if not args.no_padded:
+        log("padded drop-in op ...")
+        result["padded"] = padded_dropin(dev)
     from tmol_amd import topology_bench
 
     log("topology ...")
@@ -230,7 +344,8 @@ def main():
     if not args.no_cpu:
         log("cpu baseline ...")
         result["cpu_baseline"] = bench_cpu.ljlk_poses(args.workload, poses, args.cpu_stride)
-        result["cpu_baseline"]["all_cores"] = bench_cpu.ljlk_poses_all_cores(poses, args.cpu_stride)
+        result["cpu_baseline"]["all_cores"] = bench_cpu.ljlk_poses_all_cores(
+            args.workload, poses, args.cpu_stride)
     print(json.dumps(result), flush=True)
 
 

@@ -37,6 +37,15 @@ def _cpu(args):
     return [a.detach().cpu().contiguous() if isinstance(a, torch.Tensor) else a for a in args]
 
 
+def _child_env():
+    env = dict(os.environ)
+    env["PYTHONPATH"] = ROOT + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
+    # the child is a CPU process: keep it off any GPU
+    env["HIP_VISIBLE_DEVICES"] = ""
+    env["CUDA_VISIBLE_DEVICES"] = ""
+    return env
+
+
 def _run(job: str, payload: dict, timeout: float = 900.0):
     if not available():
         raise FileNotFoundError(REF_LIB_PATH)
@@ -44,16 +53,33 @@ def _run(job: str, payload: dict, timeout: float = 900.0):
         fin = os.path.join(d, "in.pt")
         fout = os.path.join(d, "out.pt")
         torch.save(dict(payload, job=job), fin)
-        env = dict(os.environ)
-        env["PYTHONPATH"] = ROOT + (os.pathsep + env["PYTHONPATH"] if env.get("PYTHONPATH") else "")
-        # the child is a CPU process: keep it off any GPU
-        env["HIP_VISIBLE_DEVICES"] = ""
-        env["CUDA_VISIBLE_DEVICES"] = ""
-        r = subprocess.run([sys.executable, "-m", "oracle.refop", fin, fout], cwd=ROOT, env=env,
-                           capture_output=True, text=True, timeout=timeout)
+        r = subprocess.run([sys.executable, "-m", "oracle.refop", fin, fout], cwd=ROOT,
+                           env=_child_env(), capture_output=True, text=True, timeout=timeout)
         if r.returncode != 0:
             raise RuntimeError("reference op child failed (%d):\n%s" % (r.returncode, r.stderr[-4000:]))
         return torch.load(fout, weights_only=True)
+
+
+def _run_many(job: str, payloads, timeout: float = 900.0):
+    """``_run`` of several payloads in concurrent child processes."""
+    if not available():
+        raise FileNotFoundError(REF_LIB_PATH)
+    with tempfile.TemporaryDirectory(prefix="tmol_refop_") as d:
+        procs = []
+        for k, payload in enumerate(payloads):
+            fin = os.path.join(d, "in%d.pt" % k)
+            fout = os.path.join(d, "out%d.pt" % k)
+            torch.save(dict(payload, job=job), fin)
+            procs.append((subprocess.Popen(
+                [sys.executable, "-m", "oracle.refop", fin, fout], cwd=ROOT, env=_child_env(),
+                stdout=subprocess.DEVNULL, stderr=subprocess.PIPE, text=True), fout))
+        outs = []
+        for p, fout in procs:
+            _, err = p.communicate(timeout=timeout)
+            if p.returncode != 0:
+                raise RuntimeError("reference op child failed (%d):\n%s" % (p.returncode, err[-4000:]))
+            outs.append(torch.load(fout, weights_only=True))
+        return outs
 
 
 def pose_scores(args, grad_weights=()):
@@ -72,16 +98,36 @@ def rotamer_scores(args, grad_weights=()):
     return out["score"], out["aux"], out["grads"]
 
 
-def time_pose_workload(workload: str, stride: int, weights, budget_s: float, threads: int = 1):
+def time_pose_workload(workload: str, stride: int, weights, budget_s: float, threads: int = 1,
+                       offset: int = 0, start_at: float = 0.0):
     """Times the reference op, forward + backward of (score * weights).sum(),
-    on every ``stride``-th pose of the named generator workload
+    on poses offset, offset + stride, ... of the named generator workload
     (tmol_amd.generator.workload_poses), each pose scored alone, until
     ``budget_s`` seconds of op time are used.  The child builds the poses and
-    their op arguments itself (tmol_amd's host code, which registers no op).
+    their op arguments itself (tmol_amd's host code, which registers no op)
+    and starts timing at wall-clock time ``start_at`` (0: at once).
     Returns {"n_done", "n_residues", "seconds"}."""
-    return _run("time_pose", {"workload": workload, "stride": int(stride),
-                              "weights": weights.detach().cpu(), "budget": float(budget_s),
-                              "threads": int(threads)})
+    return _run("time_pose", _time_pose_payload(workload, stride, weights, budget_s, threads,
+                                                offset, start_at))
+
+
+def _time_pose_payload(workload, stride, weights, budget_s, threads, offset, start_at):
+    return {"workload": workload, "stride": int(stride), "offset": int(offset),
+            "weights": weights.detach().cpu(), "budget": float(budget_s),
+            "threads": int(threads), "start_at": float(start_at)}
+
+
+def time_pose_workload_parallel(workload: str, stride: int, weights, budget_s: float,
+                                n_proc: int, setup_s: float = 30.0):
+    """``n_proc`` single-threaded reference processes at once, process k on
+    poses k * stride, (k + n_proc) * stride, ... of the workload, all
+    starting their timed loops at the same wall-clock instant (``setup_s``
+    after launch, time for each to import torch and build its poses).
+    Returns one result per process, as ``time_pose_workload``."""
+    t0 = time.time() + setup_s
+    payloads = [_time_pose_payload(workload, stride * n_proc, weights, budget_s, 1, k * stride, t0)
+                for k in range(n_proc)]
+    return _run_many("time_pose", payloads)
 
 
 def time_rotamer_scores(args, threads: int = 1):
@@ -129,8 +175,12 @@ def _child(fin: str, fout: str):
         w = req["weights"]
         term = LJLKEnergyTerm()
         t_total, n_done, n_res = 0.0, 0, 0
-        for p in workload_poses(req["workload"])[:: req["stride"]]:
-            a = pose_op_args(PoseStack([p]), term)
+        sample = workload_poses(req["workload"])[req.get("offset", 0):: req["stride"]]
+        prepared = [(p, pose_op_args(PoseStack([p]), term)) for p in sample]
+        late = req.get("start_at", 0.0) - time.time()
+        if late > 0:
+            time.sleep(late)
+        for p, a in prepared:
             a[0] = a[0].clone().requires_grad_(True)
             t0 = time.perf_counter()
             s, _ = ops.ljlk_pose_scores(*a)
@@ -140,7 +190,8 @@ def _child(fin: str, fout: str):
             n_res += p.n_blocks
             if t_total > req["budget"]:
                 break
-        out = {"n_done": n_done, "n_residues": n_res, "seconds": t_total}
+        out = {"n_done": n_done, "n_residues": n_res, "seconds": t_total,
+               "started_late_s": max(0.0, -late)}
     elif job == "time_rotamer":
         t0 = time.perf_counter()
         _, idx = ops.ljlk_rotamer_scores(*req["args"])
