@@ -45,6 +45,11 @@ extern "C" {
 #define TMOL_LJLK_NO_ENERGY 16 /* skip the energy output (backward re-walk) */
 #define TMOL_LJLK_ACCUMULATE 32 /* add into dvdx / grad instead of overwriting */
 #define TMOL_LJLK_GENERAL_KERNEL 128 /* force the general (any block size) kernel */
+#define TMOL_LJLK_DETERMINISTIC 1024 /* float32 DVDX / WGRAD (whole pose): gradients and
+                                        per-pose energies summed in 64-bit fixed point in the
+                                        workspace, then converted once: bit-identical results
+                                        run to run (the workspace must come from
+                                        tmol_ljlk_workspace_bytes_flags with this flag) */
 /* launch variants of the scoring kernel, chosen by an autotuner (float32
  * whole-pose energy and weighted-gradient modes; other modes keep their
  * defaults): waves per SIMD the kernel is compiled for (3, 4 or 5; 0 = the
@@ -117,6 +122,13 @@ int tmol_ljlk_fill_neighbor_tiles(const int32_t* pose_block_start_host,
 int64_t tmol_ljlk_workspace_bytes(const tmol_ljlk_stack* stack,
                                   const tmol_ljlk_block_types* bt,
                                   int64_t row_capacity, int32_t dtype);
+
+/* Workspace bytes for score flags: with TMOL_LJLK_DETERMINISTIC, room for the
+ * fixed-point gradient planes (after the regions tmol_ljlk_workspace_bytes
+ * lays out, which keep their offsets); otherwise as tmol_ljlk_workspace_bytes. */
+int64_t tmol_ljlk_workspace_bytes_flags(const tmol_ljlk_stack* stack,
+                                        const tmol_ljlk_block_types* bt,
+                                        int64_t row_capacity, int32_t dtype, int32_t flags);
 
 /* Device address of the block spheres [G, 4] (dtype) inside the workspace,
  * written by tmol_ljlk_neighbors (centre x, y, z, radius). */

@@ -31,6 +31,7 @@ constexpr int M_BP = TMOL_LJLK_BLOCK_PAIR;
 constexpr int M_NOE = TMOL_LJLK_NO_ENERGY;
 constexpr int M_CENSUS = 64;  // branch census instead of energies (FLOP accounting)
 constexpr int M_ELEC = 256;   // elec scored in the same pass (fourth term)
+constexpr int M_DET = TMOL_LJLK_DETERMINISTIC;  // fixed-point global sums (float only)
 
 // census slots (counts of enumerated atom pairs)
 enum : int {
@@ -74,13 +75,14 @@ struct WsLayout {
   int64_t e_intra;    // int32[T] + uint16[T * INTRA_MAX]: elec's intra-block pairs per type
   int64_t row_count;  // uint32[G]
   int64_t row_items;  // int32[G * row_cap]
+  int64_t fixed;      // DETERMINISTIC: int64[det_planes * NA * 3] fixed-point gradients
   int64_t total;
 };
 
 inline int64_t align256(int64_t x) { return (x + 255) & ~int64_t(255); }
 
 inline WsLayout ws_layout(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
-                          int64_t row_cap, int dtype) {
+                          int64_t row_cap, int dtype, int det_planes = 0) {
   int64_t rs = dtype == TMOL_LJLK_F64 ? 8 : 4;
   int64_t G = s->n_blocks_total;
   WsLayout L;
@@ -107,6 +109,8 @@ inline WsLayout ws_layout(const tmol_ljlk_stack* s, const tmol_ljlk_block_types*
   o = align256(o + 4 * G);
   L.row_items = o;
   o = align256(o + 4 * G * row_cap);
+  L.fixed = o;
+  o = align256(o + 8 * int64_t(det_planes) * 3 * s->n_atoms_total);
   L.total = o;
   return L;
 }
@@ -295,6 +299,14 @@ struct LdsAcc<double> {
   __device__ static __forceinline__ double dec(T x) { return x; }
 };
 
+// M_DET: per-pose energies as 64-bit fixed point with 2^-24 kcal/mol
+// resolution (range +-2^39), summed with integer atomics (order-independent)
+constexpr double DET_E_SCALE = 16777216.0;  // 2^24
+__device__ __forceinline__ unsigned long long det_energy_enc(double v) {
+  v = fmin(fmax(v * DET_E_SCALE, -9.0e18), 9.0e18);
+  return (unsigned long long)__double2ll_rn(v);
+}
+
 // one lane's running energy sum, accumulated in FP64 (each FP32 pair term is
 // added exactly; FP64 adds run at the FP32 rate on MI355X)
 struct LaneSum {
@@ -338,6 +350,7 @@ struct ScoreArgs {
   Real* energies_bp;           // [3, P, B, B] block pair
   Real* dvdx;                  // [3, NA, 3]
   Real* grad;                  // [NA, 3]
+  unsigned long long* fixed;   // M_DET: fixed-point gradient planes [NG, NA, 3]
   int64_t n_atoms;
   unsigned long long* census;  // [16] with M_CENSUS
   // M_ELEC: elec's tables (charges [T, Amax]; packed representative path

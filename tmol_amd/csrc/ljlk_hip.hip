@@ -233,9 +233,18 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
   const bool bp = flags & TMOL_LJLK_BLOCK_PAIR;
   const bool noe = flags & TMOL_LJLK_NO_ENERGY;
   const bool acc = flags & TMOL_LJLK_ACCUMULATE;
+  const bool det = flags & TMOL_LJLK_DETERMINISTIC;
   const bool general = (flags & TMOL_LJLK_GENERAL_KERNEL) || btp->max_n_atoms > TILE;
   const int P = st->n_poses;
   const int64_t NA3 = st->n_atoms_total * 3;
+  if (det) {
+    // float32 whole-pose DVDX / WGRAD (row kernel; not the general one)
+    if (!std::is_same<Real, float>::value || general || bp || ek) return TMOL_LJLK_ENOTSUP;
+    if (!(flags & (TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD))) return TMOL_LJLK_ENOTSUP;
+    const int planes = (flags & TMOL_LJLK_DVDX) ? 3 : 1;
+    a.fixed = (unsigned long long*)((char*)ws + Lw.fixed);
+    fill<unsigned long long>(a.fixed, planes * NA3, stream);
+  }
   if (!noe) {
     if (bp)
       fill<Real>((Real*)energies,
@@ -246,14 +255,34 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
     else
       fill<double>(a.energies64, int64_t(n_terms) * P, stream);
   }
-  if ((flags & TMOL_LJLK_DVDX) && !acc) fill<Real>((Real*)dvdx, 3 * NA3, stream);
-  if ((flags & TMOL_LJLK_WGRAD) && !acc) fill<Real>((Real*)grad, NA3, stream);
+  if ((flags & TMOL_LJLK_DVDX) && !acc && !det) fill<Real>((Real*)dvdx, 3 * NA3, stream);
+  if ((flags & TMOL_LJLK_WGRAD) && !acc && !det) fill<Real>((Real*)grad, NA3, stream);
   const int n_wg = n_wave_groups;
   int mode = flags & (TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY);
   if (ek) mode |= M_ELEC;
   // launch variants (autotuner): waves per SIMD and rows per queue grab
   const int occ = (flags >> 16) & 15;
   a.row_grab = (flags >> 20) & 15;
+  if (det) {
+    if constexpr (std::is_same<Real, float>::value) {
+      switch (mode) {
+        case M_DVDX: launch_rows<Real, M_DVDX | M_DET>(a, n_wg, stream); break;
+        case M_WGRAD: launch_rows<Real, M_WGRAD | M_DET>(a, n_wg, stream); break;
+        case M_WGRAD | M_NOE: launch_rows<Real, M_WGRAD | M_NOE | M_DET>(a, n_wg, stream); break;
+        default: return TMOL_LJLK_ENOTSUP;
+      }
+      const int64_t n = (mode == M_DVDX ? 3 : 1) * NA3;
+      hipLaunchKernelGGL(fixed_to_real_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream,
+                         (const unsigned long long*)a.fixed,
+                         (Real*)(mode == M_DVDX ? dvdx : grad), n, acc);
+      if (!noe)
+        hipLaunchKernelGGL(finalize_fixed_kernel<Real>, dim3(grid_for(3 * P)), dim3(256), 0,
+                           stream, (const unsigned long long*)a.energies64, (Real*)energies,
+                           3 * P);
+      return check(hipGetLastError());
+    }
+    return TMOL_LJLK_ENOTSUP;
+  }
   if (occ && !general && !ek && (mode == 0 || mode == M_WGRAD)) {
     if (mode == 0)
       launch_rows_occ<Real, 0>(a, occ, n_wg, stream);
@@ -705,6 +734,14 @@ int64_t tmol_ljlk_workspace_bytes(const tmol_ljlk_stack* s, const tmol_ljlk_bloc
   return ws_layout(s, bt, row_capacity, dtype).total;
 }
 
+int64_t tmol_ljlk_workspace_bytes_flags(const tmol_ljlk_stack* s,
+                                        const tmol_ljlk_block_types* bt, int64_t row_capacity,
+                                        int32_t dtype, int32_t flags) {
+  if (!s || !bt || row_capacity < 0) return -1;
+  const int planes = (flags & TMOL_LJLK_DETERMINISTIC) ? ((flags & TMOL_LJLK_DVDX) ? 3 : 1) : 0;
+  return ws_layout(s, bt, row_capacity, dtype, planes).total;
+}
+
 int tmol_ljlk_work_list(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                         void* workspace, int64_t row_capacity, int32_t dtype,
                         uint32_t** row_count_out, int32_t** row_items_out) {
@@ -750,7 +787,9 @@ int tmol_ljlk_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
   int v = validate(s, bt, dtype);
   if (v) return v;
   if (!coords || !workspace || row_capacity <= 0) return TMOL_LJLK_EINVAL;
-  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  const int det_planes =
+      (flags & TMOL_LJLK_DETERMINISTIC) ? ((flags & TMOL_LJLK_DVDX) ? 3 : 1) : 0;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype, det_planes);
   if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
   if (!(flags & TMOL_LJLK_NO_ENERGY) && !energies) return TMOL_LJLK_EINVAL;
   if ((flags & TMOL_LJLK_DVDX) && !dvdx) return TMOL_LJLK_EINVAL;

@@ -272,6 +272,27 @@ __global__ void finalize_kernel(const double* __restrict__ e64, Real* __restrict
   if (i < n) out[i] = Real(e64[i]);
 }
 
+// DETERMINISTIC: fixed-point per-pose energies -> output dtype
+template <typename Real>
+__global__ void finalize_fixed_kernel(const unsigned long long* __restrict__ e, Real* __restrict__ out,
+                                      int n) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) out[i] = Real(double((long long)e[i]) * (1.0 / DET_E_SCALE));
+}
+
+// DETERMINISTIC: fixed-point gradient planes -> output (added to it when
+// accumulate), one rounding per component
+template <typename Real>
+__global__ void fixed_to_real_kernel(const unsigned long long* __restrict__ fx,
+                                     Real* __restrict__ out, int64_t n, bool accumulate) {
+  int64_t i = int64_t(blockIdx.x) * blockDim.x + threadIdx.x;
+  const int64_t stride = int64_t(gridDim.x) * blockDim.x;
+  for (; i < n; i += stride) {
+    const Real v = LdsAcc<Real>::dec(fx[i]);
+    out[i] = accumulate ? out[i] + v : v;
+  }
+}
+
 // whole-pose backward: grad[a, k] = sum_t gs[t, pose(a)] * dvdx[t, a, k]
 template <typename Real>
 __global__ void combine_kernel(const Real* __restrict__ dvdx, const Real* __restrict__ gs,

@@ -133,6 +133,7 @@ struct RowCtx {
   static constexpr bool NOE = MODE & M_NOE;
   static constexpr bool DERIV = DVDX || WGRAD;
   static constexpr bool ELEC = MODE & M_ELEC;  // whole-pose energies / WGRAD only
+  static constexpr bool DET = MODE & M_DET;    // fixed-point global sums (float)
   static constexpr int NG = DVDX ? 3 : (WGRAD ? 1 : 0);
   // gradient accumulator copies: fewer when the planes are many or wide
   static constexpr int R = 1;
@@ -169,8 +170,8 @@ __device__ __forceinline__ uint16_t make_entry(int a1, int s2, bool w02) {
 // owner of an entry's second slot: 0/1 ring slot, 2 = g1 (intra)
 __device__ __forceinline__ int entry_owner(int s2) { return s2 < TILE ? 2 : (s2 - TILE) >> 5; }
 
-template <typename Real, int MODE>
-__device__ __forceinline__ void grad_scatter(RowLDS<Real, MODE>& L, int term, int s1, int s2,
+template <typename Real, int MODE, typename LDS>
+__device__ __forceinline__ void grad_scatter(LDS& L, int term, int s1, int s2,
                                              Real gx, Real gy, Real gz, int lane) {
   using C = RowCtx<Real, MODE>;
   using Acc = LdsAcc<Real>;
@@ -186,8 +187,8 @@ __device__ __forceinline__ void grad_scatter(RowLDS<Real, MODE>& L, int term, in
 }
 
 // Evaluate LJ entries [0, consume) of the FIFO, one per lane.
-template <typename Real, int MODE>
-__device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
+template <typename Real, int MODE, typename LDS>
+__device__ __forceinline__ void eval_lj(LDS& L, int consume,
                                         const GlobalParams<Real>& gp, const Real* wrow,
                                         LaneSum& e_atr, LaneSum& e_rep, int lane) {
   using C = RowCtx<Real, MODE>;
@@ -225,14 +226,14 @@ __device__ __forceinline__ void eval_lj(RowLDS<Real, MODE>& L, int consume,
           term = inside ? 1 : 0;
         else
           gs *= C::BP ? L.wt[entry_owner(s2)][inside ? 1 : 0] : wrow[inside ? 1 : 0];
-        grad_scatter(L, term, s1, s2, gs * dx, gs * dy, gs * dz, lane);
+        grad_scatter<Real, MODE>(L, term, s1, s2, gs * dx, gs * dy, gs * dz, lane);
       }
     }
   }
 }
 
-template <typename Real, int MODE>
-__device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
+template <typename Real, int MODE, typename LDS>
+__device__ __forceinline__ void eval_lk(LDS& L, int consume,
                                         const GlobalParams<Real>& gp, const Real* wrow,
                                         LaneSum& e_lk, int lane) {
   using C = RowCtx<Real, MODE>;
@@ -269,7 +270,7 @@ __device__ __forceinline__ void eval_lk(RowLDS<Real, MODE>& L, int consume,
         if (dlk != Real(0)) {
           Real gs = dlk * inv_d;
           if (C::WGRAD) gs *= C::BP ? L.wt[entry_owner(s2)][2] : wrow[2];
-          grad_scatter(L, C::DVDX ? 2 : 0, s1, s2, gs * dx, gs * dy, gs * dz, lane);
+          grad_scatter<Real, MODE>(L, C::DVDX ? 2 : 0, s1, s2, gs * dx, gs * dy, gs * dz, lane);
         }
       }
     }
@@ -305,7 +306,7 @@ __device__ __forceinline__ void eval_el(RowLDS<Real, MODE>& L, int consume,
         const Real dE = wcp * (qq * dv);
         if (dE != Real(0)) {
           const Real gs = dE * inv_d * wel;
-          grad_scatter(L, 0, s1, s2, gs * dx, gs * dy, gs * dz, lane);
+          grad_scatter<Real, MODE>(L, 0, s1, s2, gs * dx, gs * dy, gs * dz, lane);
         }
       }
     }
@@ -342,6 +343,12 @@ __device__ __forceinline__ void flush_slots(RowLDS<Real, MODE>& L, const ScoreAr
       for (int c = 0; c < C::R; ++c) {
         acc += L.grad[c][t][k][slot];
         L.grad[c][t][k][slot] = 0;
+      }
+      if (C::DET) {
+        // the exact fixed-point sum goes to global memory as it is: integer
+        // adds in any order give the same total
+        if (acc != 0) atomicAdd(&A.fixed[(int64_t(t) * A.n_atoms + atom0) * 3 + i], acc);
+        continue;
       }
       const Real v = LdsAcc<Real>::dec(acc);
       if (v != Real(0)) {
@@ -753,7 +760,12 @@ __global__ __launch_bounds__(256, (rows_waves_per_simd<MODE, OCC>())) void score
       const double ea = wave_sum(e_atr.value());
       const double er = wave_sum(e_rep.value());
       const double el = wave_sum(e_lk.value());
-      if (lane == 0) {
+      if (lane == 0 && C::DET) {
+        auto* e = (unsigned long long*)A.energies64;
+        atomicAdd(&e[p], det_energy_enc(ea));
+        atomicAdd(&e[s.P + p], det_energy_enc(er));
+        atomicAdd(&e[2 * s.P + p], det_energy_enc(el));
+      } else if (lane == 0) {
         unsafeAtomicAdd(&A.energies64[p], ea);
         unsafeAtomicAdd(&A.energies64[s.P + p], er);
         unsafeAtomicAdd(&A.energies64[2 * s.P + p], el);

@@ -97,12 +97,14 @@ class SegmentedLBFGS:
         return padded.reshape(-1)[self.flat_index]
 
     def _seg_sum(self, v: torch.Tensor) -> torch.Tensor:
-        out = torch.zeros(self.P, dtype=torch.float64, device=v.device)
-        return out.index_add_(0, self.seg, v.to(torch.float64))
+        # a row reduction of the padded layout: no atomics, the same order on
+        # every run
+        return self._padded(v).sum(1)
 
     def _seg_max(self, v: torch.Tensor) -> torch.Tensor:
-        out = torch.full((self.P,), -math.inf, dtype=torch.float64, device=v.device)
-        return out.scatter_reduce(0, self.seg, v.to(torch.float64), "amax", include_self=True)
+        pad = torch.full((self.P * self.S,), -math.inf, dtype=torch.float64, device=v.device)
+        pad[self.flat_index] = v.to(torch.float64)
+        return pad.view(self.P, self.S).amax(1)
 
     def _per_elem(self, per_seg: torch.Tensor) -> torch.Tensor:
         return per_seg[self.seg]
@@ -113,19 +115,23 @@ class SegmentedLBFGS:
     # chronological order, so only they move when the oldest entry is dropped.
     def _reset_history(self, dev):
         m, P, S = self.m, self.P, self.S
-        self.Sh = torch.zeros((m, P, S), dtype=torch.float64, device=dev)
-        self.Yh = torch.zeros((m, P, S), dtype=torch.float64, device=dev)
+        # [P, m, S]: each segment's history rows contiguous, so the products
+        # with the history are batched matrix-vector products over segments
+        self.Sh = torch.zeros((P, m, S), dtype=torch.float64, device=dev)
+        self.Yh = torch.zeros((P, m, S), dtype=torch.float64, device=dev)
         self.SY = torch.zeros((P, m, m), dtype=torch.float64, device=dev)  # s_i . y_j
         self.YY = torch.zeros((P, m, m), dtype=torch.float64, device=dev)  # y_i . y_j
         self.order = []
         self.n_hist = 0
+        self._idx = None
 
     def _used(self):
         """Physical slots in use (a prefix until the ring wraps) and the
         chronological order as indices into them."""
         n_phys = self.m if self.n_hist == self.m else self.n_hist
-        idx = torch.tensor(self.order, dtype=torch.int64, device=self.Sh.device)
-        return n_phys, idx
+        if self._idx is None:
+            self._idx = torch.tensor(self.order, dtype=torch.int64, device=self.Sh.device)
+        return n_phys, self._idx
 
     def _push(self, s: torch.Tensor, y: torch.Tensor):
         """Append one (s, y) pair [P, S] (rows of segments that did not keep
@@ -140,28 +146,27 @@ class SegmentedLBFGS:
             k = self.n_hist
             self.n_hist += 1
         self.order.append(phys)
-        self.Sh[phys] = s
-        self.Yh[phys] = y
+        self._idx = None
+        self.Sh[:, phys] = s
+        self.Yh[:, phys] = y
         n_phys, idx = self._used()
-        Sv, Yv = self.Sh[:n_phys], self.Yh[:n_phys]
-        s_y = torch.einsum("ips,ps->pi", Sv, y)[:, idx]  # s_i . y_new, chronological
-        y_s = torch.einsum("ips,ps->pi", Yv, s)[:, idx]  # y_i . s_new
-        y_y = torch.einsum("ips,ps->pi", Yv, y)[:, idx]  # y_i . y_new
+        Sv, Yv = self.Sh[:, :n_phys], self.Yh[:, :n_phys]
+        s_y = torch.bmm(Sv, y.unsqueeze(-1)).squeeze(-1)[:, idx]  # s_i . y_new, chronological
+        ys_yy = torch.bmm(Yv, torch.stack([s, y], dim=2))[:, idx]  # y_i . s_new, y_i . y_new
         n = self.n_hist
         self.SY[:, :n, k] = s_y
-        self.SY[:, k, :n] = y_s
-        self.YY[:, :n, k] = y_y
-        self.YY[:, k, :n] = y_y
+        self.SY[:, k, :n] = ys_yy[..., 0]
+        self.YY[:, :n, k] = ys_yy[..., 1]
+        self.YY[:, k, :n] = ys_yy[..., 1]
 
     def _drop_segments(self, mask: torch.Tensor):
-        """Forget the history of the segments in ``mask``."""
-        sel = mask.nonzero(as_tuple=False).flatten()
-        if sel.numel() == 0:
-            return
-        self.Sh[:, sel] = 0.0
-        self.Yh[:, sel] = 0.0
-        self.SY[sel] = 0.0
-        self.YY[sel] = 0.0
+        """Forget the history of the segments in ``mask`` (device mask; no
+        host synchronisation)."""
+        keep = (~mask).to(torch.float64)
+        self.Sh.mul_(keep[:, None, None])
+        self.Yh.mul_(keep[:, None, None])
+        self.SY.mul_(keep[:, None, None])
+        self.YY.mul_(keep[:, None, None])
 
     def _hg(self, g: torch.Tensor) -> torch.Tensor:
         """H g for padded g [P, S] (float64), compact representation."""
@@ -169,29 +174,30 @@ class SegmentedLBFGS:
         if n == 0:
             return g.clone()
         n_phys, idx = self._used()
-        Sv, Yv = self.Sh[:n_phys], self.Yh[:n_phys]
-        a = torch.einsum("ips,ps->pi", Sv, g)[:, idx]
-        b = torch.einsum("ips,ps->pi", Yv, g)[:, idx]
+        Sv, Yv = self.Sh[:, :n_phys], self.Yh[:, :n_phys]
+        gc = g.unsqueeze(-1)
+        a = torch.bmm(Sv, gc).squeeze(-1)[:, idx]
+        b = torch.bmm(Yv, gc).squeeze(-1)[:, idx]
         SY = self.SY[:, :n, :n]
         D = torch.diagonal(SY, dim1=1, dim2=2)
         # absent entries (zero rows) get a unit diagonal: they then contribute 0
         R = torch.triu(SY) + torch.diag_embed((D == 0).to(SY.dtype))
         u = torch.linalg.solve_triangular(R, a.unsqueeze(-1), upper=True).squeeze(-1)
-        v = torch.einsum("pij,pj->pi", self.YY[:, :n, :n], u) + D * u - b
+        v = torch.bmm(self.YY[:, :n, :n], u.unsqueeze(-1)).squeeze(-1) + D * u - b
         p1 = torch.linalg.solve_triangular(R.transpose(1, 2), v.unsqueeze(-1),
                                            upper=False).squeeze(-1)
         # back to physical slot order for the products with the history
-        p1_phys = torch.zeros((self.P, n_phys), dtype=g.dtype, device=g.device)
-        u_phys = torch.zeros_like(p1_phys)
-        p1_phys[:, idx] = p1
-        u_phys[:, idx] = u
-        return g + torch.einsum("pi,ips->ps", p1_phys, Sv) - torch.einsum("pi,ips->ps", u_phys, Yv)
+        coef = torch.zeros((self.P, 2, n_phys), dtype=g.dtype, device=g.device)
+        coef[:, 0, idx] = p1
+        coef[:, 1, idx] = -u
+        return g + torch.bmm(coef[:, 0:1], Sv).squeeze(1) + torch.bmm(coef[:, 1:2], Yv).squeeze(1)
 
     # -- line search ------------------------------------------------------------------
     def _armijo(self, phi_at, gtd, phi0, alpha0, searching):
         """Per-segment Armijo search; returns (accepted step, energies at it,
         evaluations, failed mask, whether the last evaluation was at the
-        accepted steps)."""
+        accepted steps).  The one host synchronisation per trial is the
+        decision whether to evaluate again."""
         zero = torch.zeros_like(alpha0)
         alpha = torch.where(searching, alpha0, zero)
         phi = phi_at(alpha)
@@ -204,7 +210,10 @@ class SegmentedLBFGS:
         step = torch.where(ok, alpha, zero)
         phi_step = torch.where(ok, phi, phi0)
         failed = torch.zeros_like(searching)
-        while bool((grow | shrink).any()):
+        while True:
+            more, at_step = torch.stack([(grow | shrink).any(), (alpha == step).all()]).tolist()
+            if not more:
+                break
             trial = torch.where(grow, alpha * 2.0, step)
             trial = torch.where(shrink, alpha * 0.25, trial)
             phi_t = phi_at(trial)
@@ -226,10 +235,14 @@ class SegmentedLBFGS:
             failed = failed | fail
             shrink = shrink & ~armijo & ~low
             alpha, phi = trial, phi_t
-        return step, phi_step, evals, failed, bool(torch.equal(alpha, step))
+        return step, phi_step, evals, failed, bool(at_step)
 
     # -- driver ------------------------------------------------------------------------
     def run(self, x0: torch.Tensor, evaluate: Evaluate) -> Tuple[torch.Tensor, Trace]:
+        """Host synchronisations: one per line-search trial (whether to
+        evaluate again) and one per iteration (convergence, and whether the
+        next iteration adds a history pair or resets a segment); the trace
+        stays on the device until the run ends."""
         x = x0.detach().reshape(-1).clone()
         dev = x.device
         eps = torch.finfo(x.dtype).eps
@@ -242,21 +255,23 @@ class SegmentedLBFGS:
         stalled = torch.zeros(P, **flags)
         needs_reset = torch.zeros(P, **flags)
         trace = Trace()
+        rec_E, rec_gn, rec_gm = [], [], []
 
         E, g = evaluate(x)
         E, g = E.to(torch.float64), g.reshape(-1).clone()
-        n_evals = [1]
 
         def record(E, g, evals):
-            trace.energy.append(E.detach().cpu())
-            trace.grad_norm.append(self._seg_sum(g * g).sqrt().cpu())
-            trace.grad_max.append(self._seg_max(g.abs()).cpu())
+            rec_E.append(E.detach())
+            rec_gn.append(self._seg_sum(g * g).sqrt())
+            rec_gm.append(self._seg_max(g.abs()))
             trace.evals.append(evals)
 
         record(E, g, 1)
         t = torch.full((P,), self.lr, dtype=torch.float64, device=dev)
         g_prev = None
         x_ref = x.clone()
+        keep = None
+        keep_any = reset_any = False
         it = 0
         while it < self.max_iter:
             it += 1
@@ -264,17 +279,16 @@ class SegmentedLBFGS:
             if g_prev is None:
                 d = -g
             else:
-                y = g - g_prev
-                s = x - x_ref
-                keep = self._seg_sum(y * s) > 1e-6
-                if bool(keep.any()):
+                if keep_any:
                     ke = self._per_elem(keep)
+                    y = g - g_prev
+                    s = x - x_ref
                     self._push(self._padded(torch.where(ke, s, torch.zeros_like(s))),
                                self._padded(torch.where(ke, y, torch.zeros_like(y))))
                     x_ref = torch.where(ke, x, x_ref)
                 d = -self._flat(self._hg(self._padded(g))).to(x.dtype)
             was_reset = needs_reset.clone()
-            if bool(needs_reset.any()):
+            if reset_any:
                 self._drop_segments(needs_reset)
                 re = self._per_elem(needs_reset)
                 d = torch.where(re, -g, d)
@@ -285,15 +299,16 @@ class SegmentedLBFGS:
             g_prev = g.clone()
             E_prev = E.clone()
             # ---- directional derivative, with repair of uphill directions
+            # (flip the uphill components, then steepest descent), masked per
+            # segment: a repair leaves segments without an uphill direction
+            # untouched
             gtd = self._seg_sum(g * d)
             for repair in (1, 2):
                 bad = (gtd > -1e-5) & ~inactive
-                if not bool(bad.any()):
-                    break
                 be = self._per_elem(bad)
                 fixed = d * -torch.sign(g * d) if repair == 1 else -g
                 d = torch.where(be, fixed, d)
-                gtd = self._seg_sum(g * d)
+                gtd = torch.where(bad, self._seg_sum(g * d), gtd)
             # ---- line search
             searching = ~inactive & (gtd < 0)
             t0 = torch.clamp(t / 0.5, max=1.0)
@@ -320,19 +335,27 @@ class SegmentedLBFGS:
             stalled = stalled | (failed & was_reset)
             needs_reset = needs_reset | (failed & ~was_reset)
             record(E, g, evals)
-            # ---- convergence
+            # ---- convergence; and, for the next iteration, which segments
+            # keep the new (s, y) pair (s.y > 1e-6), fetched in the same
+            # synchronisation
             converged = converged | (self._seg_max(g.abs()) <= self.gradtol)
             dE = (E - E_prev).abs()
             rdiff = 2 * dE / (E.abs() + E_prev.abs() + 1e-10)
             done = converged | stalled | (dE <= atol) | (rdiff <= rtol)
+            keep = self._seg_sum((g - g_prev) * (x - x_ref)) > 1e-6
+            all_done, keep_any, reset_any = torch.stack(
+                [done.all(), keep.any(), needs_reset.any()]).tolist()
             if self.verbose:
                 print("iter %4d  E=%.6f  evals=%d  done=%d/%d" % (it, float(E.sum()), evals,
                                                                   int(done.sum()), P))
-            if bool(done.all()):
+            if all_done:
                 break
         trace.iterations = it
         trace.converged = converged.cpu()
         trace.stalled = stalled.cpu()
+        n = len(rec_E)
+        for name, rec in (("energy", rec_E), ("grad_norm", rec_gn), ("grad_max", rec_gm)):
+            getattr(trace, name).extend(torch.stack(rec).cpu().unbind(0) if n else [])
         return x, trace
 
 

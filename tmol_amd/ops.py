@@ -33,6 +33,7 @@ OK, EINVAL, EOVERFLOW, EHIP, ENOTSUP = 0, 1, 2, 3, 4
 F32, F64 = 0, 1
 PADDED, RAGGED = 0, 1
 DVDX, WGRAD, BLOCK_PAIR, NO_ENERGY, ACCUMULATE, GENERAL_KERNEL = 2, 4, 8, 16, 32, 128
+DETERMINISTIC = 1024  # float32 DVDX / WGRAD: fixed-point global sums, bit-identical run to run
 WAVES_PER_SIMD, ROW_GRAB = 1 << 16, 1 << 20  # multipliers of the launch-variant fields
 NBR_TILE = 64
 
@@ -113,6 +114,9 @@ def lib():
             L.tmol_ljlk_fill_neighbor_tiles.argtypes = [_P32, ctypes.c_int32, _P32]
             L.tmol_ljlk_workspace_bytes.argtypes = [SP, BP, ctypes.c_int64, ctypes.c_int32]
             L.tmol_ljlk_workspace_bytes.restype = ctypes.c_int64
+            L.tmol_ljlk_workspace_bytes_flags.argtypes = [SP, BP, ctypes.c_int64, ctypes.c_int32,
+                                                          ctypes.c_int32]
+            L.tmol_ljlk_workspace_bytes_flags.restype = ctypes.c_int64
             L.tmol_ljlk_neighbors.argtypes = [
                 SP, BP, _VP, ctypes.c_int32, ctypes.c_double, _VP, ctypes.c_int64, _VP,
                 ctypes.c_int64, ctypes.c_int64, _VP, _VP,
@@ -240,9 +244,10 @@ class Workspace:
     """Caller-owned device workspace; ``capacity`` is the work list's row
     capacity (most neighbor blocks g2 >= g1 of any block g1)."""
 
-    def __init__(self, cstack, cbt, capacity, dtype_code, device):
+    def __init__(self, cstack, cbt, capacity, dtype_code, device, flags=0):
         L = lib()
-        nbytes = L.tmol_ljlk_workspace_bytes(ctypes.byref(cstack), ctypes.byref(cbt), capacity, dtype_code)
+        nbytes = L.tmol_ljlk_workspace_bytes_flags(ctypes.byref(cstack), ctypes.byref(cbt),
+                                                   capacity, dtype_code, flags)
         if nbytes < 0:
             raise LJLKLibraryError("workspace size query failed")
         self.buf = torch.empty(int(nbytes), dtype=torch.uint8, device=device)

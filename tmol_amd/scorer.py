@@ -64,12 +64,17 @@ class LJLKRaggedScorer:
     sum_t weights[t, p] dE_t/dx [NA, 3] in one scoring pass."""
 
     def __init__(self, stack: RaggedPoseStack, term, device="cuda", dtype=torch.float32,
-                 capacity: Optional[int] = None, wave_groups: Optional[int] = None):
+                 capacity: Optional[int] = None, wave_groups: Optional[int] = None,
+                 deterministic: bool = False):
         dev = torch.device(device)
         self.device = dev
         self.dtype = dtype
         self.dcode = ops._dtype_code(dtype)
         self.stack = stack
+        # deterministic: gradients (and per-pose energies) summed in 64-bit
+        # fixed point, so float32 results are bit-identical run to run
+        # (float32 only; float64 sums are already far below its tolerance)
+        self.det_flags = ops.DETERMINISTIC if (deterministic and dtype == torch.float32) else 0
         pbt = stack.packed_block_types
         self.P = stack.n_poses
         self.NA = stack.n_atoms_total
@@ -91,7 +96,9 @@ class LJLKRaggedScorer:
             # first neighbor pass
             capacity = max_nb
         self.capacity = int(capacity)
-        self.ws = ops.Workspace(self.cstack, self.bt.c, self.capacity, self.dcode, dev)
+        # a deterministic workspace has room for the fixed-point DVDX planes
+        self.ws = ops.Workspace(self.cstack, self.bt.c, self.capacity, self.dcode, dev,
+                                flags=self.det_flags | (ops.DVDX if self.det_flags else 0))
         self.energies = torch.zeros((3, self.P), dtype=dtype, device=dev)
         self.grad = torch.zeros((self.NA, 3), dtype=dtype, device=dev)
         self.set_launch(wave_groups=wave_groups or 0)
@@ -109,6 +116,8 @@ class LJLKRaggedScorer:
 
     def score(self, coords, flags, weights=None, energies=None, dvdx=None, grad=None):
         flags = flags | self.launch_flags()
+        if self.det_flags and flags & (ops.DVDX | ops.WGRAD) and not flags & ops.BLOCK_PAIR:
+            flags |= self.det_flags
         ops._check(ops.lib().tmol_ljlk_score(
             ctypes.byref(self.cstack), ctypes.byref(self.bt.c), ops._p(coords), self.dcode,
             ops._p(self.ws.buf), self.ws.nbytes, self.ws.capacity, flags, ops._p(weights),
@@ -352,12 +361,17 @@ class ElecRaggedScorer:
     sum_p weights[p] dE_p/dx [NA, 3] in one pass."""
 
     def __init__(self, stack: RaggedPoseStack, term, device="cuda", dtype=torch.float32,
-                 capacity: Optional[int] = None, wave_groups: Optional[int] = None):
+                 capacity: Optional[int] = None, wave_groups: Optional[int] = None,
+                 deterministic: bool = False):
         dev = torch.device(device)
         self.device = dev
         self.dtype = dtype
         self.dcode = ops._dtype_code(dtype)
         self.stack = stack
+        # deterministic: gradients (and per-pose energies) summed in 64-bit
+        # fixed point, so float32 results are bit-identical run to run
+        # (float32 only; float64 sums are already far below its tolerance)
+        self.det_flags = ops.DETERMINISTIC if (deterministic and dtype == torch.float32) else 0
         pbt = stack.packed_block_types
         self.P = stack.n_poses
         self.NA = stack.n_atoms_total
