@@ -28,6 +28,8 @@ extern "C" {
 #define TMOL_LJLK_EOVERFLOW 2 /* work list capacity exceeded (see read_counters) */
 #define TMOL_LJLK_EHIP 3      /* HIP runtime error (launch) */
 #define TMOL_LJLK_ENOTSUP 4   /* unsupported configuration */
+#define TMOL_LJLK_ESCRATCH 5  /* tmol_ljlk_score_noalloc: the stack needs more scratch than
+                                 TMOL_LJLK_NOALLOC_SCRATCH_BYTES */
 
 /* coordinate / parameter dtypes */
 #define TMOL_LJLK_F32 0
@@ -163,6 +165,31 @@ int tmol_ljlk_score(const tmol_ljlk_stack* stack,
                     int64_t row_capacity, int32_t flags,
                     const void* weights, void* energies, void* dvdx,
                     void* grad, int32_t n_wave_groups, void* stream);
+
+/* No-allocation variant of tmol_ljlk_neighbors + tmol_ljlk_score, in one
+ * call.  It has no counterpart in the reference: it is for callers that
+ * cannot allocate device memory at all (code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc), which therefore cannot
+ * hand the other entry points a workspace.  Instead of a workspace argument it
+ * uses a device array of TMOL_LJLK_NOALLOC_SCRATCH_BYTES that the library
+ * holds on each GPU it runs on until the process ends (its address is looked
+ * up on the first call on a device); it calls no allocating routine.
+ *   - The stack's workspace (tmol_ljlk_workspace_bytes for row_capacity) must
+ *     fit the array, else TMOL_LJLK_ESCRATCH and nothing is launched.
+ *   - row_capacity must be at least stack->max_n_blocks, so the work list
+ *     cannot overflow (no host read-back).
+ *   - flags as for tmol_ljlk_score, without GENERAL_KERNEL and the launch
+ *     variant fields; dense_neighbors as for tmol_ljlk_neighbors.
+ *   - Every call on a GPU uses that GPU's one array and rewrites all of it
+ *     that it reads: calls that overlap on one GPU (other streams, other
+ *     threads) race and must be serialised by the caller. */
+#define TMOL_LJLK_NOALLOC_SCRATCH_BYTES (64ll << 20)
+int64_t tmol_ljlk_noalloc_scratch_bytes(void);
+int tmol_ljlk_score_noalloc(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                            const void* coords, int32_t dtype, double max_dis,
+                            const int32_t* tiles, int64_t n_tiles, int64_t row_capacity,
+                            int32_t flags, const void* weights, void* energies, void* dvdx,
+                            void* grad, int32_t* dense_neighbors, void* stream);
 
 /* Branch census of the work list (float32 only): counts of enumerated atom
  * pairs per potential region, for algorithmic FLOP accounting.  census[16]

@@ -292,3 +292,39 @@ def test_elec_rotamer_validation():
     assert score(flags=ops.WGRAD | ops.NO_ENERGY, dTdV=None) == ops.EINVAL
     b.max_n_atoms, b.max_n_tiles = 40, 2
     assert nbr() == ops.ENOTSUP and score() == ops.ENOTSUP
+
+
+def test_noalloc_validation():
+    """tmol_ljlk_score_noalloc rejects, before any device work, a row capacity
+    below the longest pose and a stack whose workspace exceeds its scratch."""
+    from tmol_amd import ops
+
+    L = ops.lib()
+    assert L.tmol_ljlk_noalloc_scratch_bytes() == 64 << 20
+    dummy = ctypes.c_void_p(16)
+    s = ops.CStack()
+    s.layout = ops.PADDED
+    s.n_poses = 2
+    s.max_n_blocks = 10
+    s.n_blocks_total = 20
+    s.n_atoms_total = 300
+    for k in ("block_atom_offset", "block_type", "min_bondsep", "inter_block_bondsep"):
+        setattr(s, k, dummy)
+    b = ops.CBlockTypes()
+    b.n_block_types = 3
+    b.max_n_atoms = 24
+    b.max_n_tiles = 1
+    b.max_n_conn = 2
+    b.n_atom_types = 105
+    b.dtype = ops.F32
+
+    def call(cap):
+        return L.tmol_ljlk_score_noalloc(ctypes.byref(s), ctypes.byref(b), dummy, ops.F32, 6.0,
+                                         dummy, 1, cap, 0, None, dummy, None, None, None, None)
+
+    assert call(9) == ops.EINVAL  # below the longest pose
+    s.n_poses = 20000
+    s.max_n_blocks = 1000
+    s.n_blocks_total = 20000 * 1000
+    assert L.tmol_ljlk_workspace_bytes(ctypes.byref(s), ctypes.byref(b), 1000, ops.F32) > 64 << 20
+    assert call(1000) == ops.ESCRATCH

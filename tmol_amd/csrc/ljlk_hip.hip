@@ -16,6 +16,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <mutex>
 #include <type_traits>
 
 #include "../../include/tmol_ljlk.h"
@@ -312,6 +313,36 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
     hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(n_terms * P)), dim3(256), 0, stream,
                        (const double*)a.energies64, (Real*)energies, n_terms * P);
   return check(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ no-allocation entry point
+
+// Device scratch of tmol_ljlk_score_noalloc: one array per GPU, held from the
+// library's load until the process ends.  Host code never passes the array's
+// name to a kernel (on the host it is only a stand-in for the symbol): the
+// device address is looked up once per device with hipGetSymbolAddress and
+// that address is used everywhere.
+static __device__ __align__(256) unsigned char scratch[TMOL_LJLK_NOALLOC_SCRATCH_BYTES];
+
+constexpr int NOALLOC_MAX_DEVICES = 64;
+static void* noalloc_ptr[NOALLOC_MAX_DEVICES];
+static std::mutex noalloc_mu;
+
+// the current device's scratch address (looked up on the first call on that
+// device, with that device current)
+static int noalloc_scratch(void** out) {
+  int dev = -1;
+  if (hipGetDevice(&dev) != hipSuccess) return TMOL_LJLK_EHIP;
+  if (dev < 0 || dev >= NOALLOC_MAX_DEVICES) return TMOL_LJLK_ENOTSUP;
+  std::lock_guard<std::mutex> lock(noalloc_mu);
+  if (!noalloc_ptr[dev]) {
+    void* p = nullptr;
+    if (hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)) != hipSuccess || !p)
+      return TMOL_LJLK_EHIP;
+    noalloc_ptr[dev] = p;
+  }
+  *out = noalloc_ptr[dev];
+  return TMOL_LJLK_OK;
 }
 
 // ------------------------------------------------------------------ rotamers
@@ -808,6 +839,33 @@ int tmol_ljlk_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                             dvdx, grad, n_wave_groups, st, L);
   return run_score<double>(s, bt, coords, workspace, row_capacity, flags, weights, energies,
                            dvdx, grad, n_wave_groups, st, L);
+}
+
+int64_t tmol_ljlk_noalloc_scratch_bytes(void) { return TMOL_LJLK_NOALLOC_SCRATCH_BYTES; }
+
+int tmol_ljlk_score_noalloc(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                            const void* coords, int32_t dtype, double max_dis,
+                            const int32_t* tiles, int64_t n_tiles, int64_t row_capacity,
+                            int32_t flags, const void* weights, void* energies, void* dvdx,
+                            void* grad, int32_t* dense_neighbors, void* stream) {
+  int v = validate(s, bt, dtype);
+  if (v) return v;
+  if (!coords || (n_tiles > 0 && !tiles) || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  // a row holds the neighbors g2 >= g1 of one pose: with a capacity of the
+  // longest pose the work list cannot overflow, and no host read-back of the
+  // overflow counter is needed
+  if (row_capacity < s->max_n_blocks) return TMOL_LJLK_EINVAL;
+  if (flags & (TMOL_LJLK_GENERAL_KERNEL | (0xff << 16))) return TMOL_LJLK_EINVAL;
+  const int64_t need = ws_layout(s, bt, row_capacity, dtype).total;
+  if (need > TMOL_LJLK_NOALLOC_SCRATCH_BYTES) return TMOL_LJLK_ESCRATCH;
+  void* ws = nullptr;
+  v = noalloc_scratch(&ws);
+  if (v) return v;
+  v = tmol_ljlk_neighbors(s, bt, coords, dtype, max_dis, tiles, n_tiles, ws, need, row_capacity,
+                          dense_neighbors, stream);
+  if (v) return v;
+  return tmol_ljlk_score(s, bt, coords, dtype, ws, need, row_capacity, flags, weights, energies,
+                         dvdx, grad, 0, stream);
 }
 
 int tmol_ljlk_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,

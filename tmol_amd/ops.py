@@ -29,7 +29,7 @@ import torch
 _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libtmol_ljlk_hip.so")
 
-OK, EINVAL, EOVERFLOW, EHIP, ENOTSUP = 0, 1, 2, 3, 4
+OK, EINVAL, EOVERFLOW, EHIP, ENOTSUP, ESCRATCH = 0, 1, 2, 3, 4, 5
 F32, F64 = 0, 1
 PADDED, RAGGED = 0, 1
 DVDX, WGRAD, BLOCK_PAIR, NO_ENERGY, ACCUMULATE, GENERAL_KERNEL = 2, 4, 8, 16, 32, 128
@@ -42,6 +42,7 @@ _ERRORS = {
     EOVERFLOW: "work list overflow",
     EHIP: "HIP runtime error",
     ENOTSUP: "unsupported configuration",
+    ESCRATCH: "the stack needs more device scratch than the no-allocation entry point holds",
 }
 
 
@@ -139,6 +140,12 @@ def lib():
             L.tmol_ljlk_device_cu_count.argtypes = [_P32]
             L.tmol_ljlk_census.argtypes = [
                 SP, BP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64, _VP, _VP,
+            ]
+            L.tmol_ljlk_noalloc_scratch_bytes.argtypes = []
+            L.tmol_ljlk_noalloc_scratch_bytes.restype = ctypes.c_int64
+            L.tmol_ljlk_score_noalloc.argtypes = [
+                SP, BP, _VP, ctypes.c_int32, ctypes.c_double, _VP, ctypes.c_int64,
+                ctypes.c_int64, ctypes.c_int32, _VP, _VP, _VP, _VP, _VP, _VP,
             ]
             _lib = L
         return _lib
