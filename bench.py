@@ -247,9 +247,11 @@ def main():
 
     achieved_tflops = flops / (score_ms * 1e-3) / 1e12
     abytes = algorithmic_bytes(stack.n_atoms_total, stack.n_blocks_total, stack.n_poses, n_items)
-    traffic = _load_profile("score_kernel_ragged512_traffic.json")
-    counters = _load_profile("score_kernel_ragged512_counters.json")
-    hbm_bytes = traffic.get("hbm_bytes_per_launch") if traffic else None
+    # the committed rocprofv3 summary of the kernel this run measured
+    kname = scorer.kernel_name()
+    counters = _load_profile(kname + "_ragged512_counters.json")
+    traffic = counters
+    hbm_bytes = counters.get("hbm_bytes_per_launch") if counters else None
 
     result = {
         "metric": METRIC,
@@ -273,7 +275,8 @@ def main():
             "atom_pair_evals_per_step": pair_evals,
             "atom_pair_evals_per_s": pair_evals / (ms_per_step * 1e-3),
             "graph": use_graph,
-            "launch": {"waves_per_simd": scorer.waves_per_simd, "wave_groups": scorer.wave_groups,
+            "launch": {"kernel": kname, "waves_per_simd": scorer.waves_per_simd,
+                       "wave_groups": scorer.wave_groups,
                        "row_grab": scorer.row_grab},
             "autotune": ({"choice": tune["choice"],
                           "timings_ms": {k: round(v * 1e3, 4) for k, v in tune["timings_s"].items()}}
@@ -293,15 +296,15 @@ def main():
             "unit": "TFLOP/s",
             "frac": achieved_tflops / FP32_PEAK_TFLOPS,
             "traffic": hbm_bytes,
-            "kernel": "score_rows_kernel<float, WGRAD> (FP32 vector ALU; peak = MI355X FP32 vector)",
+            "kernel": kname + "<float, WGRAD> (FP32 vector ALU; peak = MI355X FP32 vector)",
             "algorithmic_flops_per_launch": flops,
             "algorithmic_bytes_per_launch": abytes["score"],
             "hbm_frac": (hbm_bytes / (score_ms * 1e-3) / (HBM_PEAK_GBS * 1e9)
                          if hbm_bytes else None),
             "hbm_frac_algorithmic": abytes["score"] / (score_ms * 1e-3) / (HBM_PEAK_GBS * 1e9),
             "traffic_profile": ({k: traffic.get(k) for k in (
-                "commit", "hbm_read_bytes_per_launch", "hbm_write_bytes_per_launch")}
-                if traffic else None),
+                "commit", "hbm_read_bytes_per_launch", "hbm_write_bytes_per_launch",
+                "copy_calibration")} if traffic else None),
             "counters": counters.get("derived") if counters else None,
             "counters_commit": counters.get("commit") if counters else None,
         },

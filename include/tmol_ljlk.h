@@ -47,6 +47,8 @@ extern "C" {
 #define TMOL_LJLK_NO_ENERGY 16 /* skip the energy output (backward re-walk) */
 #define TMOL_LJLK_ACCUMULATE 32 /* add into dvdx / grad instead of overwriting */
 #define TMOL_LJLK_GENERAL_KERNEL 128 /* force the general (any block size) kernel */
+#define TMOL_LJLK_ROW_KERNEL 4096 /* energies / WGRAD: score block pair by block pair (the
+                                     row kernel) instead of streaming each row's atoms */
 #define TMOL_LJLK_DETERMINISTIC 1024 /* float32 DVDX / WGRAD (whole pose): gradients and
                                         per-pose energies summed in 64-bit fixed point in the
                                         workspace, then converted once: bit-identical results

@@ -1,0 +1,135 @@
+"""The learned pruner on the 1024 x 300-residue workload (reduced to 128
+held-out poses; training on 48 poses of the 512-pose batch's distribution,
+calibration on 32 poses of 300 residues):
+
+* the judge: the exact kernel's per-residue total-energy error of the pruned
+  batch is within 0.05 kcal/mol, or the fallback to exact scoring fires (and a
+  deliberately over-pruned batch shows that it does);
+* the kernel-level times of the exact and pruned scoring and the pruned
+  fraction;
+* what the trained network gains over the method without it: the pairs it
+  removes within the bound against (a) exact scoring, which removes none,
+  (b) the same network untrained, (c) a rule on the sphere gap alone, tuned to
+  the same bound; and the bound on any classifier (perfect labels)."""
+
+import json
+import sys
+
+import pytest
+import torch
+
+pytestmark = pytest.mark.gpu
+DEV = torch.device("cuda:0")
+
+
+def _say(msg):
+    print("[gpu-case] " + str(msg), flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def trained():
+    from tmol_amd import prune_bench
+
+    params, rep = prune_bench.train_pruner(DEV, n_pairs=400_000, epochs=4, n_poses=48, log=_say)
+    tau, table = prune_bench.calibrate(params, DEV, n_poses=32, log=_say)
+    return params, rep, tau
+
+
+def _held_out(n_poses=128):
+    from tmol_amd import prune_bench
+    from tmol_amd.generator import workload_poses
+
+    poses = workload_poses("pruner_1024x300", n_poses)
+    st, sc = prune_bench._scorer(poses, DEV)
+    x = st.coords.to(DEV).contiguous()
+    n_res = torch.tensor([p.n_blocks for p in poses], dtype=torch.float64)
+    return st, sc, x, n_res
+
+
+def test_pruned_batch_within_bound_or_fallback(trained):
+    from tmol_amd import prune_bench
+
+    params, rep, tau = trained
+    res = prune_bench.measure(params, tau, DEV, steps=5, warmup=2, n_poses=128, log=_say)
+    keys = ("threshold", "pruned_fraction", "max_err_per_res", "mean_err_per_res", "within_bound",
+            "fallback", "kernel_ms_exact", "kernel_ms_pruned", "prune_ms", "ms_exact", "ms_pruned",
+            "speedup", "perfect_labels")
+    _say(json.dumps({k: res[k] for k in keys}))
+    assert rep["pairs"] >= 300_000
+    assert res["fallback"] == (res["max_err_per_res"] > prune_bench.BOUND)
+    if res["fallback"]:
+        assert res["effective_poses_per_s"] == res["exact_poses_per_s"]
+    else:
+        assert res["max_err_per_res"] <= prune_bench.BOUND
+    assert 0.0 < res["pruned_fraction"] < 1.0
+    # removing pairs shortens the scoring kernel; what the network costs is
+    # reported beside it (prune_ms)
+    assert res["kernel_ms_pruned"] < res["kernel_ms_exact"]
+    pl = res["perfect_labels"]
+    assert pl["max_err_per_res"] <= prune_bench.BOUND
+    assert pl["pruned_fraction"] >= res["pruned_fraction"] - 0.05
+    assert pl["kernel_ms"] <= res["kernel_ms_exact"]
+
+
+def test_fallback_fires_on_an_over_pruned_batch(trained):
+    """Pruning every pair the network may touch (threshold 0, no gap guard)
+    breaks the bound; the judge then falls back to exact scoring."""
+    from tmol_amd import prune_bench
+
+    params, _, _ = trained
+    res = prune_bench.measure(params, 0.0, DEV, steps=2, warmup=1, n_poses=64, min_gap=-1e9,
+                              log=_say)
+    _say("over-pruned: max err/res %.3f, pruned %.3f, fallback %s"
+         % (res["max_err_per_res"], res["pruned_fraction"], res["fallback"]))
+    assert res["max_err_per_res"] > prune_bench.BOUND
+    assert res["fallback"] and res["effective_poses_per_s"] == res["exact_poses_per_s"]
+
+
+def test_network_gain_over_untrained_and_gap_rule(trained):
+    from tmol_amd import prune, prune_bench
+
+    params, _, tau = trained
+    st, sc, x, n_res = _held_out(128)
+    res = {}
+    pr = prune.Pruner(sc, params, threshold=tau)
+    err, frac = prune_bench.pose_errors(pr, sc, x, n_res)
+    res["trained"] = (float(err.max()), frac)
+    untrained = prune.PrunerParams(DEV, seed=7)
+    pu = prune.Pruner(sc, untrained, threshold=tau)
+    err_u, frac_u = prune_bench.pose_errors(pu, sc, x, n_res)
+    res["untrained"] = (float(err_u.max()), frac_u)
+    # the rule on the sphere gap alone: remove every pair (not a block with
+    # itself) whose gap is at least g
+    e_exact = prune_bench._weighted(sc.energy_and_grad(x)[0].clone())
+    sc.neighbors(x)
+    pairs = prune.work_list_pairs(sc)
+    sph = pr.inputs.spheres()
+    c1, c2 = sph[pairs[:, 0].long()], sph[pairs[:, 1].long()]
+    gap = (c1[:, :3] - c2[:, :3]).norm(dim=1) - c1[:, 3] - c2[:, 3]
+    w = sc._weights(None)
+    table = []
+    for g in (5.5, 5.0, 4.5, 4.2, 4.0, 3.9, 3.8, 3.7, 3.6, 3.5, 3.4, 3.3, 3.2, 3.1, 3.0, 2.5):
+        keep = ((gap < g) | (pairs[:, 0] == pairs[:, 1])).to(torch.uint8)
+        keep = torch.cat([keep, torch.ones(pr.keep.numel() - keep.numel(), dtype=torch.uint8,
+                                           device=DEV)])
+        sc.neighbors(x)
+        pr.compact_with(x, keep)
+        sc.score(x, 4, weights=w, energies=sc.energies, grad=sc.grad)
+        e = prune_bench._weighted(sc.energies.clone())
+        e_max = float(((e - e_exact).abs().cpu() / n_res).max())
+        f = 1.0 - float(keep[:pairs.shape[0]].sum()) / pairs.shape[0]
+        table.append((g, f, e_max))
+        _say("gap rule g=%.1f: pruned %.3f, max err/res %.4f" % (g, f, e_max))
+    err_t, frac_t = res["trained"]
+    res["gap_rule_at_bound"] = max(((f, g) for g, f, e in table if e <= prune_bench.BOUND),
+                                   default=(0.0, None))
+    res["gap_rule_at_network_error"] = max(((f, g) for g, f, e in table if e <= err_t),
+                                           default=(0.0, None))
+    _say("network gain: %s" % json.dumps(res))
+    assert err_t <= prune_bench.BOUND
+    # training lowers the error at the threshold: the untrained network (which
+    # removes whatever the gap guard lets it) errs more
+    assert res["untrained"][0] > err_t
+    # at the network's own error, the gap rule removes no more pairs than it
+    assert frac_t >= res["gap_rule_at_network_error"][0]

@@ -24,6 +24,7 @@
 #include "ljlk_neighbors.hh"
 #include "ljlk_score_general.hh"
 #include "ljlk_score_rows.hh"
+#include "ljlk_score_stream.hh"
 #include "ljlk_prune.hh"
 #include "ljlk_rotamer.hh"
 #include "ljlk_elec_rotamer.hh"
@@ -210,6 +211,26 @@ static void launch_rows_occ(const ScoreArgs<Real>& a, int occ, int n_wg, hipStre
   go(score_rows_kernel<Real, MODE>);
 }
 
+// the stream kernel (whole-pose energies / WGRAD, blocks of <= 32 atoms),
+// compiled for `occ` waves per SIMD (float: 3, 4 or 5; 0 = 4)
+template <typename Real, int MODE>
+static void launch_stream(const ScoreArgs<Real>& a, int occ, int n_wg, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
+                     8 * QSTRIDE, 0u);
+  auto go = [&](auto k) {
+    if (n_wg <= 0) n_wg = resident_workgroups(k);
+    hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+  };
+  if constexpr (std::is_same<Real, float>::value) {
+    switch (occ) {
+      case 3: go(score_stream_kernel<Real, MODE, 3>); return;
+      case 5: go(score_stream_kernel<Real, MODE, 5>); return;
+      default: break;
+    }
+  }
+  go(score_stream_kernel<Real, MODE>);
+}
+
 // ek != nullptr: elec is scored in the same pass (M_ELEC; tables prepared in
 // the workspace by tmol_ljlk_elec_prepare), energies and weights [4, P]
 template <typename Real>
@@ -239,7 +260,7 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
   const int P = st->n_poses;
   const int64_t NA3 = st->n_atoms_total * 3;
   if (det) {
-    // float32 whole-pose DVDX / WGRAD (row kernel; not the general one)
+    // float32 whole-pose DVDX / WGRAD (stream or row kernel; not the general one)
     if (!std::is_same<Real, float>::value || general || bp || ek) return TMOL_LJLK_ENOTSUP;
     if (!(flags & (TMOL_LJLK_DVDX | TMOL_LJLK_WGRAD))) return TMOL_LJLK_ENOTSUP;
     const int planes = (flags & TMOL_LJLK_DVDX) ? 3 : 1;
@@ -264,13 +285,23 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
   // launch variants (autotuner): waves per SIMD and rows per queue grab
   const int occ = (flags >> 16) & 15;
   a.row_grab = (flags >> 20) & 15;
+  // whole-pose energies and the weighted gradient stream each row's atoms
+  const bool stream_k = !general && !ek && !(flags & TMOL_LJLK_ROW_KERNEL) &&
+                        (mode == 0 || mode == M_WGRAD || mode == (M_WGRAD | M_NOE));
   if (det) {
     if constexpr (std::is_same<Real, float>::value) {
-      switch (mode) {
-        case M_DVDX: launch_rows<Real, M_DVDX | M_DET>(a, n_wg, stream); break;
-        case M_WGRAD: launch_rows<Real, M_WGRAD | M_DET>(a, n_wg, stream); break;
-        case M_WGRAD | M_NOE: launch_rows<Real, M_WGRAD | M_NOE | M_DET>(a, n_wg, stream); break;
-        default: return TMOL_LJLK_ENOTSUP;
+      if (stream_k) {
+        if (mode == M_WGRAD)
+          launch_stream<Real, M_WGRAD | M_DET>(a, occ, n_wg, stream);
+        else
+          launch_stream<Real, M_WGRAD | M_NOE | M_DET>(a, occ, n_wg, stream);
+      } else {
+        switch (mode) {
+          case M_DVDX: launch_rows<Real, M_DVDX | M_DET>(a, n_wg, stream); break;
+          case M_WGRAD: launch_rows<Real, M_WGRAD | M_DET>(a, n_wg, stream); break;
+          case M_WGRAD | M_NOE: launch_rows<Real, M_WGRAD | M_NOE | M_DET>(a, n_wg, stream); break;
+          default: return TMOL_LJLK_ENOTSUP;
+        }
       }
       const int64_t n = (mode == M_DVDX ? 3 : 1) * NA3;
       hipLaunchKernelGGL(fixed_to_real_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream,
@@ -284,7 +315,14 @@ static int run_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp
     }
     return TMOL_LJLK_ENOTSUP;
   }
-  if (occ && !general && !ek && (mode == 0 || mode == M_WGRAD)) {
+  if (stream_k) {
+    switch (mode) {
+      case 0: launch_stream<Real, 0>(a, occ, n_wg, stream); break;
+      case M_WGRAD: launch_stream<Real, M_WGRAD>(a, occ, n_wg, stream); break;
+      default: launch_stream<Real, M_WGRAD | M_NOE>(a, occ, n_wg, stream); break;
+    }
+    mode = -1;  // launched
+  } else if (occ && !general && !ek && (mode == 0 || mode == M_WGRAD)) {
     if (mode == 0)
       launch_rows_occ<Real, 0>(a, occ, n_wg, stream);
     else

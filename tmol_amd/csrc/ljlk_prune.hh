@@ -402,18 +402,24 @@ __device__ __forceinline__ void prune_layer(uint16_t (*act)[PSH], const uint16_t
   wave_sync();
 }
 
-// Scores every pair of a flat list [n, 2] (n read from device memory, so the
+// Scores the pairs of a flat list [n, 2] (n read from device memory, so the
 // launch shape does not depend on it) with the MLP; keep[i] = 0 when the pair
 // is predicted negligible with probability above the threshold
 // (logit > logit_cut), is not a block with itself, and its sphere gap
 // |c1 - c2| - r1 - r2 -- a lower bound on every atom-pair distance of the two
 // blocks -- is at least min_gap (pairs whose atoms may come closer are always
-// scored: a clash there is worth far more than the bound).  logits may be null.
+// scored: a clash there is worth far more than the bound).
+// Pairs the rule keeps whatever the network says (a block with itself, a gap
+// below min_gap) are marked kept without running the network: each wave
+// walks chunks of 64 pairs, tests them one per lane, and queues only the
+// others; the network runs on 32 queued pairs at a time.  With logits
+// (analysis) every pair goes through the network and gets its logit.
 __global__ __launch_bounds__(PI_WAVES * 64) void prune_infer_kernel(
     const int64_t* __restrict__ n_pairs, const int32_t* __restrict__ pairs, PruneBlocks B,
     PruneNet net, float logit_cut, float min_gap, uint8_t* __restrict__ keep,
     float* __restrict__ logits) {
   __shared__ PruneLDS L;
+  __shared__ int32_t queue_all[PI_WAVES][96];
   const int64_t n = *n_pairs;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   for (int i = tid; i < PRUNE_H * PRUNE_K1; i += blockDim.x) L.w1[i / PRUNE_K1][i % PRUNE_K1] = net.W1[i];
@@ -430,77 +436,110 @@ __global__ __launch_bounds__(PI_WAVES * 64) void prune_infer_kernel(
   if (tid == 0) L.b4 = net.b4[0];
   __syncthreads();
   uint16_t (*act)[PSH] = L.act[wave];
-  const int64_t n_tiles = (n + 31) / 32;
+  int32_t* queue = queue_all[wave];
+  const bool all = logits != nullptr;
+  const int64_t n_chunks = (n + 63) / 64;
   const int r = lane & 31, half = lane >> 5;
-  for (int64_t t = int64_t(blockIdx.x) * PI_WAVES + wave; t < n_tiles;
-       t += int64_t(gridDim.x) * PI_WAVES) {
-    const int64_t i = t * 32 + r;
-    const bool valid = i < n;
-    float gap = 0.f;  // sphere gap of row r (half-0 lanes)
-    int g1 = 0, g2 = 0;
-    if (valid) {
-      g1 = pairs[2 * i];
-      g2 = pairs[2 * i + 1];
-    }
-    // features: lane r writes the columns of block g1 of row r, lane r + 32
-    // those of g2 (same code on both halves, only the offsets differ); the
-    // centroid distance and sequence separation use the partner lane's block
-    {
-      const int gb = half ? g2 : g1;
-      const float4 sph = *(const float4*)(B.sphere + 4 * int64_t(gb));
-      const float4 e0 = *(const float4*)(B.emb + B.emb_row[gb] * PRUNE_EMB);
-      const float4 e1 = *(const float4*)(B.emb + B.emb_row[gb] * PRUNE_EMB + 4);
-      const float* sc = B.sincos + int64_t(gb) * 2 * PRUNE_TORS;
-      float t[2 * PRUNE_TORS];
-#pragma unroll
-      for (int k = 0; k < 2 * PRUNE_TORS; k += 2) {
-        const float2 v = *(const float2*)(sc + k);
-        t[k] = v.x;
-        t[k + 1] = v.y;
+  int nq = 0;  // queued pairs (wave-uniform)
+  for (int64_t c = int64_t(blockIdx.x) * PI_WAVES + wave;; c += int64_t(gridDim.x) * PI_WAVES) {
+    const bool more = c < n_chunks;
+    if (more) {
+      // ---- which pairs of this chunk need the network
+      const int64_t i = c * 64 + lane;
+      bool cand = false;
+      if (i < n) {
+        const int a = pairs[2 * i], b = pairs[2 * i + 1];
+        const float4 s1 = *(const float4*)(B.sphere + 4 * int64_t(a));
+        const float4 s2 = *(const float4*)(B.sphere + 4 * int64_t(b));
+        const float dx = s1.x - s2.x, dy = s1.y - s2.y, dz = s1.z - s2.z;
+        const float gap = sqrtf(dx * dx + dy * dy + dz * dz) - s1.w - s2.w;
+        cand = all || (a != b && !(gap < min_gap));
+        if (!cand) keep[i] = 1;
       }
-      const int sp = B.seqpos[gb];
-      const float ox = __shfl_xor(sph.x, 32, WAVE), oy = __shfl_xor(sph.y, 32, WAVE);
-      const float oz = __shfl_xor(sph.z, 32, WAVE);
-      const float ow = __shfl_xor(sph.w, 32, WAVE);
-      const int osp = __shfl_xor(sp, 32, WAVE);
-      const float dx = sph.x - ox, dy = sph.y - oy, dz = sph.z - oz;
-      gap = sqrtf(dx * dx + dy * dy + dz * dz) - sph.w - ow;
-      const float ev[PRUNE_EMB] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
-      const uint16_t z = 0;
-      // columns: 0 distance, 1 + half radius, 3 + 8 half embedding,
-      // 19 + 10 half torsions, 39 sequence separation, 40..47 zero
-      if (half == 0) {
-        act[r][0] = valid ? to_bf16(sqrtf(dx * dx + dy * dy + dz * dz) * 0.1f) : z;
-        int d = sp - osp;
-        d = d < 0 ? -d : d;
-        act[r][39] = valid ? to_bf16(float(d < 32 ? d : 32) * (1.0f / 32.0f)) : z;
-      } else {
-#pragma unroll
-        for (int k = PRUNE_IN; k < PRUNE_K1; ++k) act[r][k] = z;
+      const uint64_t m = __ballot(cand);
+      if (cand) queue[nq + lanes_below(m)] = int32_t(i);
+      nq += __popcll(m);
+      wave_sync();
+    }
+    while (nq >= 32 || (!more && nq > 0)) {
+      // ---- the network on queued pairs 0 .. take - 1
+      const int take = nq < 32 ? nq : 32;
+      const bool valid = r < take;
+      const int64_t i = valid ? queue[r] : 0;
+      float gap = 0.f;  // sphere gap of row r (half-0 lanes)
+      int g1 = 0, g2 = 0;
+      if (valid) {
+        g1 = pairs[2 * i];
+        g2 = pairs[2 * i + 1];
       }
-      act[r][1 + half] = valid ? to_bf16(sph.w * 0.1f) : z;
+      // features: lane r writes the columns of block g1 of row r, lane r + 32
+      // those of g2 (same code on both halves, only the offsets differ); the
+      // centroid distance and sequence separation use the partner lane's block
+      {
+        const int gb = half ? g2 : g1;
+        const float4 sph = *(const float4*)(B.sphere + 4 * int64_t(gb));
+        const float4 e0 = *(const float4*)(B.emb + B.emb_row[gb] * PRUNE_EMB);
+        const float4 e1 = *(const float4*)(B.emb + B.emb_row[gb] * PRUNE_EMB + 4);
+        const float* sc = B.sincos + int64_t(gb) * 2 * PRUNE_TORS;
+        float t[2 * PRUNE_TORS];
 #pragma unroll
-      for (int k = 0; k < PRUNE_EMB; ++k) act[r][3 + 8 * half + k] = valid ? to_bf16(ev[k]) : z;
+        for (int k = 0; k < 2 * PRUNE_TORS; k += 2) {
+          const float2 v = *(const float2*)(sc + k);
+          t[k] = v.x;
+          t[k + 1] = v.y;
+        }
+        const int sp = B.seqpos[gb];
+        const float ox = __shfl_xor(sph.x, 32, WAVE), oy = __shfl_xor(sph.y, 32, WAVE);
+        const float oz = __shfl_xor(sph.z, 32, WAVE);
+        const float ow = __shfl_xor(sph.w, 32, WAVE);
+        const int osp = __shfl_xor(sp, 32, WAVE);
+        const float dx = sph.x - ox, dy = sph.y - oy, dz = sph.z - oz;
+        gap = sqrtf(dx * dx + dy * dy + dz * dz) - sph.w - ow;
+        const float ev[PRUNE_EMB] = {e0.x, e0.y, e0.z, e0.w, e1.x, e1.y, e1.z, e1.w};
+        const uint16_t z = 0;
+        // columns: 0 distance, 1 + half radius, 3 + 8 half embedding,
+        // 19 + 10 half torsions, 39 sequence separation, 40..47 zero
+        if (half == 0) {
+          act[r][0] = valid ? to_bf16(sqrtf(dx * dx + dy * dy + dz * dz) * 0.1f) : z;
+          int d = sp - osp;
+          d = d < 0 ? -d : d;
+          act[r][39] = valid ? to_bf16(float(d < 32 ? d : 32) * (1.0f / 32.0f)) : z;
+        } else {
 #pragma unroll
-      for (int k = 0; k < 2 * PRUNE_TORS; ++k) act[r][19 + 10 * half + k] = valid ? to_bf16(t[k]) : z;
+          for (int k = PRUNE_IN; k < PRUNE_K1; ++k) act[r][k] = z;
+        }
+        act[r][1 + half] = valid ? to_bf16(sph.w * 0.1f) : z;
+#pragma unroll
+        for (int k = 0; k < PRUNE_EMB; ++k) act[r][3 + 8 * half + k] = valid ? to_bf16(ev[k]) : z;
+#pragma unroll
+        for (int k = 0; k < 2 * PRUNE_TORS; ++k) act[r][19 + 10 * half + k] = valid ? to_bf16(t[k]) : z;
+      }
+      wave_sync();
+      prune_layer<PRUNE_K1, PS1>(act, L.w1, L.b[0], lane);
+      prune_layer<PRUNE_H, PSH>(act, L.w2, L.b[1], lane);
+      prune_layer<PRUNE_H, PSH>(act, L.w3, L.b[2], lane);
+      // head: lanes r and r + 32 each sum half of row r
+      float sum = 0.f;
+      for (int k = 64 * half; k < 64 * half + 64; k += 2) {
+        const uint32_t two = *(const uint32_t*)&act[r][k];
+        sum += from_bf16(uint16_t(two & 0xffffu)) * L.w4[k] + from_bf16(uint16_t(two >> 16)) * L.w4[k + 1];
+      }
+      sum += __shfl_xor(sum, 32, WAVE);
+      if (half == 0 && valid) {
+        const float logit = sum + L.b4;
+        keep[i] = (g1 == g2 || !(logit > logit_cut) || gap < min_gap) ? 1 : 0;
+        if (logits) logits[i] = logit;
+      }
+      // ---- drop the taken pairs from the queue (at most 63 remain)
+      const int rem = nq - take;
+      int32_t v = 0;
+      if (lane < rem) v = queue[take + lane];
+      wave_sync();
+      if (lane < rem) queue[lane] = v;
+      nq = rem;
+      wave_sync();
     }
-    wave_sync();
-    prune_layer<PRUNE_K1, PS1>(act, L.w1, L.b[0], lane);
-    prune_layer<PRUNE_H, PSH>(act, L.w2, L.b[1], lane);
-    prune_layer<PRUNE_H, PSH>(act, L.w3, L.b[2], lane);
-    // head: lanes r and r + 32 each sum half of row r
-    float s = 0.f;
-    for (int k = 64 * half; k < 64 * half + 64; k += 2) {
-      const uint32_t two = *(const uint32_t*)&act[r][k];
-      s += from_bf16(uint16_t(two & 0xffffu)) * L.w4[k] + from_bf16(uint16_t(two >> 16)) * L.w4[k + 1];
-    }
-    s += __shfl_xor(s, 32, WAVE);
-    if (half == 0 && valid) {
-      const float logit = s + L.b4;
-      keep[i] = (g1 == g2 || !(logit > logit_cut) || gap < min_gap) ? 1 : 0;
-      if (logits) logits[i] = logit;
-    }
-    wave_sync();
+    if (!more) break;
   }
 }
 

@@ -377,7 +377,7 @@ WORKLOADS = {
     # pruner training set: the 512-pose batch's distribution (log-uniform
     # 50-1200 residues); calibration set: poses like the held-out batch (300
     # residues); each with its own seed
-    "pruner_train": dict(n_poses=160, res_range=(50, 1200), log_uniform=True, seed=MASTER_SEED + 5),
+    "pruner_train": dict(n_poses=192, res_range=(50, 1200), log_uniform=True, seed=MASTER_SEED + 5),
     "pruner_calib": dict(n_poses=128, res_range=(300, 300), log_uniform=False, seed=MASTER_SEED + 6),
     # rotamer-pair scoring (repacking): poses whose every residue gets
     # chi-grid rotamers (tmol_amd.rotamers)

@@ -34,6 +34,7 @@ F32, F64 = 0, 1
 PADDED, RAGGED = 0, 1
 DVDX, WGRAD, BLOCK_PAIR, NO_ENERGY, ACCUMULATE, GENERAL_KERNEL = 2, 4, 8, 16, 32, 128
 DETERMINISTIC = 1024  # float32 DVDX / WGRAD: fixed-point global sums, bit-identical run to run
+ROW_KERNEL = 4096  # energies / WGRAD: the block-pair row kernel instead of the stream kernel
 WAVES_PER_SIMD, ROW_GRAB = 1 << 16, 1 << 20  # multipliers of the launch-variant fields
 NBR_TILE = 64
 

@@ -407,6 +407,19 @@ class Pruner:
         ops._check(L.tmol_ljlk_prune_compact(self.G, pc, pi, self.cap, _p(self.row_offset),
                                              _p(self.keep), st), "compact")
 
+    def compact_with(self, coords: torch.Tensor, keep: torch.Tensor):
+        """After the scorer's neighbor pass: flatten and compact the work list
+        with a given keep flag per flattened pair (uint8 [n]) instead of the
+        network's -- the same flattening and compaction kernels as
+        ``prune``."""
+        L = _lib()
+        st = ops._stream(coords.device)
+        pc, pi = self._rows()
+        ops._check(L.tmol_ljlk_prune_flatten(self.G, pc, pi, self.cap, _p(self.scratch),
+                                             _p(self.row_offset), _p(self.pairs), st), "flatten")
+        ops._check(L.tmol_ljlk_prune_compact(self.G, pc, pi, self.cap, _p(self.row_offset),
+                                             _p(keep), st), "compact")
+
     def n_pairs_host(self) -> int:
         return int(self.n_pairs.item())
 

@@ -41,8 +41,8 @@ def _weighted(E):
     return (W.to(E.device)[:, None] * E.double()).sum(0)
 
 
-def train_pruner(dev, n_pairs=1_000_000, epochs=6, seed=0, log=print):
-    poses = workload_poses("pruner_train")
+def train_pruner(dev, n_pairs=1_000_000, epochs=6, seed=0, log=print, n_poses=None):
+    poses = workload_poses("pruner_train", n_poses)
     st, sc = _scorer(poses, dev)
     x = st.coords.to(dev)
     pairs, labels, Ew = prune.build_training_set(sc, x)
@@ -82,8 +82,8 @@ def pose_errors(pruner, sc, x, n_res):
 
 def calibrate(params, dev, candidates=(0.5, 0.7, 0.8, 0.9, 0.95, 0.98, 0.99, 0.995, 0.999),
               min_gap=prune.MIN_GAP,
-              log=print):
-    poses = workload_poses("pruner_calib")
+              log=print, n_poses=None):
+    poses = workload_poses("pruner_calib", n_poses)
     st, sc = _scorer(poses, dev)
     x = st.coords.to(dev)
     n_res = torch.tensor([p.n_blocks for p in poses], dtype=torch.float64)
@@ -126,8 +126,9 @@ def _time(g, steps, warmup):
     return (time.perf_counter() - t0) / steps
 
 
-def measure(params, threshold, dev, steps=20, warmup=3, log=print, min_gap=prune.MIN_GAP):
-    poses = workload_poses("pruner_1024x300")
+def measure(params, threshold, dev, steps=20, warmup=3, log=print, min_gap=prune.MIN_GAP,
+            n_poses=None):
+    poses = workload_poses("pruner_1024x300", n_poses)
     st, sc = _scorer(poses, dev)
     x = st.coords.to(dev).contiguous()
     n_res = torch.tensor([p.n_blocks for p in poses], dtype=torch.float64)
@@ -163,6 +164,62 @@ def measure(params, threshold, dev, steps=20, warmup=3, log=print, min_gap=prune
     n_after = int(cnt.to(torch.int64).clamp(max=sc.capacity).sum())
     within = bool(float(err.max()) <= BOUND)
     P = len(poses)
+    # kernel-level times of the scoring kernel on the exact and pruned lists
+    w_ = w
+
+    def kernel_ms(prep):
+        sc.neighbors(x)
+        prep()
+        torch.cuda.synchronize()
+        a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        a.record()
+        for _ in range(steps):
+            sc.score(x, 4, weights=w_, energies=sc.energies, grad=sc.grad)
+        b.record()
+        torch.cuda.synchronize()
+        return a.elapsed_time(b) / steps
+
+    k_exact = kernel_ms(lambda: None)
+    k_pruned = kernel_ms(lambda: pr.prune(x))
+    # the bound on what any classifier can gain: remove exactly the pairs the
+    # exact kernel finds negligible (|E_pair| < 1e-3 kcal/mol, weighted),
+    # through the same flattening and compaction kernels
+    Ew, labels = prune.pair_labels(sc, x)
+    pairs_all = prune.work_list_pairs(sc)
+    perfect_keep = ((labels == 0) | (pairs_all[:, 0] == pairs_all[:, 1])).to(torch.uint8)
+    perfect_keep = torch.cat([perfect_keep, torch.ones(pr.keep.numel() - perfect_keep.numel(),
+                                                       dtype=torch.uint8, device=dev)])
+
+    def perfect():
+        sc.neighbors(x)
+        pr.compact_with(x, perfect_keep)
+        sc.score(x, 4, weights=w, energies=sc.energies, grad=sc.grad)
+
+    g_perfect = _graph(perfect)
+    t_perfect = _time(g_perfect, steps, warmup)
+    g_perfect.replay()
+    torch.cuda.synchronize()
+    e_perfect = _weighted(sc.energies.clone())
+    err_perfect = (e_perfect - e_exact).abs().cpu() / n_res
+    k_perfect = kernel_ms(lambda: pr.compact_with(x, perfect_keep))
+    n_perfect = int(perfect_keep[:pairs_all.shape[0]].to(torch.int64).sum())
+    # inference alone (flatten + network + compaction)
+    sc.neighbors(x)
+    torch.cuda.synchronize()
+    a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    a.record()
+    for _ in range(steps):
+        sc.neighbors(x)
+        pr.prune(x)
+    b.record()
+    torch.cuda.synchronize()
+    t_nbr_prune = a.elapsed_time(b) / steps
+    a.record()
+    for _ in range(steps):
+        sc.neighbors(x)
+    b.record()
+    torch.cuda.synchronize()
+    t_nbr = a.elapsed_time(b) / steps
     out = {
         "workload": "pruner_1024x300", "poses": P, "atoms": st.n_atoms_total,
         "threshold": threshold, "min_gap": min_gap,
@@ -175,6 +232,16 @@ def measure(params, threshold, dev, steps=20, warmup=3, log=print, min_gap=prune
         "within_bound": within, "fallback": not within,
         # a batch over the bound is rescored exactly: its throughput is the exact one
         "effective_poses_per_s": P / t_pruned if within else P / t_exact,
+        "kernel_ms_exact": k_exact, "kernel_ms_pruned": k_pruned,
+        "prune_ms": t_nbr_prune - t_nbr,
+        "perfect_labels": {
+            "ms": 1e3 * t_perfect, "speedup": t_exact / t_perfect,
+            "kernel_ms": k_perfect, "pairs": n_perfect,
+            "pruned_fraction": 1.0 - n_perfect / max(n_before, 1),
+            # share of the scoring kernel's time the negligible pairs cost
+            "negligible_cost_share": 1.0 - k_perfect / k_exact,
+            "max_err_per_res": float(err_perfect.max()),
+        },
     }
     log(json.dumps(out))
     return out

@@ -229,10 +229,15 @@ class LJLKRaggedScorer:
 
     # -- autotuner -------------------------------------------------------------
     def launch_flags(self) -> int:
-        """Score-flag bits of the chosen launch variant (waves per SIMD the
-        scoring kernel is compiled for, rows per queue grab)."""
+        """Score-flag bits of the chosen launch variant (the kernel, waves per
+        SIMD it is compiled for, rows per queue grab)."""
         return (ops.WAVES_PER_SIMD * int(self.waves_per_simd)
-                | ops.ROW_GRAB * int(self.row_grab))
+                | ops.ROW_GRAB * int(self.row_grab)
+                | (ops.ROW_KERNEL if self.kernel == "row" else 0))
+
+    def kernel_name(self) -> str:
+        """The scoring kernel the energy + gradient step runs."""
+        return "score_rows_kernel" if self.kernel == "row" else "score_stream_kernel"
 
     def _time_step(self, coords, w, reps):
         ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True))
@@ -255,8 +260,10 @@ class LJLKRaggedScorer:
         (workgroups of 4 waves; 0 = one per resident slot) and the rows taken
         per queue grab, searched one after the other from the library's
         defaults.  Results are cached per device and workload signature in
-        the user's cache directory."""
-        key = "%s|G%d|NA%d|P%d|%s|v2" % (torch.cuda.get_device_name(self.device), self.G,
+        the user's cache directory.  The kernel itself is the first choice:
+        the stream kernel (each row's atoms streamed through test chunks) or
+        the row kernel (block pair by block pair)."""
+        key = "%s|G%d|NA%d|P%d|%s|v3" % (torch.cuda.get_device_name(self.device), self.G,
                                          self.NA, self.P, self.dtype)
         table = {}
         path = _tune_cache_path()
@@ -272,25 +279,26 @@ class LJLKRaggedScorer:
         ops._check(ops.lib().tmol_ljlk_device_cu_count(ctypes.byref(cu)), "cu count")
         n_cu = max(int(cu.value), 1)
         w = self._weights(weights)
-        choice = {"waves_per_simd": 0, "wave_groups": 0, "row_grab": 0}
+        choice = {"kernel": "stream", "waves_per_simd": 0, "wave_groups": 0, "row_grab": 0}
         timings = {}
 
         def run(**kw):
             c = dict(choice, **kw)
             self.set_launch(**c)
             t = self._time_step(coords, w, reps)
-            timings["simd%d|wg%d|grab%d" % (c["waves_per_simd"], c["wave_groups"],
-                                            c["row_grab"])] = t
+            timings["%s|simd%d|wg%d|grab%d" % (c["kernel"], c["waves_per_simd"],
+                                               c["wave_groups"], c["row_grab"])] = t
             if log:
                 log("autotune %s: %.4f ms" % (c, 1e3 * t))
             return t
 
-        stages = [("waves_per_simd", [0, 3, 4, 5] if self.dtype == torch.float32 else [0]),
+        stages = [("kernel", ["stream", "row"]),
+                  ("waves_per_simd", [0, 3, 4, 5] if self.dtype == torch.float32 else [0]),
                   ("wave_groups", [0] + [n_cu * k for k in (2, 4, 8)]),
                   ("row_grab", [0, 2, 4])]
         for name, values in stages:
             best = min(values, key=lambda v: run(**{name: v}))
-            choice[name] = int(best)
+            choice[name] = best if name == "kernel" else int(best)
         self.set_launch(**choice)
         entry = {"choice": choice, "cu": n_cu, "timings_s": timings,
                  "best_s": min(timings.values())}
@@ -304,7 +312,8 @@ class LJLKRaggedScorer:
                 pass
         return entry
 
-    def set_launch(self, waves_per_simd=0, wave_groups=0, row_grab=0):
+    def set_launch(self, kernel="stream", waves_per_simd=0, wave_groups=0, row_grab=0):
+        self.kernel = str(kernel)
         self.waves_per_simd = int(waves_per_simd)
         self.wave_groups = int(wave_groups)
         self.row_grab = int(row_grab)
