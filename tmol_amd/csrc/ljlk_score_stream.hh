@@ -50,6 +50,9 @@ struct StreamLDS {
   using C = RowCtx<Real, MODE>;
   using T = typename LdsAcc<Real>::T;
   T grad[1][C::NG > 0 ? C::NG : 1][3][C::DERIV ? SSLOTS : 1];
+  // second copy of g1's accumulators, taken by odd lanes: consecutive FIFO
+  // entries share their g1 atom, and LDS atomics on one address serialize
+  T g1b[C::NG > 0 ? C::NG : 1][3][C::DERIV ? TILE : 1];
   T ebp[1][3];  // block-pair modes run the row kernel; kept for the evaluators
   Real wt[1][3];
   alignas(4 * sizeof(Real)) Real pos[SSLOTS][4];  // x, y, z, lj_radius
@@ -106,6 +109,28 @@ __device__ __forceinline__ void conn_rows(const StackView& s, int g1, int g2, in
   for (int c1 = 0; c1 < nc1; ++c1) cs[c1] = (uint32_t)e[4 + c1];
 }
 
+// gradient of an entry's pair: + to the g1 atom's slot s1, - to slot s2;
+// odd lanes add g1 slots into the second copy
+template <typename Real, int MODE>
+__device__ __forceinline__ void grad_scatter(StreamLDS<Real, MODE>& L, int term, int s1, int s2,
+                                             Real gx, Real gy, Real gz, int lane) {
+  using Acc = LdsAcc<Real>;
+  using T = typename Acc::T;
+  const bool odd = lane & 1;
+  const T ex = Acc::enc(gx), ey = Acc::enc(gy), ez = Acc::enc(gz);
+  T* b1 = odd ? &L.g1b[term][0][s1] : &L.grad[0][term][0][s1];
+  const int k1 = odd ? TILE : SSLOTS;
+  const bool g2_in_g1 = s2 < TILE;
+  T* b2 = (odd && g2_in_g1) ? &L.g1b[term][0][s2] : &L.grad[0][term][0][s2];
+  const int k2 = (odd && g2_in_g1) ? TILE : SSLOTS;
+  atomicAdd(b1, ex);
+  atomicAdd(b1 + k1, ey);
+  atomicAdd(b1 + 2 * k1, ez);
+  atomicAdd(b2, T(0) - ex);
+  atomicAdd(b2 + k2, T(0) - ey);
+  atomicAdd(b2 + 2 * k2, T(0) - ez);
+}
+
 // flush the gradients of ring chunk r (n atoms, slots TILE + SCH r ...) to
 // their atoms and zero them; lane l takes component l % 3 of atom l / 3
 template <typename Real, int MODE>
@@ -146,8 +171,9 @@ __device__ __forceinline__ void flush_g1(StreamLDS<Real, MODE>& L, const ScoreAr
     const int a = i / 3, k = i - 3 * a;
 #pragma unroll
     for (int t = 0; t < C::NG; ++t) {
-      const typename LdsAcc<Real>::T acc = L.grad[0][t][k][a];
+      const typename LdsAcc<Real>::T acc = L.grad[0][t][k][a] + L.g1b[t][k][a];
       L.grad[0][t][k][a] = 0;
+      L.g1b[t][k][a] = 0;
       if (C::DET) {
         if (acc != 0) atomicAdd(&A.fixed[(int64_t(t) * A.n_atoms + atom0) * 3 + i], acc);
         continue;
@@ -187,6 +213,8 @@ __global__ __launch_bounds__(256, (stream_waves_per_simd<MODE, OCC>())) void sco
   if (C::DERIV) {
     auto* g = &L.grad[0][0][0][0];
     for (int i = lane; i < (C::NG > 0 ? C::NG : 1) * 3 * SSLOTS; i += WAVE) g[i] = 0;
+    auto* gb = &L.g1b[0][0][0];
+    for (int i = lane; i < (C::NG > 0 ? C::NG : 1) * 3 * TILE; i += WAVE) gb[i] = 0;
   }
   wave_sync();
 
