@@ -318,7 +318,9 @@ def main():
         result["pruned"] = {k: pr[k] for k in (
             "workload", "poses", "threshold", "exact_poses_per_s", "pruned_poses_per_s", "speedup",
             "pruned_fraction", "max_err_per_res", "mean_err_per_res", "within_bound", "fallback",
-            "effective_poses_per_s")}
+            "pays_off", "effective_poses_per_s", "kernel_ms_exact", "kernel_ms_pruned", "prune_ms",
+            "perfect_labels")}
+        result["pruned"]["train_pairs"] = pr["training"]["pairs"]
         result["pruned"]["train_seconds"] = pr["training"]["train_seconds"]
         result["pruned"]["train_accuracy"] = pr["training"]["train_accuracy"]
     if not args.no_rotamer:

@@ -513,6 +513,22 @@ int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
                             int64_t n_units, int64_t n_pairs, int32_t flags, const void* dTdV,
                             void* energies, void* grad, int32_t n_workgroups, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Segmented L-BFGS history products (float64), for the minimizer that
+ * drives this op (no op counterpart: the reference's LBFGS_Armijo,
+ * tmol/optimization/lbfgs_armijo.py:6-58, does this algebra in torch).  The
+ * history is [n_segments, history, seg_len] (rows contiguous); rows
+ * 0 .. n_used - 1 are used.
+ *   tmol_lbfgs_dots:    out[k, p, i] = H[p, i, :] . vecs[k, p, :], k < n_vecs (<= 2)
+ *   tmol_lbfgs_combine: out[p, :] = g[p, :] + sum_i c1[p, i] H1[p, i, :]
+ *                                             + c2[p, i] H2[p, i, :]   (c: [P, n_used]) */
+int tmol_lbfgs_dots(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,
+                    const double* H, const double* vecs, int32_t n_vecs, double* out,
+                    void* stream);
+int tmol_lbfgs_combine(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,
+                       const double* g, const double* H1, const double* c1, const double* H2,
+                       const double* c2, double* out, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

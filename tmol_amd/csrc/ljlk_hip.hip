@@ -30,6 +30,7 @@
 #include "ljlk_elec_rotamer.hh"
 #include "ljlk_topology.hh"
 #include "ljlk_elec.hh"
+#include "lbfgs.hh"
 
 #include <cmath>
 
@@ -1481,6 +1482,33 @@ int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
                                             n_pairs, dTdV, energies, grad, acc, n_workgroups, st);
   return run_elec_rot_score<double, false>(rs, bt, e, coords, workspace, L, units, n_units,
                                            n_pairs, dTdV, energies, grad, acc, n_workgroups, st);
+}
+
+// ------------------------------------------------------------------ L-BFGS history products
+
+int tmol_lbfgs_dots(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,
+                    const double* H, const double* vecs, int32_t n_vecs, double* out,
+                    void* stream) {
+  if (n_segments <= 0 || history <= 0 || seg_len <= 0 || n_used < 0 || n_used > history ||
+      !H || !vecs || !out || n_vecs < 1 || n_vecs > LBFGS_K)
+    return TMOL_LJLK_EINVAL;
+  if (n_used == 0) return TMOL_LJLK_OK;
+  hipLaunchKernelGGL(lbfgs_dots_kernel, dim3(unsigned(n_segments) * unsigned(n_used)), dim3(256),
+                     0, (hipStream_t)stream, H, history, seg_len, n_used, n_segments, vecs, n_vecs,
+                     out);
+  return check(hipGetLastError());
+}
+
+int tmol_lbfgs_combine(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,
+                       const double* g, const double* H1, const double* c1, const double* H2,
+                       const double* c2, double* out, void* stream) {
+  if (n_segments <= 0 || history <= 0 || seg_len <= 0 || n_used < 0 || n_used > history || !g ||
+      !H1 || !c1 || !H2 || !c2 || !out)
+    return TMOL_LJLK_EINVAL;
+  hipLaunchKernelGGL(lbfgs_combine_kernel, dim3(unsigned((seg_len + 255) / 256), n_segments),
+                     dim3(256), 0, (hipStream_t)stream, g, H1, c1, H2, c2, history, seg_len,
+                     n_used, n_segments, out);
+  return check(hipGetLastError());
 }
 
 }  // extern "C"

@@ -37,6 +37,42 @@ import torch
 
 Evaluate = Callable[[torch.Tensor], Tuple[torch.Tensor, torch.Tensor]]
 
+_LBFGS_LIB = None
+
+
+def _lbfgs_lib():
+    """The HIP library's history-product entry points (tmol_lbfgs_*)."""
+    global _LBFGS_LIB
+    if _LBFGS_LIB is None:
+        import ctypes
+
+        from . import ops
+
+        L = ops.lib()
+        I32, I64, VP = ctypes.c_int32, ctypes.c_int64, ctypes.c_void_p
+        L.tmol_lbfgs_dots.argtypes = [I32, I32, I64, I32, VP, VP, I32, VP, VP]
+        L.tmol_lbfgs_combine.argtypes = [I32, I32, I64, I32, VP, VP, VP, VP, VP, VP, VP]
+        _LBFGS_LIB = L
+    return _LBFGS_LIB
+
+
+def _vp(t: torch.Tensor):
+    import ctypes
+
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream_of(t: torch.Tensor):
+    from . import ops
+
+    return ops._stream(t.device)
+
+
+def _check_lbfgs(status, what):
+    from . import ops
+
+    ops._check(status, what)
+
 
 @dataclass
 class Trace:
@@ -109,6 +145,31 @@ class SegmentedLBFGS:
     def _per_elem(self, per_seg: torch.Tensor) -> torch.Tensor:
         return per_seg[self.seg]
 
+    # -- products with the history (HIP kernels on the device, torch on the host)
+    def _dots(self, H: torch.Tensor, n_phys: int, vecs) -> torch.Tensor:
+        """[K, P, n_phys]: H[p, i, :] . v_k[p, :] for the used rows i."""
+        if H.is_cuda:
+            V = torch.stack(vecs).contiguous()
+            out = torch.empty((len(vecs), self.P, n_phys), dtype=torch.float64, device=H.device)
+            _check_lbfgs(_lbfgs_lib().tmol_lbfgs_dots(
+                self.P, self.m, self.S, n_phys, _vp(H), _vp(V), len(vecs), _vp(out),
+                _stream_of(H)), "lbfgs dots")
+            return out
+        Hv = H[:, :n_phys]
+        return torch.stack([torch.bmm(Hv, v.unsqueeze(-1)).squeeze(-1) for v in vecs])
+
+    def _combine(self, g: torch.Tensor, n_phys: int, c1: torch.Tensor, c2: torch.Tensor):
+        """g + sum_i c1[:, i] Sh[:, i] + c2[:, i] Yh[:, i] for the used rows."""
+        if g.is_cuda:
+            out = torch.empty_like(g)
+            _check_lbfgs(_lbfgs_lib().tmol_lbfgs_combine(
+                self.P, self.m, self.S, n_phys, _vp(g.contiguous()), _vp(self.Sh),
+                _vp(c1.contiguous()), _vp(self.Yh), _vp(c2.contiguous()), _vp(out),
+                _stream_of(g)), "lbfgs combine")
+            return out
+        Sv, Yv = self.Sh[:, :n_phys], self.Yh[:, :n_phys]
+        return g + torch.bmm(c1.unsqueeze(1), Sv).squeeze(1) + torch.bmm(c2.unsqueeze(1), Yv).squeeze(1)
+
     # -- history ------------------------------------------------------------------
     # (s, y) pairs live in a ring of m physical slots; ``order`` lists the
     # slots from oldest to newest.  The small Gram matrices are kept in that
@@ -150,14 +211,13 @@ class SegmentedLBFGS:
         self.Sh[:, phys] = s
         self.Yh[:, phys] = y
         n_phys, idx = self._used()
-        Sv, Yv = self.Sh[:, :n_phys], self.Yh[:, :n_phys]
-        s_y = torch.bmm(Sv, y.unsqueeze(-1)).squeeze(-1)[:, idx]  # s_i . y_new, chronological
-        ys_yy = torch.bmm(Yv, torch.stack([s, y], dim=2))[:, idx]  # y_i . s_new, y_i . y_new
+        s_y = self._dots(self.Sh, n_phys, [y])[0][:, idx]  # s_i . y_new, chronological
+        y_s, y_y = self._dots(self.Yh, n_phys, [s, y])[:, :, idx]  # y_i . s_new, y_i . y_new
         n = self.n_hist
         self.SY[:, :n, k] = s_y
-        self.SY[:, k, :n] = ys_yy[..., 0]
-        self.YY[:, :n, k] = ys_yy[..., 1]
-        self.YY[:, k, :n] = ys_yy[..., 1]
+        self.SY[:, k, :n] = y_s
+        self.YY[:, :n, k] = y_y
+        self.YY[:, k, :n] = y_y
 
     def _drop_segments(self, mask: torch.Tensor):
         """Forget the history of the segments in ``mask`` (device mask; no
@@ -174,10 +234,8 @@ class SegmentedLBFGS:
         if n == 0:
             return g.clone()
         n_phys, idx = self._used()
-        Sv, Yv = self.Sh[:, :n_phys], self.Yh[:, :n_phys]
-        gc = g.unsqueeze(-1)
-        a = torch.bmm(Sv, gc).squeeze(-1)[:, idx]
-        b = torch.bmm(Yv, gc).squeeze(-1)[:, idx]
+        a = self._dots(self.Sh, n_phys, [g])[0][:, idx]
+        b = self._dots(self.Yh, n_phys, [g])[0][:, idx]
         SY = self.SY[:, :n, :n]
         D = torch.diagonal(SY, dim1=1, dim2=2)
         # absent entries (zero rows) get a unit diagonal: they then contribute 0
@@ -187,10 +245,10 @@ class SegmentedLBFGS:
         p1 = torch.linalg.solve_triangular(R.transpose(1, 2), v.unsqueeze(-1),
                                            upper=False).squeeze(-1)
         # back to physical slot order for the products with the history
-        coef = torch.zeros((self.P, 2, n_phys), dtype=g.dtype, device=g.device)
-        coef[:, 0, idx] = p1
-        coef[:, 1, idx] = -u
-        return g + torch.bmm(coef[:, 0:1], Sv).squeeze(1) + torch.bmm(coef[:, 1:2], Yv).squeeze(1)
+        coef = torch.zeros((2, self.P, n_phys), dtype=g.dtype, device=g.device)
+        coef[0][:, idx] = p1
+        coef[1][:, idx] = -u
+        return self._combine(g, n_phys, coef[0], coef[1])
 
     # -- line search ------------------------------------------------------------------
     def _armijo(self, phi_at, gtd, phi0, alpha0, searching):
@@ -242,8 +300,13 @@ class SegmentedLBFGS:
         """Host synchronisations: one per line-search trial (whether to
         evaluate again) and one per iteration (convergence, and whether the
         next iteration adds a history pair or resets a segment); the trace
-        stays on the device until the run ends."""
-        x = x0.detach().reshape(-1).clone()
+        stays on the device until the run ends.  The vectors live in the
+        padded per-segment layout [P, S] (pads hold 0 in x, g and d), so
+        segment sums are row reductions and per-segment scalars broadcast
+        along rows; ``evaluate`` still sees flat vectors."""
+        f64 = torch.float64
+        x0f = x0.detach().reshape(-1)
+        x = self._padded(x0f, dtype=x0f.dtype)
         dev = x.device
         eps = torch.finfo(x.dtype).eps
         rtol = self.rtol if self.rtol is not None else math.sqrt(eps)
@@ -257,17 +320,30 @@ class SegmentedLBFGS:
         trace = Trace()
         rec_E, rec_gn, rec_gm = [], [], []
 
-        E, g = evaluate(x)
-        E, g = E.to(torch.float64), g.reshape(-1).clone()
+        def ev(xp):
+            E_, gf = evaluate(self._flat(xp))
+            gf = gf.reshape(-1)
+            return E_.to(f64), self._padded(gf, dtype=gf.dtype)
+
+        def ssum(v):  # per-segment sum, float64
+            return v.to(f64).sum(1)
+
+        def smax_abs(v):  # per-segment max |v| (pads are 0)
+            return v.abs().to(f64).amax(1)
+
+        def col(v):  # a per-segment value broadcast over the segment's elements
+            return v[:, None]
+
+        E, g = ev(x)
 
         def record(E, g, evals):
             rec_E.append(E.detach())
-            rec_gn.append(self._seg_sum(g * g).sqrt())
-            rec_gm.append(self._seg_max(g.abs()))
+            rec_gn.append(ssum(g * g).sqrt())
+            rec_gm.append(smax_abs(g))
             trace.evals.append(evals)
 
         record(E, g, 1)
-        t = torch.full((P,), self.lr, dtype=torch.float64, device=dev)
+        t = torch.full((P,), self.lr, dtype=f64, device=dev)
         g_prev = None
         x_ref = x.clone()
         keep = None
@@ -280,35 +356,35 @@ class SegmentedLBFGS:
                 d = -g
             else:
                 if keep_any:
-                    ke = self._per_elem(keep)
+                    ke = col(keep)
                     y = g - g_prev
                     s = x - x_ref
-                    self._push(self._padded(torch.where(ke, s, torch.zeros_like(s))),
-                               self._padded(torch.where(ke, y, torch.zeros_like(y))))
+                    self._push(torch.where(ke, s, torch.zeros_like(s)).to(f64),
+                               torch.where(ke, y, torch.zeros_like(y)).to(f64))
                     x_ref = torch.where(ke, x, x_ref)
-                d = -self._flat(self._hg(self._padded(g))).to(x.dtype)
+                d = -self._hg(g.to(f64)).to(x.dtype)
             was_reset = needs_reset.clone()
             if reset_any:
                 self._drop_segments(needs_reset)
-                re = self._per_elem(needs_reset)
+                re = col(needs_reset)
                 d = torch.where(re, -g, d)
                 x_ref = torch.where(re, x, x_ref)
                 needs_reset = torch.zeros_like(needs_reset)
             inactive = converged | stalled
-            d = d * (~self._per_elem(inactive)).to(d.dtype)
+            d = d * (~col(inactive)).to(d.dtype)
             g_prev = g.clone()
             E_prev = E.clone()
             # ---- directional derivative, with repair of uphill directions
             # (flip the uphill components, then steepest descent), masked per
             # segment: a repair leaves segments without an uphill direction
             # untouched
-            gtd = self._seg_sum(g * d)
+            gtd = ssum(g * d)
             for repair in (1, 2):
                 bad = (gtd > -1e-5) & ~inactive
-                be = self._per_elem(bad)
+                be = col(bad)
                 fixed = d * -torch.sign(g * d) if repair == 1 else -g
                 d = torch.where(be, fixed, d)
-                gtd = torch.where(bad, self._seg_sum(g * d), gtd)
+                gtd = torch.where(bad, ssum(g * d), gtd)
             # ---- line search
             searching = ~inactive & (gtd < 0)
             t0 = torch.clamp(t / 0.5, max=1.0)
@@ -316,18 +392,17 @@ class SegmentedLBFGS:
             last = {}
 
             def phi_at(alpha):
-                xt = x_base + d * self._per_elem(alpha).to(x.dtype)
-                Et, gt = evaluate(xt)
-                last["x"], last["E"], last["g"] = xt, Et.to(torch.float64), gt.reshape(-1).clone()
-                return last["E"]
+                xt = x_base + d * col(alpha).to(x.dtype)
+                Et, gt = ev(xt)
+                last["x"], last["E"], last["g"] = xt, Et, gt
+                return Et
 
             step, E_step, evals, failed, at_step = self._armijo(phi_at, gtd, E, t0, searching)
-            x = x_base + d * self._per_elem(step).to(x.dtype)
+            x = x_base + d * col(step).to(x.dtype)
             if at_step:
                 E, g = last["E"], last["g"]
             else:
-                E, g = evaluate(x)
-                E, g = E.to(torch.float64), g.reshape(-1).clone()
+                E, g = ev(x)
                 evals += 1
             retry = torch.clamp((-gtd).clamp(min=self.minstep).rsqrt(), max=1.0)
             t = torch.where(failed, retry, step)
@@ -338,11 +413,11 @@ class SegmentedLBFGS:
             # ---- convergence; and, for the next iteration, which segments
             # keep the new (s, y) pair (s.y > 1e-6), fetched in the same
             # synchronisation
-            converged = converged | (self._seg_max(g.abs()) <= self.gradtol)
+            converged = converged | (smax_abs(g) <= self.gradtol)
             dE = (E - E_prev).abs()
             rdiff = 2 * dE / (E.abs() + E_prev.abs() + 1e-10)
             done = converged | stalled | (dE <= atol) | (rdiff <= rtol)
-            keep = self._seg_sum((g - g_prev) * (x - x_ref)) > 1e-6
+            keep = ssum((g - g_prev) * (x - x_ref)) > 1e-6
             all_done, keep_any, reset_any = torch.stack(
                 [done.all(), keep.any(), needs_reset.any()]).tolist()
             if self.verbose:
@@ -356,21 +431,33 @@ class SegmentedLBFGS:
         n = len(rec_E)
         for name, rec in (("energy", rec_E), ("grad_norm", rec_gn), ("grad_max", rec_gm)):
             getattr(trace, name).extend(torch.stack(rec).cpu().unbind(0) if n else [])
-        return x, trace
+        return self._flat(x), trace
 
 
 # ---------------------------------------------------------------------------
 # evaluators
 
 
-def ragged_evaluator(scorer, weights=(1.0, 0.55, 1.0)) -> Evaluate:
-    """Energy + weighted gradient of a ``LJLKRaggedScorer`` (fused kernel)."""
+def ragged_evaluator(scorer, weights=(1.0, 0.55, 1.0), graph: bool = True) -> Evaluate:
+    """Energy + weighted gradient of a ``LJLKRaggedScorer`` (fused kernel).
+    With ``graph`` the neighbor pass and the scoring launch are captured once
+    in a hipGraph and each evaluation copies the coordinates in and replays
+    it (the scorer's default work-list capacity cannot overflow, whatever the
+    trial coordinates)."""
     w = torch.tensor(weights, dtype=torch.float64)
     wd = w.to(scorer.device)
+    wk = scorer._weights(w)
+    state = {"captured": False}
 
     def evaluate(x):
         coords = x.view(-1, 3)
-        E, g = scorer.energy_and_grad(coords, w)
+        if graph and coords.is_cuda:
+            if not state["captured"]:
+                scorer.capture(coords, wk)
+                state["captured"] = True
+            E, g = scorer.replay(coords)
+        else:
+            E, g = scorer.energy_and_grad(coords, wk)
         return (wd[:, None] * E.double()).sum(0), g.reshape(-1).clone()
 
     return evaluate

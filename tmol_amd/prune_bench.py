@@ -230,8 +230,11 @@ def measure(params, threshold, dev, steps=20, warmup=3, log=print, min_gap=prune
         "pruned_fraction": 1.0 - n_after / max(n_before, 1),
         "max_err_per_res": float(err.max()), "mean_err_per_res": float(err.mean()),
         "within_bound": within, "fallback": not within,
-        # a batch over the bound is rescored exactly: its throughput is the exact one
-        "effective_poses_per_s": P / t_pruned if within else P / t_exact,
+        # a batch over the bound is rescored exactly, and a pruned step slower
+        # than the exact one is not used (the cost check): either way the
+        # throughput is the exact one
+        "pays_off": bool(t_pruned < t_exact),
+        "effective_poses_per_s": P / t_pruned if (within and t_pruned < t_exact) else P / t_exact,
         "kernel_ms_exact": k_exact, "kernel_ms_pruned": k_pruned,
         "prune_ms": t_nbr_prune - t_nbr,
         "perfect_labels": {
