@@ -179,22 +179,46 @@ def test_minimization_matches_oracle_trajectory(diff32):
     assert abs(float(E[n - 1].sum() - Eo[n - 1].sum())) <= 0.01 * abs(float(Eo[n - 1].sum()))
 
 
-def test_minimization_f32_200_steps(diff32):
-    """The f32 kernels minimize the same poses: energy never increases for
-    any pose, gradient norms are tracked, the first iterations follow the
-    oracle-driven f64 run pose by pose (1e-4 relative), and after the paths
-    separate the batch's total energy stays within 5% of the oracle's and
-    ends within 1%, the mean energy per residue within 0.1 kcal/mol (0.7% of
-    the final -14.9 kcal/mol per residue).  The f32 gradient sums use
-    order-dependent atomics, so the separated paths differ from run to run:
-    mid-path gaps of 2.4% and final per-residue gaps from under 0.05 up to
-    0.056 kcal/mol have been seen on unchanged code."""
-    gold = _fixture()
-    st32, sc32 = diff32[torch.float32]
-    _say("minimize f32 200")
+@pytest.fixture(scope="module")
+def det32(diff32):
+    """The f32 scorer in deterministic mode (fixed-point global sums):
+    repeated minimizations follow the same path to the last bit."""
+    from tmol_amd.scorer import LJLKRaggedScorer
+
+    st32, _ = diff32[torch.float32]
+    return st32, LJLKRaggedScorer(st32, LJLKEnergyTerm(device=DEV), device=DEV,
+                                  deterministic=True)
+
+
+def _min_f32(det32, iters=200):
+    st32, sc = det32
     x0 = st32.coords.to(DEV).reshape(-1)
     seg = st32.atom_pose.to(DEV).repeat_interleave(3).long()
-    x, tr = SegmentedLBFGS(seg, max_iter=200).run(x0, ragged_evaluator(sc32))
+    return SegmentedLBFGS(seg, max_iter=iters).run(x0, ragged_evaluator(sc))
+
+
+def test_minimization_f32_deterministic_is_reproducible(det32):
+    """Two deterministic f32 minimizations of 60 iterations: identical
+    energies, gradient norms and final coordinates."""
+    _say("minimize f32 deterministic x2")
+    x1, t1 = _min_f32(det32, 60)
+    x2, t2 = _min_f32(det32, 60)
+    assert torch.equal(t1.energies(), t2.energies())
+    assert torch.equal(t1.grad_norms(), t2.grad_norms())
+    assert torch.equal(x1, x2)
+
+
+def test_minimization_f32_200_steps(det32, diff32):
+    """The f32 kernels (deterministic mode) minimize the same poses: energy
+    never increases for any pose, gradient norms are tracked, the first
+    iterations follow the oracle-driven f64 run pose by pose (1e-4
+    relative), and after the paths separate (f32 and f64 line-search
+    decisions part) the batch's total energy stays close to the oracle's.
+    The bounds are those of this one reproducible path (the deterministic
+    sums make it the same on every run), with a margin."""
+    gold = _fixture()
+    _say("minimize f32 200")
+    x, tr = _min_f32(det32)
     E = tr.energies()
     assert bool((E[1:] <= E[:-1] + 1e-6 * E[:-1].abs()).all())
     assert bool(torch.isfinite(tr.grad_norms()).all())
@@ -204,10 +228,15 @@ def test_minimization_f32_200_steps(diff32):
     tot, toto = E[:n].sum(1), Eo[:n].sum(1)
     n_res = torch.tensor([p.n_blocks for p in diff32["poses"]], dtype=torch.float64)
     dmean = abs(float((E[n - 1] / n_res).mean() - (Eo[n - 1] / n_res).mean()))
-    _say("f32: iterations %d, first-5 max rel %.3g, final total %.1f vs %.1f, mean/res diff %.4f"
-         % (tr.iterations, float(rel[:6].max()), float(tot[-1]), float(toto[-1]), dmean))
-    assert float(rel[:6].max()) < 1e-4
     late = slice(30, n)
-    assert float(((tot[late] - toto[late]).abs() / toto[late].abs()).max()) < 0.05
-    assert abs(float(tot[-1] - toto[-1])) <= 0.01 * abs(float(toto[-1]))
-    assert dmean < 0.1
+    mid = float(((tot[late] - toto[late]).abs() / toto[late].abs()).max())
+    end = abs(float(tot[-1] - toto[-1])) / abs(float(toto[-1]))
+    _say("f32: iterations %d, first-5 max rel %.3g, final total %.1f vs %.1f, mean/res diff %.4f, "
+         "mid-path max rel %.4f, final rel %.5f"
+         % (tr.iterations, float(rel[:6].max()), float(tot[-1]), float(toto[-1]), dmean, mid, end))
+    # measured on this path: mid-path 0.017, final 0.00078, mean per
+    # residue 0.012 kcal/mol (of -14.9)
+    assert float(rel[:6].max()) < 1e-4
+    assert mid < 0.03
+    assert end < 0.002
+    assert dmean < 0.025
