@@ -45,14 +45,27 @@ constexpr uint32_t Q_NONE = 0x64646464u;  // q bytes of blocks more than 4 bonds
 constexpr uint32_t Q_HEAVY = 0x80000000u; // bit 7 of byte 3: heavy atom
 constexpr uint32_t Q_LIG = 0x00800000u;   // bit 7 of byte 2: ligand-fragment block
 
+// Copies of the LDS gradient accumulators, chosen by lane: LDS atomics on one
+// address serialize, and consecutive FIFO entries share their g1 atom (and,
+// less often, their ring atom).  g1 slots: G1COPIES copies (lane % G1COPIES);
+// ring slots: RINGCOPIES copies ((lane / 2) % RINGCOPIES).  Flushes add the
+// copies.
+#ifndef TMOL_STREAM_G1COPIES
+#define TMOL_STREAM_G1COPIES 2
+#endif
+#ifndef TMOL_STREAM_RINGCOPIES
+#define TMOL_STREAM_RINGCOPIES 1
+#endif
+constexpr int G1COPIES = TMOL_STREAM_G1COPIES;
+constexpr int RINGCOPIES = TMOL_STREAM_RINGCOPIES;
+
 template <typename Real, int MODE>
 struct StreamLDS {
   using C = RowCtx<Real, MODE>;
   using T = typename LdsAcc<Real>::T;
-  T grad[1][C::NG > 0 ? C::NG : 1][3][C::DERIV ? SSLOTS : 1];
-  // second copy of g1's accumulators, taken by odd lanes: consecutive FIFO
-  // entries share their g1 atom, and LDS atomics on one address serialize
-  T g1b[C::NG > 0 ? C::NG : 1][3][C::DERIV ? TILE : 1];
+  static constexpr int NGP = C::NG > 0 ? C::NG : 1;
+  T gring[RINGCOPIES][NGP][3][C::DERIV ? SRING : 1];  // ring slots TILE.. (index - TILE)
+  T g1acc[G1COPIES][NGP][3][C::DERIV ? TILE : 1];     // g1 slots 0..31
   T ebp[1][3];  // block-pair modes run the row kernel; kept for the evaluators
   Real wt[1][3];
   alignas(4 * sizeof(Real)) Real pos[SSLOTS][4];  // x, y, z, lj_radius
@@ -109,23 +122,23 @@ __device__ __forceinline__ void conn_rows(const StackView& s, int g1, int g2, in
   for (int c1 = 0; c1 < nc1; ++c1) cs[c1] = (uint32_t)e[4 + c1];
 }
 
-// gradient of an entry's pair: + to the g1 atom's slot s1, - to slot s2;
-// odd lanes add g1 slots into the second copy
+// gradient of an entry's pair: + to the g1 atom's slot s1, - to slot s2
+// (a ring slot, or a g1 slot for intra-block pairs), into the lane's copies
 template <typename Real, int MODE>
 __device__ __forceinline__ void grad_scatter(StreamLDS<Real, MODE>& L, int term, int s1, int s2,
                                              Real gx, Real gy, Real gz, int lane) {
   using Acc = LdsAcc<Real>;
   using T = typename Acc::T;
-  const bool odd = lane & 1;
+  const int c1 = lane & (G1COPIES - 1);
   const T ex = Acc::enc(gx), ey = Acc::enc(gy), ez = Acc::enc(gz);
-  T* b1 = odd ? &L.g1b[term][0][s1] : &L.grad[0][term][0][s1];
-  const int k1 = odd ? TILE : SSLOTS;
+  T* b1 = &L.g1acc[c1][term][0][s1];
   const bool g2_in_g1 = s2 < TILE;
-  T* b2 = (odd && g2_in_g1) ? &L.g1b[term][0][s2] : &L.grad[0][term][0][s2];
-  const int k2 = (odd && g2_in_g1) ? TILE : SSLOTS;
+  T* b2 = g2_in_g1 ? &L.g1acc[c1][term][0][s2]
+                   : &L.gring[(lane >> 1) & (RINGCOPIES - 1)][term][0][s2 - TILE];
+  const int k2 = g2_in_g1 ? TILE : SRING;
   atomicAdd(b1, ex);
-  atomicAdd(b1 + k1, ey);
-  atomicAdd(b1 + 2 * k1, ez);
+  atomicAdd(b1 + TILE, ey);
+  atomicAdd(b1 + 2 * TILE, ez);
   atomicAdd(b2, T(0) - ex);
   atomicAdd(b2 + k2, T(0) - ey);
   atomicAdd(b2 + 2 * k2, T(0) - ez);
@@ -140,12 +153,16 @@ __device__ __forceinline__ void flush_chunk(StreamLDS<Real, MODE>& L, const Scor
   if (!C::DERIV) return;
   for (int i = lane; i < 3 * n; i += WAVE) {
     const int a = i / 3, k = i - 3 * a;
-    const int slot = TILE + SCH * r + a;
+    const int slot = SCH * r + a;
     const int64_t atom = L.atom[SCH * r + a];
 #pragma unroll
     for (int t = 0; t < C::NG; ++t) {
-      const typename LdsAcc<Real>::T acc = L.grad[0][t][k][slot];
-      L.grad[0][t][k][slot] = 0;
+      typename LdsAcc<Real>::T acc = 0;
+#pragma unroll
+      for (int c = 0; c < RINGCOPIES; ++c) {
+        acc += L.gring[c][t][k][slot];
+        L.gring[c][t][k][slot] = 0;
+      }
       if (C::DET) {
         if (acc != 0) atomicAdd(&A.fixed[(int64_t(t) * A.n_atoms + atom) * 3 + k], acc);
         continue;
@@ -171,9 +188,12 @@ __device__ __forceinline__ void flush_g1(StreamLDS<Real, MODE>& L, const ScoreAr
     const int a = i / 3, k = i - 3 * a;
 #pragma unroll
     for (int t = 0; t < C::NG; ++t) {
-      const typename LdsAcc<Real>::T acc = L.grad[0][t][k][a] + L.g1b[t][k][a];
-      L.grad[0][t][k][a] = 0;
-      L.g1b[t][k][a] = 0;
+      typename LdsAcc<Real>::T acc = 0;
+#pragma unroll
+      for (int c = 0; c < G1COPIES; ++c) {
+        acc += L.g1acc[c][t][k][a];
+        L.g1acc[c][t][k][a] = 0;
+      }
       if (C::DET) {
         if (acc != 0) atomicAdd(&A.fixed[(int64_t(t) * A.n_atoms + atom0) * 3 + i], acc);
         continue;
@@ -211,10 +231,10 @@ __global__ __launch_bounds__(256, (stream_waves_per_simd<MODE, OCC>())) void sco
   const int u = lane & (SCH - 1), half = lane >> 5;
 
   if (C::DERIV) {
-    auto* g = &L.grad[0][0][0][0];
-    for (int i = lane; i < (C::NG > 0 ? C::NG : 1) * 3 * SSLOTS; i += WAVE) g[i] = 0;
-    auto* gb = &L.g1b[0][0][0];
-    for (int i = lane; i < (C::NG > 0 ? C::NG : 1) * 3 * TILE; i += WAVE) gb[i] = 0;
+    auto* g = &L.gring[0][0][0][0];
+    for (int i = lane; i < RINGCOPIES * L.NGP * 3 * SRING; i += WAVE) g[i] = 0;
+    auto* gb = &L.g1acc[0][0][0][0];
+    for (int i = lane; i < G1COPIES * L.NGP * 3 * TILE; i += WAVE) gb[i] = 0;
   }
   wave_sync();
 
