@@ -1,6 +1,7 @@
 #!/bin/bash
 # Build the reference's own CPU LJ+LK op (tmol_ljlk::ljlk_pose_scores and
-# ljlk_rotamer_scores), its elec op (tmol_elec::elec_pose_scores) and its
+# ljlk_rotamer_scores), its elec op (tmol_elec::elec_pose_scores), its
+# lk_ball ops (tmol_lk_ball::gen_pose_waters, lk_ball_pose_score) and its
 # all-pairs-shortest-paths op (tmol_apsp::apsp_op)
 # from the sources where they lie under the reference checkout, into
 # oracle/_ref/.
@@ -52,3 +53,16 @@ g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
   -o "$OUT3.tmp"
 mv "$OUT3.tmp" "$OUT3"
 echo "built $OUT3"
+# the reference's lk_ball ops (tmol_lk_ball::gen_pose_waters,
+# lk_ball_pose_score, lk_ball_rotamer_score)
+OUT4="$HERE/_ref/tmol_lk_ball_ref.so"
+g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
+  -DTORCH_EXTENSION_NAME=tmol_lk_ball_ref \
+  -I"$R" -I"$R/tmol/extern" $TI -I"$PYI" \
+  "$R/tmol/score/lk_ball/potentials/compiled.ops.cpp" \
+  "$R/tmol/score/lk_ball/potentials/lk_ball_pose_score.cpu.cpp" \
+  "$R/tmol/score/lk_ball/potentials/gen_pose_waters.cpu.cpp" \
+  -L"$TL" -ltorch -ltorch_cpu -lc10 -Wl,-rpath,"$TL" \
+  -o "$OUT4.tmp"
+mv "$OUT4.tmp" "$OUT4"
+echo "built $OUT4"

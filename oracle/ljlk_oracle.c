@@ -1,5 +1,6 @@
 /* CPU oracle for the LJ+LK pose-score op -- TEST INFRASTRUCTURE ONLY.
- * Float and double instantiations of ljlk_oracle_impl.h, plus the integer
+ * Float and double instantiations of ljlk_oracle_impl.h, elec_oracle_impl.h
+ * and lkball_oracle_impl.h, plus the integer
  * count-pair dump.  Built by oracle/Makefile with -O2 -ffp-contract=off
  * (no fused multiply-add, like the reference's x86-64 CPU build). */
 #include <math.h>
@@ -37,26 +38,40 @@ static float elec_eps_float(float dist, float D, float D0, float S) {
 #define SQRT sqrtf
 #define EXP expf
 #define FLOOR floorf
+#define LOG logf
+#define COS cosf
+#define SIN sinf
 #include "ljlk_oracle_impl.h"
 #include "elec_oracle_impl.h"
+#include "lkball_oracle_impl.h"
 #undef REAL
 #undef RSUF
 #undef SQRT
 #undef EXP
 #undef FLOOR
+#undef LOG
+#undef COS
+#undef SIN
 
 #define REAL double
 #define RSUF _d
 #define SQRT sqrt
 #define EXP exp
 #define FLOOR floor
+#define LOG log
+#define COS cos
+#define SIN sin
 #include "ljlk_oracle_impl.h"
 #include "elec_oracle_impl.h"
+#include "lkball_oracle_impl.h"
 #undef REAL
 #undef RSUF
 #undef SQRT
 #undef EXP
 #undef FLOOR
+#undef LOG
+#undef COS
+#undef SIN
 
 /* integer count-pair separation for one atom pair of a block pair
  * (b1 != b2: interres rule; b1 == b2: intra-block path distance) */

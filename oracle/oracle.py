@@ -92,6 +92,25 @@ def lib():
                 ctypes.POINTER(OracleStack), PR, ctypes.c_int64, PR, _P, PR, ctypes.c_int64, _P,
                 _P, _P, _P, PR, PR, PR,
             ]
+        for suf, ct in (("_f", ctypes.c_float), ("_d", ctypes.c_double)):
+            PR = ctypes.POINTER(ct)
+            LS = ctypes.POINTER(OracleLKBStack)
+            getattr(L, "oracle_lkb_waters" + suf).argtypes = [LS, PR, ctypes.c_int64, PR, PR,
+                                                               PR, PR, PR]
+            getattr(L, "oracle_lkb_waters_backward" + suf).argtypes = [LS, PR, PR, PR, PR, PR,
+                                                                        PR, PR]
+            getattr(L, "oracle_lkb_score" + suf).argtypes = [
+                LS, PR, PR, ctypes.c_int64, PR, PR, _P, ctypes.c_int, PR, PR, PR, PR]
+            getattr(L, "oracle_lkb_pair" + suf).argtypes = [PR, PR, PR, PR, ctypes.c_int, PR,
+                                                             PR, PR, PR, PR]
+            f = getattr(L, "oracle_lkb_fraction" + suf)
+            f.argtypes = [PR, PR, ct, PR, PR]
+            f.restype = ct
+            f = getattr(L, "oracle_lkb_bridge" + suf)
+            f.argtypes = [PR, PR, PR, PR, ct, PR, PR, PR, PR]
+            f.restype = ct
+            getattr(L, "oracle_lkb_build_water" + suf).argtypes = [
+                ctypes.c_int, PR, PR, PR, ct, ct, ct, PR, PR, PR, PR, PR]
         L.oracle_rot_dispatch.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P,
                                           ctypes.c_int64]
         L.oracle_rot_dispatch.restype = ctypes.c_int64
@@ -603,3 +622,230 @@ def pose_oracle_evaluator(poses, weights=(1.0, 0.55, 1.0), threads=8, dtype=nump
         return E.to(x.device), G.to(device=x.device, dtype=x.dtype)
 
     return evaluate
+
+
+# ---------------------------------------------------------------- lk_ball
+
+class OracleLKBStack(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int32) for n in (
+        "n_poses", "max_n_blocks", "n_block_types", "max_n_block_atoms", "max_n_tiles",
+        "max_n_conn", "max_n_all_bonds")] + [(n, _P) for n in (
+            "rot_coord_offset", "block_type", "irc", "min_bondsep", "inter_block_bondsep",
+            "bt_n_atoms", "bt_n_conn", "bt_conn_atoms", "bt_all_bonds", "bt_atom_all_bond_ranges",
+            "tile_n_donH", "tile_n_acc", "tile_donH_inds", "tile_donH_hvy_inds", "tile_which_donH",
+            "tile_acc_inds", "tile_acc_hyb", "tile_acc_nH", "is_hydrogen", "tile_n_polar",
+            "tile_n_occ", "tile_pol_occ_inds", "path_distance")] + [
+        ("n_tors", ctypes.c_int32 * 3)]
+
+
+class _LKBStack:
+    """The lk_ball oracle's view of the two reference ops' arguments: the 33
+    of gen_pose_waters (lk_ball_energy_term.py:244-268) and, for scoring, the
+    29 of lk_ball_pose_score (:272-296)."""
+
+    def __init__(self, wargs, sargs=None):
+        (coords, rco, _pia, frfb, frbt, _bifr, _pifr, btifr, _nrp, _rop, _nrb, _rob, _mx,
+         irc, bt_n_atoms, bt_n_conn, bt_conn_atoms, _nab, all_bonds, ranges, t_ndonH, t_nacc,
+         t_donH, t_donHhvy, t_which, t_acc, t_hyb, t_nH, is_h, wg, sp2, sp3, ring) = wargs
+        P, B = frfb.shape
+        a = {
+            "rot_coord_offset": _np_i32(rco), "block_type": _np_i32(btifr), "irc": _np_i32(irc),
+            "bt_n_atoms": _np_i32(bt_n_atoms), "bt_n_conn": _np_i32(bt_n_conn),
+            "bt_conn_atoms": _np_i32(bt_conn_atoms), "bt_all_bonds": _np_i32(all_bonds),
+            "bt_atom_all_bond_ranges": _np_i32(ranges), "tile_n_donH": _np_i32(t_ndonH),
+            "tile_n_acc": _np_i32(t_nacc), "tile_donH_inds": _np_i32(t_donH),
+            "tile_donH_hvy_inds": _np_i32(t_donHhvy), "tile_which_donH": _np_i32(t_which),
+            "tile_acc_inds": _np_i32(t_acc), "tile_acc_hyb": _np_i32(t_hyb),
+            "tile_acc_nH": _np_i32(t_nH), "is_hydrogen": _np_i32(is_h),
+        }
+        self.block_pair = False
+        if sargs is not None:
+            (_c, _rco, _pia, _frfb, _frbt, _bifr, _pifr, _btifr, _nrp, _rop, _nrb, _rob, _mx,
+             _irc, mbs, ibs, _bna, _bnc, _bca, t_npol, t_nocc, t_poi, t_par, pd, gp, max_dis,
+             _w, block_pair) = sargs
+            a.update({"min_bondsep": _np_i32(mbs), "inter_block_bondsep": _np_i32(ibs),
+                      "tile_n_polar": _np_i32(t_npol), "tile_n_occ": _np_i32(t_nocc),
+                      "tile_pol_occ_inds": _np_i32(t_poi), "path_distance": _np_i32(pd)})
+            self.tile_params, self.gp = t_par, gp
+            self.max_dis, self.block_pair = float(max_dis), bool(block_pair)
+        self.arrays = a
+        st = OracleLKBStack()
+        st.n_poses, st.max_n_blocks = P, B
+        st.n_block_types = bt_n_atoms.shape[0]
+        st.max_n_block_atoms = is_h.shape[1]
+        st.max_n_tiles = t_ndonH.shape[1]
+        st.max_n_conn = bt_conn_atoms.shape[1]
+        st.max_n_all_bonds = all_bonds.shape[1]
+        for k, v in a.items():
+            setattr(st, k, _ptr(v, ctypes.c_int32))
+        st.n_tors[0], st.n_tors[1], st.n_tors[2] = len(sp2), len(sp3), len(ring)
+        self.struct = st
+        self.P, self.B = P, B
+        self.coords, self.wg, self.tors = coords, wg, (sp2, sp3, ring)
+        self.bt_n_atoms = a["bt_n_atoms"]
+
+    def host(self, t, dtype):
+        return numpy.ascontiguousarray(t.detach().cpu().numpy().astype(dtype))
+
+
+def _lkb_ct(dtype):
+    return ("_d", ctypes.c_double) if dtype == numpy.float64 else ("_f", ctypes.c_float)
+
+
+def oracle_lkb_waters(*wargs, dtype=None):
+    """gen_pose_waters on the oracle: [NA, 4, 3], NaN where no water."""
+    S = _LKBStack(wargs)
+    if dtype is None:
+        dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
+    suf, ct = _lkb_ct(dtype)
+    X = S.host(S.coords, dtype)
+    wg = S.host(S.wg, dtype).reshape(-1)
+    tors = [S.host(t, dtype) for t in S.tors]
+    W = numpy.zeros((X.shape[0], 4, 3), dtype=dtype)
+    getattr(lib(), "oracle_lkb_waters" + suf)(
+        ctypes.byref(S.struct), _ptr(X, ct), X.shape[0], _ptr(wg, ct),
+        *[_ptr(t, ct) for t in tors], _ptr(W, ct))
+    return W
+
+
+def oracle_lkb_waters_backward(wargs, dW, dtype=None):
+    """dE/dx of the water builders for dE/dW [NA, 4, 3]."""
+    S = _LKBStack(wargs)
+    if dtype is None:
+        dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
+    suf, ct = _lkb_ct(dtype)
+    X = S.host(S.coords, dtype)
+    wg = S.host(S.wg, dtype).reshape(-1)
+    tors = [S.host(t, dtype) for t in S.tors]
+    g = numpy.ascontiguousarray(numpy.asarray(dW, dtype=dtype))
+    dX = numpy.zeros((X.shape[0], 3), dtype=dtype)
+    getattr(lib(), "oracle_lkb_waters_backward" + suf)(
+        ctypes.byref(S.struct), _ptr(X, ct), _ptr(wg, ct), *[_ptr(t, ct) for t in tors],
+        _ptr(g, ct), _ptr(dX, ct))
+    return dX
+
+
+def oracle_lkb_score(wargs, sargs, waters=None, dTdV=None, dtype=None):
+    """lk_ball_pose_score on the oracle.  Returns (scores [4, P] or
+    [4, P, B, B], block_neighbors, dX [NA, 3] or None, dW [NA, 4, 3] or
+    None); with dTdV (shape of scores) the derivatives are the weighted
+    backward of the op, dX without the water chain rule."""
+    S = _LKBStack(wargs, sargs)
+    if dtype is None:
+        dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
+    suf, ct = _lkb_ct(dtype)
+    L = lib()
+    X = S.host(S.coords, dtype)
+    NA = X.shape[0]
+    if waters is None:
+        waters = S.host(sargs[26], dtype)
+    W = numpy.ascontiguousarray(numpy.asarray(waters, dtype=dtype).reshape(NA, 4, 3))
+    PB = S.P * S.B
+    spheres = numpy.zeros((PB, 4), dtype=dtype)
+    getattr(L, "oracle_block_spheres" + suf)(
+        _ptr(X, ct), PB, _ptr(S.arrays["rot_coord_offset"], ctypes.c_int32),
+        _ptr(S.arrays["block_type"], ctypes.c_int32), _ptr(S.bt_n_atoms, ctypes.c_int32),
+        _ptr(spheres, ct))
+    nbr = numpy.zeros((S.P, S.B, S.B), dtype=numpy.int32)
+    getattr(L, "oracle_block_neighbors" + suf)(
+        S.P, S.B, _ptr(S.arrays["block_type"], ctypes.c_int32), _ptr(spheres, ct),
+        ct(dtype(S.max_dis)), _ptr(nbr, ctypes.c_int32))
+    shape = (4, S.P, S.B, S.B) if S.block_pair else (4, S.P)
+    out = numpy.zeros(shape, dtype=dtype)
+    tp = S.host(S.tile_params, dtype)
+    gp = S.host(S.gp, dtype).reshape(-1)
+    dX = dW = wts = None
+    if dTdV is not None:
+        wts = numpy.ascontiguousarray(numpy.asarray(dTdV, dtype=dtype).reshape(shape))
+        dX = numpy.zeros((NA, 3), dtype=dtype)
+        dW = numpy.zeros((NA, 4, 3), dtype=dtype)
+    getattr(L, "oracle_lkb_score" + suf)(
+        ctypes.byref(S.struct), _ptr(X, ct), _ptr(W, ct), NA, _ptr(tp, ct), _ptr(gp, ct),
+        _ptr(nbr, ctypes.c_int32), int(S.block_pair), _ptr(out, ct),
+        _ptr(wts, ct) if wts is not None else None, _ptr(dX, ct) if dX is not None else None,
+        _ptr(dW, ct) if dW is not None else None)
+    return out, nbr, dX, dW
+
+
+def oracle_lkb_pair(I, J, WI, WJ, sep, params_i, params_j, global_params, deriv=False,
+                    dtype=numpy.float64):
+    """lk_ball_score::V (and dV) of one polar / occluder pair: E[4] and, with
+    deriv, (dI [4, 3], dJ [4, 3], dWI [4, 4, 3], dWJ [4, 4, 3])."""
+    suf, ct = _lkb_ct(dtype)
+    f = lambda x, n: numpy.ascontiguousarray(numpy.asarray(x, dtype=dtype).reshape(n))  # noqa
+    I, J = f(I, 3), f(J, 3)
+    WI, WJ = f(WI, 12), f(WJ, 12)
+    pi, pj, gp = f(params_i, 9), f(params_j, 9), f(global_params, 5)
+    E = numpy.zeros(4, dtype=dtype)
+    d = numpy.zeros(4 * 3 * 2 + 4 * 12 * 2, dtype=dtype) if deriv else None
+    getattr(lib(), "oracle_lkb_pair" + suf)(
+        _ptr(I, ct), _ptr(J, ct), _ptr(WI, ct), _ptr(WJ, ct), int(sep), _ptr(pi, ct),
+        _ptr(pj, ct), _ptr(gp, ct), _ptr(E, ct), _ptr(d, ct) if d is not None else None)
+    if not deriv:
+        return E
+    return E, (d[:12].reshape(4, 3), d[12:24].reshape(4, 3), d[24:72].reshape(4, 4, 3),
+               d[72:].reshape(4, 4, 3))
+
+
+def load_reference_lk_ball_ops():
+    """The reference's own CPU lk_ball ops built by build_ref.sh
+    (``torch.ops.tmol_lk_ball``: gen_pose_waters, lk_ball_pose_score)."""
+    path = os.path.join(HERE, "_ref", "tmol_lk_ball_ref.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    if not hasattr(torch.ops, "tmol_lk_ball") or not hasattr(torch.ops.tmol_lk_ball,
+                                                              "gen_pose_waters"):
+        torch.ops.load_library(path)
+    return torch.ops.tmol_lk_ball
+
+
+def oracle_lkb_build_water(kind, P, G, GG=None, dist=2.65, angle=0.0, tors=0.0, g=None,
+                           dtype=numpy.float64):
+    """One water: kind 0 donor (P = D, G = H), otherwise build_coordinate
+    (P = A, G = B, GG = B0).  With g = dE/dW also returns (dE/dP, dE/dG,
+    dE/dGG)."""
+    suf, ct = _lkb_ct(dtype)
+    f = lambda x: numpy.ascontiguousarray(numpy.asarray(x, dtype=dtype).reshape(3))  # noqa
+    P, G = f(P), f(G)
+    GG = f(GG if GG is not None else numpy.zeros(3))
+    W = numpy.zeros(3, dtype=dtype)
+    outs = [numpy.zeros(3, dtype=dtype) for _ in range(3)]
+    gg = f(g) if g is not None else None
+    getattr(lib(), "oracle_lkb_build_water" + suf)(
+        int(kind), _ptr(P, ct), _ptr(G, ct), _ptr(GG, ct), ct(dist), ct(angle), ct(tors),
+        _ptr(W, ct), _ptr(gg, ct) if gg is not None else None, *[_ptr(o, ct) for o in outs])
+    return W if g is None else (W, outs)
+
+
+def _waters4(W, dtype):
+    """[n <= 4, 3] waters -> [4, 3], NaN rows appended"""
+    W = numpy.asarray(W, dtype=dtype).reshape(-1, 3)
+    out = numpy.full((4, 3), numpy.nan, dtype=dtype)
+    out[: W.shape[0]] = W
+    return out
+
+
+def oracle_lkb_fraction(WI, J, lj_radius_j, deriv=False, dtype=numpy.float64):
+    """lk_fraction::V (and dV: (dWI [4, 3], dJ [3]))."""
+    suf, ct = _lkb_ct(dtype)
+    WI = _waters4(WI, dtype)
+    J = numpy.ascontiguousarray(numpy.asarray(J, dtype=dtype).reshape(3))
+    dWI, dJ = numpy.zeros((4, 3), dtype=dtype), numpy.zeros(3, dtype=dtype)
+    v = getattr(lib(), "oracle_lkb_fraction" + suf)(
+        _ptr(WI, ct), _ptr(J, ct), ct(lj_radius_j), _ptr(dWI, ct) if deriv else None,
+        _ptr(dJ, ct) if deriv else None)
+    return (v, dWI, dJ) if deriv else v
+
+
+def oracle_lkb_bridge(I, J, WI, WJ, water_dist, deriv=False, dtype=numpy.float64):
+    """lk_bridge_fraction::V (and dV: (dI, dJ, dWI, dWJ))."""
+    suf, ct = _lkb_ct(dtype)
+    f = lambda x: numpy.ascontiguousarray(numpy.asarray(x, dtype=dtype).reshape(3))  # noqa
+    I, J = f(I), f(J)
+    WI, WJ = _waters4(WI, dtype), _waters4(WJ, dtype)
+    outs = [numpy.zeros(3, dtype=dtype), numpy.zeros(3, dtype=dtype),
+            numpy.zeros((4, 3), dtype=dtype), numpy.zeros((4, 3), dtype=dtype)]
+    v = getattr(lib(), "oracle_lkb_bridge" + suf)(
+        _ptr(I, ct), _ptr(J, ct), _ptr(WI, ct), _ptr(WJ, ct), ct(water_dist),
+        *[_ptr(o, ct) if deriv else None for o in outs])
+    return (v, *outs) if deriv else v

@@ -6,6 +6,8 @@ Reads (only when run by hand, in a checkout that has the reference tree):
   tmol/database/default/scoring/ljlk.yaml       (LJ/LK global + per-atom-type params)
   tmol/database/default/scoring/elec.yaml       (elec global params, partial charges
                                                  and count-pair representatives)
+  tmol/database/default/scoring/hbond.yaml      (donor / acceptor atom types, which
+                                                 place lk_ball's waters)
 and writes ``default_db.json`` next to this file.  The JSON is data only: the
 same numbers, names and connectivity as the YAML, restricted to what the
 20 canonical amino acids with termini need.  Nothing at run time reads the
@@ -127,6 +129,21 @@ def extract(ref_root):
                 "lj_hbond_hdis",
                 "lj_hbond_dis",
                 "lk_min_dis2sigma",
+                "lkb_water_dist",
+            )
+        },
+        # lk_ball water geometry, degrees as written (ljlk.yaml: "134.5 deg")
+        "lkb_water": {
+            k: (
+                [_angle(x) for x in g[k]] if isinstance(g[k], list) else _angle(g[k])
+            )
+            for k in (
+                "lkb_water_angle_sp2",
+                "lkb_water_angle_sp3",
+                "lkb_water_angle_ring",
+                "lkb_water_tors_sp2",
+                "lkb_water_tors_sp3",
+                "lkb_water_tors_ring",
             )
         },
         "atom_type_parameters": [
@@ -164,13 +181,24 @@ def extract(ref_root):
             if _res(x["res"])[0] in keep
         ],
     }
+    hb = yaml.safe_load(
+        open(os.path.join(ref_root, "tmol/database/default/scoring/hbond.yaml"))
+    )
+    hbond_out = {
+        "donor_atom_types": [[x["d"], x["donor_type"]] for x in hb["donor_atom_types"]],
+        "acceptor_atom_types": [
+            [x["a"], x["acceptor_type"]] for x in hb["acceptor_atom_types"]
+        ],
+    }
     return {
-        "source": "tmol/database/default (chemical.yaml, scoring/ljlk.yaml, scoring/elec.yaml)",
+        "source": "tmol/database/default (chemical.yaml, scoring/ljlk.yaml, scoring/elec.yaml,"
+        " scoring/hbond.yaml)",
         "atom_types": atom_types,
         "residues": residues,
         "variants": variants,
         "ljlk": ljlk_out,
         "elec": elec_out,
+        "hbond": hbond_out,
     }
 
 
