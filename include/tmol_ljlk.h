@@ -514,6 +514,82 @@ int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
                             void* energies, void* grad, int32_t n_workgroups, void* stream);
 
 /* ---------------------------------------------------------------------
+ * lk_ball on the same scheduler.  Replaces, for its pose path, the
+ * reference's ops
+ *   tmol_lk_ball::gen_pose_waters     tmol/score/lk_ball/potentials/compiled.ops.cpp:22-353
+ *                                     (PoseWaterGen forward/backward, pose_watergen_op:283)
+ *   tmol_lk_ball::lk_ball_pose_score  compiled.ops.cpp:356-907 (LKBallPoseScoreOp
+ *                                     forward/backward, lkball_pose_score:840)
+ * and their dispatch
+ *   GeneratePoseWaters::forward / backward  gen_pose_waters.impl.hh:28-432
+ *   LKBallPoseScoreDispatch::forward / backward  lk_ball_pose_score.impl.hh:358-1000
+ * The stack and block-type structs are the LJ+LK ones (block types: n_atoms,
+ * n_conn, conn_atoms).  lk_ball's own tables (all device pointers, dtype of
+ * the coordinates for the float ones):
+ *   occluders: per block type the heavy atoms, those that carry waters
+ *     ("polar": donor heavy atoms and acceptors) first, at most 32 per type
+ *     (ENOTSUP beyond), with the reference's 9 per-atom parameters
+ *     (lj_radius, lk_dgfree, lk_lambda, lk_volume, is_donor, is_hydroxyl,
+ *     is_polarh, is_acceptor, is_carbon_lk);
+ *   recipes: one row per water slot, TMOL_LKBALL_RECIPE_WIDTH int32:
+ *     {atom, slot (0..3), kind, a1, a2, a3, torsion index, 0}; kind 0 donor
+ *     (a1 = heavy atom D, a2 = its polar H), 1 sp2 / 2 sp3 / 3 ring acceptor
+ *     (a1 = A, a2 = B, a3 = B0; ring: built from (B + B0) / 2 and B0), all
+ *     atoms of the same block;
+ *   path_distance: [T, Amax, Amax] intra-block bond separations (no ligand
+ *     rule: lk_ball counts pairs by plain path distance). */
+#define TMOL_LKBALL_RECIPE_WIDTH 8
+typedef struct {
+  int32_t max_n_occ;             /* Omax (<= 32) */
+  int32_t max_n_recipes;         /* Rmax */
+  const int32_t* n_occ;          /* [T] */
+  const int32_t* n_polar;        /* [T] */
+  const int32_t* occ_atoms;      /* [T, Omax] block atom index, polar first */
+  const void* occ_params;        /* [T, Omax, 9] */
+  const int32_t* n_recipes;      /* [T] */
+  const int32_t* recipes;        /* [T, Rmax, 8] */
+  const int32_t* path_distance;  /* [T, Amax, Amax] */
+  const void* global_params;     /* [5] lj_hbond_dis, lj_hbond_OH_donor_dis, lj_hbond_hdis,
+                                    lkb_water_dist, max_dis */
+  const void* watergen_params;   /* [4] lkb_water_dist, lkb_water_angle_sp2, _sp3, _ring (rad) */
+  const void* water_tors;        /* [3, 4] torsions (rad) of sp2, sp3, ring waters */
+} tmol_lkball_tables;
+
+/* Waters [NA, 4, 3] (NaN where no water is built), the reference's
+ * gen_pose_waters output. */
+int tmol_lkball_waters(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                       const tmol_lkball_tables* lkb, const void* coords, int32_t dtype,
+                       void* waters, void* stream);
+
+/* Backward of the waters: dxyz [NA, 3] += (dWaters/dx)^T dwaters [NA, 4, 3]. */
+int tmol_lkball_waters_backward(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                                const tmol_lkball_tables* lkb, const void* coords,
+                                int32_t dtype, const void* dwaters, void* dxyz, void* stream);
+
+/* Neighbor pass for lk_ball (spheres + work list, as tmol_ljlk_neighbors,
+ * without the LJ+LK atom records): blocks within max_dis (the reference's
+ * host scalar) of each other. */
+int tmol_lkball_neighbors(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                          const void* coords, int32_t dtype, double max_dis,
+                          const int32_t* tiles, int64_t n_tiles, void* workspace,
+                          int64_t ws_bytes, int64_t row_capacity, int32_t* dense_neighbors,
+                          void* stream);
+
+/* Score the work list.
+ *   flags 0: energies [4, P] (lk_ball_iso, lk_ball, lk_bridge,
+ *     lk_bridge_uncpl); BLOCK_PAIR: [4, P, B, B] padded / [4, G, row_cap]
+ *     ragged.
+ *   WGRAD (with weights = dTdV of the energies' shape): dxyz [NA, 3] and
+ *     dwaters [NA, 4, 3] of sum_t w[t] V_t (zeroed first unless ACCUMULATE);
+ *     with NO_ENERGY the backward alone.  The full coordinate gradient adds
+ *     tmol_lkball_waters_backward of dwaters. */
+int tmol_lkball_score(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                      const tmol_lkball_tables* lkb, const void* coords, const void* waters,
+                      int32_t dtype, void* workspace, int64_t ws_bytes, int64_t row_capacity,
+                      int32_t flags, const void* weights, void* energies, void* dxyz,
+                      void* dwaters, int32_t n_wave_groups, void* stream);
+
+/* ---------------------------------------------------------------------
  * Segmented L-BFGS history products (float64), for the minimizer that
  * drives this op (no op counterpart: the reference's LBFGS_Armijo,
  * tmol/optimization/lbfgs_armijo.py:6-58, does this algebra in torch).  The

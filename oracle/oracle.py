@@ -641,7 +641,7 @@ class OracleLKBStack(ctypes.Structure):
 class _LKBStack:
     """The lk_ball oracle's view of the two reference ops' arguments: the 33
     of gen_pose_waters (lk_ball_energy_term.py:244-268) and, for scoring, the
-    29 of lk_ball_pose_score (:272-296)."""
+    28 of lk_ball_pose_score (:272-296)."""
 
     def __init__(self, wargs, sargs=None):
         (coords, rco, _pia, frfb, frbt, _bifr, _pifr, btifr, _nrp, _rop, _nrb, _rob, _mx,

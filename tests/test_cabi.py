@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "tmol_ljlk.h")
 
 def _declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(tmol_(?:ljlk|elec|lbfgs)_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(tmol_(?:ljlk|elec|lbfgs|lkball)_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_header_symbols():

@@ -31,6 +31,7 @@
 #include "ljlk_topology.hh"
 #include "ljlk_elec.hh"
 #include "lbfgs.hh"
+#include "lkball.hh"
 
 #include <cmath>
 
@@ -768,6 +769,164 @@ static int run_elec_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types
   return check(hipGetLastError());
 }
 
+// ------------------------------------------------------------------ lk_ball
+
+// scoring = false: the waters and their backward, which need no count-pair
+// tables
+static int validate_lkb(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                        const tmol_lkball_tables* k, int dtype, bool scoring) {
+  if (!s || !bt || !k) return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32 && dtype != TMOL_LJLK_F64) return TMOL_LJLK_EINVAL;
+  if (bt->max_n_conn > MAX_CONN || bt->max_n_conn < 0) return TMOL_LJLK_ENOTSUP;
+  if (bt->n_block_types <= 0 || bt->max_n_atoms <= 0) return TMOL_LJLK_EINVAL;
+  if (!bt->n_atoms || !bt->n_conn || (bt->max_n_conn > 0 && !bt->conn_atoms))
+    return TMOL_LJLK_EINVAL;
+  if (k->max_n_occ < 0 || k->max_n_recipes < 0) return TMOL_LJLK_EINVAL;
+  if (k->max_n_occ > LKB_OMAX) return TMOL_LJLK_ENOTSUP;
+  if (!k->n_occ || !k->n_polar || !k->occ_atoms || !k->occ_params || !k->n_recipes ||
+      (k->max_n_recipes > 0 && !k->recipes) || !k->path_distance || !k->global_params ||
+      !k->watergen_params || !k->water_tors)
+    return TMOL_LJLK_EINVAL;
+  if (s->n_poses <= 0 || s->n_blocks_total <= 0) return TMOL_LJLK_EINVAL;
+  if (s->layout == TMOL_LJLK_PADDED) {
+    if (int64_t(s->n_poses) * s->max_n_blocks != s->n_blocks_total) return TMOL_LJLK_EINVAL;
+    if (scoring && (!s->min_bondsep || !s->inter_block_bondsep)) return TMOL_LJLK_EINVAL;
+  } else if (s->layout == TMOL_LJLK_RAGGED) {
+    if (!s->block_pose || (scoring && (!s->bonded_start || !s->bonded))) return TMOL_LJLK_EINVAL;
+  } else {
+    return TMOL_LJLK_EINVAL;
+  }
+  if (!s->block_atom_offset || !s->block_type) return TMOL_LJLK_EINVAL;
+  if (s->n_atoms_total >= (int64_t(1) << 31) / 12) return TMOL_LJLK_ENOTSUP;
+  return TMOL_LJLK_OK;
+}
+
+template <typename Real>
+static int run_lkb_waters(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
+                          const tmol_lkball_tables* k, const void* coords, void* waters,
+                          bool backward, const void* dwat, void* dxyz, hipStream_t stream) {
+  StackView s = make_stack_view(st, btp->max_n_conn);
+  const int R = k->max_n_recipes;
+  if (!backward) {
+    const int64_t n = st->n_atoms_total * 12;
+    hipLaunchKernelGGL(fill_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream, (Real*)waters,
+                       n, Real(NAN));
+  }
+  if (R == 0) return check(hipGetLastError());
+  const int64_t n = int64_t(s.G) * R;
+  if (!backward)
+    hipLaunchKernelGGL(lkb_waters_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream, s,
+                       k->n_recipes, k->recipes, R, (const Real*)coords,
+                       (const Real*)k->watergen_params, (const Real*)k->water_tors,
+                       (Real*)waters);
+  else
+    hipLaunchKernelGGL(lkb_waters_backward_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream,
+                       s, k->n_recipes, k->recipes, R, (const Real*)coords,
+                       (const Real*)k->watergen_params, (const Real*)k->water_tors,
+                       (const Real*)dwat, (Real*)dxyz);
+  return check(hipGetLastError());
+}
+
+template <typename Real>
+static int run_lkb_neighbors(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
+                             const void* coords, double max_dis, const int32_t* tiles,
+                             int64_t n_tiles, void* ws, int64_t row_cap, int32_t* dense,
+                             hipStream_t stream, const WsLayout& Lw) {
+  StackView s = make_stack_view(st, btp->max_n_conn);
+  BTView bt = make_bt_view(btp);
+  char* w = (char*)ws;
+  uint32_t* counters = (uint32_t*)(w + Lw.counters);
+  Real* spheres = (Real*)(w + Lw.spheres);
+  uint32_t* row_count = (uint32_t*)(w + Lw.row_count);
+  int32_t* row_items = (int32_t*)(w + Lw.row_items);
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(1), dim3(64), 0, stream, counters, 16,
+                     (uint32_t)0);
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(grid_for(s.G)), dim3(256), 0, stream,
+                     row_count, int64_t(s.G), (uint32_t)0);
+  hipLaunchKernelGGL(spheres_kernel<Real>, dim3(grid_for(s.G)), dim3(256), 0, stream, s, bt,
+                     (const Real*)coords, spheres);
+  if (dense) {
+    int64_t n = int64_t(s.P) * s.B * s.B;
+    hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream, dense, n, 0);
+  }
+  if (n_tiles > 0)
+    hipLaunchKernelGGL(neighbor_rows_kernel<Real>, dim3((unsigned)n_tiles), dim3(256), 0, stream, s,
+                       (const int4*)tiles, n_tiles, spheres, (Real)max_dis, row_count, row_items,
+                       row_cap, counters, dense);
+  return check(hipGetLastError());
+}
+
+template <typename Real, int MODE>
+static void launch_lkb(const LkbArgs<Real>& a, int n_wg, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
+                     8 * QSTRIDE, 0u);
+  auto k = lkb_rows_kernel<Real, MODE>;
+  if (n_wg <= 0) n_wg = resident_workgroups(k);
+  hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+}
+
+template <typename Real>
+static int run_lkb_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
+                         const tmol_lkball_tables* k, const void* coords, const void* waters,
+                         void* ws, int64_t row_cap, int flags, const void* weights,
+                         void* energies, void* dxyz, void* dwat, int n_wg, hipStream_t stream,
+                         const WsLayout& Lw) {
+  LkbArgs<Real> a{};
+  a.s = make_stack_view(st, btp->max_n_conn);
+  a.bt = make_bt_view(btp);
+  char* w = (char*)ws;
+  a.w.row_count = (const uint32_t*)(w + Lw.row_count);
+  a.w.row_items = (const int32_t*)(w + Lw.row_items);
+  a.w.row_cap = row_cap;
+  a.coords = (const Real*)coords;
+  a.waters = (const Real*)waters;
+  a.n_occ = k->n_occ;
+  a.n_pol = k->n_polar;
+  a.occ_atoms = k->occ_atoms;
+  a.occ_params = (const Real*)k->occ_params;
+  a.path_distance = k->path_distance;
+  a.Omax = k->max_n_occ;
+  a.gp = (const Real*)k->global_params;
+  a.row_cursor = (uint32_t*)(w + Lw.cursors);
+  a.weights = (const Real*)weights;
+  a.energies64 = (double*)(w + Lw.energies);
+  a.energies_bp = (Real*)energies;
+  a.dxyz = (Real*)dxyz;
+  a.dwat = (Real*)dwat;
+  const bool bp = flags & TMOL_LJLK_BLOCK_PAIR;
+  const bool noe = flags & TMOL_LJLK_NO_ENERGY;
+  const bool acc = flags & TMOL_LJLK_ACCUMULATE;
+  const int P = st->n_poses;
+  if (!noe) {
+    if (bp)
+      fill<Real>((Real*)energies,
+                 4 * (st->layout == TMOL_LJLK_PADDED
+                          ? int64_t(P) * st->max_n_blocks * st->max_n_blocks
+                          : int64_t(st->n_blocks_total) * row_cap),
+                 stream);
+    else
+      fill<double>(a.energies64, 4 * int64_t(P), stream);
+  }
+  if ((flags & TMOL_LJLK_WGRAD) && !acc) {
+    fill<Real>((Real*)dxyz, st->n_atoms_total * 3, stream);
+    fill<Real>((Real*)dwat, st->n_atoms_total * 12, stream);
+  }
+  const int mode = flags & (TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY);
+  switch (mode) {
+    case 0: launch_lkb<Real, 0>(a, n_wg, stream); break;
+    case M_WGRAD: launch_lkb<Real, M_WGRAD>(a, n_wg, stream); break;
+    case M_WGRAD | M_NOE: launch_lkb<Real, M_WGRAD | M_NOE>(a, n_wg, stream); break;
+    case M_BP: launch_lkb<Real, M_BP>(a, n_wg, stream); break;
+    case M_BP | M_WGRAD: launch_lkb<Real, M_BP | M_WGRAD>(a, n_wg, stream); break;
+    case M_BP | M_WGRAD | M_NOE: launch_lkb<Real, M_BP | M_WGRAD | M_NOE>(a, n_wg, stream); break;
+    default: return TMOL_LJLK_ENOTSUP;
+  }
+  if (!noe && !bp)
+    hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(4 * int64_t(P))), dim3(256), 0,
+                       stream, (const double*)a.energies64, (Real*)energies, 4 * P);
+  return check(hipGetLastError());
+}
+
 extern "C" {
 
 int64_t tmol_ljlk_count_neighbor_tiles(const int32_t* pose_block_start_host, int32_t n_poses) {
@@ -1485,6 +1644,77 @@ int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
 }
 
 // ------------------------------------------------------------------ L-BFGS history products
+
+int tmol_lkball_waters(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                       const tmol_lkball_tables* k, const void* coords, int32_t dtype,
+                       void* waters, void* stream) {
+  int v = validate_lkb(s, bt, k, dtype, false);
+  if (v) return v;
+  if (!coords || !waters) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_lkb_waters<float>(s, bt, k, coords, waters, false, nullptr, nullptr, st);
+  return run_lkb_waters<double>(s, bt, k, coords, waters, false, nullptr, nullptr, st);
+}
+
+int tmol_lkball_waters_backward(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                                const tmol_lkball_tables* k, const void* coords, int32_t dtype,
+                                const void* dwaters, void* dxyz, void* stream) {
+  int v = validate_lkb(s, bt, k, dtype, false);
+  if (v) return v;
+  if (!coords || !dwaters || !dxyz) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_lkb_waters<float>(s, bt, k, coords, nullptr, true, dwaters, dxyz, st);
+  return run_lkb_waters<double>(s, bt, k, coords, nullptr, true, dwaters, dxyz, st);
+}
+
+int tmol_lkball_neighbors(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                          const void* coords, int32_t dtype, double max_dis,
+                          const int32_t* tiles, int64_t n_tiles, void* workspace,
+                          int64_t ws_bytes, int64_t row_capacity, int32_t* dense, void* stream) {
+  if (!s || !bt) return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32 && dtype != TMOL_LJLK_F64) return TMOL_LJLK_EINVAL;
+  if (s->n_poses <= 0 || s->n_blocks_total <= 0 || !bt->n_atoms) return TMOL_LJLK_EINVAL;
+  if (!coords || !workspace || (n_tiles > 0 && !tiles) || row_capacity <= 0)
+    return TMOL_LJLK_EINVAL;
+  if (!std::isfinite(max_dis) || max_dis < 0) return TMOL_LJLK_EINVAL;
+  if (n_tiles >= (int64_t(1) << 31)) return TMOL_LJLK_ENOTSUP;
+  if (dense && s->layout != TMOL_LJLK_PADDED) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_lkb_neighbors<float>(s, bt, coords, max_dis, tiles, n_tiles, workspace,
+                                    row_capacity, dense, st, L);
+  return run_lkb_neighbors<double>(s, bt, coords, max_dis, tiles, n_tiles, workspace,
+                                   row_capacity, dense, st, L);
+}
+
+int tmol_lkball_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                      const tmol_lkball_tables* k, const void* coords, const void* waters,
+                      int32_t dtype, void* workspace, int64_t ws_bytes, int64_t row_capacity,
+                      int32_t flags, const void* weights, void* energies, void* dxyz,
+                      void* dwaters, int32_t n_wave_groups, void* stream) {
+  int v = validate_lkb(s, bt, k, dtype, true);
+  if (v) return v;
+  if (!coords || !waters || !workspace || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  if (flags & ~(TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY |
+                TMOL_LJLK_ACCUMULATE))
+    return TMOL_LJLK_EINVAL;
+  if (!(flags & TMOL_LJLK_NO_ENERGY) && !energies) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_NO_ENERGY) && !(flags & TMOL_LJLK_WGRAD)) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_WGRAD) && (!dxyz || !dwaters || !weights)) return TMOL_LJLK_EINVAL;
+  if (n_wave_groups < 0 || n_wave_groups > (1 << 20)) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_lkb_score<float>(s, bt, k, coords, waters, workspace, row_capacity, flags,
+                                weights, energies, dxyz, dwaters, n_wave_groups, st, L);
+  return run_lkb_score<double>(s, bt, k, coords, waters, workspace, row_capacity, flags,
+                               weights, energies, dxyz, dwaters, n_wave_groups, st, L);
+}
 
 int tmol_lbfgs_dots(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,
                     const double* H, const double* vecs, int32_t n_vecs, double* out,

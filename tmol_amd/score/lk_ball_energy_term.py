@@ -178,89 +178,87 @@ class LKBallBlockType:
         self.recipes = self._recipes(rt, term)
 
     def _recipes(self, rt, term):
-        """One row per water slot: [atom, slot, kind, a1, a2, a3, torsion
-        index, 0].  Donor: a1 = D, a2 = H.  Acceptor: a1 = A, a2 = B,
-        a3 = B0 (ring: the frame is built from (B + B0) / 2 and B0)."""
-        rows = []
-        for h, d, w in zip(self.donH, self.D_for_H, self.which_H):
-            rows.append([d, w, W_DONOR, d, h, -1, 0, 0])
-        nbr_rows = self.atom_all_bond_ranges
-        ab = self.all_bonds
+        return water_recipes(
+            self.donH, self.D_for_H, self.which_H, self.acc, self.acc_hyb, self.n_H,
+            self.all_bonds, self.atom_all_bond_ranges, self.is_hydrogen,
+            (len(term.tors_sp2), len(term.tors_sp3), len(term.tors_ring)), rt.name)
 
-        def bound(a):
-            # (atom or -1 for an inter-block partner) in the reference's order
-            return [int(ab[k, 1]) for k in range(nbr_rows[a, 0], nbr_rows[a, 1])]
 
-        def cross(lst, upto):
-            # the walk would read an inter-block partner before settling
-            return any(x < 0 for x in lst[: upto + 1])
+def water_recipes(donH, D_for_H, which_H, acc, acc_hyb, n_H, all_bonds, ranges, is_h, n_tors,
+                  name="block type"):
+    """Rows [atom, slot, kind, a1, a2, a3, torsion index, 0], one per water
+    of a block type: donor waters (a1 = D, a2 = H) in slot which_H; acceptor
+    waters (a1 = A, a2 = B, a3 = B0) in slots n_H[A] + torsion index, the
+    bases by the reference's walk over the all-bonds table
+    (tmol/score/hbond/identification.hh:26-167).  Acceptors with no water
+    frame (no heavy neighbor; sp3 without a second distinct base) get no
+    rows.  A walk that would read an inter-block partner raises
+    NotImplementedError."""
+    rows = []
+    for h, d, w in zip(donH, D_for_H, which_H):
+        rows.append([int(d), int(w), W_DONOR, int(d), int(h), -1, 0, 0])
 
-        for a in self.acc:
-            hyb = self.acc_hyb[a]
-            tors = {1: term.tors_sp2, 2: term.tors_sp3, 3: term.tors_ring}.get(hyb)
-            if tors is None:
-                continue
-            nb = bound(a)
-            B = B0 = -1
-            # B: first heavy neighbor (identification.hh:40-45, 85-90, 125-130)
-            for k, x in enumerate(nb):
-                if x < 0 or not self.is_hydrogen[x]:
+    def bound(a):
+        return [int(all_bonds[k, 1]) for k in range(ranges[a, 0], ranges[a, 1])]
+
+    def fail(a):
+        raise NotImplementedError(
+            "lk_ball: acceptor %d of %s has its base across an inter-block bond" % (a, name))
+
+    for a in acc:
+        hyb = int(acc_hyb[a])
+        if hyb not in (1, 2, 3):
+            continue
+        nb = bound(a)
+        B = B0 = -1
+        # B: first heavy neighbor (identification.hh:40-45, 85-90, 125-130)
+        for x in nb:
+            if x < 0:
+                fail(a)
+            if not is_h[x]:
+                B = x
+                break
+        if B < 0:
+            continue
+        if hyb == 1:  # sp2: first neighbor of B other than A
+            for x in bound(B):
+                if x != a:
                     if x < 0:
-                        raise NotImplementedError(
-                            "lk_ball: acceptor %s of %s has its base across an inter-block bond"
-                            % (rt.atom_names[a], rt.name))
-                    B = x
+                        fail(a)
+                    B0 = x
                     break
-            if B < 0:
+            if B0 < 0:
                 continue
-            if hyb == 1:  # sp2: first neighbor of B other than A
-                nbB = bound(B)
-                for k, x in enumerate(nbB):
-                    if x != a:
-                        if x < 0:
-                            raise NotImplementedError(
-                                "lk_ball: acceptor %s of %s has its base across an inter-block"
-                                " bond" % (rt.atom_names[a], rt.name))
-                        B0 = x
-                        break
-                if B0 < 0:
-                    continue
-            elif hyb == 2:  # sp3: a bonded hydrogen, else another neighbor, else B
+        elif hyb == 2:  # sp3: a bonded hydrogen, else another neighbor, else B
+            for x in nb:
+                if x >= 0 and x != B and is_h[x]:
+                    B0 = x
+                    break
+            if B0 < 0:
                 for x in nb:
-                    if x >= 0 and x != B and self.is_hydrogen[x]:
-                        B0 = x
-                        break
-                if B0 < 0:
-                    for x in nb:
-                        if x != B:
-                            if x < 0:
-                                raise NotImplementedError(
-                                    "lk_ball: acceptor %s of %s has its base across an"
-                                    " inter-block bond" % (rt.atom_names[a], rt.name))
-                            B0 = x
-                            break
-                if B0 < 0:
-                    B0 = B
-                if B0 == B:
-                    continue  # no distinct second base: no waters
-            else:  # ring: another heavy neighbor
-                for x in nb:
-                    if x != B and (x < 0 or not self.is_hydrogen[x]):
+                    if x != B:
                         if x < 0:
-                            raise NotImplementedError(
-                                "lk_ball: acceptor %s of %s has its base across an inter-block"
-                                " bond" % (rt.atom_names[a], rt.name))
+                            fail(a)
                         B0 = x
                         break
-                if B0 < 0:
-                    continue
-            kind = {1: W_SP2, 2: W_SP3, 3: W_RING}[hyb]
-            for w in range(len(tors)):
-                slot = w + int(self.n_H[a])
-                if slot >= MAX_N_WATER:
-                    raise ValueError("lk_ball: more than 4 waters on %s" % rt.atom_names[a])
-                rows.append([a, slot, kind, a, B, B0, w, 0])
-        return numpy.array(rows, dtype=numpy.int32).reshape(-1, RECIPE_WIDTH)
+            if B0 < 0 or B0 == B:
+                continue  # no distinct second base: no waters
+        else:  # ring: another heavy neighbor
+            for x in nb:
+                if x != B and (x < 0 or not is_h[x]):
+                    if x < 0:
+                        fail(a)
+                    B0 = x
+                    break
+            if B0 < 0:
+                continue
+        kind = {1: W_SP2, 2: W_SP3, 3: W_RING}[hyb]
+        for w in range(n_tors[hyb - 1]):
+            slot = w + int(n_H[a])
+            if slot >= MAX_N_WATER:
+                raise ValueError("lk_ball: more than 4 waters on atom %d of %s" % (a, name))
+            rows.append([int(a), slot, kind, int(a), B, B0, w, 0])
+    return numpy.array(rows, dtype=numpy.int32).reshape(-1, RECIPE_WIDTH)
 
 
 class LKBallEnergyTerm:
@@ -405,7 +403,7 @@ def watergen_op_args(pose_stack, term: LKBallEnergyTerm, coords=None):
 
 
 def score_op_args(pose_stack, term: LKBallEnergyTerm, waters, coords=None, block_pair=False):
-    """The 29 arguments of ``tmol_lk_ball::lk_ball_pose_score`` for a padded
+    """The 28 arguments of ``tmol_lk_ball::lk_ball_pose_score`` for a padded
     stack (lk_ball_energy_term.py:272-296)."""
     ps = pose_stack
     if coords is None:
