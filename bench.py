@@ -149,6 +149,8 @@ def main():
                     help="skip the rotamer-pair scoring leg (16 x 300-residue repacking)")
     ap.add_argument("--no-elec", action="store_true",
                     help="skip the elec leg (same 512-pose batch, elec energy + gradient)")
+    ap.add_argument("--no-lkball", action="store_true",
+                    help="skip the lk_ball leg (same 512-pose batch, waters + energy + gradient)")
     ap.add_argument("--no-padded", action="store_true",
                     help="skip the padded drop-in op leg (64 poses, per-call set-up included)")
     args = ap.parse_args()
@@ -338,6 +340,12 @@ def main():
         result["elec"] = elec_bench.run(dev, poses=poses,
                                         cpu=None if args.no_cpu else bench_cpu.elec_poses, log=log)
         result["elec"].pop("census", None)
+    if not args.no_lkball:
+        from tmol_amd import lk_ball_bench
+
+        log("lk_ball ...")
+        result["lk_ball"] = lk_ball_bench.run(
+            dev, poses=poses, cpu=None if args.no_cpu else bench_cpu.lk_ball_poses, log=log)
     if not args.no_padded:
         log("padded drop-in op ...")
         result["padded"] = padded_dropin(dev)

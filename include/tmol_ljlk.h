@@ -528,7 +528,8 @@ int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
  * the coordinates for the float ones):
  *   occluders: per block type the heavy atoms, those that carry waters
  *     ("polar": donor heavy atoms and acceptors) first, at most 32 per type
- *     (ENOTSUP beyond), with the reference's 9 per-atom parameters
+ *     of which at most 16 polar (ENOTSUP beyond), with the reference's 9
+ *     per-atom parameters
  *     (lj_radius, lk_dgfree, lk_lambda, lk_volume, is_donor, is_hydroxyl,
  *     is_polarh, is_acceptor, is_carbon_lk);
  *   recipes: one row per water slot, TMOL_LKBALL_RECIPE_WIDTH int32:
@@ -542,6 +543,7 @@ int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
 typedef struct {
   int32_t max_n_occ;             /* Omax (<= 32) */
   int32_t max_n_recipes;         /* Rmax */
+  int32_t max_n_polar;           /* most polar atoms of a block type (<= 16) */
   const int32_t* n_occ;          /* [T] */
   const int32_t* n_polar;        /* [T] */
   const int32_t* occ_atoms;      /* [T, Omax] block atom index, polar first */
@@ -554,6 +556,12 @@ typedef struct {
   const void* watergen_params;   /* [4] lkb_water_dist, lkb_water_angle_sp2, _sp3, _ring (rad) */
   const void* water_tors;        /* [3, 4] torsions (rad) of sp2, sp3, ring waters */
 } tmol_lkball_tables;
+
+/* Workspace bytes of tmol_lkball_neighbors + tmol_lkball_score (the LJ+LK
+ * layout for row_capacity, then per-launch block records). */
+int64_t tmol_lkball_workspace_bytes(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                                    const tmol_lkball_tables* lkb, int64_t row_capacity,
+                                    int32_t dtype);
 
 /* Waters [NA, 4, 3] (NaN where no water is built), the reference's
  * gen_pose_waters output. */
@@ -568,7 +576,11 @@ int tmol_lkball_waters_backward(const tmol_ljlk_stack* stack, const tmol_ljlk_bl
 
 /* Neighbor pass for lk_ball (spheres + work list, as tmol_ljlk_neighbors,
  * without the LJ+LK atom records): blocks within max_dis (the reference's
- * host scalar) of each other. */
+ * host scalar) of each other.  Rows are full: row g lists every neighbor
+ * block of g in its pose (both triangles; the row capacity is the most
+ * blocks of any pose, as for tmol_ljlk_neighbors); ragged stacks need
+ * pose_block_start.  dense_neighbors receives the upper triangle (the
+ * reference's flags). */
 int tmol_lkball_neighbors(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
                           const void* coords, int32_t dtype, double max_dis,
                           const int32_t* tiles, int64_t n_tiles, void* workspace,
@@ -577,8 +589,8 @@ int tmol_lkball_neighbors(const tmol_ljlk_stack* stack, const tmol_ljlk_block_ty
 
 /* Score the work list.
  *   flags 0: energies [4, P] (lk_ball_iso, lk_ball, lk_bridge,
- *     lk_bridge_uncpl); BLOCK_PAIR: [4, P, B, B] padded / [4, G, row_cap]
- *     ragged.
+ *     lk_bridge_uncpl); BLOCK_PAIR (padded stacks; ENOTSUP for ragged):
+ *     [4, P, B, B], upper triangle.
  *   WGRAD (with weights = dTdV of the energies' shape): dxyz [NA, 3] and
  *     dwaters [NA, 4, 3] of sum_t w[t] V_t (zeroed first unless ACCUMULATE);
  *     with NO_ENERGY the backward alone.  The full coordinate gradient adds
@@ -588,6 +600,15 @@ int tmol_lkball_score(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types*
                       int32_t dtype, void* workspace, int64_t ws_bytes, int64_t row_capacity,
                       int32_t flags, const void* weights, void* energies, void* dxyz,
                       void* dwaters, int32_t n_wave_groups, void* stream);
+
+/* Census of the lk_ball work list: census[4] (device uint64) = candidate
+ * (polar, occluder) pairs, pairs at separation <= 3, evaluated pairs (within
+ * max_dis, polar atom with waters), of which with the water bridge
+ * evaluated; for algorithmic FLOP accounting. */
+int tmol_lkball_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                       const tmol_lkball_tables* lkb, const void* coords, const void* waters,
+                       int32_t dtype, void* workspace, int64_t ws_bytes, int64_t row_capacity,
+                       unsigned long long* census, void* stream);
 
 /* ---------------------------------------------------------------------
  * Segmented L-BFGS history products (float64), for the minimizer that

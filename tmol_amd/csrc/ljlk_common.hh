@@ -207,8 +207,10 @@ constexpr int BONDED_W = TMOL_LJLK_BONDED_WIDTH;
 // two blocks are more than 5 bonds apart (rows are sorted by g2 and short:
 // a handful per block)
 __device__ __forceinline__ const int32_t* bonded_row(const StackView& s, int lo, int hi) {
-  const int k1 = s.bonded_start[lo + 1];
-  for (int k = s.bonded_start[lo]; k < k1; ++k) {
+  const int k0 = s.bonded_start[lo], k1 = s.bonded_start[lo + 1];
+  // rows are sorted by g2: most pairs are past the row's last entry
+  if (k1 == k0 || s.bonded[int64_t(k1 - 1) * BONDED_W] < hi) return nullptr;
+  for (int k = k0; k < k1; ++k) {
     const int g = s.bonded[int64_t(k) * BONDED_W];
     if (g == hi) return s.bonded + int64_t(k) * BONDED_W;
     if (g > hi) break;

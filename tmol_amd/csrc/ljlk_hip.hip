@@ -782,7 +782,8 @@ static int validate_lkb(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* b
   if (!bt->n_atoms || !bt->n_conn || (bt->max_n_conn > 0 && !bt->conn_atoms))
     return TMOL_LJLK_EINVAL;
   if (k->max_n_occ < 0 || k->max_n_recipes < 0) return TMOL_LJLK_EINVAL;
-  if (k->max_n_occ > LKB_OMAX) return TMOL_LJLK_ENOTSUP;
+  if (k->max_n_occ > LKB_OMAX || k->max_n_polar > LKB_PMAX || k->max_n_polar < 0)
+    return TMOL_LJLK_ENOTSUP;
   if (!k->n_occ || !k->n_polar || !k->occ_atoms || !k->occ_params || !k->n_recipes ||
       (k->max_n_recipes > 0 && !k->recipes) || !k->path_distance || !k->global_params ||
       !k->watergen_params || !k->water_tors)
@@ -849,10 +850,12 @@ static int run_lkb_neighbors(const tmol_ljlk_stack* st, const tmol_ljlk_block_ty
     int64_t n = int64_t(s.P) * s.B * s.B;
     hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream, dense, n, 0);
   }
-  if (n_tiles > 0)
-    hipLaunchKernelGGL(neighbor_rows_kernel<Real>, dim3((unsigned)n_tiles), dim3(256), 0, stream, s,
-                       (const int4*)tiles, n_tiles, spheres, (Real)max_dis, row_count, row_items,
-                       row_cap, counters, dense);
+  if (n_tiles > 0) {
+    auto k = neighbor_rows_kernel<Real, true>;  // full rows
+    hipLaunchKernelGGL(k, dim3((unsigned)n_tiles), dim3(256), 0, stream, s, (const int4*)tiles,
+                       n_tiles, spheres, (Real)max_dis, row_count, row_items, row_cap, counters,
+                       dense, st->pose_block_start);
+  }
   return check(hipGetLastError());
 }
 
@@ -860,7 +863,7 @@ template <typename Real, int MODE>
 static void launch_lkb(const LkbArgs<Real>& a, int n_wg, hipStream_t stream) {
   hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
                      8 * QSTRIDE, 0u);
-  auto k = lkb_rows_kernel<Real, MODE>;
+  auto k = lkb_gather_kernel<Real, MODE>;
   if (n_wg <= 0) n_wg = resident_workgroups(k);
   hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
 }
@@ -870,7 +873,7 @@ static int run_lkb_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types*
                          const tmol_lkball_tables* k, const void* coords, const void* waters,
                          void* ws, int64_t row_cap, int flags, const void* weights,
                          void* energies, void* dxyz, void* dwat, int n_wg, hipStream_t stream,
-                         const WsLayout& Lw) {
+                         const WsLayout& Lw, unsigned long long* census = nullptr) {
   LkbArgs<Real> a{};
   a.s = make_stack_view(st, btp->max_n_conn);
   a.bt = make_bt_view(btp);
@@ -893,6 +896,24 @@ static int run_lkb_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types*
   a.energies_bp = (Real*)energies;
   a.dxyz = (Real*)dxyz;
   a.dwat = (Real*)dwat;
+  a.census = census;
+  const LkbWs Lk = lkb_ws_layout(Lw, st->n_blocks_total, k->max_n_occ);
+  a.info = (const int4*)(w + Lk.info);
+  a.opos = (const float4*)(w + Lk.opos);
+  a.oatom = (const int32_t*)(w + Lk.oatom);
+  {
+    const int64_t n = int64_t(st->n_blocks_total) * k->max_n_occ;
+    hipLaunchKernelGGL(lkb_prep_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream, a.s,
+                       k->n_occ, k->n_polar, btp->n_conn, k->occ_atoms, k->max_n_occ,
+                       (const Real*)coords, (int4*)(w + Lk.info), (float4*)(w + Lk.opos),
+                       (int32_t*)(w + Lk.oatom));
+  }
+  if (census) {
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, stream, census, 4,
+                       0ull);
+    launch_lkb<Real, M_CENSUS>(a, n_wg, stream);
+    return check(hipGetLastError());
+  }
   const bool bp = flags & TMOL_LJLK_BLOCK_PAIR;
   const bool noe = flags & TMOL_LJLK_NO_ENERGY;
   const bool acc = flags & TMOL_LJLK_ACCUMULATE;
@@ -1645,6 +1666,14 @@ int tmol_elec_rotamer_score(const tmol_ljlk_rotamer_set* rs, const tmol_ljlk_blo
 
 // ------------------------------------------------------------------ L-BFGS history products
 
+int64_t tmol_lkball_workspace_bytes(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                                    const tmol_lkball_tables* k, int64_t row_capacity,
+                                    int32_t dtype) {
+  if (!s || !bt || !k || row_capacity < 0 || k->max_n_occ < 0) return -1;
+  return lkb_ws_layout(ws_layout(s, bt, row_capacity, dtype), s->n_blocks_total, k->max_n_occ)
+      .total;
+}
+
 int tmol_lkball_waters(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                        const tmol_lkball_tables* k, const void* coords, int32_t dtype,
                        void* waters, void* stream) {
@@ -1681,6 +1710,7 @@ int tmol_lkball_neighbors(const tmol_ljlk_stack* s, const tmol_ljlk_block_types*
   if (!std::isfinite(max_dis) || max_dis < 0) return TMOL_LJLK_EINVAL;
   if (n_tiles >= (int64_t(1) << 31)) return TMOL_LJLK_ENOTSUP;
   if (dense && s->layout != TMOL_LJLK_PADDED) return TMOL_LJLK_EINVAL;
+  if (s->layout == TMOL_LJLK_RAGGED && !s->pose_block_start) return TMOL_LJLK_EINVAL;
   WsLayout L = ws_layout(s, bt, row_capacity, dtype);
   if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
   hipStream_t st = (hipStream_t)stream;
@@ -1700,13 +1730,14 @@ int tmol_lkball_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
   if (v) return v;
   if (!coords || !waters || !workspace || row_capacity <= 0) return TMOL_LJLK_EINVAL;
   WsLayout L = ws_layout(s, bt, row_capacity, dtype);
-  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  if (ws_bytes < lkb_ws_layout(L, s->n_blocks_total, k->max_n_occ).total) return TMOL_LJLK_EINVAL;
   if (flags & ~(TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY |
                 TMOL_LJLK_ACCUMULATE))
     return TMOL_LJLK_EINVAL;
   if (!(flags & TMOL_LJLK_NO_ENERGY) && !energies) return TMOL_LJLK_EINVAL;
   if ((flags & TMOL_LJLK_NO_ENERGY) && !(flags & TMOL_LJLK_WGRAD)) return TMOL_LJLK_EINVAL;
   if ((flags & TMOL_LJLK_WGRAD) && (!dxyz || !dwaters || !weights)) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_BLOCK_PAIR) && s->layout != TMOL_LJLK_PADDED) return TMOL_LJLK_ENOTSUP;
   if (n_wave_groups < 0 || n_wave_groups > (1 << 20)) return TMOL_LJLK_EINVAL;
   hipStream_t st = (hipStream_t)stream;
   if (dtype == TMOL_LJLK_F32)
@@ -1714,6 +1745,23 @@ int tmol_lkball_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                                 weights, energies, dxyz, dwaters, n_wave_groups, st, L);
   return run_lkb_score<double>(s, bt, k, coords, waters, workspace, row_capacity, flags,
                                weights, energies, dxyz, dwaters, n_wave_groups, st, L);
+}
+
+int tmol_lkball_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                       const tmol_lkball_tables* k, const void* coords, const void* waters,
+                       int32_t dtype, void* workspace, int64_t ws_bytes, int64_t row_capacity,
+                       unsigned long long* census, void* stream) {
+  int v = validate_lkb(s, bt, k, dtype, true);
+  if (v) return v;
+  if (!coords || !waters || !workspace || !census || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < lkb_ws_layout(L, s->n_blocks_total, k->max_n_occ).total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_lkb_score<float>(s, bt, k, coords, waters, workspace, row_capacity, 0, nullptr,
+                                nullptr, nullptr, nullptr, 0, st, L, census);
+  return run_lkb_score<double>(s, bt, k, coords, waters, workspace, row_capacity, 0, nullptr,
+                               nullptr, nullptr, nullptr, 0, st, L, census);
 }
 
 int tmol_lbfgs_dots(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,

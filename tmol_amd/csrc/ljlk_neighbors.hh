@@ -161,12 +161,16 @@ __device__ __forceinline__ void chunk_box(const Real (*sph)[4], const int* typ, 
   }
 }
 
-template <typename Real>
+// FULL: every neighbor g2 of the pose goes to row g1 (both triangles; the
+// rows of a gather kernel that owns its block's atoms), columns from the
+// pose's first block; pose_start (ragged) gives it.  dense still receives
+// the upper triangle only.
+template <typename Real, bool FULL = false>
 __global__ __launch_bounds__(256) void neighbor_rows_kernel(
     StackView s, const int4* __restrict__ tiles, int64_t n_tiles,
     const Real* __restrict__ spheres, Real reach, uint32_t* __restrict__ row_count,
     int32_t* __restrict__ row_items, int64_t row_cap, uint32_t* __restrict__ counters,
-    int32_t* __restrict__ dense) {
+    int32_t* __restrict__ dense, const int32_t* __restrict__ pose_start = nullptr) {
 #pragma clang fp contract(off)
   __shared__ Real sph_r[NBR_TILE][4];
   __shared__ Real sph_c[NBR_TILE][4];
@@ -194,7 +198,9 @@ __global__ __launch_bounds__(256) void neighbor_rows_kernel(
 #pragma unroll
   for (int r = 0; r < 16; ++r) cnt[r] = 0;
   bool overflow = false;
-  for (int col0 = row0; col0 < end; col0 += NBR_TILE) {
+  const int col_begin =
+      !FULL ? row0 : (s.layout == TMOL_LJLK_PADDED ? tl.w * s.B : pose_start[tl.w]);
+  for (int col0 = col_begin; col0 < end; col0 += NBR_TILE) {
     __syncthreads();  // the previous column chunk is no longer read
     if (tid < NBR_TILE) {
       const int g = col0 + tid;
@@ -219,7 +225,7 @@ __global__ __launch_bounds__(256) void neighbor_rows_kernel(
     for (int r = 0; r < 16; ++r) {
       const int lr = wave * 16 + r;
       const int g1 = row0 + lr;
-      const bool ok = col_ok && typ_r[lr] >= 0 && g1 <= g2;
+      const bool ok = col_ok && typ_r[lr] >= 0 && (FULL || g1 <= g2);
       const Real dx = sph_r[lr][0] - c0;
       const Real dy = sph_r[lr][1] - c1;
       const Real dz = sph_r[lr][2] - c2;
@@ -238,7 +244,7 @@ __global__ __launch_bounds__(256) void neighbor_rows_kernel(
           row_items[int64_t(g1) * row_cap + idx] = g2;
         else
           overflow = true;
-        if (dense) {
+        if (dense && g1 <= g2) {
           const int b2 = g2 - (g2 / s.B) * s.B;
           dense[int64_t(g1) * s.B + b2] = 1;
         }

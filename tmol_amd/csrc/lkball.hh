@@ -15,15 +15,18 @@
 // vector-Jacobian product through the frame's normalisations and cross
 // products.
 //
-// Score: persistent waves take rows of the work list (row g1: its neighbor
-// blocks g2 >= g1).  A wave stages g1's occluders (heavy atoms, polar ones
-// first) with coordinates, waters and parameters in LDS once per row, then
-// each g2's; lanes run over the (polar, occluder) pairs of the block pair in
-// both directions (intra-block: ordered pairs of distinct atoms).  Pairs
-// beyond max_dis or at count-pair separation <= 3 score zero.  Gradients
-// (weighted by dTdV) accumulate per staged atom in LDS -- coordinates and the
-// four waters, 15 sums -- and are flushed to global memory once per block
-// pair (g2) and once per row (g1).
+// Score: persistent waves take rows of a full work list (row g: every
+// neighbor block of g, both triangles).  A row owns its block's atoms: it
+// stages them (coordinates, parameters, waters, path distances) in LDS with
+// gradient sums, tests the (polar, occluder) pairs against each neighbor in
+// both directions -- g's polar atoms against the neighbor's occluders and
+// the neighbor's polar atoms against g's occluders -- four neighbors per pass
+// (16 lanes stage each), and queues the pairs within max_dis at count-pair
+// separation > 3 in a FIFO that is evaluated 64 pairs at a time.  Each pair
+// is thus evaluated in both blocks' rows; a row keeps only its own atoms'
+// gradient, and the energy is counted in the row of the lower block (intra-
+// block pairs once).  Gradients need no global atomics: every atom is
+// written by its own row.
 #pragma once
 
 #include "ljlk_common.hh"
@@ -32,6 +35,8 @@
 namespace tmol_ljlk {
 
 constexpr int LKB_OMAX = 32;  // occluders (heavy atoms) of one block the score kernel stages
+constexpr int LKB_PMAX = 16;  // polar atoms (with waters) of one block
+constexpr int LKB_ITEMS = 4;  // neighbor blocks staged per pass (16 lanes each)
 constexpr int LKB_RW = TMOL_LKBALL_RECIPE_WIDTH;
 
 template <typename Real>
@@ -236,7 +241,7 @@ template <typename Real>
 struct LkbArgs {
   StackView s;
   BTView bt;  // n_atoms, n_conn, conn_atoms
-  WorkView w;
+  WorkView w;  // full rows: every neighbor block of g (both triangles) in row g
   const Real* coords;         // [NA, 3]
   const Real* waters;         // [NA, 4, 3], NaN where absent
   const int32_t* n_occ;       // [T]
@@ -248,143 +253,189 @@ struct LkbArgs {
   const Real* gp;             // lj_hbond_dis, lj_hbond_OH_donor_dis, lj_hbond_hdis,
                               // lkb_water_dist, max_dis
   uint32_t* row_cursor;
-  const Real* weights;        // [4, P] or [4, block-pair plane]
+  const Real* weights;        // [4, P] or [4, P, B, B]
   double* energies64;         // [4, P]
-  Real* energies_bp;          // [4, block-pair plane]
+  Real* energies_bp;          // [4, P, B, B] (padded)
   Real* dxyz;                 // [NA, 3]
   Real* dwat;                 // [NA, 4, 3]
+  unsigned long long* census;  // M_CENSUS: [4] candidate pairs, separation <= 3,
+                               // evaluated pairs, bridge evaluations
+  // per-launch block records (lkb_prep_kernel, in the workspace)
+  const int4* info;     // [G] block type, n_occ | n_pol << 8 | n_conn << 16, atom offset,
+                        // last bonded partner (ragged; -1 none)
+  const float4* opos;   // [G, Omax] occluder coordinates (float)
+  const int32_t* oatom; // [G, Omax] occluder atom index
 };
 
+// workspace regions of the lk_ball score after the shared layout
+struct LkbWs {
+  int64_t info, opos, oatom, total;
+};
+inline LkbWs lkb_ws_layout(const WsLayout& L, int64_t G, int Omax) {
+  LkbWs w;
+  int64_t o = L.total;
+  w.info = o;
+  o = align256(o + 16 * G);
+  w.opos = o;
+  o = align256(o + 16 * G * int64_t(Omax));
+  w.oatom = o;
+  o = align256(o + 4 * G * int64_t(Omax));
+  w.total = o;
+  return w;
+}
+
+// block records for a launch: one thread per (block, occluder slot)
+template <typename Real>
+__global__ __launch_bounds__(256) void lkb_prep_kernel(StackView s, const int32_t* n_occ,
+                                                       const int32_t* n_pol,
+                                                       const int32_t* n_conn,
+                                                       const int32_t* occ_atoms, int Omax,
+                                                       const Real* coords, int4* info,
+                                                       float4* opos, int32_t* oatom) {
+  const int64_t n = int64_t(s.G) * Omax;
+  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
+       t += int64_t(gridDim.x) * blockDim.x) {
+    const int g = int(t / Omax), k = int(t - int64_t(g) * Omax);
+    const int bt = s.block_type[g];
+    const int no = bt >= 0 ? n_occ[bt] : 0;
+    const int off = s.block_atom_offset[g];
+    if (k == 0) {
+      int4 r;
+      r.x = bt;
+      r.y = bt >= 0 ? (no | (n_pol[bt] << 8) | (n_conn[bt] << 16)) : 0;
+      r.z = off;
+      r.w = -1;
+      if (s.layout == TMOL_LJLK_RAGGED) {
+        // the forward bonded table's rows are sorted by partner
+        const int k0 = s.bonded_start[g], k1 = s.bonded_start[g + 1];
+        if (k1 > k0) r.w = s.bonded[int64_t(k1 - 1) * BONDED_W];
+      }
+      info[g] = r;
+    }
+    if (k < no) {
+      const int64_t at = off + occ_atoms[int64_t(bt) * Omax + k];
+      opos[t] = make_float4(float(coords[3 * at]), float(coords[3 * at + 1]),
+                            float(coords[3 * at + 2]), 0.f);
+      oatom[t] = int32_t(at);
+    }
+  }
+}
+
+template <int MODE>
+constexpr bool lkb_deriv_mode() {
+  return (MODE & M_WGRAD) != 0;
+}
+
+// one atom of a pair, in registers
+template <typename Real>
+struct LkbAtom {
+  Real x, y, z;
+  Real r, K, il2, vol;  // lj_radius, lk_dgfree / (2 pi^1.5 lk_lambda), 1 / lk_lambda^2, lk_volume
+  int flg;              // F_DONOR ... F_CARBON_LK
+  Real w[12];           // waters, NaN where absent
+};
+
+template <typename Real>
+__device__ __forceinline__ void lkb_params(const Real* p, Real& r, Real& K, Real& il2, Real& vol,
+                                           int& f) {
+  const Real pi_pow1p5 = Real(5.56832799683f);  // lk_isotropic.hh:40
+  r = p[0];
+  K = p[1] * fast_rcp(Real(2) * pi_pow1p5 * p[2]);
+  il2 = fast_rcp(p[2] * p[2]);
+  vol = p[3];
+  f = 0;
+  if (p[4] != Real(0)) f |= F_DONOR;
+  if (p[4] != Real(0) && p[5] != Real(0)) f |= F_OH_DONOR;
+  if (p[6] != Real(0)) f |= F_POLARH;
+  if (p[7] != Real(0)) f |= F_ACCEPTOR;
+  if (p[8] != Real(0)) f |= F_CARBON_LK;
+}
+
+constexpr int LKB_FIFO = 128;  // compacted pairs awaiting evaluation (two chunks of 64)
+
+// the row's own block (g): its occluders, with the row's gradient sums
 template <typename Real, bool DERIV>
-struct LkbBlock {
+struct LkbRow {
   using T = typename LdsAcc<Real>::T;
-  Real x[LKB_OMAX], y[LKB_OMAX], z[LKB_OMAX];
-  Real w[12][LKB_OMAX];  // waters, component-major; NaN where absent
-  Real r[LKB_OMAX];      // lj_radius
-  Real K[LKB_OMAX];      // lk_dgfree / (2 pi^1.5 lk_lambda)
-  Real il2[LKB_OMAX];    // 1 / lk_lambda^2
-  Real vol[LKB_OMAX];    // lk_volume
+  alignas(16) Real pos[LKB_OMAX][4];  // x, y, z, lj_radius
+  alignas(16) Real prm[LKB_OMAX][4];  // K, 1 / lambda^2, lk_volume, flags
+  alignas(16) Real w[LKB_PMAX][12];   // waters of the polar slots, NaN where absent
   int32_t atom[LKB_OMAX];
-  uint8_t loc[LKB_OMAX];  // atom index within the block
-  uint8_t flg[LKB_OMAX];  // F_DONOR ... F_CARBON_LK
-  uint8_t pd[MAX_CONN][LKB_OMAX];  // path distance connection atom -> atom
-  T acc[DERIV ? 15 : 1][DERIV ? LKB_OMAX : 1];  // dE/dx (3), dE/dwaters (12)
-  int n_occ, n_pol, bt, nc;
+  uint8_t loc[LKB_OMAX];
+  uint8_t pd[MAX_CONN][LKB_OMAX];
+  T acc[DERIV ? 3 : 1][DERIV ? LKB_OMAX : 1];    // dE/dx
+  T accw[DERIV ? 12 : 1][DERIV ? LKB_PMAX : 1];  // dE/dwaters (polar slots)
+};
+
+// LKB_ITEMS neighbor blocks staged for the pair tests (16 lanes each)
+struct LkbNbr {
+  float x[LKB_ITEMS][LKB_OMAX], y[LKB_ITEMS][LKB_OMAX], z[LKB_ITEMS][LKB_OMAX];
+  int32_t atom[LKB_ITEMS][LKB_OMAX];
+  uint8_t pd[LKB_ITEMS][MAX_CONN][LKB_OMAX];
+  uint8_t cs[LKB_ITEMS][MAX_CONN * MAX_CONN];
+};
+
+// a compacted pair: the other atom (global index, its block type and slot)
+// and meta = slot i of g (5 bits) | dir << 5 (0: g's polar atom against the
+// other occluder, 1: the other polar atom against g's occluder) | counted << 6
+// (energy of this pair belongs to this row) | sep4 << 7 (count-pair weight
+// 0.2) | intra << 8 | slot j of g << 9 (intra pairs: the occluder)
+struct LkbFifo {
+  int32_t atom[LKB_FIFO];
+  int32_t tslot[LKB_FIFO];  // block type * Omax + slot of the other atom
+  int32_t blk[LKB_FIFO];    // the other atom's block
+  uint32_t meta[LKB_FIFO];
 };
 
 template <typename Real, bool DERIV>
 struct LkbLDS {
-  LkbBlock<Real, DERIV> b[2];
-  uint8_t cs[MAX_CONN * MAX_CONN];
+  LkbRow<Real, DERIV> row;
+  LkbNbr nb;
+  LkbFifo q;
 };
 
 template <typename Real, bool DERIV>
-__device__ void lkb_stage(LkbBlock<Real, DERIV>& B, const LkbArgs<Real>& a, int g, int lane) {
-  const int bt = a.s.block_type[g];
-  const int no = a.n_occ[bt], np = a.n_pol[bt], nc = a.bt.n_conn[bt];
-  const int64_t off = a.s.block_atom_offset[g];
-  if (lane < no) {
-    const int la = a.occ_atoms[int64_t(bt) * a.Omax + lane];
-    const int64_t at = off + la;
-    B.x[lane] = a.coords[3 * at];
-    B.y[lane] = a.coords[3 * at + 1];
-    B.z[lane] = a.coords[3 * at + 2];
-    const Real nan = __builtin_nan("");
-#pragma unroll
-    for (int k = 0; k < 12; ++k) B.w[k][lane] = lane < np ? a.waters[12 * at + k] : Real(nan);
-    const Real* p = a.occ_params + (int64_t(bt) * a.Omax + lane) * 9;
-    const Real pi_pow1p5 = Real(5.56832799683f);  // lk_isotropic.hh:40
-    B.r[lane] = p[0];
-    B.K[lane] = p[1] / (Real(2) * pi_pow1p5 * p[2]);
-    B.il2[lane] = Real(1) / (p[2] * p[2]);
-    B.vol[lane] = p[3];
-    int f = 0;
-    if (p[4] != Real(0)) f |= F_DONOR;
-    if (p[4] != Real(0) && p[5] != Real(0)) f |= F_OH_DONOR;
-    if (p[6] != Real(0)) f |= F_POLARH;
-    if (p[7] != Real(0)) f |= F_ACCEPTOR;
-    if (p[8] != Real(0)) f |= F_CARBON_LK;
-    B.flg[lane] = uint8_t(f);
-    B.atom[lane] = int32_t(at);
-    B.loc[lane] = uint8_t(la);
-#pragma unroll
-    for (int c = 0; c < MAX_CONN; ++c) {
-      int v = 6;
-      if (c < nc) {
-        const int ca = a.bt.conn_atoms[bt * a.bt.C + c];
-        v = a.path_distance[(int64_t(bt) * a.bt.Amax + ca) * a.bt.Amax + la];
-      }
-      B.pd[c][lane] = uint8_t(v);
-    }
-    if (DERIV)
-#pragma unroll
-      for (int k = 0; k < 15; ++k) B.acc[k][lane] = 0;
-  }
-  if (lane == 0) {
-    B.n_occ = no;
-    B.n_pol = np;
-    B.bt = bt;
-    B.nc = nc;
-  }
+__device__ __forceinline__ void lkb_acc(LkbRow<Real, DERIV>& R, int k, int slot, Real v) {
+  if (!DERIV || v == Real(0)) return;
+  if (k < 3)
+    atomicAdd(&R.acc[k][slot], LdsAcc<Real>::enc(v));
+  else if (slot < LKB_PMAX)  // waters exist on polar slots only
+    atomicAdd(&R.accw[k - 3][slot], LdsAcc<Real>::enc(v));
 }
 
+// The four lk_ball energies of polar atom I against occluder J at count-pair
+// weight wcp (separation > 3) (lk_ball_score::V / dV, lk_ball.hh:335-494),
+// added into e4.  With DERIV the gradient of sum_t w4[t] E_t is added into
+// the row's LDS sums: I's coordinates and waters at slot si, J's at slot sj
+// (-1: not this row's atom, not computed).  Returns 0 at or beyond max_dis
+// (nothing added), 1, or 2 when the water bridge was evaluated.
 template <typename Real, bool DERIV>
-__device__ void lkb_flush(LkbBlock<Real, DERIV>& B, const LkbArgs<Real>& a, int lane) {
-  if (!DERIV) return;
-  if (lane < B.n_occ) {
-    const int64_t at = B.atom[lane];
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-      const auto v = B.acc[k][lane];
-      if (v != 0) atomicAdd(&a.dxyz[3 * at + k], LdsAcc<Real>::dec(v));
-    }
-    if (lane < B.n_pol)
-#pragma unroll
-      for (int k = 0; k < 12; ++k) {
-        const auto v = B.acc[3 + k][lane];
-        if (v != 0) atomicAdd(&a.dwat[12 * at + k], LdsAcc<Real>::dec(v));
-      }
-  }
-}
-
-template <typename Real>
-__device__ __forceinline__ void lds_add(typename LdsAcc<Real>::T* p, Real v) {
-  if (v != Real(0)) atomicAdd(p, LdsAcc<Real>::enc(v));
-}
-
-// one (polar i of block X, occluder j of block Y) pair at separation sep
-// (> 3) and distance below max_dis: energies into e4, weighted derivatives
-// into the blocks' LDS sums
-template <typename Real, bool DERIV>
-__device__ __forceinline__ void lkb_pair(LkbBlock<Real, DERIV>& X, int i, LkbBlock<Real, DERIV>& Y,
-                                         int j, int sep, Real dx, Real dy, Real dz, Real d2,
-                                         const GlobalParams<Real>& g, Real water_dist,
-                                         const Real* w4, Real* e4) {
-  const Real dist = sqrt(d2);
-  const Real wi0 = X.w[0][i];
-  if (!(wi0 == wi0)) return;  // polar atom without waters
-  const int fi = X.flg[i], fj = Y.flg[j];
-  const Real sigma = lj_sigma(fi, fj, X.r[i], Y.r[j], g);
+__device__ __forceinline__ int lkb_eval(const LkbAtom<Real>& I, const LkbAtom<Real>& J, Real wcp,
+                                        const GlobalParams<Real>& g, Real water_dist,
+                                        const Real* w4, Real* e4, LkbRow<Real, DERIV>& R, int si,
+                                        int sj) {
+  const Real dx = I.x - J.x, dy = I.y - J.y, dz = I.z - J.z;
+  const Real d2 = dx * dx + dy * dy + dz * dz;
+  const Real dist = fast_sqrt(d2);
+  if (!(dist < g.max_dis)) return 0;  // the reference's cutoff, dist >= max_dis
+  const Real sigma = lj_sigma(I.flg, J.flg, I.r, J.r, g);
   Real d_min, cdmin, cdmax;
-  lk_knots(sigma, bool((fi & F_CARBON_LK) && (fj & F_CARBON_LK)), d_min, cdmin, cdmax);
+  lk_knots(sigma, bool((I.flg & F_CARBON_LK) && (J.flg & F_CARBON_LK)), d_min, cdmin, cdmax);
   Real iso, diso;
-  lk_dir<Real, DERIV>(dist, d_min, cdmin, cdmax, -Y.vol[j] * X.K[i], X.r[i], X.il2[i], g, iso,
-                      diso);
-  const Real wcp = cp_weight<Real>(sep);
+  lk_dir<Real, DERIV>(dist, d_min, cdmin, cdmax, -J.vol * I.K, I.r, I.il2, g, iso, diso);
   iso *= wcp;
   diso *= wcp;
-  const Real Jx = Y.x[j], Jy = Y.y[j], Jz = Y.z[j];
   // lk_fraction (lk_ball.hh:42-124)
-  const Real ramp = Real(3.709);
-  Real t = Real(1.4 + double(Y.r[j]));
+  const Real ramp = Real(3.709), inv_ramp = Real(1) / ramp;
+  const Real t = Real(1.4 + double(J.r));
   Real d2_low = t * t - ramp;
   d2_low = d2_low < Real(0) ? Real(0) : d2_low;
   Real fsum = 0, eW[4];
 #pragma unroll
   for (int w = 0; w < 4; ++w) {
-    const Real dx = Jx - X.w[3 * w][i], dy = Jy - X.w[3 * w + 1][i], dz = Jz - X.w[3 * w + 2][i];
-    const Real d2 = (dx * dx + dy * dy + dz * dz) - d2_low;
-    eW[w] = d2 == d2 ? fast_exp(-d2) : Real(0);
+    const Real ex = J.x - I.w[3 * w], ey = J.y - I.w[3 * w + 1], ez = J.z - I.w[3 * w + 2];
+    const Real q = (ex * ex + ey * ey + ez * ez) - d2_low;
+    eW[w] = q == q ? fast_exp(-q) : Real(0);
     fsum += eW[w];
   }
   const Real fwd = -lkb_log(fsum);
@@ -392,36 +443,37 @@ __device__ __forceinline__ void lkb_pair(LkbBlock<Real, DERIV>& X, int i, LkbBlo
   if (fwd < Real(0)) {
     fr = Real(1);
   } else if (fwd < ramp) {
-    const Real q = fwd / ramp;
+    const Real q = fwd * inv_ramp;
     fr = (Real(1) - q * q) * (Real(1) - q * q);
-    if (fwd > Real(0)) dfr = Real(-4) * fwd * (ramp * ramp - fwd * fwd) / ((ramp * ramp) * (ramp * ramp));
+    if (fwd > Real(0)) dfr = Real(-4) * fwd * (ramp * ramp - fwd * fwd) * (inv_ramp * inv_ramp) *
+                             (inv_ramp * inv_ramp);
   }
-  // lk_bridge_fraction (lk_ball.hh:126-277), when j is a donor or acceptor
-  Real ov = Real(0);
-  Real bsum = Real(0), bI = Real(0);  // bI: d(ov)/d|I-J|^2 scale
-  Real bW[4][4];
-  Real afrac = Real(0), dof = Real(0), ofrac = Real(0);
-  const bool bridge = (fj & (F_DONOR | F_ACCEPTOR)) && Y.w[0][j] == Y.w[0][j];
+  // lk_bridge_fraction (lk_ball.hh:126-277), when J is a donor or acceptor
+  // with waters
+  Real ov = Real(0), bsum = Real(0), bI = Real(0), afrac = Real(0), dof = Real(0);
+  const bool bridge = (J.flg & (F_DONOR | F_ACCEPTOR)) && J.w[0] == J.w[0];
+  const Real gap = Real(0.5), width = Real(2.6), aw = Real(2.8 * double(Real(2.6)));
+  const Real inv_width = Real(1) / width, inv_aw = Real(1) / aw;
   if (bridge) {
-    const Real gap = Real(0.5), width = Real(2.6), aw = Real(2.8 * double(Real(2.6)));
 #pragma unroll
     for (int wi = 0; wi < 4; ++wi)
 #pragma unroll
       for (int wj = 0; wj < 4; ++wj) {
-        const Real dx = X.w[3 * wi][i] - Y.w[3 * wj][j];
-        const Real dy = X.w[3 * wi + 1][i] - Y.w[3 * wj + 1][j];
-        const Real dz = X.w[3 * wi + 2][i] - Y.w[3 * wj + 2][j];
-        const Real d2 = (dx * dx + dy * dy + dz * dz) - gap;
-        bW[wi][wj] = d2 == d2 ? fast_exp(-d2) : Real(0);
-        bsum += bW[wi][wj];
+        const Real ex = I.w[3 * wi] - J.w[3 * wj];
+        const Real ey = I.w[3 * wi + 1] - J.w[3 * wj + 1];
+        const Real ez = I.w[3 * wi + 2] - J.w[3 * wj + 2];
+        const Real q = (ex * ex + ey * ey + ez * ez) - gap;
+        bsum += q == q ? fast_exp(-q) : Real(0);
       }
     const Real wd = -lkb_log(bsum);
+    Real ofrac;
     if (wd > width) {
       ofrac = Real(0);
     } else if (wd > Real(0)) {
-      const Real q = wd / width;
+      const Real q = wd * inv_width;
       ofrac = (Real(1) - q * q) * (Real(1) - q * q);
-      dof = Real(-4) * wd * (width * width - wd * wd) / ((width * width) * (width * width));
+      dof = Real(-4) * wd * (width * width - wd * wd) * (inv_width * inv_width) *
+            (inv_width * inv_width);
     } else {
       ofrac = Real(1);
     }
@@ -430,9 +482,10 @@ __device__ __forceinline__ void lkb_pair(LkbBlock<Real, DERIV>& X, int i, LkbBlo
     const Real abd = bd < Real(0) ? -bd : bd;
     Real daf = Real(0);
     if (abd <= aw) {
-      const Real q = bd / aw;
+      const Real q = bd * inv_aw;
       afrac = (Real(1) - q * q) * (Real(1) - q * q);
-      if (abd > Real(0)) daf = Real(-4) * bd * (aw * aw - bd * bd) / ((aw * aw) * (aw * aw));
+      if (abd > Real(0))
+        daf = Real(-4) * bd * (aw * aw - bd * bd) * (inv_aw * inv_aw) * (inv_aw * inv_aw);
     }
     ov = ofrac * afrac;
     bI = ofrac * daf;  // d(ov)/dI = 2 bI (I - J)
@@ -441,67 +494,179 @@ __device__ __forceinline__ void lkb_pair(LkbBlock<Real, DERIV>& X, int i, LkbBlo
   e4[1] += iso * fr;
   e4[2] += iso * ov;
   e4[3] += ov / Real(2);
-  if (!DERIV) return;
-  // weighted derivatives (score types iso, lk_ball, bridge, bridge_uncpl)
-  const Real inv = dist > Real(0) ? Real(1) / dist : Real(0);
+  if (!DERIV) return bridge ? 2 : 1;
+  const Real inv = dist > Real(0) ? fast_rcp(dist) : Real(0);
   const Real cI = (w4[0] + w4[1] * fr + w4[2] * ov) * diso;  // along (I - J) / d
   const Real cB = w4[2] * iso + w4[3] / Real(2);              // on d(ov)
   const Real cF = w4[1] * iso;                                // on d(fr)
   const Real cIJ = cI * inv + cB * bI * Real(2);              // on I - J
-  const Real gIx = cIJ * dx, gIy = cIJ * dy, gIz = cIJ * dz;
-  Real gJx = -gIx, gJy = -gIy, gJz = -gIz;
+  Real gJx = -cIJ * dx, gJy = -cIJ * dy, gJz = -cIJ * dz;
+  if (si >= 0) {
+    lkb_acc(R, 0, si, cIJ * dx);
+    lkb_acc(R, 1, si, cIJ * dy);
+    lkb_acc(R, 2, si, cIJ * dz);
+  }
   if (cF != Real(0) && dfr != Real(0) && fsum != Real(0)) {
-    const Real s = cF * dfr / fsum;
+    const Real sf = cF * dfr * fast_rcp(fsum);
 #pragma unroll
     for (int w = 0; w < 4; ++w) {
       if (eW[w] == Real(0)) continue;
-      const Real dx = Jx - X.w[3 * w][i], dy = Jy - X.w[3 * w + 1][i], dz = Jz - X.w[3 * w + 2][i];
-      const Real c = Real(2) * eW[w] * s;
-      gJx += c * dx;
-      gJy += c * dy;
-      gJz += c * dz;
-      lds_add<Real>(&X.acc[3 + 3 * w][i], -c * dx);
-      lds_add<Real>(&X.acc[4 + 3 * w][i], -c * dy);
-      lds_add<Real>(&X.acc[5 + 3 * w][i], -c * dz);
+      const Real c = Real(2) * eW[w] * sf;
+      const Real ex = J.x - I.w[3 * w], ey = J.y - I.w[3 * w + 1], ez = J.z - I.w[3 * w + 2];
+      gJx += c * ex;
+      gJy += c * ey;
+      gJz += c * ez;
+      if (si >= 0) {
+        lkb_acc(R, 3 + 3 * w, si, -c * ex);
+        lkb_acc(R, 4 + 3 * w, si, -c * ey);
+        lkb_acc(R, 5 + 3 * w, si, -c * ez);
+      }
     }
   }
+  if (sj >= 0) {
+    lkb_acc(R, 0, sj, gJx);
+    lkb_acc(R, 1, sj, gJy);
+    lkb_acc(R, 2, sj, gJz);
+  }
   if (bridge && cB != Real(0) && dof != Real(0) && bsum != Real(0)) {
-    const Real s = cB * afrac * dof / bsum;
-#pragma unroll
-    for (int wi = 0; wi < 4; ++wi)
+    const Real sb = cB * afrac * dof * fast_rcp(bsum);
+#pragma unroll 1
+    for (int wi = 0; wi < 4; ++wi) {
+      Real gx = 0, gy = 0, gz = 0;
 #pragma unroll
       for (int wj = 0; wj < 4; ++wj) {
-        if (bW[wi][wj] == Real(0)) continue;
-        const Real c = Real(2) * bW[wi][wj] * s;
-        const Real dx = X.w[3 * wi][i] - Y.w[3 * wj][j];
-        const Real dy = X.w[3 * wi + 1][i] - Y.w[3 * wj + 1][j];
-        const Real dz = X.w[3 * wi + 2][i] - Y.w[3 * wj + 2][j];
-        lds_add<Real>(&X.acc[3 + 3 * wi][i], c * dx);
-        lds_add<Real>(&X.acc[4 + 3 * wi][i], c * dy);
-        lds_add<Real>(&X.acc[5 + 3 * wi][i], c * dz);
-        lds_add<Real>(&Y.acc[3 + 3 * wj][j], -c * dx);
-        lds_add<Real>(&Y.acc[4 + 3 * wj][j], -c * dy);
-        lds_add<Real>(&Y.acc[5 + 3 * wj][j], -c * dz);
+        const Real ex = I.w[3 * wi] - J.w[3 * wj];
+        const Real ey = I.w[3 * wi + 1] - J.w[3 * wj + 1];
+        const Real ez = I.w[3 * wi + 2] - J.w[3 * wj + 2];
+        const Real q = (ex * ex + ey * ey + ez * ez) - gap;
+        if (!(q == q)) continue;
+        const Real c = Real(2) * fast_exp(-q) * sb;
+        gx += c * ex;
+        gy += c * ey;
+        gz += c * ez;
+        if (sj >= 0) {
+          lkb_acc(R, 3 + 3 * wj, sj, -c * ex);
+          lkb_acc(R, 4 + 3 * wj, sj, -c * ey);
+          lkb_acc(R, 5 + 3 * wj, sj, -c * ez);
+        }
       }
+      if (si >= 0) {
+        lkb_acc(R, 3 + 3 * wi, si, gx);
+        lkb_acc(R, 4 + 3 * wi, si, gy);
+        lkb_acc(R, 5 + 3 * wi, si, gz);
+      }
+    }
   }
-  lds_add<Real>(&X.acc[0][i], gIx);
-  lds_add<Real>(&X.acc[1][i], gIy);
-  lds_add<Real>(&X.acc[2][i], gIz);
-  lds_add<Real>(&Y.acc[0][j], gJx);
-  lds_add<Real>(&Y.acc[1][j], gJy);
-  lds_add<Real>(&Y.acc[2][j], gJz);
+  return bridge ? 2 : 1;
 }
 
-template <int MODE>
-constexpr bool lkb_deriv() {
-  return (MODE & M_WGRAD) != 0;
+template <typename Real, bool DERIV>
+__device__ __forceinline__ void lkb_load_row_atom(const LkbRow<Real, DERIV>& R, int i,
+                                                  LkbAtom<Real>& A) {
+  A.x = R.pos[i][0];
+  A.y = R.pos[i][1];
+  A.z = R.pos[i][2];
+  A.r = R.pos[i][3];
+  A.K = R.prm[i][0];
+  A.il2 = R.prm[i][1];
+  A.vol = R.prm[i][2];
+  A.flg = int(R.prm[i][3]);
+#pragma unroll
+  for (int k = 0; k < 12; ++k) A.w[k] = i < LKB_PMAX ? R.w[i][k] : Real(__builtin_nan(""));
 }
 
+template <typename Real>
+__device__ __forceinline__ void lkb_load_global_atom(const LkbArgs<Real>& a, int64_t at,
+                                                     int tslot, bool polar, LkbAtom<Real>& A) {
+  A.x = a.coords[3 * at];
+  A.y = a.coords[3 * at + 1];
+  A.z = a.coords[3 * at + 2];
+  lkb_params(a.occ_params + int64_t(tslot) * 9, A.r, A.K, A.il2, A.vol, A.flg);
+  // waters: of the polar partner, or of an occluder that may bridge
+  const bool want = polar || (A.flg & (F_DONOR | F_ACCEPTOR));
+  const Real nan = __builtin_nan("");
+#pragma unroll
+  for (int k = 0; k < 12; ++k) A.w[k] = want ? a.waters[12 * at + k] : Real(nan);
+}
+
+// evaluate n (<= 64) entries of the FIFO starting at head (mod LKB_FIFO)
 template <typename Real, int MODE>
-__global__ __launch_bounds__(256) void lkb_rows_kernel(LkbArgs<Real> a) {
-  constexpr bool DERIV = lkb_deriv<MODE>();
+__device__ __forceinline__ void lkb_phase_b(LkbLDS<Real, lkb_deriv_mode<MODE>()>& L,
+                                            const LkbArgs<Real>& a, int g1, int head, int n,
+                                            int lane, const GlobalParams<Real>& g,
+                                            Real water_dist, const Real* w4row, Real* e4,
+                                            uint64_t* cnt) {
+  constexpr bool DERIV = lkb_deriv_mode<MODE>();
   constexpr bool BP = (MODE & M_BP) != 0;
   constexpr bool NOE = (MODE & M_NOE) != 0;
+  constexpr bool CENSUS = (MODE & M_CENSUS) != 0;
+  if (lane >= n) return;
+  const int e = (head + lane) & (LKB_FIFO - 1);
+  const uint32_t m = L.q.meta[e];
+  // block-pair cell (padded layout): (pose, min block, max block)
+  int64_t cell = 0;
+  Real w4[4];
+  if (BP) {
+    const int g2 = L.q.blk[e];
+    const int lo = g1 < g2 ? g1 : g2, hi = g1 < g2 ? g2 : g1;
+    const int p = lo / a.s.B;
+    cell = (int64_t(p) * a.s.B + (lo - p * a.s.B)) * a.s.B + (hi - p * a.s.B);
+    if (DERIV) {
+      const int64_t plane = int64_t(a.s.P) * a.s.B * a.s.B;
+#pragma unroll
+      for (int st = 0; st < 4; ++st) w4[st] = a.weights[int64_t(st) * plane + cell];
+    }
+  } else {
+#pragma unroll
+    for (int st = 0; st < 4; ++st) w4[st] = w4row[st];
+  }
+  const int i = m & 31, dir = (m >> 5) & 1;
+  const bool counted = (m >> 6) & 1, intra = (m >> 8) & 1;
+  const Real wcp = (m >> 7) & 1 ? Real(0.2) : Real(1);
+  LkbAtom<Real> I, J;
+  int si = -1, sj = -1;  // this row's slots on the I and J sides
+  if (intra) {
+    si = i;
+    sj = (m >> 9) & 31;
+    lkb_load_row_atom(L.row, si, I);
+    lkb_load_row_atom(L.row, sj, J);
+  } else if (dir == 0) {
+    si = i;
+    lkb_load_row_atom(L.row, i, I);
+    lkb_load_global_atom(a, L.q.atom[e], L.q.tslot[e], false, J);
+  } else {
+    sj = i;
+    lkb_load_global_atom(a, L.q.atom[e], L.q.tslot[e], true, I);
+    lkb_load_row_atom(L.row, i, J);
+  }
+  if (!(I.w[0] == I.w[0])) return;  // polar atom without waters
+  Real e4p[4] = {0, 0, 0, 0};
+  const int ev = lkb_eval<Real, DERIV>(I, J, wcp, g, water_dist, w4, e4p, L.row, si, sj);
+  if (ev == 0) return;
+  if (CENSUS && counted) {
+    cnt[2] += 1;
+    cnt[3] += ev > 1;
+  }
+  if (counted && !NOE) {
+    if (BP) {
+      const int64_t plane = int64_t(a.s.P) * a.s.B * a.s.B;
+#pragma unroll
+      for (int st = 0; st < 4; ++st)
+        if (e4p[st] != Real(0)) atomicAdd(&a.energies_bp[int64_t(st) * plane + cell], e4p[st]);
+    } else {
+#pragma unroll
+      for (int st = 0; st < 4; ++st) e4[st] += e4p[st];
+    }
+  }
+}
+
+// 4 waves per SIMD: the gradient mode otherwise takes ~200 VGPRs (2 waves)
+template <typename Real, int MODE>
+__global__ __launch_bounds__(256, 4) void lkb_gather_kernel(LkbArgs<Real> a) {
+  constexpr bool DERIV = lkb_deriv_mode<MODE>();
+  constexpr bool BP = (MODE & M_BP) != 0;
+  constexpr bool NOE = (MODE & M_NOE) != 0;
+  constexpr bool CENSUS = (MODE & M_CENSUS) != 0;
   __shared__ LkbLDS<Real, DERIV> lds_all[WAVES_PER_WG];
   const int wave = threadIdx.x >> 6, lane = lane_id();
   LkbLDS<Real, DERIV>& L = lds_all[wave];
@@ -512,106 +677,299 @@ __global__ __launch_bounds__(256) void lkb_rows_kernel(LkbArgs<Real> a) {
   g.hbond_hdis = a.gp[2];
   g.spline_start = g.max_dis - Real(1.5);
   g.max_dis2 = g.max_dis * g.max_dis;
+  const float max_dis2f = float(g.max_dis2);
   const Real water_dist = a.gp[3];
   const int P = a.s.P;
-  const int64_t plane = BP ? bp_plane(a.s, a.w) : int64_t(P);
-  RowQueue q(a.row_cursor, a.s.G);
-  for (int64_t row = q.next(); row >= 0; row = q.next()) {
+  uint64_t cnt[4] = {0, 0, 0, 0};
+  RowQueue rq(a.row_cursor, a.s.G);
+  for (int64_t row = rq.next(); row >= 0; row = rq.next()) {
     const int g1 = int(row);
+    const int4 inf1 = a.info[g1];
+    const int bt1 = inf1.x;
+    if (bt1 < 0) continue;
     const int n_items = int(min<int64_t>(a.w.row_count[g1], a.w.row_cap));  // count may overflow
     if (n_items == 0) continue;
-    if (a.s.block_type[g1] < 0) continue;
     const int pose = a.s.pose_of(g1);
-    lkb_stage<Real, DERIV>(L.b[0], a, g1, lane);
-    Real w4[4] = {0, 0, 0, 0};
+    const int no1 = inf1.y & 255, np1 = (inf1.y >> 8) & 255, nc1 = inf1.y >> 16;
+    const int64_t off1 = inf1.z;
+    const int32_t* items = a.w.row_items + int64_t(g1) * a.w.row_cap;
+    // the row's block
+    if (lane < no1) {
+      const int la = a.occ_atoms[int64_t(bt1) * a.Omax + lane];
+      const int64_t at = off1 + la;
+      L.row.pos[lane][0] = a.coords[3 * at];
+      L.row.pos[lane][1] = a.coords[3 * at + 1];
+      L.row.pos[lane][2] = a.coords[3 * at + 2];
+      int f;
+      lkb_params(a.occ_params + (int64_t(bt1) * a.Omax + lane) * 9, L.row.pos[lane][3],
+                 L.row.prm[lane][0], L.row.prm[lane][1], L.row.prm[lane][2], f);
+      L.row.prm[lane][3] = Real(f);
+      const Real nan = __builtin_nan("");
+      if (lane < LKB_PMAX)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) L.row.w[lane][k] = lane < np1 ? a.waters[12 * at + k] : nan;
+      L.row.atom[lane] = int32_t(at);
+      L.row.loc[lane] = uint8_t(la);
+#pragma unroll
+      for (int c = 0; c < MAX_CONN; ++c) {
+        int v = 6;
+        if (c < nc1) {
+          const int ca = a.bt.conn_atoms[bt1 * a.bt.C + c];
+          v = a.path_distance[(int64_t(bt1) * a.bt.Amax + ca) * a.bt.Amax + la];
+        }
+        L.row.pd[c][lane] = uint8_t(v);
+      }
+      if (DERIV) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) L.row.acc[k][lane] = 0;
+        if (lane < LKB_PMAX)
+#pragma unroll
+          for (int k = 0; k < 12; ++k) L.row.accw[k][lane] = 0;
+      }
+    }
+    Real w4row[4] = {0, 0, 0, 0};
     if (DERIV && !BP)
 #pragma unroll
-      for (int st = 0; st < 4; ++st) w4[st] = a.weights[int64_t(st) * P + pose];
-    double erow[4] = {0, 0, 0, 0};
-    for (int k = 0; k < n_items; ++k) {
-      const int g2 = a.w.row_items[int64_t(g1) * a.w.row_cap + k];
-      const bool intra = g2 == g1;
-      int64_t bpi = 0;
-      if (BP) {
-        bpi = bp_index(a.s, a.w, g1, g2, k);
-        if (DERIV)
+      for (int st = 0; st < 4; ++st) w4row[st] = a.weights[int64_t(st) * P + pose];
+    Real e4[4] = {0, 0, 0, 0};
+    int head = 0, tail = 0;  // FIFO positions (mod LKB_FIFO), wave-uniform
+    // the next LKB_ITEMS items, loaded one pass ahead
+    int nx[LKB_ITEMS];
 #pragma unroll
-          for (int st = 0; st < 4; ++st) w4[st] = a.weights[int64_t(st) * plane + bpi];
+    for (int h = 0; h < LKB_ITEMS; ++h) nx[h] = h < n_items ? items[h] : -1;
+    const int quarter = lane >> 4, ql = lane & 15;
+    for (int k0 = 0; k0 < n_items; k0 += LKB_ITEMS) {
+      // the items' block data, in every lane
+      int g2v[LKB_ITEMS], bt2v[LKB_ITEMS], no2v[LKB_ITEMS], np2v[LKB_ITEMS], nc2v[LKB_ITEMS],
+          msep[LKB_ITEMS], cand[LKB_ITEMS];
+      int4 inf2[LKB_ITEMS];
+#pragma unroll
+      for (int h = 0; h < LKB_ITEMS; ++h) {
+        g2v[h] = nx[h];
+        inf2[h] = g2v[h] >= 0 ? a.info[g2v[h]] : make_int4(-1, 0, 0, -1);
       }
-      int min_sep = 6;
-      if (!intra) {
-        wave_sync();
-        lkb_stage<Real, DERIV>(L.b[1], a, g2, lane);
-        const int nc1 = a.bt.n_conn[L.b[0].bt];
-        const int nc2 = a.bt.n_conn[a.s.block_type[g2]];
-        min_sep = min_block_sep(a.s, g1, g2, nc1, nc2);
-        if (min_sep <= 4 && lane < MAX_CONN * MAX_CONN) {
-          const int c1 = lane / MAX_CONN, c2 = lane - c1 * MAX_CONN;
-          L.cs[lane] = uint8_t(c1 < nc1 && c2 < nc2 ? conn_sep(a.s, g1, g2, c1, c2) : 6);
+      // this quarter's neighbor slots (ql, ql + 16), loaded without waiting
+      // for the block's count
+      const int gs = g2v[quarter];
+      const bool stage = gs >= 0 && gs != g1;
+      float4 sp[2];
+      int32_t sat[2];
+#pragma unroll
+      for (int r = 0; r < 2; ++r) {
+        const int sl = ql + 16 * r;
+        sp[r] = make_float4(0.f, 0.f, 0.f, 0.f);
+        sat[r] = 0;
+        if (stage && sl < a.Omax) {
+          sp[r] = a.opos[int64_t(gs) * a.Omax + sl];
+          sat[r] = a.oatom[int64_t(gs) * a.Omax + sl];
+        }
+      }
+#pragma unroll
+      for (int h = 0; h < LKB_ITEMS; ++h)
+        nx[h] = k0 + LKB_ITEMS + h < n_items ? items[k0 + LKB_ITEMS + h] : -1;
+#pragma unroll
+      for (int h = 0; h < LKB_ITEMS; ++h) {
+        bt2v[h] = inf2[h].x;
+        no2v[h] = inf2[h].y & 255;
+        np2v[h] = (inf2[h].y >> 8) & 255;
+        nc2v[h] = inf2[h].y >> 16;
+        msep[h] = 6;
+        if (g2v[h] == g1) {
+          cand[h] = np1 * no1;
+        } else if (g2v[h] >= 0) {
+          // ragged: the last bonded partner of the lower block rules out most
+          // pairs without a search
+          const int lo = g1 < g2v[h] ? g1 : g2v[h], hi = g1 < g2v[h] ? g2v[h] : g1;
+          const int bmax = lo == g1 ? inf1.w : inf2[h].w;
+          if (a.s.layout == TMOL_LJLK_PADDED || hi <= bmax)
+            msep[h] = min_block_sep(a.s, g1, g2v[h], nc1, nc2v[h]);
+          cand[h] = np1 * no2v[h] + np2v[h] * no1;
+        } else {
+          cand[h] = 0;
+        }
+      }
+      // stage the neighbor blocks, one per quarter wave
+      {
+        const int h = quarter;
+#pragma unroll
+        for (int r = 0; r < 2; ++r) {
+          const int sl = ql + 16 * r;
+          if (stage && sl < no2v[h]) {
+            L.nb.x[h][sl] = sp[r].x;
+            L.nb.y[h][sl] = sp[r].y;
+            L.nb.z[h][sl] = sp[r].z;
+            L.nb.atom[h][sl] = sat[r];
+            if (msep[h] <= 4) {
+              const int la = int(sat[r] - inf2[h].z);
+#pragma unroll
+              for (int c = 0; c < MAX_CONN; ++c) {
+                int v = 6;
+                if (c < nc2v[h]) {
+                  const int ca = a.bt.conn_atoms[bt2v[h] * a.bt.C + c];
+                  v = a.path_distance[(int64_t(bt2v[h]) * a.bt.Amax + ca) * a.bt.Amax + la];
+                }
+                L.nb.pd[h][c][sl] = uint8_t(v);
+              }
+            }
+          }
+        }
+        if (stage && msep[h] <= 4) {
+          const int c1 = ql / MAX_CONN, c2 = ql - c1 * MAX_CONN;
+          L.nb.cs[h][ql] = uint8_t(c1 < nc1 && c2 < nc2v[h] ? conn_sep(a.s, g1, gs, c1, c2) : 6);
         }
       }
       wave_sync();
-      LkbBlock<Real, DERIV>& B1 = L.b[0];
-      LkbBlock<Real, DERIV>& B2 = intra ? L.b[0] : L.b[1];
-      const int np1 = B1.n_pol, no1 = B1.n_occ, np2 = B2.n_pol, no2 = B2.n_occ;
-      const int na = np1 * no2;
-      const int n_pairs = intra ? na : na + np2 * no1;
-      Real e4[4] = {0, 0, 0, 0};
-      for (int p = lane; p < n_pairs; p += WAVE) {
-        const bool dir1 = p < na;  // polar in g1, occluder in g2
-        const int pp = dir1 ? p : p - na;
-        const int nY = dir1 ? no2 : no1;
-        const int i = pp / nY, j = pp - i * nY;
-        LkbBlock<Real, DERIV>& X = dir1 ? B1 : B2;
-        LkbBlock<Real, DERIV>& Y = dir1 ? B2 : B1;
-        int sep;
-        if (intra) {
-          if (i == j) continue;
-          sep = a.path_distance[(int64_t(B1.bt) * a.bt.Amax + B1.loc[i]) * a.bt.Amax + B1.loc[j]];
-        } else {
-          sep = min_sep;
-          if (sep <= 4) {
-            const int a1 = dir1 ? i : j, a2 = dir1 ? j : i;  // slots in g1, g2
-            int best = 5;
-            for (int c1 = 0; c1 < B1.nc; ++c1)
-              for (int c2 = 0; c2 < B2.nc; ++c2) {
-                const int v = B1.pd[c1][a1] + L.cs[c1 * MAX_CONN + c2] + B2.pd[c2][a2];
-                best = v < best ? v : best;
+      // pair tests over the items; passing pairs into the FIFO
+      int ctot = 0, cbeg[LKB_ITEMS];
+#pragma unroll
+      for (int h = 0; h < LKB_ITEMS; ++h) {
+        cbeg[h] = ctot;
+        ctot += cand[h];
+      }
+      if (CENSUS) {
+#pragma unroll
+        for (int h = 0; h < LKB_ITEMS; ++h)
+          if (g2v[h] == g1)
+            cnt[0] += uint64_t(np1) * (no1 - 1);
+          else if (g2v[h] > g1)
+            cnt[0] += uint64_t(cand[h]);
+      }
+      for (int base = 0; base < ctot; base += WAVE) {
+        const int p = base + lane;
+        bool pass = false;
+        uint32_t meta = 0;
+        int32_t oatom = 0, otslot = 0, oblk = 0;
+        if (p < ctot) {
+          int h = 0;
+#pragma unroll
+          for (int q = 1; q < LKB_ITEMS; ++q) h = p >= cbeg[q] ? q : h;
+          const int pp = p - cbeg[h];
+          const int g2 = g2v[h];
+          const bool counted = g2 >= g1;
+          if (g2 == g1) {
+            const int i = pp / no1, j = pp - i * no1;
+            if (i != j) {
+              const int sep = a.path_distance[(int64_t(bt1) * a.bt.Amax + L.row.loc[i]) *
+                                                  a.bt.Amax + L.row.loc[j]];
+              if (sep > 3) {
+                const Real dx = L.row.pos[i][0] - L.row.pos[j][0],
+                           dy = L.row.pos[i][1] - L.row.pos[j][1],
+                           dz = L.row.pos[i][2] - L.row.pos[j][2];
+                pass = dx * dx + dy * dy + dz * dz < g.max_dis2;
+                meta = uint32_t(i) | (1u << 6) | (sep == 4 ? 1u << 7 : 0u) | (1u << 8) |
+                       (uint32_t(j) << 9);
+                oblk = g1;
+              } else if (CENSUS) {
+                cnt[1] += 1;
               }
-            sep = best;
+            }
+          } else {
+            const int nA = np1 * no2v[h];
+            const int dir = pp < nA ? 0 : 1;
+            int i, j;  // i: slot of the row block, j: slot of the neighbor
+            if (dir == 0) {
+              i = pp / no2v[h];
+              j = pp - i * no2v[h];
+            } else {
+              const int q = pp - nA;
+              j = q / no1;
+              i = q - j * no1;
+            }
+            int sep = msep[h];
+            if (sep <= 4) {
+              int best = 5;
+              for (int c1 = 0; c1 < nc1; ++c1)
+                for (int c2 = 0; c2 < nc2v[h]; ++c2) {
+                  const int v = L.row.pd[c1][i] + L.nb.cs[h][c1 * MAX_CONN + c2] +
+                                L.nb.pd[h][c2][j];
+                  best = v < best ? v : best;
+                }
+              sep = best;
+            }
+            if (sep > 3) {
+              const float dx = float(L.row.pos[i][0]) - L.nb.x[h][j];
+              const float dy = float(L.row.pos[i][1]) - L.nb.y[h][j];
+              const float dz = float(L.row.pos[i][2]) - L.nb.z[h][j];
+              // a float pre-test with margin; the exact cutoff is applied in
+              // the evaluation's precision below
+              pass = dx * dx + dy * dy + dz * dz < max_dis2f * 1.0001f;
+              if (pass && !(sizeof(Real) == 4)) {
+                const int64_t at = L.nb.atom[h][j];
+                const Real ex = L.row.pos[i][0] - a.coords[3 * at],
+                           ey = L.row.pos[i][1] - a.coords[3 * at + 1],
+                           ez = L.row.pos[i][2] - a.coords[3 * at + 2];
+                pass = ex * ex + ey * ey + ez * ez < g.max_dis2;
+              } else if (pass) {
+                pass = dx * dx + dy * dy + dz * dz < max_dis2f;
+              }
+              meta = uint32_t(i) | (uint32_t(dir) << 5) | (counted ? 1u << 6 : 0u) |
+                     (sep == 4 ? 1u << 7 : 0u);
+              oatom = L.nb.atom[h][j];
+              otslot = bt2v[h] * a.Omax + j;
+              oblk = g2;
+            } else if (CENSUS && counted) {
+              cnt[1] += 1;
+            }
           }
         }
-        if (sep <= 3) continue;
-        const Real dx = X.x[i] - Y.x[j], dy = X.y[i] - Y.y[j], dz = X.z[i] - Y.z[j];
-        const Real d2 = dx * dx + dy * dy + dz * dz;
-        if (!(d2 < g.max_dis2)) continue;
-        lkb_pair<Real, DERIV>(X, i, Y, j, sep, dx, dy, dz, d2, g, water_dist, w4, e4);
-      }
-      if (BP && !NOE) {
-#pragma unroll
-        for (int st = 0; st < 4; ++st) {
-          const double v = wave_sum(double(e4[st]));
-          if (lane == 0) a.energies_bp[int64_t(st) * plane + bpi] = Real(v);
+        const uint64_t mask = __ballot(pass);
+        if (pass) {
+          const int e = (tail + lanes_below(mask)) & (LKB_FIFO - 1);
+          L.q.atom[e] = oatom;
+          L.q.tslot[e] = otslot;
+          L.q.blk[e] = oblk;
+          L.q.meta[e] = meta;
         }
-      } else if (!NOE) {
-#pragma unroll
-        for (int st = 0; st < 4; ++st) erow[st] += double(e4[st]);
+        tail += __popcll(mask);
+        if (tail - head >= WAVE) {
+          wave_sync();
+          lkb_phase_b<Real, MODE>(L, a, g1, head, WAVE, lane, g, water_dist, w4row, e4, cnt);
+          head += WAVE;
+        }
       }
-      if (!intra) {
-        wave_sync();
-        lkb_flush<Real, DERIV>(L.b[1], a, lane);
-      }
+      wave_sync();
+    }
+    while (tail > head) {
+      const int n = tail - head < WAVE ? tail - head : WAVE;
+      wave_sync();
+      lkb_phase_b<Real, MODE>(L, a, g1, head, n, lane, g, water_dist, w4row, e4, cnt);
+      head += n;
     }
     wave_sync();
-    lkb_flush<Real, DERIV>(L.b[0], a, lane);
-    if (!BP && !NOE) {
+    // the row owns its block's atoms: plain read-modify-write, no atomics
+    if (DERIV && lane < no1) {
+      const int64_t at = L.row.atom[lane];
+#pragma unroll
+      for (int k = 0; k < 3; ++k) {
+        const auto v = L.row.acc[k][lane];
+        if (v != 0) a.dxyz[3 * at + k] += LdsAcc<Real>::dec(v);
+      }
+      if (lane < np1)
+#pragma unroll
+        for (int k = 0; k < 12; ++k) {
+          const auto v = L.row.accw[k][lane];
+          if (v != 0) a.dwat[12 * at + k] += LdsAcc<Real>::dec(v);
+        }
+    }
+    if (!BP && !NOE && !CENSUS) {
 #pragma unroll
       for (int st = 0; st < 4; ++st) {
-        const double v = wave_sum(erow[st]);
+        const double v = wave_sum(double(e4[st]));
         if (lane == 0 && v != 0.0) atomicAdd(&a.energies64[int64_t(st) * P + pose], v);
       }
     }
     wave_sync();
+  }
+  if (CENSUS) {
+    // cnt[0] is wave-uniform (added by every lane); the others per lane
+    const unsigned long long c1 = wave_sum(cnt[1]), c2 = wave_sum(cnt[2]),
+                             c3 = wave_sum(cnt[3]);
+    if (lane == 0) {
+      atomicAdd(&a.census[0], (unsigned long long)cnt[0]);
+      atomicAdd(&a.census[1], c1);
+      atomicAdd(&a.census[2], c2);
+      atomicAdd(&a.census[3], c3);
+    }
   }
 }
 

@@ -132,3 +132,30 @@ def algorithmic_bytes(n_atoms: int, n_blocks: int, n_poses: int, n_items: int,
         "neighbors": coords + meta + 2 * spheres + items,
         "score": coords + items + 16 * n_items + 3 * real_bytes * n_atoms + 3 * 8 * n_poses,
     }
+
+
+# lk_ball (tmol/score/lk_ball/potentials/lk_ball.hh:42-494), per enumerated
+# (polar atom, occluder) pair of a neighbor block pair, same convention,
+# energy + weighted gradient of the four score types:
+#   separation <= 3                                                   0
+#   beyond max_dis, or a polar atom without waters                    8 (distance)
+#   evaluated   distance + sqrt 9; isotropic LK one direction with
+#               derivative 14; lk_fraction: 4 waters x (3 sub, 3 mul,
+#               2 add, - d2_low, exp, sum = 11) = 44, -log, ramp
+#               polynomial 7; its derivative 4 x 10 + 6 = 46; the four
+#               energies 4; weighted gradient combination 12; I, J
+#               gradient 6                                           142
+#   + bridge    (occluder a donor or acceptor with waters) 16 water
+#               pairs x 11 = 176, -log, overlap polynomial 7, base
+#               length fraction 8; derivative 16 x 11 + 10          +377
+# Energy only: evaluated 9 + 13 + 51 + 4 = 77, bridge +191.
+LKB_CENSUS_FIELDS = ["candidates", "cp0", "evaluated", "bridge"]
+LKB_FLOPS_ENERGY_GRAD = {"far": DIST, "evaluated": 142, "bridge": 377}
+LKB_FLOPS_ENERGY = {"far": DIST, "evaluated": 77, "bridge": 191}
+
+
+def lkb_algorithmic_flops(census: dict, gradient: bool = True) -> int:
+    t = LKB_FLOPS_ENERGY_GRAD if gradient else LKB_FLOPS_ENERGY
+    far = census["candidates"] - census["cp0"] - census["evaluated"]
+    return int(far * t["far"] + census["evaluated"] * t["evaluated"]
+               + census["bridge"] * t["bridge"])

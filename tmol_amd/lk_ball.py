@@ -39,12 +39,14 @@ from .score.lk_ball_energy_term import RECIPE_WIDTH, water_recipes
 _VP = ctypes.c_void_p
 _P32 = ctypes.POINTER(ctypes.c_int32)
 MAX_OCC = 32
+MAX_POLAR = 16
 
 
 class CLKBallTables(ctypes.Structure):
     _fields_ = [
         ("max_n_occ", ctypes.c_int32),
         ("max_n_recipes", ctypes.c_int32),
+        ("max_n_polar", ctypes.c_int32),
         ("n_occ", _P32),
         ("n_polar", _P32),
         ("occ_atoms", _P32),
@@ -78,6 +80,11 @@ def _lkb_lib():
             L.tmol_lkball_score.argtypes = [
                 SP, BP, KP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64,
                 ctypes.c_int32, _VP, _VP, _VP, _VP, ctypes.c_int32, _VP]
+            L.tmol_lkball_workspace_bytes.argtypes = [SP, BP, KP, ctypes.c_int64, ctypes.c_int32]
+            L.tmol_lkball_workspace_bytes.restype = ctypes.c_int64
+            L.tmol_lkball_census.argtypes = [
+                SP, BP, KP, _VP, _VP, ctypes.c_int32, _VP, ctypes.c_int64, ctypes.c_int64, _VP,
+                _VP]
             _bound = True
     return L
 
@@ -132,6 +139,10 @@ class LKBallTables:
         self.cbt = b
         k = CLKBallTables()
         k.max_n_occ, k.max_n_recipes = Omax, int(Rmax)
+        k.max_n_polar = int(numpy.asarray(n_polar).max()) if T else 0
+        if k.max_n_polar > MAX_POLAR:
+            raise ops.LJLKLibraryError("lk_ball: more than %d polar atoms in a block type"
+                                       % MAX_POLAR)
         for name in ("n_occ", "n_polar", "occ_atoms", "n_recipes", "recipes", "path_distance"):
             setattr(k, name, ctypes.cast(self.t[name].data_ptr(), _P32))
         for name in ("occ_params", "global_params", "watergen_params", "water_tors"):
@@ -159,6 +170,15 @@ class LKBallTables:
         return cls(pbt.n_atoms, pbt.n_conn, pbt.conn_atom, n_occ, n_pol, occ, prm,
                    [b.recipes for b in bts], pd, term.global_params, term.watergen_params,
                    tuple(_np(t) for t in term.water_tors()), dtype, device)
+
+
+def lkb_workspace(cstack, tables, capacity, dcode, device):
+    """A workspace for lk_ball's neighbor pass and score."""
+    n = _lkb_lib().tmol_lkball_workspace_bytes(ctypes.byref(cstack), ctypes.byref(tables.cbt),
+                                               ctypes.byref(tables.ck), capacity, dcode)
+    if n < 0:
+        raise ops.LJLKLibraryError("lk_ball workspace size query failed")
+    return ops.Workspace(cstack, tables.cbt, capacity, dcode, device, nbytes=int(n))
 
 
 # ---------------------------------------------------------------- op-argument conversion
@@ -369,8 +389,8 @@ class _ScoreCall:
         self.block_pair = bool(block_pair)
         pbs = torch.arange(0, P + 1, dtype=torch.int32) * B
         self.tiles = ops.neighbor_tiles(pbs, dev)
-        self.ws = ops.Workspace(self.cstack, self.tables.cbt,
-                                ops.padded_row_capacity(self.P, self.B), self.dcode, dev)
+        self.ws = lkb_workspace(self.cstack, self.tables, ops.padded_row_capacity(self.P, self.B),
+                                self.dcode, dev)
 
     def refs(self):
         return ctypes.byref(self.cstack), ctypes.byref(self.tables.cbt), ctypes.byref(
@@ -462,7 +482,7 @@ class LKBallRaggedScorer:
         self.tiles = ops.neighbor_tiles(stack.pose_block_start, dev)
         max_nb = int(stack.n_blocks_per_pose.max())
         self.capacity = int(capacity or max_nb)
-        self.ws = ops.Workspace(self.cstack, self.tables.cbt, self.capacity, self.dcode, dev)
+        self.ws = lkb_workspace(self.cstack, self.tables, self.capacity, self.dcode, dev)
         self.waters = torch.empty((self.NA, 4, 3), dtype=dtype, device=dev)
         self.energies = torch.zeros((4, self.P), dtype=dtype, device=dev)
         self.dxyz = torch.zeros((self.NA, 3), dtype=dtype, device=dev)
@@ -498,6 +518,16 @@ class LKBallRaggedScorer:
         ops._check(_lkb_lib().tmol_lkball_waters_backward(
             *self.refs(), ops._p(coords), self.dcode, ops._p(dwat), ops._p(dxyz),
             ops._stream(self.device)), "lk_ball waters backward")
+
+    def census(self, coords) -> dict:
+        """Pair counts of the current work list (after neighbors and
+        gen_waters): candidates, separation <= 3, evaluated, with bridge."""
+        out = torch.zeros(4, dtype=torch.int64, device=self.device)
+        ops._check(_lkb_lib().tmol_lkball_census(
+            *self.refs(), ops._p(coords), ops._p(self.waters), self.dcode, ops._p(self.ws.buf),
+            self.ws.nbytes, self.ws.capacity, ops._p(out), ops._stream(self.device)),
+            "lk_ball census")
+        return dict(zip(("candidates", "cp0", "evaluated", "bridge"), out.cpu().tolist()))
 
     def check_capacity(self):
         n, overflow = self.ws.counters(self.device)

@@ -252,10 +252,11 @@ class Workspace:
     """Caller-owned device workspace; ``capacity`` is the work list's row
     capacity (most neighbor blocks g2 >= g1 of any block g1)."""
 
-    def __init__(self, cstack, cbt, capacity, dtype_code, device, flags=0):
+    def __init__(self, cstack, cbt, capacity, dtype_code, device, flags=0, nbytes=None):
         L = lib()
-        nbytes = L.tmol_ljlk_workspace_bytes_flags(ctypes.byref(cstack), ctypes.byref(cbt),
-                                                   capacity, dtype_code, flags)
+        if nbytes is None:
+            nbytes = L.tmol_ljlk_workspace_bytes_flags(ctypes.byref(cstack), ctypes.byref(cbt),
+                                                       capacity, dtype_code, flags)
         if nbytes < 0:
             raise LJLKLibraryError("workspace size query failed")
         self.buf = torch.empty(int(nbytes), dtype=torch.uint8, device=device)
