@@ -660,9 +660,18 @@ __device__ __forceinline__ void lkb_phase_b(LkbLDS<Real, lkb_deriv_mode<MODE>()>
   }
 }
 
-// 4 waves per SIMD: the gradient mode otherwise takes ~200 VGPRs (2 waves)
+// waves per SIMD the kernel is compiled for: 4 with gradients (which
+// otherwise take ~200 VGPRs, 2 waves), 5 for energies (measured on the
+// 512-pose batch: 4.82 vs 5.33 ms at 5 with gradients; 3.11 vs 3.33 ms at 4
+// for energies)
+template <int MODE>
+constexpr int lkb_waves_per_simd() {
+  return (MODE & M_WGRAD) ? 4 : 5;
+}
+
 template <typename Real, int MODE>
-__global__ __launch_bounds__(256, 4) void lkb_gather_kernel(LkbArgs<Real> a) {
+__global__ __launch_bounds__(256, lkb_waves_per_simd<MODE>()) void lkb_gather_kernel(
+    LkbArgs<Real> a) {
   constexpr bool DERIV = lkb_deriv_mode<MODE>();
   constexpr bool BP = (MODE & M_BP) != 0;
   constexpr bool NOE = (MODE & M_NOE) != 0;
