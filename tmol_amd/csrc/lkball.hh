@@ -333,6 +333,7 @@ struct LkbAtom {
   Real r, K, il2, vol;  // lj_radius, lk_dgfree / (2 pi^1.5 lk_lambda), 1 / lk_lambda^2, lk_volume
   int flg;              // F_DONOR ... F_CARBON_LK
   Real w[12];           // waters, NaN where absent
+  const Real* wsrc;     // where w was read from (LDS or global memory; null: none)
 };
 
 template <typename Real>
@@ -367,8 +368,16 @@ struct LkbRow {
   T accw[DERIV ? 12 : 1][DERIV ? LKB_PMAX : 1];  // dE/dwaters (polar slots)
 };
 
+// a neighbor-row item: the other block, its type, first atom, occluder and
+// occluder, polar and connection counts, minimum separation from the row's
+// block (6: beyond 4) and its candidate pairs
+struct LkbItem {
+  int g2, bt2, off2, no2, np2, nc2, msep, cand;
+};
+
 // LKB_ITEMS neighbor blocks staged for the pair tests (16 lanes each)
 struct LkbNbr {
+  LkbItem it[LKB_ITEMS];
   float x[LKB_ITEMS][LKB_OMAX], y[LKB_ITEMS][LKB_OMAX], z[LKB_ITEMS][LKB_OMAX];
   int32_t atom[LKB_ITEMS][LKB_OMAX];
   uint8_t pd[LKB_ITEMS][MAX_CONN][LKB_OMAX];
@@ -530,14 +539,17 @@ __device__ __forceinline__ int lkb_eval(const LkbAtom<Real>& I, const LkbAtom<Re
   }
   if (bridge && cB != Real(0) && dof != Real(0) && bsum != Real(0)) {
     const Real sb = cB * afrac * dof * fast_rcp(bsum);
+    // a rolled loop over I's waters, read again from their source: indexing
+    // I.w by the loop counter would place it in scratch memory
 #pragma unroll 1
     for (int wi = 0; wi < 4; ++wi) {
+      const Real iwx = I.wsrc[3 * wi], iwy = I.wsrc[3 * wi + 1], iwz = I.wsrc[3 * wi + 2];
       Real gx = 0, gy = 0, gz = 0;
 #pragma unroll
       for (int wj = 0; wj < 4; ++wj) {
-        const Real ex = I.w[3 * wi] - J.w[3 * wj];
-        const Real ey = I.w[3 * wi + 1] - J.w[3 * wj + 1];
-        const Real ez = I.w[3 * wi + 2] - J.w[3 * wj + 2];
+        const Real ex = iwx - J.w[3 * wj];
+        const Real ey = iwy - J.w[3 * wj + 1];
+        const Real ez = iwz - J.w[3 * wj + 2];
         const Real q = (ex * ex + ey * ey + ez * ez) - gap;
         if (!(q == q)) continue;
         const Real c = Real(2) * fast_exp(-q) * sb;
@@ -573,6 +585,7 @@ __device__ __forceinline__ void lkb_load_row_atom(const LkbRow<Real, DERIV>& R, 
   A.flg = int(R.prm[i][3]);
 #pragma unroll
   for (int k = 0; k < 12; ++k) A.w[k] = i < LKB_PMAX ? R.w[i][k] : Real(__builtin_nan(""));
+  A.wsrc = i < LKB_PMAX ? &R.w[i][0] : nullptr;
 }
 
 template <typename Real>
@@ -587,6 +600,7 @@ __device__ __forceinline__ void lkb_load_global_atom(const LkbArgs<Real>& a, int
   const Real nan = __builtin_nan("");
 #pragma unroll
   for (int k = 0; k < 12; ++k) A.w[k] = want ? a.waters[12 * at + k] : Real(nan);
+  A.wsrc = want ? a.waters + 12 * at : nullptr;
 }
 
 // evaluate n (<= 64) entries of the FIFO starting at head (mod LKB_FIFO)
@@ -742,24 +756,14 @@ __global__ __launch_bounds__(256, lkb_waves_per_simd<MODE>()) void lkb_gather_ke
       for (int st = 0; st < 4; ++st) w4row[st] = a.weights[int64_t(st) * P + pose];
     Real e4[4] = {0, 0, 0, 0};
     int head = 0, tail = 0;  // FIFO positions (mod LKB_FIFO), wave-uniform
-    // the next LKB_ITEMS items, loaded one pass ahead
-    int nx[LKB_ITEMS];
-#pragma unroll
-    for (int h = 0; h < LKB_ITEMS; ++h) nx[h] = h < n_items ? items[h] : -1;
+    // item k0 + lane of the next pass in lanes < LKB_ITEMS, loaded one pass ahead
+    int nxt = lane < LKB_ITEMS && lane < n_items ? items[lane] : -1;
     const int quarter = lane >> 4, ql = lane & 15;
     for (int k0 = 0; k0 < n_items; k0 += LKB_ITEMS) {
-      // the items' block data, in every lane
-      int g2v[LKB_ITEMS], bt2v[LKB_ITEMS], no2v[LKB_ITEMS], np2v[LKB_ITEMS], nc2v[LKB_ITEMS],
-          msep[LKB_ITEMS], cand[LKB_ITEMS];
-      int4 inf2[LKB_ITEMS];
-#pragma unroll
-      for (int h = 0; h < LKB_ITEMS; ++h) {
-        g2v[h] = nx[h];
-        inf2[h] = g2v[h] >= 0 ? a.info[g2v[h]] : make_int4(-1, 0, 0, -1);
-      }
+      const int my = nxt;
       // this quarter's neighbor slots (ql, ql + 16), loaded without waiting
-      // for the block's count
-      const int gs = g2v[quarter];
+      // for the block's record
+      const int gs = __shfl(my, quarter, WAVE);
       const bool stage = gs >= 0 && gs != g1;
       float4 sp[2];
       int32_t sat[2];
@@ -773,75 +777,85 @@ __global__ __launch_bounds__(256, lkb_waves_per_simd<MODE>()) void lkb_gather_ke
           sat[r] = a.oatom[int64_t(gs) * a.Omax + sl];
         }
       }
-#pragma unroll
-      for (int h = 0; h < LKB_ITEMS; ++h)
-        nx[h] = k0 + LKB_ITEMS + h < n_items ? items[k0 + LKB_ITEMS + h] : -1;
-#pragma unroll
-      for (int h = 0; h < LKB_ITEMS; ++h) {
-        bt2v[h] = inf2[h].x;
-        no2v[h] = inf2[h].y & 255;
-        np2v[h] = (inf2[h].y >> 8) & 255;
-        nc2v[h] = inf2[h].y >> 16;
-        msep[h] = 6;
-        if (g2v[h] == g1) {
-          cand[h] = np1 * no1;
-        } else if (g2v[h] >= 0) {
+      nxt = lane < LKB_ITEMS && k0 + LKB_ITEMS + lane < n_items ? items[k0 + LKB_ITEMS + lane]
+                                                              : -1;
+      // the items' records, one per lane (lanes < LKB_ITEMS), into LDS: the
+      // pair tests below read them by a per-lane item index, which on private
+      // arrays would place them in scratch memory
+      if (lane < LKB_ITEMS) {
+        const int4 inf2 = my >= 0 ? a.info[my] : make_int4(-1, 0, 0, -1);
+        LkbItem it;
+        it.g2 = my;
+        it.bt2 = inf2.x;
+        it.off2 = inf2.z;
+        it.no2 = inf2.y & 255;
+        it.nc2 = inf2.y >> 16;
+        it.np2 = (inf2.y >> 8) & 255;
+        it.msep = 6;
+        if (my == g1) {
+          it.cand = np1 * no1;
+        } else if (my >= 0) {
           // ragged: the last bonded partner of the lower block rules out most
           // pairs without a search
-          const int lo = g1 < g2v[h] ? g1 : g2v[h], hi = g1 < g2v[h] ? g2v[h] : g1;
-          const int bmax = lo == g1 ? inf1.w : inf2[h].w;
+          const int lo = g1 < my ? g1 : my, hi = g1 < my ? my : g1;
+          const int bmax = lo == g1 ? inf1.w : inf2.w;
           if (a.s.layout == TMOL_LJLK_PADDED || hi <= bmax)
-            msep[h] = min_block_sep(a.s, g1, g2v[h], nc1, nc2v[h]);
-          cand[h] = np1 * no2v[h] + np2v[h] * no1;
+            it.msep = min_block_sep(a.s, g1, my, nc1, it.nc2);
+          it.cand = np1 * it.no2 + it.np2 * no1;
         } else {
-          cand[h] = 0;
+          it.cand = 0;
         }
+        L.nb.it[lane] = it;
+      }
+      wave_sync();
+      int ctot = 0, cbeg[LKB_ITEMS];
+#pragma unroll
+      for (int h = 0; h < LKB_ITEMS; ++h) {
+        cbeg[h] = ctot;
+        ctot += L.nb.it[h].cand;
       }
       // stage the neighbor blocks, one per quarter wave
       {
         const int h = quarter;
+        const LkbItem it = L.nb.it[h];
 #pragma unroll
         for (int r = 0; r < 2; ++r) {
           const int sl = ql + 16 * r;
-          if (stage && sl < no2v[h]) {
+          if (stage && sl < it.no2) {
             L.nb.x[h][sl] = sp[r].x;
             L.nb.y[h][sl] = sp[r].y;
             L.nb.z[h][sl] = sp[r].z;
             L.nb.atom[h][sl] = sat[r];
-            if (msep[h] <= 4) {
-              const int la = int(sat[r] - inf2[h].z);
+            if (it.msep <= 4) {
+              const int la = sat[r] - it.off2;
 #pragma unroll
               for (int c = 0; c < MAX_CONN; ++c) {
                 int v = 6;
-                if (c < nc2v[h]) {
-                  const int ca = a.bt.conn_atoms[bt2v[h] * a.bt.C + c];
-                  v = a.path_distance[(int64_t(bt2v[h]) * a.bt.Amax + ca) * a.bt.Amax + la];
+                if (c < it.nc2) {
+                  const int ca = a.bt.conn_atoms[it.bt2 * a.bt.C + c];
+                  v = a.path_distance[(int64_t(it.bt2) * a.bt.Amax + ca) * a.bt.Amax + la];
                 }
                 L.nb.pd[h][c][sl] = uint8_t(v);
               }
             }
           }
         }
-        if (stage && msep[h] <= 4) {
+        if (stage && it.msep <= 4) {
           const int c1 = ql / MAX_CONN, c2 = ql - c1 * MAX_CONN;
-          L.nb.cs[h][ql] = uint8_t(c1 < nc1 && c2 < nc2v[h] ? conn_sep(a.s, g1, gs, c1, c2) : 6);
+          L.nb.cs[h][ql] = uint8_t(c1 < nc1 && c2 < it.nc2 ? conn_sep(a.s, g1, gs, c1, c2) : 6);
         }
       }
       wave_sync();
       // pair tests over the items; passing pairs into the FIFO
-      int ctot = 0, cbeg[LKB_ITEMS];
-#pragma unroll
-      for (int h = 0; h < LKB_ITEMS; ++h) {
-        cbeg[h] = ctot;
-        ctot += cand[h];
-      }
       if (CENSUS) {
 #pragma unroll
-        for (int h = 0; h < LKB_ITEMS; ++h)
-          if (g2v[h] == g1)
+        for (int h = 0; h < LKB_ITEMS; ++h) {
+          const int g2 = L.nb.it[h].g2;
+          if (g2 == g1)
             cnt[0] += uint64_t(np1) * (no1 - 1);
-          else if (g2v[h] > g1)
-            cnt[0] += uint64_t(cand[h]);
+          else if (g2 > g1)
+            cnt[0] += uint64_t(L.nb.it[h].cand);
+        }
       }
       for (int base = 0; base < ctot; base += WAVE) {
         const int p = base + lane;
@@ -849,11 +863,15 @@ __global__ __launch_bounds__(256, lkb_waves_per_simd<MODE>()) void lkb_gather_ke
         uint32_t meta = 0;
         int32_t oatom = 0, otslot = 0, oblk = 0;
         if (p < ctot) {
-          int h = 0;
+          int h = 0, pp = p;
 #pragma unroll
-          for (int q = 1; q < LKB_ITEMS; ++q) h = p >= cbeg[q] ? q : h;
-          const int pp = p - cbeg[h];
-          const int g2 = g2v[h];
+          for (int q = 1; q < LKB_ITEMS; ++q)
+            if (p >= cbeg[q]) {
+              h = q;
+              pp = p - cbeg[q];
+            }
+          const LkbItem& it = L.nb.it[h];
+          const int g2 = it.g2;
           const bool counted = g2 >= g1;
           if (g2 == g1) {
             const int i = pp / no1, j = pp - i * no1;
@@ -873,22 +891,23 @@ __global__ __launch_bounds__(256, lkb_waves_per_simd<MODE>()) void lkb_gather_ke
               }
             }
           } else {
-            const int nA = np1 * no2v[h];
+            const int no2 = it.no2, nc2 = it.nc2;
+            const int nA = np1 * no2;
             const int dir = pp < nA ? 0 : 1;
             int i, j;  // i: slot of the row block, j: slot of the neighbor
             if (dir == 0) {
-              i = pp / no2v[h];
-              j = pp - i * no2v[h];
+              i = pp / no2;
+              j = pp - i * no2;
             } else {
               const int q = pp - nA;
               j = q / no1;
               i = q - j * no1;
             }
-            int sep = msep[h];
+            int sep = it.msep;
             if (sep <= 4) {
               int best = 5;
               for (int c1 = 0; c1 < nc1; ++c1)
-                for (int c2 = 0; c2 < nc2v[h]; ++c2) {
+                for (int c2 = 0; c2 < nc2; ++c2) {
                   const int v = L.row.pd[c1][i] + L.nb.cs[h][c1 * MAX_CONN + c2] +
                                 L.nb.pd[h][c2][j];
                   best = v < best ? v : best;
@@ -914,7 +933,7 @@ __global__ __launch_bounds__(256, lkb_waves_per_simd<MODE>()) void lkb_gather_ke
               meta = uint32_t(i) | (uint32_t(dir) << 5) | (counted ? 1u << 6 : 0u) |
                      (sep == 4 ? 1u << 7 : 0u);
               oatom = L.nb.atom[h][j];
-              otslot = bt2v[h] * a.Omax + j;
+              otslot = it.bt2 * a.Omax + j;
               oblk = g2;
             } else if (CENSUS && counted) {
               cnt[1] += 1;

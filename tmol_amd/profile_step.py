@@ -3,10 +3,11 @@ the scoring kernel a fixed number of times (no autotuning, no CPU work), so
 that ``rocprofv3`` sees a short, repeatable kernel sequence.
 
     python -m tmol_amd.profile_step --workload ragged_512x50_1200 --reps 5 \
-        [--mode wgrad|dvdx|energy] [--general] [--wave-groups N] [--term ljlk|elec]
+        [--mode wgrad|dvdx|energy] [--general] [--wave-groups N] [--term ljlk|elec|lk_ball]
 
 ``--term elec`` runs the elec term's neighbor pass and row kernel instead
-(weights 1 per pose in wgrad mode).
+(weights 1 per pose in wgrad mode); ``--term lk_ball`` lk_ball's neighbor
+pass, waters and gather kernel (``--mode wgrad|energy``).
 
 ``--copy-bytes N`` instead copies an N-byte device buffer ``reps`` times with
 torch's copy kernel: the reference point that scales the HBM counters
@@ -103,7 +104,7 @@ def main(argv=None):
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--copy-bytes", type=int, default=0)
-    ap.add_argument("--term", default="ljlk", choices=["ljlk", "elec"])
+    ap.add_argument("--term", default="ljlk", choices=["ljlk", "elec", "lk_ball"])
     args = ap.parse_args(argv)
 
     if args.summarize:
@@ -135,6 +136,30 @@ def main(argv=None):
     dtype = torch.float32 if args.dtype == "f32" else torch.float64
     poses = workload_poses(args.workload, args.n_poses)
     stack = RaggedPoseStack(poses, dtype=dtype)
+    if args.term == "lk_ball":
+        from .lk_ball import LKBallRaggedScorer
+        from .score.lk_ball_energy_term import LKBallEnergyTerm
+
+        sc = LKBallRaggedScorer(stack, LKBallEnergyTerm(), device=dev, dtype=dtype)
+        x = stack.coords.to(dev)
+        w = sc._weights(None)
+        flags = {"wgrad": ops.WGRAD, "energy": 0}[args.mode]
+        sc.neighbors(x)
+        n_items = sc.check_capacity()
+        sc.gen_waters(x)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(args.reps):
+            t0 = time.perf_counter()
+            sc.neighbors(x)
+            sc.gen_waters(x)
+            sc.score(x, flags, weights=w, energies=sc.energies, dxyz=sc.dxyz, dwat=sc.dwat)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        print(json.dumps({"workload": args.workload, "term": args.term, "mode": args.mode,
+                          "block_pairs": n_items, "step_ms": [1e3 * t for t in times]}),
+              flush=True)
+        return 0
     if args.term == "elec":
         from .score.elec_energy_term import ElecEnergyTerm
         from .scorer import ElecRaggedScorer
