@@ -66,3 +66,16 @@ g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
   -o "$OUT4.tmp"
 mv "$OUT4.tmp" "$OUT4"
 echo "built $OUT4"
+# the reference's hbond ops (tmol_hbond::gen_hbond_bases, hbond_pose_scores,
+# hbond_rotamer_scores)
+OUT5="$HERE/_ref/tmol_hbond_ref.so"
+g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
+  -DTORCH_EXTENSION_NAME=tmol_hbond_ref \
+  -I"$R" -I"$R/tmol/extern" $TI -I"$PYI" \
+  "$R/tmol/score/hbond/potentials/compiled.ops.cpp" \
+  "$R/tmol/score/hbond/potentials/hbond_pose_score.cpu.cpp" \
+  "$R/tmol/score/hbond/potentials/gen_hbond_bases.cpu.cpp" \
+  -L"$TL" -ltorch -ltorch_cpu -lc10 -Wl,-rpath,"$TL" \
+  -o "$OUT5.tmp"
+mv "$OUT5.tmp" "$OUT5"
+echo "built $OUT5"
