@@ -1,6 +1,6 @@
 /* CPU oracle for the LJ+LK pose-score op -- TEST INFRASTRUCTURE ONLY.
- * Float and double instantiations of ljlk_oracle_impl.h, elec_oracle_impl.h
- * and lkball_oracle_impl.h, plus the integer
+ * Float and double instantiations of ljlk_oracle_impl.h, elec_oracle_impl.h,
+ * lkball_oracle_impl.h and hbond_oracle_impl.h, plus the integer
  * count-pair dump.  Built by oracle/Makefile with -O2 -ffp-contract=off
  * (no fused multiply-add, like the reference's x86-64 CPU build). */
 #include <math.h>
@@ -41,9 +41,14 @@ static float elec_eps_float(float dist, float D, float D0, float S) {
 #define LOG logf
 #define COS cosf
 #define SIN sinf
+#define ATAN2 atan2f
+#define ACOS acosf
+#define FMIN fminf
+#define FMAX fmaxf
 #include "ljlk_oracle_impl.h"
 #include "elec_oracle_impl.h"
 #include "lkball_oracle_impl.h"
+#include "hbond_oracle_impl.h"
 #undef REAL
 #undef RSUF
 #undef SQRT
@@ -52,6 +57,10 @@ static float elec_eps_float(float dist, float D, float D0, float S) {
 #undef LOG
 #undef COS
 #undef SIN
+#undef ATAN2
+#undef ACOS
+#undef FMIN
+#undef FMAX
 
 #define REAL double
 #define RSUF _d
@@ -61,9 +70,14 @@ static float elec_eps_float(float dist, float D, float D0, float S) {
 #define LOG log
 #define COS cos
 #define SIN sin
+#define ATAN2 atan2
+#define ACOS acos
+#define FMIN fmin
+#define FMAX fmax
 #include "ljlk_oracle_impl.h"
 #include "elec_oracle_impl.h"
 #include "lkball_oracle_impl.h"
+#include "hbond_oracle_impl.h"
 #undef REAL
 #undef RSUF
 #undef SQRT
@@ -72,6 +86,10 @@ static float elec_eps_float(float dist, float D, float D0, float S) {
 #undef LOG
 #undef COS
 #undef SIN
+#undef ATAN2
+#undef ACOS
+#undef FMIN
+#undef FMAX
 
 /* integer count-pair separation for one atom pair of a block pair
  * (b1 != b2: interres rule; b1 == b2: intra-block path distance) */

@@ -111,6 +111,16 @@ def lib():
             f.restype = ct
             getattr(L, "oracle_lkb_build_water" + suf).argtypes = [
                 ctypes.c_int, PR, PR, PR, ct, ct, ct, PR, PR, PR, PR, PR]
+        for suf, ct in (("_f", ctypes.c_float), ("_d", ctypes.c_double)):
+            PR = ctypes.POINTER(ct)
+            PD = ctypes.POINTER(ctypes.c_double)
+            HS = ctypes.POINTER(OracleHBStack)
+            getattr(L, "oracle_hb_bases" + suf).argtypes = [HS, PR, ctypes.c_int64, PR, _P]
+            getattr(L, "oracle_hb_score_pose" + suf).argtypes = [
+                HS, PR, ctypes.c_int64, PR, _P, PR, PD, PR, _P, ctypes.c_int, PR, PR, PR]
+            f = getattr(L, "oracle_hb_score" + suf)
+            f.argtypes = [PR, PR, PR, PR, PR, PR, PD, PR, PR]
+            f.restype = ct
         L.oracle_rot_dispatch.argtypes = [ctypes.c_int, ctypes.c_int, _P, _P, _P, _P,
                                           ctypes.c_int64]
         L.oracle_rot_dispatch.restype = ctypes.c_int64
@@ -849,3 +859,159 @@ def oracle_lkb_bridge(I, J, WI, WJ, water_dist, deriv=False, dtype=numpy.float64
         _ptr(I, ct), _ptr(J, ct), _ptr(WI, ct), _ptr(WJ, ct), ct(water_dist),
         *[_ptr(o, ct) if deriv else None for o in outs])
     return (v, *outs) if deriv else v
+
+
+# ---------------------------------------------------------------- hbond
+
+class OracleHBStack(ctypes.Structure):
+    _fields_ = [("base", OracleLKBStack), ("tile_don_type", _P), ("tile_acc_type", _P),
+                ("n_don_types", ctypes.c_int32), ("n_acc_types", ctypes.c_int32)]
+
+
+class _HBStack:
+    """The hbond oracle's view of the reference ops' arguments: the 20 of
+    gen_hbond_bases (hbond_energy_term.py:76-97) and, for scoring, the 37 of
+    hbond_pose_scores (:98-132)."""
+
+    def __init__(self, bargs, sargs=None):
+        (coords, rco, frfb, _frbt, _bifr, _pifr, btifr, irc, bt_n_atoms, bt_n_conn,
+         bt_conn_atoms, _nab, all_bonds, ranges, t_ndonH, t_nacc, t_donH, t_acc, t_hyb,
+         is_h) = bargs
+        P, B = frfb.shape
+        zero_t = numpy.zeros(1, dtype=numpy.int32)
+        a = {
+            "rot_coord_offset": _np_i32(rco), "block_type": _np_i32(btifr), "irc": _np_i32(irc),
+            "bt_n_atoms": _np_i32(bt_n_atoms), "bt_n_conn": _np_i32(bt_n_conn),
+            "bt_conn_atoms": _np_i32(bt_conn_atoms), "bt_all_bonds": _np_i32(all_bonds),
+            "bt_atom_all_bond_ranges": _np_i32(ranges), "tile_n_donH": _np_i32(t_ndonH),
+            "tile_n_acc": _np_i32(t_nacc), "tile_donH_inds": _np_i32(t_donH),
+            "tile_acc_inds": _np_i32(t_acc), "tile_acc_hyb": _np_i32(t_hyb),
+            "is_hydrogen": _np_i32(is_h),
+        }
+        hb = {}
+        self.block_pair = False
+        if sargs is not None:
+            (_c, _rco, _pia, _frfb, _frbt, _bifr, _pifr, _btifr, _nrp, _rop, _nrb, _rob, _mx,
+             _irc, mbs, ibs, _bna, _bnc, _bca, _nab2, _ab2, _rng2, pd, _tnd, _tna, _tdi, _tai,
+             t_dtype, t_atype, _thyb, _ish, pair_params, pair_polys, gp, derived, dinds,
+             block_pair) = sargs
+            a.update({"min_bondsep": _np_i32(mbs), "inter_block_bondsep": _np_i32(ibs),
+                      "path_distance": _np_i32(pd)})
+            hb = {"tile_don_type": _np_i32(t_dtype), "tile_acc_type": _np_i32(t_atype)}
+            self.pair_params, self.pair_polys, self.gp = pair_params, pair_polys, gp
+            self.derived, self.dinds = derived, dinds
+            self.block_pair = bool(block_pair)
+            self.n_types = tuple(pair_params.shape[:2])
+        self.arrays = dict(a, **hb)
+        st = OracleHBStack()
+        b = st.base
+        b.n_poses, b.max_n_blocks = P, B
+        b.n_block_types = bt_n_atoms.shape[0]
+        b.max_n_block_atoms = is_h.shape[1]
+        b.max_n_tiles = t_ndonH.shape[1]
+        b.max_n_conn = bt_conn_atoms.shape[1]
+        b.max_n_all_bonds = all_bonds.shape[1]
+        for k, v in a.items():
+            setattr(b, k, _ptr(v, ctypes.c_int32))
+        for k in ("tile_donH_hvy_inds", "tile_which_donH", "tile_acc_nH", "tile_n_polar",
+                  "tile_n_occ", "tile_pol_occ_inds"):
+            setattr(b, k, _ptr(zero_t, ctypes.c_int32))
+        self._zero = zero_t
+        for k, v in hb.items():
+            setattr(st, k, _ptr(v, ctypes.c_int32))
+        if sargs is not None:
+            st.n_don_types, st.n_acc_types = self.n_types
+        self.struct = st
+        self.P, self.B = P, B
+        self.coords = coords
+        self.bt_n_atoms = a["bt_n_atoms"]
+
+    def host(self, t, dtype):
+        return numpy.ascontiguousarray(t.detach().cpu().numpy().astype(dtype))
+
+
+def oracle_hb_bases(*bargs, dtype=None):
+    """gen_hbond_bases on the oracle: (derived coords [NA, 3, 3], source
+    atoms [NA, 3] int32)."""
+    S = _HBStack(bargs)
+    if dtype is None:
+        dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
+    suf, ct = _lkb_ct(dtype)
+    X = S.host(S.coords, dtype)
+    NA = X.shape[0]
+    D = numpy.zeros((NA, 3, 3), dtype=dtype)
+    I = numpy.zeros((NA, 3), dtype=numpy.int32)
+    getattr(lib(), "oracle_hb_bases" + suf)(ctypes.byref(S.struct), _ptr(X, ct), NA,
+                                            _ptr(D, ct), _ptr(I, ctypes.c_int32))
+    return D, I
+
+
+def oracle_hb_score(sargs, bargs, dTdV=None, dtype=None, derived=None, inds=None):
+    """hbond_pose_scores on the oracle.  Returns (scores [1, P] or
+    [1, P, B, B], block_neighbors [P, B, B], dX [NA, 3] or None); with dTdV
+    (shape of scores) dX is the op's weighted backward."""
+    S = _HBStack(bargs, sargs)
+    if dtype is None:
+        dtype = numpy.float64 if S.coords.dtype == torch.float64 else numpy.float32
+    suf, ct = _lkb_ct(dtype)
+    L = lib()
+    X = S.host(S.coords, dtype)
+    NA = X.shape[0]
+    Dv = numpy.ascontiguousarray(numpy.asarray(
+        derived if derived is not None else S.host(S.derived, dtype), dtype=dtype))
+    Iv = numpy.ascontiguousarray(numpy.asarray(
+        inds if inds is not None else S.dinds.detach().cpu().numpy(), dtype=numpy.int32))
+    PB = S.P * S.B
+    spheres = numpy.zeros((PB, 4), dtype=dtype)
+    getattr(L, "oracle_block_spheres" + suf)(
+        _ptr(X, ct), PB, _ptr(S.arrays["rot_coord_offset"], ctypes.c_int32),
+        _ptr(S.arrays["block_type"], ctypes.c_int32), _ptr(S.bt_n_atoms, ctypes.c_int32),
+        _ptr(spheres, ct))
+    nbr = numpy.zeros((S.P, S.B, S.B), dtype=numpy.int32)
+    getattr(L, "oracle_block_neighbors" + suf)(
+        S.P, S.B, _ptr(S.arrays["block_type"], ctypes.c_int32), _ptr(spheres, ct),
+        ct(dtype(5.5)), _ptr(nbr, ctypes.c_int32))
+    shape = (1, S.P, S.B, S.B) if S.block_pair else (1, S.P)
+    out = numpy.zeros(shape, dtype=dtype)
+    pp = S.host(S.pair_params, dtype)
+    polys = S.host(S.pair_polys, numpy.float64)
+    gp = S.host(S.gp, dtype).reshape(-1)
+    dX = wts = None
+    if dTdV is not None:
+        wts = numpy.ascontiguousarray(numpy.asarray(dTdV, dtype=dtype).reshape(shape))
+        dX = numpy.zeros((NA, 3), dtype=dtype)
+    getattr(L, "oracle_hb_score_pose" + suf)(
+        ctypes.byref(S.struct), _ptr(X, ct), NA, _ptr(Dv, ct), _ptr(Iv, ctypes.c_int32),
+        _ptr(pp, ct), _ptr(polys, ctypes.c_double), _ptr(gp, ct), _ptr(nbr, ctypes.c_int32),
+        int(S.block_pair), _ptr(out, ct), _ptr(wts, ct) if wts is not None else None,
+        _ptr(dX, ct) if dX is not None else None)
+    return out, nbr, dX
+
+
+def oracle_hb_point(D, H, A, B, B0, pair_params, polys, global_params, deriv=False,
+                    dtype=numpy.float64):
+    """The five-atom hbond energy of one donor / acceptor geometry; with
+    deriv also its gradient [5, 3] (D, H, A, B, B0)."""
+    suf, ct = _lkb_ct(dtype)
+    f = lambda x, n: numpy.ascontiguousarray(numpy.asarray(x, dtype=dtype).reshape(n))  # noqa
+    pts = [f(x, 3) for x in (D, H, A, B, B0)]
+    pp = f(pair_params, 3)
+    P = numpy.ascontiguousarray(numpy.asarray(polys, dtype=numpy.float64).reshape(48))
+    gp = f(global_params, 6)
+    g = numpy.zeros(15, dtype=dtype) if deriv else None
+    E = getattr(lib(), "oracle_hb_score" + suf)(
+        *[_ptr(x, ct) for x in pts], _ptr(pp, ct), _ptr(P, ctypes.c_double), _ptr(gp, ct),
+        _ptr(g, ct) if g is not None else None)
+    return (E, g.reshape(5, 3)) if deriv else E
+
+
+def load_reference_hbond_ops():
+    """The reference's own CPU hbond ops built by build_ref.sh
+    (``torch.ops.tmol_hbond``: gen_hbond_bases, hbond_pose_scores)."""
+    path = os.path.join(HERE, "_ref", "tmol_hbond_ref.so")
+    if not os.path.exists(path):
+        raise FileNotFoundError(path)
+    if not hasattr(torch.ops, "tmol_hbond") or not hasattr(torch.ops.tmol_hbond,
+                                                            "gen_hbond_bases"):
+        torch.ops.load_library(path)
+    return torch.ops.tmol_hbond

@@ -7,7 +7,9 @@ Reads (only when run by hand, in a checkout that has the reference tree):
   tmol/database/default/scoring/elec.yaml       (elec global params, partial charges
                                                  and count-pair representatives)
   tmol/database/default/scoring/hbond.yaml      (donor / acceptor atom types, which
-                                                 place lk_ball's waters)
+                                                 place lk_ball's waters; the hbond
+                                                 potential's weights, polynomials and
+                                                 donor x acceptor pairs)
 and writes ``default_db.json`` next to this file.  The JSON is data only: the
 same numbers, names and connectivity as the YAML, restricted to what the
 20 canonical amino acids with termini need.  Nothing at run time reads the
@@ -184,10 +186,29 @@ def extract(ref_root):
     hb = yaml.safe_load(
         open(os.path.join(ref_root, "tmol/database/default/scoring/hbond.yaml"))
     )
+    poly_keys = ["c_%d" % n for n in range(11)]
     hbond_out = {
         "donor_atom_types": [[x["d"], x["donor_type"]] for x in hb["donor_atom_types"]],
         "acceptor_atom_types": [
             [x["a"], x["acceptor_type"]] for x in hb["acceptor_atom_types"]
+        ],
+        # the hbond potential (hbond.yaml; tmol/database/scoring/hbond.py):
+        # type weights in file order (the type indices), the polynomials
+        # (coefficients c_0..c_10, absent ones 0), the donor x acceptor pairs
+        "global_parameters": dict(hb["global_parameters"]),
+        "donor_type_params": [[x["name"], x["weight"]] for x in hb["donor_type_params"]],
+        "acceptor_type_params": [[x["name"], x["weight"]] for x in hb["acceptor_type_params"]],
+        "polynomial_parameters": [
+            {
+                "name": x["name"], "dimension": x["dimension"], "xmin": x["xmin"],
+                "xmax": x["xmax"], "min_val": x["min_val"], "max_val": x["max_val"],
+                "degree": x["degree"], "c": [float(x.get(k, 0.0)) for k in poly_keys],
+            }
+            for x in hb["polynomial_parameters"]
+        ],
+        "pair_parameters": [
+            [x["donor_type"], x["acceptor_type"], x["AHdist"], x["cosBAH"], x["cosAHD"]]
+            for x in hb["pair_parameters"]
         ],
     }
     return {

@@ -13,7 +13,7 @@ unbonded above.  Missing atoms are not built: every atom of the chosen type
 must be present.
 """
 
-from typing import List, Optional, Sequence, Tuple
+from typing import List, Sequence, Tuple
 
 import numpy
 
@@ -50,21 +50,29 @@ def parse_pdb_residues(pdb_text: str):
 
 def pose_from_pdb(
     pdb_text: str,
-    res_range: Optional[Tuple[int, int]] = None,
+    res_range=None,
     db=None,
 ) -> Pose:
     """Build one pose from a PDB; ``res_range=(i, j)`` keeps residues i..j-1
-    (0-based over the whole file) with the reference's subset termini rule."""
+    (0-based over the whole file), a list of such ranges keeps each segment
+    (tmol/tests/score/common/test_energy_term.py:104-137), with the
+    reference's subset termini rule; residues of different segments are not
+    bonded."""
     db = db or default_chemical_database()
     recs = parse_pdb_residues(pdb_text)
     n_all = len(recs)
-    i0, i1 = res_range if res_range is not None else (0, n_all)
-    sel = recs[i0:i1]
+    if res_range is None:
+        ranges = [(0, n_all)]
+    elif isinstance(res_range[0], (tuple, list)):
+        ranges = [tuple(r) for r in res_range]
+    else:
+        ranges = [tuple(res_range)]
+    sel = [(gi, k == 0) for i0, i1 in ranges for k, gi in enumerate(range(i0, i1))]
     rts = []
     coords = []
     connected = []
-    for k, (chain, resseq, icode, res3, atoms) in enumerate(sel):
-        gi = i0 + k
+    for k, (gi, seg_start) in enumerate(sel):
+        chain, resseq, icode, res3, atoms = recs[gi]
         chain_start = gi == 0 or recs[gi - 1][0] != chain
         chain_end = gi == n_all - 1 or recs[gi + 1][0] != chain
         nterm = chain_start
@@ -83,5 +91,5 @@ def pose_from_pdb(
             raise ValueError("missing atoms %s in %s %d" % (missing, rt.name, resseq))
         rts.append(rt)
         coords.append(xyz)
-        connected.append(k > 0 and not chain_start)
+        connected.append(k > 0 and not chain_start and not seg_start)
     return Pose(rts, coords, connected)

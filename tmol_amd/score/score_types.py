@@ -1,6 +1,6 @@
 """Score types of the two-body terms built here, in output order within each
 term (tmol/score/score_types.py; tmol/score/terms/ljlk_creator.py:7-19,
-elec_creator.py, lk_ball_creator.py)."""
+elec_creator.py, lk_ball_creator.py, hbond_creator.py)."""
 
 import enum
 
@@ -14,3 +14,4 @@ class ScoreType(enum.IntEnum):
     lk_ball = 5
     lk_bridge = 6
     lk_bridge_uncpl = 7
+    hbond = 8
