@@ -149,6 +149,8 @@ def main():
                     help="skip the rotamer-pair scoring leg (16 x 300-residue repacking)")
     ap.add_argument("--no-elec", action="store_true",
                     help="skip the elec leg (same 512-pose batch, elec energy + gradient)")
+    ap.add_argument("--no-hbond", action="store_true",
+                    help="skip the hbond leg (same 512-pose batch, bases + energy + gradient)")
     ap.add_argument("--no-lkball", action="store_true",
                     help="skip the lk_ball leg (same 512-pose batch, waters + energy + gradient)")
     ap.add_argument("--no-padded", action="store_true",
@@ -346,6 +348,12 @@ def main():
         log("lk_ball ...")
         result["lk_ball"] = lk_ball_bench.run(
             dev, poses=poses, cpu=None if args.no_cpu else bench_cpu.lk_ball_poses, log=log)
+    if not args.no_hbond:
+        from tmol_amd import hbond_bench
+
+        log("hbond ...")
+        result["hbond"] = hbond_bench.run(
+            dev, poses=poses, cpu=None if args.no_cpu else bench_cpu.hbond_poses, log=log)
     if not args.no_padded:
         log("padded drop-in op ...")
         result["padded"] = padded_dropin(dev)

@@ -611,6 +611,90 @@ int tmol_lkball_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types
                        unsigned long long* census, void* stream);
 
 /* ---------------------------------------------------------------------
+ * hbond on the same scheduler.  Replaces, for its pose path, the
+ * reference's ops
+ *   tmol_hbond::gen_hbond_bases     tmol/score/hbond/potentials/compiled.ops.cpp:930-996
+ *                                   (GenerateHBondBases::forward,
+ *                                   gen_hbond_bases.impl.hh:380-468)
+ *   tmol_hbond::hbond_pose_scores   compiled.ops.cpp:28-375, 723-825 (HBondPoseScoresOp
+ *                                   forward / backward; HBondPoseScoreDispatch::forward,
+ *                                   hbond_pose_score.impl.hh:422-720, backward :860-1000)
+ * The stack and block-type structs are the LJ+LK ones (block types: n_atoms,
+ * n_conn, conn_atoms).  hbond's own tables (device pointers):
+ *   don: per block type its polar hydrogens in atom order, at most 32,
+ *     TMOL_HBOND_DON_WIDTH int32 each: {H, D (heavy atom), donor type, 0};
+ *   acc: its acceptors in atom order, at most 32, TMOL_HBOND_ACC_WIDTH
+ *     int32 each: {A, B, B0, acceptor type, hybridization (1 sp2, 2 sp3,
+ *     3 ring), 0, 0, 0}, bases -1 where missing, all atoms of the block;
+ *   base_src: [T, Amax, 3] the source atom of each derived slot of each atom
+ *     (slot 0: D of a polar H; 1, 2: B, B0 of an acceptor), -1 a missing
+ *     base, -2 no role;
+ *   path_distance: [T, Amax, Amax] intra-block bond separations;
+ *   the potential's tables as the reference stacks them: pair_params
+ *     [nD, nA, 3] {hybridization, acceptor weight, donor weight} and
+ *     global_params [6] {sp2_range_span, sp2_BAH180_rise, sp2_outer_width,
+ *     sp3_softmax_fade, threshold_distance, max_ha_dis} in the coordinates'
+ *     dtype, pair_polys [nD, nA, 3, 16] float64 (per polynomial c10..c0, pad,
+ *     xmin, xmax, min_val, max_val; A-H distance, cos BAH, AHD angle).
+ * Block types of more than 32 atoms are refused (ENOTSUP). */
+#define TMOL_HBOND_DON_WIDTH 4
+#define TMOL_HBOND_ACC_WIDTH 8
+typedef struct {
+  int32_t max_n_don;             /* Dmax (<= 32) */
+  int32_t max_n_acc;             /* Xmax (<= 32) */
+  int32_t n_don_types;           /* nD */
+  int32_t n_acc_types;           /* nA */
+  const int32_t* n_don;          /* [T] */
+  const int32_t* n_acc;          /* [T] */
+  const int32_t* don;            /* [T, Dmax, 4] */
+  const int32_t* acc;            /* [T, Xmax, 8] */
+  const int32_t* base_src;       /* [T, Amax, 3] */
+  const int32_t* path_distance;  /* [T, Amax, Amax] */
+  const void* pair_params;       /* [nD, nA, 3] */
+  const double* pair_polys;      /* [nD, nA, 48] */
+  const void* global_params;     /* [6] */
+} tmol_hbond_tables;
+
+/* Derived coordinates [NA, 3, 3] and their source atoms [NA, 3] (int32):
+ * the reference's gen_hbond_bases output (NaN / -1 for atoms with no role;
+ * (0, 0, 0) / -1 for a base that does not exist). */
+int tmol_hbond_bases(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                     const tmol_hbond_tables* hb, const void* coords, int32_t dtype,
+                     void* derived_coords, int32_t* derived_atom_inds, void* stream);
+
+/* Neighbor pass for hbond (spheres + half-triangle work list, as
+ * tmol_ljlk_neighbors without the LJ+LK atom records): blocks within
+ * max_dis (the reference's 5.5) of each other; row g1 lists g2 >= g1 of its
+ * pose.  Workspace: tmol_ljlk_workspace_bytes. */
+int tmol_hbond_neighbors(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                         const void* coords, int32_t dtype, double max_dis,
+                         const int32_t* tiles, int64_t n_tiles, void* workspace,
+                         int64_t ws_bytes, int64_t row_capacity, int32_t* dense_neighbors,
+                         void* stream);
+
+/* Score the work list.
+ *   flags 0: energies [P]; BLOCK_PAIR (padded stacks; ENOTSUP for ragged):
+ *     [P, B, B], upper triangle.
+ *   WGRAD (with weights = dTdV of the energies' shape): dxyz [NA, 3] of
+ *     sum w V through D, H, A, B, B0 (zeroed first unless ACCUMULATE); with
+ *     NO_ENERGY the backward alone. */
+int tmol_hbond_score(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                     const tmol_hbond_tables* hb, const void* coords,
+                     const void* derived_coords, const int32_t* derived_atom_inds,
+                     int32_t dtype, void* workspace, int64_t ws_bytes, int64_t row_capacity,
+                     int32_t flags, const void* weights, void* energies, void* dxyz,
+                     int32_t n_wave_groups, void* stream);
+
+/* Census of the hbond work list: census[3] (device uint64) = candidate
+ * donor x acceptor pairs, pairs below separation 5, evaluated pairs (H-A
+ * within max_ha_dis); for algorithmic FLOP accounting. */
+int tmol_hbond_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types* bt,
+                      const tmol_hbond_tables* hb, const void* coords,
+                      const void* derived_coords, const int32_t* derived_atom_inds,
+                      int32_t dtype, void* workspace, int64_t ws_bytes, int64_t row_capacity,
+                      unsigned long long* census, void* stream);
+
+/* ---------------------------------------------------------------------
  * Segmented L-BFGS history products (float64), for the minimizer that
  * drives this op (no op counterpart: the reference's LBFGS_Armijo,
  * tmol/optimization/lbfgs_armijo.py:6-58, does this algebra in torch).  The

@@ -32,6 +32,7 @@
 #include "ljlk_elec.hh"
 #include "lbfgs.hh"
 #include "lkball.hh"
+#include "hbond.hh"
 
 #include <cmath>
 
@@ -68,9 +69,9 @@ static unsigned grid_for(int64_t n, int block = 256) {
 }
 
 template <typename Real>
-static void fill(Real* p, int64_t n, hipStream_t stream) {
+static void fill(Real* p, int64_t n, hipStream_t stream, Real v = Real(0)) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(fill_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream, p, n, Real(0));
+  hipLaunchKernelGGL(fill_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream, p, n, v);
 }
 
 template <typename Real>
@@ -828,7 +829,9 @@ static int run_lkb_waters(const tmol_ljlk_stack* st, const tmol_ljlk_block_types
   return check(hipGetLastError());
 }
 
-template <typename Real>
+// spheres + work list without the LJ+LK atom records; FULL: rows list
+// both triangles (lk_ball), otherwise g2 >= g1 (hbond)
+template <typename Real, bool FULL = true>
 static int run_lkb_neighbors(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
                              const void* coords, double max_dis, const int32_t* tiles,
                              int64_t n_tiles, void* ws, int64_t row_cap, int32_t* dense,
@@ -851,7 +854,7 @@ static int run_lkb_neighbors(const tmol_ljlk_stack* st, const tmol_ljlk_block_ty
     hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(n)), dim3(256), 0, stream, dense, n, 0);
   }
   if (n_tiles > 0) {
-    auto k = neighbor_rows_kernel<Real, true>;  // full rows
+    auto k = neighbor_rows_kernel<Real, FULL>;
     hipLaunchKernelGGL(k, dim3((unsigned)n_tiles), dim3(256), 0, stream, s, (const int4*)tiles,
                        n_tiles, spheres, (Real)max_dis, row_count, row_items, row_cap, counters,
                        dense, st->pose_block_start);
@@ -945,6 +948,133 @@ static int run_lkb_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types*
   if (!noe && !bp)
     hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(4 * int64_t(P))), dim3(256), 0,
                        stream, (const double*)a.energies64, (Real*)energies, 4 * P);
+  return check(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ hbond
+
+static int validate_hb(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                       const tmol_hbond_tables* h, int dtype, bool scoring) {
+  if (!s || !bt || !h) return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32 && dtype != TMOL_LJLK_F64) return TMOL_LJLK_EINVAL;
+  if (bt->max_n_conn > MAX_CONN || bt->max_n_conn < 0) return TMOL_LJLK_ENOTSUP;
+  if (bt->n_block_types <= 0 || bt->max_n_atoms <= 0) return TMOL_LJLK_EINVAL;
+  if (bt->max_n_atoms > TILE) return TMOL_LJLK_ENOTSUP;
+  if (!bt->n_atoms || !bt->n_conn || (bt->max_n_conn > 0 && !bt->conn_atoms))
+    return TMOL_LJLK_EINVAL;
+  if (h->max_n_don < 0 || h->max_n_acc < 0 || h->n_don_types <= 0 || h->n_acc_types <= 0)
+    return TMOL_LJLK_EINVAL;
+  if (h->max_n_don > TILE || h->max_n_acc > TILE) return TMOL_LJLK_ENOTSUP;
+  if (h->n_don_types >= (1 << 15) || h->n_acc_types >= (1 << 15)) return TMOL_LJLK_ENOTSUP;
+  if (!h->n_don || !h->n_acc || (h->max_n_don > 0 && !h->don) ||
+      (h->max_n_acc > 0 && !h->acc) || !h->base_src || !h->path_distance || !h->pair_params ||
+      !h->pair_polys || !h->global_params)
+    return TMOL_LJLK_EINVAL;
+  if (s->n_poses <= 0 || s->n_blocks_total <= 0) return TMOL_LJLK_EINVAL;
+  if (s->layout == TMOL_LJLK_PADDED) {
+    if (int64_t(s->n_poses) * s->max_n_blocks != s->n_blocks_total) return TMOL_LJLK_EINVAL;
+    if (scoring && (!s->min_bondsep || !s->inter_block_bondsep)) return TMOL_LJLK_EINVAL;
+  } else if (s->layout == TMOL_LJLK_RAGGED) {
+    if (!s->block_pose || (scoring && (!s->bonded_start || !s->bonded))) return TMOL_LJLK_EINVAL;
+  } else {
+    return TMOL_LJLK_EINVAL;
+  }
+  if (!s->block_atom_offset || !s->block_type) return TMOL_LJLK_EINVAL;
+  if (s->n_atoms_total >= (int64_t(1) << 31) / 9) return TMOL_LJLK_ENOTSUP;
+  return TMOL_LJLK_OK;
+}
+
+template <typename Real>
+static int run_hb_bases(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
+                        const tmol_hbond_tables* h, const void* coords, void* derived,
+                        int32_t* inds, hipStream_t stream) {
+  StackView s = make_stack_view(st, btp->max_n_conn);
+  const int64_t na = st->n_atoms_total;
+  fill<Real>((Real*)derived, 9 * na, stream, Real(NAN));
+  hipLaunchKernelGGL(fill_kernel<int32_t>, dim3(grid_for(3 * na)), dim3(256), 0, stream, inds,
+                     3 * na, -1);
+  const int64_t n = int64_t(s.G) * btp->max_n_atoms;
+  hipLaunchKernelGGL(hb_bases_kernel<Real>, dim3(grid_for(n)), dim3(256), 0, stream, s,
+                     btp->n_atoms, h->base_src, btp->max_n_atoms, (const Real*)coords,
+                     (Real*)derived, inds);
+  return check(hipGetLastError());
+}
+
+template <typename Real, int MODE>
+static void launch_hb(const HbArgs<Real>& a, int n_wg, hipStream_t stream) {
+  hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3(2), dim3(256), 0, stream, a.row_cursor,
+                     8 * QSTRIDE, 0u);
+  auto k = hb_score_kernel<Real, MODE>;
+  if (n_wg <= 0) n_wg = resident_workgroups(k);
+  hipLaunchKernelGGL(k, dim3(n_wg), dim3(256), 0, stream, a);
+}
+
+template <typename Real>
+static int run_hb_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* btp,
+                        const tmol_hbond_tables* h, const void* coords, const void* derived,
+                        const int32_t* dinds, void* ws, int64_t row_cap, int flags,
+                        const void* weights, void* energies, void* dxyz, int n_wg,
+                        hipStream_t stream, const WsLayout& Lw,
+                        unsigned long long* census = nullptr) {
+  HbArgs<Real> a{};
+  a.s = make_stack_view(st, btp->max_n_conn);
+  char* w = (char*)ws;
+  a.w.row_count = (const uint32_t*)(w + Lw.row_count);
+  a.w.row_items = (const int32_t*)(w + Lw.row_items);
+  a.w.row_cap = row_cap;
+  a.coords = (const Real*)coords;
+  a.derived = (const Real*)derived;
+  a.dinds = dinds;
+  a.n_conn = btp->n_conn;
+  a.conn_atoms = btp->conn_atoms;
+  a.n_don = h->n_don;
+  a.n_acc = h->n_acc;
+  a.don = h->don;
+  a.acc = h->acc;
+  a.Dmax = h->max_n_don;
+  a.Xmax = h->max_n_acc;
+  a.Am = btp->max_n_atoms;
+  a.path_distance = h->path_distance;
+  a.pair_params = (const Real*)h->pair_params;
+  a.polys = h->pair_polys;
+  a.n_acc_types = h->n_acc_types;
+  a.gp = (const Real*)h->global_params;
+  a.row_cursor = (uint32_t*)(w + Lw.cursors);
+  a.weights = (const Real*)weights;
+  a.energies64 = (double*)(w + Lw.energies);
+  a.energies_bp = (Real*)energies;
+  a.dxyz = (Real*)dxyz;
+  a.census = census;
+  if (census) {
+    hipLaunchKernelGGL(fill_kernel<unsigned long long>, dim3(1), dim3(64), 0, stream, census, 3,
+                       0ull);
+    launch_hb<Real, M_CENSUS>(a, n_wg, stream);
+    return check(hipGetLastError());
+  }
+  const bool bp = flags & TMOL_LJLK_BLOCK_PAIR;
+  const bool noe = flags & TMOL_LJLK_NO_ENERGY;
+  const bool acc = flags & TMOL_LJLK_ACCUMULATE;
+  const int P = st->n_poses;
+  if (!noe) {
+    if (bp)
+      fill<Real>((Real*)energies, int64_t(P) * st->max_n_blocks * st->max_n_blocks, stream);
+    else
+      fill<double>(a.energies64, int64_t(P), stream);
+  }
+  if ((flags & TMOL_LJLK_WGRAD) && !acc) fill<Real>((Real*)dxyz, st->n_atoms_total * 3, stream);
+  const int mode = flags & (TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY);
+  switch (mode) {
+    case 0: launch_hb<Real, 0>(a, n_wg, stream); break;
+    case M_WGRAD: launch_hb<Real, M_WGRAD>(a, n_wg, stream); break;
+    case M_WGRAD | M_NOE: launch_hb<Real, M_WGRAD | M_NOE>(a, n_wg, stream); break;
+    case M_BP: launch_hb<Real, M_BP>(a, n_wg, stream); break;
+    case M_BP | M_WGRAD: launch_hb<Real, M_BP | M_WGRAD>(a, n_wg, stream); break;
+    case M_BP | M_WGRAD | M_NOE: launch_hb<Real, M_BP | M_WGRAD | M_NOE>(a, n_wg, stream); break;
+    default: return TMOL_LJLK_ENOTSUP;
+  }
+  if (!noe && !bp)
+    hipLaunchKernelGGL(finalize_kernel<Real>, dim3(grid_for(int64_t(P))), dim3(256), 0, stream,
+                       (const double*)a.energies64, (Real*)energies, P);
   return check(hipGetLastError());
 }
 
@@ -1762,6 +1892,83 @@ int tmol_lkball_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt
                                 nullptr, nullptr, nullptr, 0, st, L, census);
   return run_lkb_score<double>(s, bt, k, coords, waters, workspace, row_capacity, 0, nullptr,
                                nullptr, nullptr, nullptr, 0, st, L, census);
+}
+
+int tmol_hbond_bases(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                     const tmol_hbond_tables* h, const void* coords, int32_t dtype,
+                     void* derived, int32_t* inds, void* stream) {
+  int v = validate_hb(s, bt, h, dtype, false);
+  if (v) return v;
+  if (!coords || !derived || !inds) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32) return run_hb_bases<float>(s, bt, h, coords, derived, inds, st);
+  return run_hb_bases<double>(s, bt, h, coords, derived, inds, st);
+}
+
+int tmol_hbond_neighbors(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                         const void* coords, int32_t dtype, double max_dis,
+                         const int32_t* tiles, int64_t n_tiles, void* workspace,
+                         int64_t ws_bytes, int64_t row_capacity, int32_t* dense, void* stream) {
+  if (!s || !bt) return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32 && dtype != TMOL_LJLK_F64) return TMOL_LJLK_EINVAL;
+  if (s->n_poses <= 0 || s->n_blocks_total <= 0 || !bt->n_atoms) return TMOL_LJLK_EINVAL;
+  if (!coords || !workspace || (n_tiles > 0 && !tiles) || row_capacity <= 0)
+    return TMOL_LJLK_EINVAL;
+  if (!std::isfinite(max_dis) || max_dis < 0) return TMOL_LJLK_EINVAL;
+  if (n_tiles >= (int64_t(1) << 31)) return TMOL_LJLK_ENOTSUP;
+  if (dense && s->layout != TMOL_LJLK_PADDED) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_lkb_neighbors<float, false>(s, bt, coords, max_dis, tiles, n_tiles, workspace,
+                                           row_capacity, dense, st, L);
+  return run_lkb_neighbors<double, false>(s, bt, coords, max_dis, tiles, n_tiles, workspace,
+                                          row_capacity, dense, st, L);
+}
+
+int tmol_hbond_score(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                     const tmol_hbond_tables* h, const void* coords, const void* derived,
+                     const int32_t* dinds, int32_t dtype, void* workspace, int64_t ws_bytes,
+                     int64_t row_capacity, int32_t flags, const void* weights, void* energies,
+                     void* dxyz, int32_t n_wave_groups, void* stream) {
+  int v = validate_hb(s, bt, h, dtype, true);
+  if (v) return v;
+  if (!coords || !derived || !dinds || !workspace || row_capacity <= 0) return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  if (flags & ~(TMOL_LJLK_WGRAD | TMOL_LJLK_BLOCK_PAIR | TMOL_LJLK_NO_ENERGY |
+                TMOL_LJLK_ACCUMULATE))
+    return TMOL_LJLK_EINVAL;
+  if (!(flags & TMOL_LJLK_NO_ENERGY) && !energies) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_NO_ENERGY) && !(flags & TMOL_LJLK_WGRAD)) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_WGRAD) && (!dxyz || !weights)) return TMOL_LJLK_EINVAL;
+  if ((flags & TMOL_LJLK_BLOCK_PAIR) && s->layout != TMOL_LJLK_PADDED) return TMOL_LJLK_ENOTSUP;
+  if (n_wave_groups < 0 || n_wave_groups > (1 << 20)) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_hb_score<float>(s, bt, h, coords, derived, dinds, workspace, row_capacity, flags,
+                               weights, energies, dxyz, n_wave_groups, st, L);
+  return run_hb_score<double>(s, bt, h, coords, derived, dinds, workspace, row_capacity, flags,
+                              weights, energies, dxyz, n_wave_groups, st, L);
+}
+
+int tmol_hbond_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
+                      const tmol_hbond_tables* h, const void* coords, const void* derived,
+                      const int32_t* dinds, int32_t dtype, void* workspace, int64_t ws_bytes,
+                      int64_t row_capacity, unsigned long long* census, void* stream) {
+  int v = validate_hb(s, bt, h, dtype, true);
+  if (v) return v;
+  if (!coords || !derived || !dinds || !workspace || !census || row_capacity <= 0)
+    return TMOL_LJLK_EINVAL;
+  WsLayout L = ws_layout(s, bt, row_capacity, dtype);
+  if (ws_bytes < L.total) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  if (dtype == TMOL_LJLK_F32)
+    return run_hb_score<float>(s, bt, h, coords, derived, dinds, workspace, row_capacity, 0,
+                               nullptr, nullptr, nullptr, 0, st, L, census);
+  return run_hb_score<double>(s, bt, h, coords, derived, dinds, workspace, row_capacity, 0,
+                              nullptr, nullptr, nullptr, 0, st, L, census);
 }
 
 int tmol_lbfgs_dots(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,

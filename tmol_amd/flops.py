@@ -159,3 +159,31 @@ def lkb_algorithmic_flops(census: dict, gradient: bool = True) -> int:
     far = census["candidates"] - census["cp0"] - census["evaluated"]
     return int(far * t["far"] + census["evaluated"] * t["evaluated"]
                + census["bridge"] * t["bridge"])
+
+
+# hbond (tmol/score/hbond/potentials/potentials.hh:36-508), per donor H x
+# acceptor candidate of a neighbor block pair, same convention (one FLOP per
+# add, multiply, divide, square root or transcendental; the double-precision
+# polynomial steps count as FLOPs too), energy + weighted gradient:
+#   separation < 5                                                    0
+#   H-A distance test                                                 9
+#   evaluated   A-H distance with its polynomial (10 Horner steps of
+#               value and slope, 40) and gradient 56; A-H-D angle
+#               (two differences, cross product, normalisation, atan2,
+#               gradients of both arms) with polynomial 150; B-A-H
+#               polynomial of the cosine with gradient 95 (sp3: twice
+#               plus the soft-max, 230); sp2 chi: angle 100, dihedral
+#               with Blondel-Karplus gradient 110, the chi energy's
+#               trigonometry 120; weight, fade and the 15 scaled
+#               gradient components 35.  Backbone O (sp2) dominates the
+#               acceptors, so the sp2 total is used for every pair      670
+# Energy only: 9 + 45 + 90 + 60 + 200 + 10 = 414.
+HB_CENSUS_FIELDS = ["candidates", "cp_excluded", "evaluated"]
+HB_FLOPS_ENERGY_GRAD = {"test": DIST + 1, "evaluated": 670}
+HB_FLOPS_ENERGY = {"test": DIST + 1, "evaluated": 414}
+
+
+def hb_algorithmic_flops(census: dict, gradient: bool = True) -> int:
+    t = HB_FLOPS_ENERGY_GRAD if gradient else HB_FLOPS_ENERGY
+    tested = census["candidates"] - census["cp_excluded"]
+    return int(tested * t["test"] + census["evaluated"] * t["evaluated"])
