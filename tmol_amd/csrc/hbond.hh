@@ -324,6 +324,7 @@ struct HbArgs {
   const double* polys;           // [nD, nA, 48]
   int n_acc_types;
   const Real* gp;                // [6]
+  const Real* spheres;           // [G, 4] block spheres of the neighbor pass
   uint32_t* row_cursor;
   const Real* weights;           // dTdV: [P] or block pairs
   double* energies64;            // [P]
@@ -395,7 +396,9 @@ __device__ __forceinline__ void hb_phase_b(HbLDS& L, const HbArgs<Real>& a, int 
 
 // Persistent waves over the rows of the half-triangle work list (row g1
 // holds its neighbor blocks g2 >= g1, g1 itself for the intra-block pairs).
-// Per pass, up to 64 items are staged as records in LDS; the wave then walks
+// Per pass, up to 64 items are staged as records in LDS (an item whose block
+// spheres are farther apart than max_ha_dis gets no candidates); the wave
+// then walks
 // their donor x acceptor candidates 64 at a time (an item found by binary
 // search over the prefix sums): count pair (inter-block pairs: the
 // reference's lookup with the donor's and acceptor's list positions as atom
@@ -404,8 +407,24 @@ __device__ __forceinline__ void hb_phase_b(HbLDS& L, const HbArgs<Real>& a, int 
 // FIFO and evaluated 64 at a time.  Energies: per-pose FP64 sums (one
 // atomic per row) or block-pair cells; gradients through the derived atoms'
 // indices with global atomics (few pairs pass).
+// waves per SIMD the scoring kernel is compiled for: it is latency-bound
+// (the per-item loads form dependent chains), so occupancy pays until the
+// register cap spills.  Measured on the 512-pose batch: gradients 3 / 4 / 5 /
+// 6 waves 1.07 / 0.91 / 1.03 / 1.10 ms (5 and 6 spill); energies 4 / 5 / 6 / 8
+// waves 0.74 / 0.68 / 0.67 / 0.67 ms (6 and 8 spill)
+#ifndef TMOL_HB_WAVES_GRAD
+#define TMOL_HB_WAVES_GRAD 4
+#endif
+#ifndef TMOL_HB_WAVES_ENERGY
+#define TMOL_HB_WAVES_ENERGY 5
+#endif
+template <int MODE>
+constexpr int hb_waves_per_simd() {
+  return (MODE & M_WGRAD) ? TMOL_HB_WAVES_GRAD : TMOL_HB_WAVES_ENERGY;
+}
+
 template <typename Real, int MODE>
-__global__ __launch_bounds__(256) void hb_score_kernel(HbArgs<Real> a) {
+__global__ __launch_bounds__(256, hb_waves_per_simd<MODE>()) void hb_score_kernel(HbArgs<Real> a) {
   constexpr bool DERIV = (MODE & M_WGRAD) != 0;
   constexpr bool BP = (MODE & M_BP) != 0;
   constexpr bool NOE = (MODE & M_NOE) != 0;
@@ -445,8 +464,15 @@ __global__ __launch_bounds__(256) void hb_score_kernel(HbArgs<Real> a) {
         if (it.g2 == g1) {
           cand = nd1 * na1;
         } else if (it.bt2 >= 0) {
-          it.msep = min_block_sep(a.s, g1, it.g2, nc1, it.nc2);
-          cand = nd1 * it.na2 + na1 * it.nd2;
+          // blocks whose spheres stay farther apart than max_ha_dis (with
+          // a margin for rounding) hold no pair within it
+          const Real* s1 = a.spheres + 4 * int64_t(g1);
+          const Real* s2 = a.spheres + 4 * int64_t(it.g2);
+          const Vec3<Real> dc = {s1[0] - s2[0], s1[1] - s2[1], s1[2] - s2[2]};
+          if (norm3(dc) <= s1[3] + s2[3] + max_ha + Real(1e-3)) {
+            it.msep = min_block_sep(a.s, g1, it.g2, nc1, it.nc2);
+            cand = nd1 * it.na2 + na1 * it.nd2;
+          }
         }
         it.cand = cand;
         L.it[lane] = it;

@@ -14,11 +14,24 @@ backward), a CPU baseline on a bounded sample of the poses.
 
 import argparse
 import json
+import os
 import sys
 import time
 
 FP32_PEAK_TFLOPS = 157.3
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
+                       "hb_score_kernel_ragged512_counters.json")
+
+
+def _counters():
+    """The committed rocprofv3 counter summary of the scoring kernel on this
+    batch (HBM bytes per launch from FETCH_SIZE / WRITE_SIZE), or None."""
+    try:
+        with open(PROFILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=None, log=print):
@@ -78,6 +91,8 @@ def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=None, log=pr
     # gradient of the hbond atoms written (bounded above by all atoms)
     na = stack.n_atoms_total
     alg_bytes = 4 * n_items + na * (3 + 9) * 4 + na * 3 * 4
+    prof = _counters()
+    traffic = prof.get("hbm_bytes_per_launch") if prof else None
     return {
         "workload": "ragged_512x50_1200",
         "poses": stack.n_poses,
@@ -99,7 +114,11 @@ def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=None, log=pr
             "peak": FP32_PEAK_TFLOPS,
             "unit": "TFLOP/s",
             "frac": achieved / FP32_PEAK_TFLOPS,
-            "traffic": None,
+            "traffic": traffic,
+            "traffic_profile": ({k: prof.get(k) for k in (
+                "commit", "hbm_read_bytes_per_launch", "hbm_write_bytes_per_launch",
+                "copy_calibration")} if prof else None),
+            "counters": prof.get("derived") if prof else None,
             "kernel": "hb_score_kernel<float, WGRAD> (vector ALU; polynomials in FP64)",
             "algorithmic_flops_per_launch": flops,
             "algorithmic_bytes_per_launch": alg_bytes,

@@ -3,11 +3,13 @@ the scoring kernel a fixed number of times (no autotuning, no CPU work), so
 that ``rocprofv3`` sees a short, repeatable kernel sequence.
 
     python -m tmol_amd.profile_step --workload ragged_512x50_1200 --reps 5 \
-        [--mode wgrad|dvdx|energy] [--general] [--wave-groups N] [--term ljlk|elec|lk_ball]
+        [--mode wgrad|dvdx|energy] [--general] [--wave-groups N]
+        [--term ljlk|elec|lk_ball|hbond]
 
 ``--term elec`` runs the elec term's neighbor pass and row kernel instead
 (weights 1 per pose in wgrad mode); ``--term lk_ball`` lk_ball's neighbor
-pass, waters and gather kernel (``--mode wgrad|energy``).
+pass, waters and gather kernel; ``--term hbond`` hbond's neighbor pass,
+bases and row kernel (both ``--mode wgrad|energy``).
 
 ``--copy-bytes N`` instead copies an N-byte device buffer ``reps`` times with
 torch's copy kernel: the reference point that scales the HBM counters
@@ -104,7 +106,7 @@ def main(argv=None):
     ap.add_argument("--wave-groups", type=int, default=0)
     ap.add_argument("--dtype", default="f32", choices=["f32", "f64"])
     ap.add_argument("--copy-bytes", type=int, default=0)
-    ap.add_argument("--term", default="ljlk", choices=["ljlk", "elec", "lk_ball"])
+    ap.add_argument("--term", default="ljlk", choices=["ljlk", "elec", "lk_ball", "hbond"])
     args = ap.parse_args(argv)
 
     if args.summarize:
@@ -154,6 +156,30 @@ def main(argv=None):
             sc.neighbors(x)
             sc.gen_waters(x)
             sc.score(x, flags, weights=w, energies=sc.energies, dxyz=sc.dxyz, dwat=sc.dwat)
+            torch.cuda.synchronize()
+            times.append(time.perf_counter() - t0)
+        print(json.dumps({"workload": args.workload, "term": args.term, "mode": args.mode,
+                          "block_pairs": n_items, "step_ms": [1e3 * t for t in times]}),
+              flush=True)
+        return 0
+    if args.term == "hbond":
+        from .hbond import HBondRaggedScorer
+        from .score.hbond_energy_term import HBondEnergyTerm
+
+        sc = HBondRaggedScorer(stack, HBondEnergyTerm(), device=dev, dtype=dtype)
+        x = stack.coords.to(dev)
+        w = sc._weights(None)
+        flags = {"wgrad": ops.WGRAD, "energy": 0}[args.mode]
+        sc.neighbors(x)
+        n_items = sc.check_capacity()
+        sc.bases(x)
+        torch.cuda.synchronize()
+        times = []
+        for _ in range(args.reps):
+            t0 = time.perf_counter()
+            sc.neighbors(x)
+            sc.bases(x)
+            sc.score(x, flags, weights=w, energies=sc.energies, dxyz=sc.dxyz)
             torch.cuda.synchronize()
             times.append(time.perf_counter() - t0)
         print(json.dumps({"workload": args.workload, "term": args.term, "mode": args.mode,
