@@ -79,3 +79,16 @@ g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
   -o "$OUT5.tmp"
 mv "$OUT5.tmp" "$OUT5"
 echo "built $OUT5"
+# the reference's ingest ops (tmol/io/details/compiled: gen_pose_leaf_atoms,
+# resolve_his_taut)
+OUT6="$HERE/_ref/tmol_io_ref.so"
+g++ -O3 -w -std=c++17 -shared -fPIC -D_GLIBCXX_USE_CXX11_ABI=$ABI \
+  -DTORCH_EXTENSION_NAME=tmol_io_ref \
+  -I"$R" -I"$R/tmol/extern" $TI -I"$PYI" \
+  "$R/tmol/io/details/compiled/compiled.ops.cpp" \
+  "$R/tmol/io/details/compiled/gen_pose_leaf_atoms.cpu.cpp" \
+  "$R/tmol/io/details/compiled/resolve_his_taut.cpu.cpp" \
+  -L"$TL" -ltorch -ltorch_cpu -lc10 -Wl,-rpath,"$TL" \
+  -o "$OUT6.tmp"
+mv "$OUT6.tmp" "$OUT6"
+echo "built $OUT6"
