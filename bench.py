@@ -149,6 +149,8 @@ def main():
                     help="skip the rotamer-pair scoring leg (16 x 300-residue repacking)")
     ap.add_argument("--no-elec", action="store_true",
                     help="skip the elec leg (same 512-pose batch, elec energy + gradient)")
+    ap.add_argument("--no-leaf", action="store_true",
+                    help="skip the missing-hydrogen leg (same 512-pose batch, build + backward)")
     ap.add_argument("--no-hbond", action="store_true",
                     help="skip the hbond leg (same 512-pose batch, bases + energy + gradient)")
     ap.add_argument("--no-lkball", action="store_true",
@@ -354,6 +356,12 @@ def main():
         log("hbond ...")
         result["hbond"] = hbond_bench.run(
             dev, poses=poses, cpu=None if args.no_cpu else bench_cpu.hbond_poses, log=log)
+    if not args.no_leaf:
+        from tmol_amd import leaf_bench
+
+        log("leaf atoms ...")
+        result["leaf_atoms"] = leaf_bench.run(
+            dev, poses=poses, cpu=None if args.no_cpu else bench_cpu.leaf_atoms, log=log)
     if not args.no_padded:
         log("padded drop-in op ...")
         result["padded"] = padded_dropin(dev)

@@ -695,6 +695,45 @@ int tmol_hbond_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types*
                       unsigned long long* census, void* stream);
 
 /* ---------------------------------------------------------------------
+ * Missing leaf atoms of a pose stack (canonical-form ingest).  Replaces the
+ * reference's op
+ *   tmol_io::gen_pose_leaf_atoms   tmol/io/details/compiled/compiled.ops.cpp:19-222
+ *                                  (PoseLeafAtomGen forward / backward;
+ *                                  GeneratePoseLeafAtoms, gen_pose_leaf_atoms.impl.hh:25-336)
+ * The arguments are the op's, in the padded pose layout (device pointers):
+ * atoms given in block_atom_missing = 0 are copied, a missing atom is built
+ * from three ancestors (atom index in the block, or connection + bonds from
+ * it into the connected block) and its ideal (phi, theta, d), with a backup
+ * ancestor set where the first is incomplete; NaN when neither resolves. */
+typedef struct {
+  int32_t n_poses;                            /* P */
+  int32_t max_n_pose_atoms;                   /* NA */
+  int32_t max_n_blocks;                       /* B */
+  int32_t max_n_block_atoms;                  /* Amax */
+  int32_t max_n_conn;                         /* C */
+  const uint8_t* block_atom_missing;          /* [P, B, Amax] */
+  const uint8_t* pose_atom_missing;           /* [P, NA] */
+  const int32_t* block_coord_offset;          /* [P, B] */
+  const int32_t* block_type;                  /* [P, B], -1 = none */
+  const int32_t* inter_block_connections;     /* [P, B, C, 2] */
+  const int32_t* bt_n_atoms;                  /* [T] */
+  const int32_t* bt_atom_downstream_of_conn;  /* [T, C, Amax] */
+  const int32_t* bt_atom_ancestors;           /* [T, Amax, 3, 3] */
+  const void* bt_atom_icoors;                 /* [T, Amax, 3] phi, theta, d (coords dtype) */
+  const int32_t* bt_atom_ancestors_backup;    /* [T, Amax, 3, 3] */
+  const void* bt_atom_icoors_backup;          /* [T, Amax, 3] */
+} tmol_leaf_atoms_args;
+
+/* new_coords [P, NA, 3] from orig_coords [P, NA, 3]. */
+int tmol_leaf_atoms_build(const tmol_leaf_atoms_args* args, const void* orig_coords,
+                          int32_t dtype, void* new_coords, void* stream);
+
+/* dorig [P, NA, 3] (overwritten) from dnew [P, NA, 3]: the backward of
+ * tmol_leaf_atoms_build. */
+int tmol_leaf_atoms_backward(const tmol_leaf_atoms_args* args, const void* orig_coords,
+                             int32_t dtype, const void* dnew, void* dorig, void* stream);
+
+/* ---------------------------------------------------------------------
  * Segmented L-BFGS history products (float64), for the minimizer that
  * drives this op (no op counterpart: the reference's LBFGS_Armijo,
  * tmol/optimization/lbfgs_armijo.py:6-58, does this algebra in torch).  The

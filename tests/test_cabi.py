@@ -15,7 +15,7 @@ HEADER = os.path.join(ROOT, "include", "tmol_ljlk.h")
 
 def _declared_symbols():
     text = open(HEADER).read()
-    return sorted(set(re.findall(r"\b(tmol_(?:ljlk|elec|lbfgs|lkball)_[a-z0-9_]+)\s*\(", text)))
+    return sorted(set(re.findall(r"\b(tmol_(?:ljlk|elec|lbfgs|lkball|hbond|leaf_atoms)_[a-z0-9_]+)\s*\(", text)))
 
 
 def test_library_exports_header_symbols():
@@ -328,3 +328,27 @@ def test_noalloc_validation():
     s.n_blocks_total = 20000 * 1000
     assert L.tmol_ljlk_workspace_bytes(ctypes.byref(s), ctypes.byref(b), 1000, ops.F32) > 64 << 20
     assert call(1000) == ops.ESCRATCH
+
+
+def test_leaf_atoms_validation():
+    """tmol_leaf_atoms_build / _backward reject missing tables, a bad dtype
+    and oversized poses before touching the device."""
+    from tmol_amd import ops
+    from tmol_amd.ingest import CLeafArgs, _leaf_lib
+
+    L = _leaf_lib()
+    dummy = ctypes.c_double(0.0)
+    p = ctypes.cast(ctypes.pointer(dummy), ctypes.c_void_p).value
+    a = CLeafArgs()
+    a.n_poses, a.max_n_pose_atoms, a.max_n_blocks, a.max_n_block_atoms, a.max_n_conn = 2, 100, 8, 28, 3
+    for name, _ in CLeafArgs._fields_[5:]:
+        setattr(a, name, p)
+    assert L.tmol_leaf_atoms_build(None, p, ops.F32, p, None) == ops.EINVAL
+    assert L.tmol_leaf_atoms_build(ctypes.byref(a), p, 7, p, None) == ops.EINVAL
+    assert L.tmol_leaf_atoms_build(ctypes.byref(a), None, ops.F32, p, None) == ops.EINVAL
+    assert L.tmol_leaf_atoms_backward(ctypes.byref(a), p, ops.F64, p, None, None) == ops.EINVAL
+    a.bt_atom_ancestors_backup = None
+    assert L.tmol_leaf_atoms_build(ctypes.byref(a), p, ops.F32, p, None) == ops.EINVAL
+    a.bt_atom_ancestors_backup = p
+    a.n_poses, a.max_n_pose_atoms = 1 << 16, 1 << 14
+    assert L.tmol_leaf_atoms_build(ctypes.byref(a), p, ops.F32, p, None) == ops.ENOTSUP

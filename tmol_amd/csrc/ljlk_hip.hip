@@ -33,6 +33,7 @@
 #include "lbfgs.hh"
 #include "lkball.hh"
 #include "hbond.hh"
+#include "leaf_atoms.hh"
 
 #include <cmath>
 
@@ -1039,6 +1040,7 @@ static int run_hb_score(const tmol_ljlk_stack* st, const tmol_ljlk_block_types* 
   a.polys = h->pair_polys;
   a.n_acc_types = h->n_acc_types;
   a.gp = (const Real*)h->global_params;
+  a.spheres = (const Real*)(w + Lw.spheres);
   a.row_cursor = (uint32_t*)(w + Lw.cursors);
   a.weights = (const Real*)weights;
   a.energies64 = (double*)(w + Lw.energies);
@@ -1969,6 +1971,79 @@ int tmol_hbond_census(const tmol_ljlk_stack* s, const tmol_ljlk_block_types* bt,
                                nullptr, nullptr, nullptr, 0, st, L, census);
   return run_hb_score<double>(s, bt, h, coords, derived, dinds, workspace, row_capacity, 0,
                               nullptr, nullptr, nullptr, 0, st, L, census);
+}
+
+static int leaf_args(const tmol_leaf_atoms_args* x, int dtype, LeafArgs& a) {
+  if (!x) return TMOL_LJLK_EINVAL;
+  if (dtype != TMOL_LJLK_F32 && dtype != TMOL_LJLK_F64) return TMOL_LJLK_EINVAL;
+  if (x->n_poses <= 0 || x->max_n_blocks <= 0 || x->max_n_block_atoms <= 0 ||
+      x->max_n_pose_atoms <= 0 || x->max_n_conn < 0)
+    return TMOL_LJLK_EINVAL;
+  if (!x->block_atom_missing || !x->pose_atom_missing || !x->block_coord_offset ||
+      !x->block_type || (x->max_n_conn > 0 && !x->inter_block_connections) ||
+      !x->bt_n_atoms || !x->bt_atom_downstream_of_conn || !x->bt_atom_ancestors ||
+      !x->bt_atom_icoors || !x->bt_atom_ancestors_backup || !x->bt_atom_icoors_backup)
+    return TMOL_LJLK_EINVAL;
+  if (int64_t(x->n_poses) * x->max_n_pose_atoms >= (int64_t(1) << 31) / 3)
+    return TMOL_LJLK_ENOTSUP;
+  a.P = x->n_poses;
+  a.NA = x->max_n_pose_atoms;
+  a.B = x->max_n_blocks;
+  a.Amax = x->max_n_block_atoms;
+  a.C = x->max_n_conn;
+  a.block_missing = x->block_atom_missing;
+  a.pose_missing = x->pose_atom_missing;
+  a.block_offset = x->block_coord_offset;
+  a.block_type = x->block_type;
+  a.irc = x->inter_block_connections;
+  a.n_atoms = x->bt_n_atoms;
+  a.downstream = x->bt_atom_downstream_of_conn;
+  a.anc = x->bt_atom_ancestors;
+  a.anc_backup = x->bt_atom_ancestors_backup;
+  a.geom = x->bt_atom_icoors;
+  a.geom_backup = x->bt_atom_icoors_backup;
+  return TMOL_LJLK_OK;
+}
+
+int tmol_leaf_atoms_build(const tmol_leaf_atoms_args* x, const void* orig, int32_t dtype,
+                          void* out, void* stream) {
+  LeafArgs a;
+  int v = leaf_args(x, dtype, a);
+  if (v) return v;
+  if (!orig || !out) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = int64_t(a.P) * a.B * a.Amax;
+  // atoms of the pose layout beyond any block stay zero, as in the reference
+  if (dtype == TMOL_LJLK_F32) {
+    fill<float>((float*)out, 3 * int64_t(a.P) * a.NA, st);
+    hipLaunchKernelGGL(leaf_atoms_kernel<float>, dim3(grid_for(n)), dim3(256), 0, st, a,
+                       (const float*)orig, (float*)out);
+  } else {
+    fill<double>((double*)out, 3 * int64_t(a.P) * a.NA, st);
+    hipLaunchKernelGGL(leaf_atoms_kernel<double>, dim3(grid_for(n)), dim3(256), 0, st, a,
+                       (const double*)orig, (double*)out);
+  }
+  return check(hipGetLastError());
+}
+
+int tmol_leaf_atoms_backward(const tmol_leaf_atoms_args* x, const void* orig, int32_t dtype,
+                             const void* dnew, void* dorig, void* stream) {
+  LeafArgs a;
+  int v = leaf_args(x, dtype, a);
+  if (v) return v;
+  if (!orig || !dnew || !dorig) return TMOL_LJLK_EINVAL;
+  hipStream_t st = (hipStream_t)stream;
+  const int64_t n = int64_t(a.P) * a.B * a.Amax;
+  if (dtype == TMOL_LJLK_F32) {
+    fill<float>((float*)dorig, 3 * int64_t(a.P) * a.NA, st);
+    hipLaunchKernelGGL(leaf_atoms_backward_kernel<float>, dim3(grid_for(n)), dim3(256), 0, st,
+                       a, (const float*)orig, (const float*)dnew, (float*)dorig);
+  } else {
+    fill<double>((double*)dorig, 3 * int64_t(a.P) * a.NA, st);
+    hipLaunchKernelGGL(leaf_atoms_backward_kernel<double>, dim3(grid_for(n)), dim3(256), 0, st,
+                       a, (const double*)orig, (const double*)dnew, (double*)dorig);
+  }
+  return check(hipGetLastError());
 }
 
 int tmol_lbfgs_dots(int32_t n_segments, int32_t history, int64_t seg_len, int32_t n_used,

@@ -153,6 +153,41 @@ __global__ __launch_bounds__(256) void lkb_waters_kernel(StackView s, const int3
   }
 }
 
+// vector-Jacobian product of build_coordinate(A, B, C, dist, angle, tors)
+// (the water frame above): gradients gA, gB, gC of A, B, C for the output's
+// gradient gw.  dW/dA = I always; collinear frames pass nothing to B and C.
+template <typename Real>
+__device__ __forceinline__ void build_coord_vjp(Vec3<Real> A, Vec3<Real> B, Vec3<Real> C,
+                                                WaterCoef<Real> c, Vec3<Real> gw,
+                                                Vec3<Real>& gA, Vec3<Real>& gB,
+                                                Vec3<Real>& gC) {
+  Vec3<Real> e0, e1, e2;
+  const Real nraw = water_frame(A, B, C, e0, e1, e2);
+  gA = gw;
+  gB = {0, 0, 0};
+  gC = {0, 0, 0};
+  if (nraw == Real(0)) return;
+  Vec3<Real> g0 = scale(gw, c.k0), g1 = scale(gw, c.k1), g2 = scale(gw, c.k2);
+  // e1 = e2 x e0
+  Vec3<Real> t1 = cross(e0, g1), t2 = cross(g1, e2);
+  g2 = {g2.x + t1.x, g2.y + t1.y, g2.z + t1.z};
+  g0 = {g0.x + t2.x, g0.y + t2.y, g0.z + t2.z};
+  // e2 = n / |n|, n = e0 x (C - B)
+  const Real d2 = dot(e2, g2);
+  const Vec3<Real> gn = scale(sub(g2, scale(e2, d2)), Real(1) / nraw);
+  const Vec3<Real> cb = sub(C, B);
+  const Vec3<Real> t3 = cross(cb, gn);
+  g0 = {g0.x + t3.x, g0.y + t3.y, g0.z + t3.z};
+  const Vec3<Real> gcb = cross(gn, e0);
+  // e0 = (A - B) / |A - B|
+  const Vec3<Real> ab = sub(A, B);
+  const Real d0 = dot(e0, g0);
+  const Vec3<Real> gab = scale(sub(g0, scale(e0, d0)), Real(1) / sqrt(dot(ab, ab)));
+  gA = {gA.x + gab.x, gA.y + gab.y, gA.z + gab.z};
+  gB = {-gab.x - gcb.x, -gab.y - gcb.y, -gab.z - gcb.z};
+  gC = gcb;
+}
+
 // dE/dx += J^T dE/dW for every built water (atomics: a frame atom serves
 // several waters)
 template <typename Real>
@@ -193,32 +228,9 @@ __global__ __launch_bounds__(256) void lkb_waters_backward_kernel(
     const Vec3<Real> GG = v3(coords + 3 * igg);
     Vec3<Real> B = G;
     if (kind == 3) B = {(G.x + GG.x) / Real(2), (G.y + GG.y) / Real(2), (G.z + GG.z) / Real(2)};
-    Vec3<Real> e0, e1, e2;
-    const Real nraw = water_frame(P, B, GG, e0, e1, e2);
-    // dW/dA = I always; collinear frames pass nothing to B and B0
-    Vec3<Real> gA = gw, gB = {0, 0, 0}, gC = {0, 0, 0};
-    if (nraw != Real(0)) {
-      const WaterCoef<Real> c = water_coef(dist, wg[kind], tors[(kind - 1) * 4 + e[6]]);
-      Vec3<Real> g0 = scale(gw, c.k0), g1 = scale(gw, c.k1), g2 = scale(gw, c.k2);
-      // e1 = e2 x e0
-      Vec3<Real> t1 = cross(e0, g1), t2 = cross(g1, e2);
-      g2 = {g2.x + t1.x, g2.y + t1.y, g2.z + t1.z};
-      g0 = {g0.x + t2.x, g0.y + t2.y, g0.z + t2.z};
-      // e2 = n / |n|, n = e0 x (C - B)
-      const Real d2 = dot(e2, g2);
-      const Vec3<Real> gn = scale(sub(g2, scale(e2, d2)), Real(1) / nraw);
-      const Vec3<Real> cb = sub(GG, B);
-      const Vec3<Real> t3 = cross(cb, gn);
-      g0 = {g0.x + t3.x, g0.y + t3.y, g0.z + t3.z};
-      const Vec3<Real> gcb = cross(gn, e0);
-      // e0 = (A - B) / |A - B|
-      const Vec3<Real> ab = sub(P, B);
-      const Real d0 = dot(e0, g0);
-      const Vec3<Real> gab = scale(sub(g0, scale(e0, d0)), Real(1) / sqrt(dot(ab, ab)));
-      gA = {gA.x + gab.x, gA.y + gab.y, gA.z + gab.z};
-      gB = {-gab.x - gcb.x, -gab.y - gcb.y, -gab.z - gcb.z};
-      gC = gcb;
-    }
+    Vec3<Real> gA, gB, gC;
+    build_coord_vjp(P, B, GG, water_coef(dist, wg[kind], tors[(kind - 1) * 4 + e[6]]), gw, gA,
+                    gB, gC);
     atomicAdd(&dxyz[3 * ip + 0], gA.x);
     atomicAdd(&dxyz[3 * ip + 1], gA.y);
     atomicAdd(&dxyz[3 * ip + 2], gA.z);
