@@ -704,7 +704,10 @@ int tmol_hbond_census(const tmol_ljlk_stack* stack, const tmol_ljlk_block_types*
  * atoms given in block_atom_missing = 0 are copied, a missing atom is built
  * from three ancestors (atom index in the block, or connection + bonds from
  * it into the connected block) and its ideal (phi, theta, d), with a backup
- * ancestor set where the first is incomplete; NaN when neither resolves. */
+ * ancestor set where the first is incomplete; NaN when neither resolves.
+ * The table indices are not checked here (the caller validates them, as
+ * tmol_amd/ingest.py does).  ENOTSUP: max_n_block_atoms > 2048, or
+ * P * B * Amax >= 2^31, or P * NA * 3 >= 2^31. */
 typedef struct {
   int32_t n_poses;                            /* P */
   int32_t max_n_pose_atoms;                   /* NA */

@@ -71,7 +71,7 @@ def test_leaf_atoms_match_oracle(co, dt, frac):
     o = out.detach().cpu()
     assert torch.equal(torch.isnan(o), torch.isnan(ref))
     assert int(args[1].sum()) > 500
-    scale = float(ref.abs().nan_to_num(0).max())
+    scale = float(ref.detach().abs().nan_to_num(0).max())
     assert float((o - ref.detach()).abs().nan_to_num(0).max()) <= TOL[dt] * scale
     w = torch.randn(ref.shape, dtype=dt, generator=torch.Generator().manual_seed(4))
     (out.nan_to_num(0) * w.cuda()).sum().backward()
@@ -155,3 +155,39 @@ def test_large_batch_properties(co):
         o1 = ingest.gen_pose_leaf_atoms(*_dev(a1))
         n = o1.shape[1]
         assert torch.equal(o1[0].nan_to_num(0), out[p, :n].nan_to_num(0))
+
+
+@pytest.mark.parametrize("amax", [64, 100])
+def test_wide_block_type_tables(co, amax):
+    """The same poses with the block-type tables padded to ``amax`` atom
+    slots (one block per wave pass, lanes striding over the slots when
+    amax > 64): forward and backward identical to the unpadded call."""
+    print("leaf atoms amax", amax, flush=True)
+    ps, args = _random_missing(co, torch.float32, seed=7)
+    A0 = args[1].shape[2]
+
+    def pad(t, dim, value):
+        shape = list(t.shape)
+        shape[dim] = amax - A0
+        return torch.cat([t, torch.full(shape, value, dtype=t.dtype)], dim=dim)
+
+    wide = list(args)
+    wide[1] = pad(args[1], 2, False)
+    wide[7] = pad(args[7], 2, -1)
+    wide[8] = pad(args[8], 1, -1)
+    wide[9] = pad(args[9], 1, 0.0)
+    wide[10] = pad(args[10], 1, -1)
+    wide[11] = pad(args[11], 1, 0.0)
+    outs, grads = [], []
+    for a in (args, wide):
+        X = a[0].cuda().requires_grad_(True)
+        da = _dev(a)
+        da[0] = X
+        out = ingest.gen_pose_leaf_atoms(*da)
+        w = torch.randn(out.shape, generator=torch.Generator().manual_seed(2)).cuda()
+        (out.nan_to_num(0) * w).sum().backward()
+        outs.append(out.detach().cpu())
+        grads.append(X.grad.cpu())
+    assert torch.equal(outs[0].nan_to_num(0), outs[1].nan_to_num(0))
+    assert torch.equal(torch.isnan(outs[0]), torch.isnan(outs[1]))
+    assert torch.allclose(grads[0], grads[1], rtol=1e-6, atol=1e-6)

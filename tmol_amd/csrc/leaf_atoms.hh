@@ -70,88 +70,237 @@ __device__ __forceinline__ bool any_nan(Vec3<Real> v) {
   return v.x != v.x || v.y != v.y || v.z != v.z;
 }
 
-// one thread per (pose, block, atom slot): given atoms are copied, missing
-// ones built (NaN when no complete ancestor set exists)
+// Launch shape of all three kernels: one wave per workgroup (its barriers
+// are wave barriers, reached by the whole workgroup whatever the wave's
+// trip count).  Wave w owns the LEAF_BPW blocks q = LEAF_BPW w ... of the flat
+// (pose, block) grid: it loads their types, keeps the real ones (a ballot)
+// and takes them G = 64 / Amax at a time (Amax <= 64: lane = g * Amax + atom
+// slot), or one at a time with lanes striding over the atom slots (Amax >
+// 64).  Padding blocks cost one load per 64.
+constexpr int LEAF_WAVE = 64;
+// blocks whose types one wave loads: fewer than 64 keeps one or two passes
+// per wave, so the dependent loads of many waves overlap
+constexpr int LEAF_BPW = 16;
+constexpr int LEAF_MAX_ATOMS = 2048;  // larger block types: ENOTSUP
+
+struct LeafPass {
+  int g;       // this lane's block of the pass (-1: none)
+  int q;       // its flat (pose, block) index
+  int bt;      // its block type
+  int at0;     // first atom slot of this lane
+  int stride;  // atom-slot stride (past Amax when one slot per lane)
+};
+
+__device__ __forceinline__ int leaf_blocks_per_pass(const LeafArgs& a) {
+  return a.Amax <= LEAF_WAVE ? LEAF_WAVE / a.Amax : 1;
+}
+
+// this lane's block in the pass that takes the lowest blocks of `real`
+__device__ __forceinline__ LeafPass leaf_pass(const LeafArgs& a, uint64_t real, int64_t q0,
+                                              int bt_lane, int lane) {
+  const int G = leaf_blocks_per_pass(a);
+  LeafPass r;
+  const bool small = a.Amax <= LEAF_WAVE;
+  const int g = small ? lane / a.Amax : 0;
+  r.at0 = small ? lane - g * a.Amax : lane;
+  r.stride = small ? (1 << 30) : LEAF_WAVE;  // small: one slot per lane
+  uint64_t m = real;
+  for (int k = 0; k < g && m; ++k) m &= m - 1;
+  const int pos = (g < G && m) ? __ffsll((unsigned long long)m) - 1 : -1;
+  const int bt = __shfl(bt_lane, pos < 0 ? 0 : pos, LEAF_WAVE);
+  r.g = pos < 0 ? -1 : g;
+  r.q = pos < 0 ? -1 : int(q0 + pos);
+  r.bt = pos < 0 ? -1 : bt;
+  return r;
+}
+
+__device__ __forceinline__ uint64_t leaf_drop(const LeafArgs& a, uint64_t real) {
+  const int G = leaf_blocks_per_pass(a);
+  for (int k = 0; k < G && real; ++k) real &= real - 1;
+  return real;
+}
+
+// given atoms are copied, missing ones built (NaN when no complete ancestor
+// set exists)
 template <typename Real>
-__global__ __launch_bounds__(256) void leaf_atoms_kernel(LeafArgs a, const Real* orig,
-                                                         Real* out) {
-  const int64_t n = int64_t(a.P) * a.B * a.Amax;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
-       t += int64_t(gridDim.x) * blockDim.x) {
-    const int p = int(t / (int64_t(a.B) * a.Amax));
-    const int64_t r = t - int64_t(p) * a.B * a.Amax;
-    const int b = int(r / a.Amax), at = int(r - int64_t(b) * a.Amax);
-    const int bt = a.block_type[p * a.B + b];
-    if (bt < 0 || at >= a.n_atoms[bt]) continue;
-    const int64_t dst = int64_t(p) * a.NA + a.block_offset[p * a.B + b] + at;
-    Real* o = out + 3 * dst;
-    if (!a.block_missing[t]) {
-      o[0] = orig[3 * dst];
-      o[1] = orig[3 * dst + 1];
-      o[2] = orig[3 * dst + 2];
-      continue;
-    }
-    int anc[3];
-    Real g[3];
-    Vec3<Real> W = {Real(NAN), Real(NAN), Real(NAN)};
-    if (leaf_ancestors(a, p, b, bt, at, anc, g)) {
-      const Real* base = orig + 3 * int64_t(p) * a.NA;
-      const Vec3<Real> A = v3(base + 3 * anc[0]), B = v3(base + 3 * anc[1]),
-                       C = v3(base + 3 * anc[2]);
-      if (!any_nan(A) && !any_nan(B) && !any_nan(C)) {
-        Vec3<Real> e0, e1, e2;
-        water_frame(A, B, C, e0, e1, e2);
-        const WaterCoef<Real> c = water_coef(g[2], g[1], g[0]);
-        W = {A.x + c.k0 * e0.x + c.k1 * e1.x + c.k2 * e2.x,
-             A.y + c.k0 * e0.y + c.k1 * e1.y + c.k2 * e2.y,
-             A.z + c.k0 * e0.z + c.k1 * e1.z + c.k2 * e2.z};
+__global__ __launch_bounds__(LEAF_WAVE) void leaf_atoms_kernel(LeafArgs a, const Real* orig,
+                                                               Real* out) {
+  const int lane = threadIdx.x;
+  const int64_t nblk = int64_t(a.P) * a.B;
+  for (int64_t w = blockIdx.x; w * LEAF_BPW < nblk; w += gridDim.x) {
+    const int64_t q0 = w * LEAF_BPW;
+    const int bt_lane = lane < LEAF_BPW && q0 + lane < nblk ? a.block_type[q0 + lane] : -1;
+    uint64_t real = __ballot(bt_lane >= 0);
+    while (real) {
+      const LeafPass r = leaf_pass(a, real, q0, bt_lane, lane);
+      real = leaf_drop(a, real);
+      if (r.g < 0) continue;
+      const int p = r.q / a.B, b = r.q - p * a.B;
+      const int n_at = a.n_atoms[r.bt];
+      const int64_t off = int64_t(p) * a.NA + a.block_offset[r.q];
+      for (int at = r.at0; at < n_at; at += r.stride) {
+        const int64_t dst = off + at;
+        Real* o = out + 3 * dst;
+        if (!a.block_missing[int64_t(r.q) * a.Amax + at]) {
+          o[0] = orig[3 * dst];
+          o[1] = orig[3 * dst + 1];
+          o[2] = orig[3 * dst + 2];
+          continue;
+        }
+        int anc[3];
+        Real g[3];
+        Vec3<Real> W = {Real(NAN), Real(NAN), Real(NAN)};
+        if (leaf_ancestors(a, p, b, r.bt, at, anc, g)) {
+          const Real* base = orig + 3 * int64_t(p) * a.NA;
+          const Vec3<Real> A = v3(base + 3 * anc[0]), B = v3(base + 3 * anc[1]),
+                           C = v3(base + 3 * anc[2]);
+          if (!any_nan(A) && !any_nan(B) && !any_nan(C)) {
+            Vec3<Real> e0, e1, e2;
+            water_frame(A, B, C, e0, e1, e2);
+            const WaterCoef<Real> c = water_coef(g[2], g[1], g[0]);
+            W = {A.x + c.k0 * e0.x + c.k1 * e1.x + c.k2 * e2.x,
+                 A.y + c.k0 * e0.y + c.k1 * e1.y + c.k2 * e2.y,
+                 A.z + c.k0 * e0.z + c.k1 * e1.z + c.k2 * e2.z};
+          }
+        }
+        o[0] = W.x;
+        o[1] = W.y;
+        o[2] = W.z;
       }
     }
-    o[0] = W.x;
-    o[1] = W.y;
-    o[2] = W.z;
   }
 }
 
-// backward: the gradient of a given atom passes through; that of a built
-// atom goes to its three ancestors (atomics: an ancestor serves several
-// built atoms and is itself given)
+// the ancestors of a built atom (pose-local indices) with, for each, its
+// atom slot in the same block or -1 when named across a connection; false
+// when no complete ancestor set exists
 template <typename Real>
-__global__ __launch_bounds__(256) void leaf_atoms_backward_kernel(LeafArgs a, const Real* orig,
-                                                                  const Real* dnew,
-                                                                  Real* dorig) {
-  const int64_t n = int64_t(a.P) * a.B * a.Amax;
-  for (int64_t t = int64_t(blockIdx.x) * blockDim.x + threadIdx.x; t < n;
-       t += int64_t(gridDim.x) * blockDim.x) {
-    const int p = int(t / (int64_t(a.B) * a.Amax));
-    const int64_t r = t - int64_t(p) * a.B * a.Amax;
-    const int b = int(r / a.Amax), at = int(r - int64_t(b) * a.Amax);
-    const int bt = a.block_type[p * a.B + b];
-    if (bt < 0 || at >= a.n_atoms[bt]) continue;
-    const int64_t dst = int64_t(p) * a.NA + a.block_offset[p * a.B + b] + at;
-    const Vec3<Real> gw = v3(dnew + 3 * dst);
-    if (!a.block_missing[t]) {
-      atomicAdd(&dorig[3 * dst], gw.x);
-      atomicAdd(&dorig[3 * dst + 1], gw.y);
-      atomicAdd(&dorig[3 * dst + 2], gw.z);
-      continue;
-    }
-    int anc[3];
-    Real g[3];
-    if (!leaf_ancestors(a, p, b, bt, at, anc, g)) continue;
-    const int64_t base = int64_t(p) * a.NA;
-    const Vec3<Real> A = v3(orig + 3 * (base + anc[0])), B = v3(orig + 3 * (base + anc[1])),
-                     C = v3(orig + 3 * (base + anc[2]));
-    if (any_nan(A) || any_nan(B) || any_nan(C)) continue;
-    Vec3<Real> gA, gB, gC;
-    build_coord_vjp(A, B, C, water_coef(g[2], g[1], g[0]), gw, gA, gB, gC);
-    const Vec3<Real> gs[3] = {gA, gB, gC};
-#pragma unroll
+__device__ __forceinline__ bool leaf_ancestor_slots(const LeafArgs& a, int p, int b, int bt,
+                                                    int at, int* anc, int* slot, Real* g) {
+  const int64_t row = int64_t(bt) * a.Amax + at;
+  const int32_t* sets[2] = {a.anc + row * 9, a.anc_backup + row * 9};
+  const Real* geoms[2] = {(const Real*)a.geom + row * 3, (const Real*)a.geom_backup + row * 3};
+  for (int k = 0; k < 2; ++k) {
+    bool ok = true;
     for (int j = 0; j < 3; ++j) {
-      Real* d = dorig + 3 * (base + anc[j]);
-      atomicAdd(&d[0], gs[j].x);
-      atomicAdd(&d[1], gs[j].y);
-      atomicAdd(&d[2], gs[j].z);
+      const int32_t* u = sets[k] + 3 * j;
+      anc[j] = leaf_resolve(a, p, b, u);
+      slot[j] = u[0];
+      ok = ok && anc[j] != -1 && !a.pose_missing[int64_t(p) * a.NA + anc[j]];
+    }
+    if (ok) {
+      g[0] = geoms[k][0];
+      g[1] = geoms[k][1];
+      g[2] = geoms[k][2];
+      return true;
+    }
+  }
+  return false;
+}
+
+// the ancestor shares of the atom at slot `at` of block q if it is built
+// (false otherwise, or without ancestors or with an absent coordinate; with
+// REMOTE_ONLY also when every ancestor is in the same block)
+template <typename Real, bool REMOTE_ONLY>
+__device__ __forceinline__ bool leaf_shares(const LeafArgs& a, const Real* orig,
+                                            const Real* dnew, int q, int bt, int at, int* anc,
+                                            int* slot, Vec3<Real>* gs) {
+  if (!a.block_missing[int64_t(q) * a.Amax + at]) return false;
+  const int p = q / a.B, b = q - p * a.B;
+  Real g[3];
+  if (!leaf_ancestor_slots(a, p, b, bt, at, anc, slot, g)) return false;
+  if (REMOTE_ONLY && slot[0] >= 0 && slot[1] >= 0 && slot[2] >= 0) return false;
+  const int64_t base = int64_t(p) * a.NA;
+  const Vec3<Real> A = v3(orig + 3 * (base + anc[0])), B = v3(orig + 3 * (base + anc[1])),
+                   C = v3(orig + 3 * (base + anc[2]));
+  if (any_nan(A) || any_nan(B) || any_nan(C)) return false;
+  const int64_t dst = base + a.block_offset[q] + at;
+  build_coord_vjp(A, B, C, water_coef(g[2], g[1], g[0]), v3(dnew + 3 * dst), gs[0], gs[1],
+                  gs[2]);
+  return true;
+}
+
+// Backward, first kernel: a pass's blocks sum the shares their built atoms
+// send to ancestors in the same block in LDS (dynamic, 3 x max(64, Amax)
+// per wave), then write each atom once: its own gradient if given, plus the
+// sum.  Shares sent across a connection are added by the second kernel,
+// with global atomics, after the first has written every atom.
+template <typename Real>
+__global__ __launch_bounds__(LEAF_WAVE) void leaf_atoms_backward_local_kernel(
+    LeafArgs a, const Real* orig, const Real* dnew, Real* dorig) {
+  extern __shared__ unsigned char leaf_lds[];
+  Real* acc = (Real*)leaf_lds;
+  const int lane = threadIdx.x;
+  const int W = a.Amax <= LEAF_WAVE ? LEAF_WAVE : a.Amax;  // entries per component
+  const int64_t nblk = int64_t(a.P) * a.B;
+  for (int64_t w = blockIdx.x; w * LEAF_BPW < nblk; w += gridDim.x) {
+    const int64_t q0 = w * LEAF_BPW;
+    const int bt_lane = lane < LEAF_BPW && q0 + lane < nblk ? a.block_type[q0 + lane] : -1;
+    uint64_t real = __ballot(bt_lane >= 0);
+    while (real) {
+      const LeafPass r = leaf_pass(a, real, q0, bt_lane, lane);
+      real = leaf_drop(a, real);
+      for (int i = lane; i < 3 * W; i += LEAF_WAVE) acc[i] = Real(0);
+      __syncthreads();
+      const int n_at = r.g < 0 ? 0 : a.n_atoms[r.bt];
+      const int gbase = r.g < 0 ? 0 : r.g * a.Amax;
+      for (int at = r.at0; at < n_at; at += r.stride) {
+        int anc[3], slot[3];
+        Vec3<Real> gs[3];
+        if (!leaf_shares<Real, false>(a, orig, dnew, r.q, r.bt, at, anc, slot, gs)) continue;
+        for (int j = 0; j < 3; ++j) {
+          if (slot[j] < 0) continue;
+          const int e = gbase + slot[j];
+          atomicAdd(&acc[e], gs[j].x);
+          atomicAdd(&acc[W + e], gs[j].y);
+          atomicAdd(&acc[2 * W + e], gs[j].z);
+        }
+      }
+      __syncthreads();
+      if (r.g >= 0) {
+        const int p = r.q / a.B;
+        const int64_t off = int64_t(p) * a.NA + a.block_offset[r.q];
+        for (int at = r.at0; at < n_at; at += r.stride) {
+          const int64_t dst = off + at;
+          const int e = gbase + at;
+          const bool given = !a.block_missing[int64_t(r.q) * a.Amax + at];
+          dorig[3 * dst] = (given ? dnew[3 * dst] : Real(0)) + acc[e];
+          dorig[3 * dst + 1] = (given ? dnew[3 * dst + 1] : Real(0)) + acc[W + e];
+          dorig[3 * dst + 2] = (given ? dnew[3 * dst + 2] : Real(0)) + acc[2 * W + e];
+        }
+      }
+      __syncthreads();
+    }
+  }
+}
+
+template <typename Real>
+__global__ __launch_bounds__(LEAF_WAVE) void leaf_atoms_backward_remote_kernel(
+    LeafArgs a, const Real* orig, const Real* dnew, Real* dorig) {
+  const int lane = threadIdx.x;
+  const int64_t nblk = int64_t(a.P) * a.B;
+  for (int64_t w = blockIdx.x; w * LEAF_BPW < nblk; w += gridDim.x) {
+    const int64_t q0 = w * LEAF_BPW;
+    const int bt_lane = lane < LEAF_BPW && q0 + lane < nblk ? a.block_type[q0 + lane] : -1;
+    uint64_t real = __ballot(bt_lane >= 0);
+    while (real) {
+      const LeafPass r = leaf_pass(a, real, q0, bt_lane, lane);
+      real = leaf_drop(a, real);
+      if (r.g < 0) continue;
+      const int n_at = a.n_atoms[r.bt];
+      const int64_t base = int64_t(r.q / a.B) * a.NA;
+      for (int at = r.at0; at < n_at; at += r.stride) {
+        int anc[3], slot[3];
+        Vec3<Real> gs[3];
+        if (!leaf_shares<Real, true>(a, orig, dnew, r.q, r.bt, at, anc, slot, gs)) continue;
+        for (int j = 0; j < 3; ++j) {
+          if (slot[j] >= 0) continue;
+          Real* d = dorig + 3 * (base + anc[j]);
+          atomicAdd(&d[0], gs[j].x);
+          atomicAdd(&d[1], gs[j].y);
+          atomicAdd(&d[2], gs[j].z);
+        }
+      }
     }
   }
 }

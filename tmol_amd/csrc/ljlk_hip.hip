@@ -1984,7 +1984,9 @@ static int leaf_args(const tmol_leaf_atoms_args* x, int dtype, LeafArgs& a) {
       !x->bt_n_atoms || !x->bt_atom_downstream_of_conn || !x->bt_atom_ancestors ||
       !x->bt_atom_icoors || !x->bt_atom_ancestors_backup || !x->bt_atom_icoors_backup)
     return TMOL_LJLK_EINVAL;
-  if (int64_t(x->n_poses) * x->max_n_pose_atoms >= (int64_t(1) << 31) / 3)
+  if (int64_t(x->n_poses) * x->max_n_pose_atoms >= (int64_t(1) << 31) / 3 ||
+      int64_t(x->n_poses) * x->max_n_blocks * x->max_n_block_atoms >= (int64_t(1) << 31) ||
+      x->max_n_block_atoms > LEAF_MAX_ATOMS)
     return TMOL_LJLK_ENOTSUP;
   a.P = x->n_poses;
   a.NA = x->max_n_pose_atoms;
@@ -2005,6 +2007,12 @@ static int leaf_args(const tmol_leaf_atoms_args* x, int dtype, LeafArgs& a) {
   return TMOL_LJLK_OK;
 }
 
+// one-wave workgroups over the (pose, block) grid, LEAF_BPW blocks per wave
+static unsigned leaf_grid(const LeafArgs& a) {
+  const int64_t waves = (int64_t(a.P) * a.B + LEAF_BPW - 1) / LEAF_BPW;
+  return (unsigned)std::max<int64_t>(1, std::min<int64_t>(waves, 65536));
+}
+
 int tmol_leaf_atoms_build(const tmol_leaf_atoms_args* x, const void* orig, int32_t dtype,
                           void* out, void* stream) {
   LeafArgs a;
@@ -2012,16 +2020,15 @@ int tmol_leaf_atoms_build(const tmol_leaf_atoms_args* x, const void* orig, int32
   if (v) return v;
   if (!orig || !out) return TMOL_LJLK_EINVAL;
   hipStream_t st = (hipStream_t)stream;
-  const int64_t n = int64_t(a.P) * a.B * a.Amax;
   // atoms of the pose layout beyond any block stay zero, as in the reference
   if (dtype == TMOL_LJLK_F32) {
     fill<float>((float*)out, 3 * int64_t(a.P) * a.NA, st);
-    hipLaunchKernelGGL(leaf_atoms_kernel<float>, dim3(grid_for(n)), dim3(256), 0, st, a,
+    hipLaunchKernelGGL(leaf_atoms_kernel<float>, dim3(leaf_grid(a)), dim3(LEAF_WAVE), 0, st, a,
                        (const float*)orig, (float*)out);
   } else {
     fill<double>((double*)out, 3 * int64_t(a.P) * a.NA, st);
-    hipLaunchKernelGGL(leaf_atoms_kernel<double>, dim3(grid_for(n)), dim3(256), 0, st, a,
-                       (const double*)orig, (double*)out);
+    hipLaunchKernelGGL(leaf_atoms_kernel<double>, dim3(leaf_grid(a)), dim3(LEAF_WAVE), 0, st,
+                       a, (const double*)orig, (double*)out);
   }
   return check(hipGetLastError());
 }
@@ -2033,15 +2040,23 @@ int tmol_leaf_atoms_backward(const tmol_leaf_atoms_args* x, const void* orig, in
   if (v) return v;
   if (!orig || !dnew || !dorig) return TMOL_LJLK_EINVAL;
   hipStream_t st = (hipStream_t)stream;
-  const int64_t n = int64_t(a.P) * a.B * a.Amax;
+  const size_t lds_entries = 3 * size_t(a.Amax <= LEAF_WAVE ? LEAF_WAVE : a.Amax);
   if (dtype == TMOL_LJLK_F32) {
     fill<float>((float*)dorig, 3 * int64_t(a.P) * a.NA, st);
-    hipLaunchKernelGGL(leaf_atoms_backward_kernel<float>, dim3(grid_for(n)), dim3(256), 0, st,
-                       a, (const float*)orig, (const float*)dnew, (float*)dorig);
+    hipLaunchKernelGGL(leaf_atoms_backward_local_kernel<float>, dim3(leaf_grid(a)),
+                       dim3(LEAF_WAVE), lds_entries * sizeof(float), st, a, (const float*)orig,
+                       (const float*)dnew, (float*)dorig);
+    hipLaunchKernelGGL(leaf_atoms_backward_remote_kernel<float>, dim3(leaf_grid(a)),
+                       dim3(LEAF_WAVE), 0, st, a, (const float*)orig, (const float*)dnew,
+                       (float*)dorig);
   } else {
     fill<double>((double*)dorig, 3 * int64_t(a.P) * a.NA, st);
-    hipLaunchKernelGGL(leaf_atoms_backward_kernel<double>, dim3(grid_for(n)), dim3(256), 0, st,
-                       a, (const double*)orig, (const double*)dnew, (double*)dorig);
+    hipLaunchKernelGGL(leaf_atoms_backward_local_kernel<double>, dim3(leaf_grid(a)),
+                       dim3(LEAF_WAVE), lds_entries * sizeof(double), st, a,
+                       (const double*)orig, (const double*)dnew, (double*)dorig);
+    hipLaunchKernelGGL(leaf_atoms_backward_remote_kernel<double>, dim3(leaf_grid(a)),
+                       dim3(LEAF_WAVE), 0, st, a, (const double*)orig, (const double*)dnew,
+                       (double*)dorig);
   }
   return check(hipGetLastError());
 }

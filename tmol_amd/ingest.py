@@ -252,11 +252,12 @@ def _validate_leaf_tables(k, P, NA, B, Amax, C):
                 bad.append("inter_block_connections outside [0, max_n_blocks) x [0, max_n_conn)")
             elif bool(((ob >= 0) & (bt.gather(1, ob.clamp(min=0).view(P, -1).long()).view(ob.shape) < 0)).any()):
                 bad.append("inter_block_connections naming a padding block")
-        if bool(((k["down"] < -1) | (k["down"] >= Amax)).any()):
-            bad.append("atom_downstream_of_conn outside [-1, max_n_block_atoms)")
+        n_of_type = k["bt_n"].view(T, 1, 1)
+        if bool(((k["down"] < -1) | (k["down"] >= n_of_type)).any()):
+            bad.append("atom_downstream_of_conn outside [-1, n_atoms of its block type)")
         for key in ("anc", "anc_b"):
             u = k[key]
-            if bool(((u[..., 0] < -1) | (u[..., 0] >= Amax)).any()) or \
+            if bool(((u[..., 0] < -1) | (u[..., 0] >= n_of_type.view(T, 1, 1))).any()) or \
                     bool(((u[..., 1] < -1) | (u[..., 1] >= C)).any()) or \
                     bool(((u[..., 1] >= 0) & ((u[..., 2] < 0) | (u[..., 2] >= Amax))).any()):
                 bad.append("%s outside the block-type tables" % key)

@@ -15,10 +15,23 @@ given, the reference's CPU op on the same batch.
 
 import argparse
 import json
+import os
 import sys
 import time
 
 HBM_PEAK_GBS = 8000.0
+PROFILE = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "profiles",
+                       "leaf_atoms_ragged512_counters.json")
+
+
+def _counters():
+    """The committed rocprofv3 counter summary of the leaf kernels on this
+    batch (HBM bytes per launch from FETCH_SIZE / WRITE_SIZE), or None."""
+    try:
+        with open(PROFILE) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
 
 
 def strip_hydrogens(poses, db=None):
@@ -86,6 +99,9 @@ def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=None, log=pr
     ms = (time.perf_counter() - t0) * 1000.0 / steps
     alg_bytes = n_slots * 1 + n_real * 12 + (n_real - n_built) * 12 + n_built * (36 + 3)
     achieved = alg_bytes / (fwd_ms * 1e-3) / 1e9
+    prof = _counters()
+    kern = (prof or {}).get("kernels", {})
+    fwd_prof = kern.get("leaf_atoms_kernel<float>", {})
     return {
         "workload": "ragged_512x50_1200 hydrogens rebuilt",
         "poses": ps.n_poses,
@@ -105,9 +121,13 @@ def run(device, poses=None, steps=20, warmup=3, kernel_reps=10, cpu=None, log=pr
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
-            "kernel": "leaf_atoms_kernel<float> (one thread per atom slot)",
+            "traffic": fwd_prof.get("hbm_bytes_per_launch"),
+            "kernel": "leaf_atoms_kernel<float> (one-wave workgroups, 16 blocks each)",
             "algorithmic_bytes_per_launch": alg_bytes,
+            "backward_traffic": {k: v.get("hbm_bytes_per_launch") for k, v in kern.items()
+                                 if "backward" in k} or None,
+            "traffic_profile": ({k: prof.get(k) for k in ("commit", "copy_calibration")}
+                                if prof else None),
         },
         "cpu_baseline": cpu(args) if cpu is not None else None,
     }
