@@ -191,3 +191,20 @@ def test_wide_block_type_tables(co, amax):
     assert torch.equal(outs[0].nan_to_num(0), outs[1].nan_to_num(0))
     assert torch.equal(torch.isnan(outs[0]), torch.isnan(outs[1]))
     assert torch.allclose(grads[0], grads[1], rtol=1e-6, atol=1e-6)
+
+
+def test_pose_stack_from_pdb(co):
+    """The top-level entry from a file name with a residue range: the
+    interior residues' atoms (hydrogens included, as given) equal the
+    deposited ones and the new termini are built and finite."""
+    print("pose_stack_from_pdb 1ubq 10..40", flush=True)
+    ps = ingest.pose_stack_from_pdb(os.path.join(GOLDEN, "1ubq.pdb"), "cuda",
+                                    residue_start=10, residue_end=40, co=co)
+    assert ps.n_poses == 1 and ps.max_n_blocks == 30
+    assert not torch.isnan(ps.coords).any()
+    rts = ps.poses[0].residue_types
+    assert rts[0].is_nterm and rts[-1].is_cterm
+    full = ingest.pose_stack_from_layouts(ingest.block_layout_from_canonical_form(
+        co, *ingest.canonical_form_from_pdb(_pdb("1ubq.pdb"), co)))
+    for b in range(1, 29):
+        assert numpy.allclose(ps.poses[0].coords[b], full.poses[0].coords[b + 10], atol=1e-4)

@@ -212,3 +212,17 @@ def test_table_validation_rejects_bad_indices(co):
         k2[key] = edit(k[key].clone())
         with pytest.raises(LJLKLibraryError):
             ingest._validate_leaf_tables(k2, P, NA, B, Amax, C)
+
+
+def test_canonical_form_residue_range(co):
+    """residue_start / residue_end keep residues start .. end-1 in file
+    order; the range's first and last residues become the termini."""
+    full = ingest.canonical_form_from_pdb(_pdb("1ubq.pdb"), co)
+    sub = ingest.canonical_form_from_pdb(_pdb("1ubq.pdb"), co, residue_start=10, residue_end=40)
+    assert sub[1].shape == (1, 30)
+    assert torch.equal(sub[1][0], full[1][0, 10:40])
+    assert torch.equal(sub[2][0].nan_to_num(0), full[2][0, 10:40].nan_to_num(0))
+    lay = ingest.block_layout_from_canonical_form(co, *sub)[0]
+    assert lay["rts"][0].is_nterm and lay["rts"][-1].is_cterm
+    with pytest.raises(ValueError, match="no residues"):
+        ingest.canonical_form_from_pdb(_pdb("1ubq.pdb"), co, residue_start=80)

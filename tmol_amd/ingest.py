@@ -375,12 +375,19 @@ class CanonicalOrdering:
         self.max_n_canonical_atoms = max(len(a) for a in self.atoms)
 
 
-def canonical_form_from_pdb(pdb_text: str, co: Optional[CanonicalOrdering] = None):
+def canonical_form_from_pdb(pdb_text: str, co: Optional[CanonicalOrdering] = None,
+                            residue_start: Optional[int] = None,
+                            residue_end: Optional[int] = None):
     """(chain_id [1, L] int32, res_types [1, L] int32, coords [1, L, A, 3]
     float32 with NaN for absent atoms) from the ATOM records of a PDB
-    (canonical_ordering.py:464-614); unknown residues and atoms raise."""
+    (canonical_ordering.py:464-614); unknown residues and atoms raise.
+    ``residue_start`` / ``residue_end`` keep residues start .. end-1 of the
+    file, counted in file order (canonical_ordering.py:490-517)."""
     co = co or CanonicalOrdering()
     recs = parse_pdb_residues(pdb_text)
+    recs = recs[(residue_start or 0): (len(recs) if residue_end is None else residue_end)]
+    if not recs:
+        raise ValueError("no residues in the requested range")
     L, A = len(recs), co.max_n_canonical_atoms
     chain_id = numpy.zeros((1, L), dtype=numpy.int32)
     res_types = numpy.full((1, L), -1, dtype=numpy.int32)
@@ -398,6 +405,26 @@ def canonical_form_from_pdb(pdb_text: str, co: Optional[CanonicalOrdering] = Non
                 raise ValueError("atom %s not in %s (%s %d)" % (nm, res3, chain, resseq))
             coords[0, i, co.atom_index[t][nm]] = x
     return (torch.tensor(chain_id), torch.tensor(res_types), torch.tensor(coords))
+
+
+def pose_stack_from_pdb(pdb_lines_or_fname, device="cuda", *, residue_start=None,
+                        residue_end=None, res_not_connected=None, dtype=torch.float32,
+                        co: Optional[CanonicalOrdering] = None) -> PoseStack:
+    """A PoseStack from the text of a PDB or its file name, missing leaf
+    atoms built on ``device`` (tmol/io/__init__.py:24-60): the canonical
+    form of residues start .. end-1, then pose_stack_from_canonical_form."""
+    import os
+
+    text = pdb_lines_or_fname
+    if isinstance(text, (list, tuple)):
+        text = "\n".join(text)
+    elif "\n" not in text and os.path.exists(text):
+        with open(text) as f:
+            text = f.read()
+    co = co or CanonicalOrdering()
+    ch, rt, xyz = canonical_form_from_pdb(text, co, residue_start, residue_end)
+    return pose_stack_from_canonical_form(co, ch, rt, xyz, res_not_connected, device=device,
+                                          dtype=dtype)
 
 
 def _disulfides(co, res_types, coords):
