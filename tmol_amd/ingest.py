@@ -226,41 +226,53 @@ def _leaf_lib():
     return L
 
 
+def _failed(checks):
+    """The messages of the checks whose condition holds anywhere; the
+    conditions are reduced on their device and read back together (one
+    synchronization)."""
+    if not checks:
+        return []
+    flags = torch.stack([c.any() for _, c in checks]).tolist()
+    return [m for (m, _), f in zip(checks, flags) if f]
+
+
 def _validate_leaf_tables(k, P, NA, B, Amax, C):
     """Every index the kernels follow stays inside its table: block types,
     block offsets plus sizes inside the pose, connection partners, ancestor
-    ids and the downstream atoms."""
+    ids and the downstream atoms.  Two read-backs: the range checks, then
+    the checks that index with the values the first ones cleared."""
     T = int(k["bt_n"].shape[0])
-    bad = []
     bt = k["btype"]
-    if bool(((bt < -1) | (bt >= T)).any()):
-        bad.append("block_type outside [-1, n_block_types)")
     if tuple(k["down"].shape) != (T, C, Amax) or tuple(k["anc"].shape) != (T, Amax, 3, 3) \
             or tuple(k["anc_b"].shape) != (T, Amax, 3, 3) \
             or tuple(k["geom"].shape) != (T, Amax, 3) or tuple(k["geom_b"].shape) != (T, Amax, 3):
-        bad.append("block-type table shapes")
-    if bool(((k["bt_n"] < 0) | (k["bt_n"] > Amax)).any()):
-        bad.append("bt_n_atoms outside [0, max_n_block_atoms]")
+        raise ops.LJLKLibraryError("leaf-atom op: block-type table shapes")
+    n_of_type = k["bt_n"].view(T, 1, 1)
+    checks = [("block_type outside [-1, n_block_types)", (bt < -1) | (bt >= T)),
+              ("bt_n_atoms outside [0, max_n_block_atoms]", (k["bt_n"] < 0) | (k["bt_n"] > Amax)),
+              ("atom_downstream_of_conn outside [-1, n_atoms of its block type)",
+               (k["down"] < -1) | (k["down"] >= n_of_type))]
+    for key in ("anc", "anc_b"):
+        u = k[key]
+        checks.append(("%s outside the block-type tables" % key,
+                       (u[..., 0] < -1) | (u[..., 0] >= n_of_type) | (u[..., 1] < -1)
+                       | (u[..., 1] >= C) | ((u[..., 1] >= 0) & ((u[..., 2] < 0) | (u[..., 2] >= Amax)))))
+    if C > 0:
+        ob, oc = k["irc"][..., 0], k["irc"][..., 1]
+        checks.append(("inter_block_connections outside [0, max_n_blocks) x [0, max_n_conn)",
+                       (ob < -1) | (ob >= B) | ((ob >= 0) & ((oc < 0) | (oc >= C)))))
+    bad = _failed(checks)
     if not bad:
         real = bt >= 0
         n = torch.where(real, k["bt_n"][bt.clamp(min=0).long()], torch.zeros_like(bt))
-        if bool((real & ((k["boff"] < 0) | (k["boff"] + n > NA))).any()):
-            bad.append("block_coord_offset + n_atoms outside the pose")
+        checks = [("block_coord_offset + n_atoms outside the pose",
+                   real & ((k["boff"] < 0) | (k["boff"] + n > NA)))]
         if C > 0:
-            ob, oc = k["irc"][..., 0], k["irc"][..., 1]
-            if bool(((ob < -1) | (ob >= B)).any()) or bool(((ob >= 0) & ((oc < 0) | (oc >= C))).any()):
-                bad.append("inter_block_connections outside [0, max_n_blocks) x [0, max_n_conn)")
-            elif bool(((ob >= 0) & (bt.gather(1, ob.clamp(min=0).view(P, -1).long()).view(ob.shape) < 0)).any()):
-                bad.append("inter_block_connections naming a padding block")
-        n_of_type = k["bt_n"].view(T, 1, 1)
-        if bool(((k["down"] < -1) | (k["down"] >= n_of_type)).any()):
-            bad.append("atom_downstream_of_conn outside [-1, n_atoms of its block type)")
-        for key in ("anc", "anc_b"):
-            u = k[key]
-            if bool(((u[..., 0] < -1) | (u[..., 0] >= n_of_type.view(T, 1, 1))).any()) or \
-                    bool(((u[..., 1] < -1) | (u[..., 1] >= C)).any()) or \
-                    bool(((u[..., 1] >= 0) & ((u[..., 2] < 0) | (u[..., 2] >= Amax))).any()):
-                bad.append("%s outside the block-type tables" % key)
+            ob = k["irc"][..., 0]
+            partner = bt.gather(1, ob.clamp(min=0).view(P, -1).long()).view(ob.shape)
+            checks.append(("inter_block_connections naming a padding block",
+                           (ob >= 0) & (partner < 0)))
+        bad = _failed(checks)
     if bad:
         raise ops.LJLKLibraryError("leaf-atom op: " + "; ".join(bad))
 
