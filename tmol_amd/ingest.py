@@ -516,9 +516,12 @@ def block_layout_from_canonical_form(co, chain_id, res_types, coords, res_not_co
 def pose_stack_from_canonical_form(co, chain_id, res_types, coords, res_not_connected=None,
                                    device="cuda", dtype=torch.float32):
     """A PoseStack from a canonical form, the missing leaf atoms built on
-    the device (the reference's pose_stack_from_canonical_form, for the
-    canonical amino acids).  Missing non-leaf atoms raise ValueError; an
-    atom whose ancestors are all missing stays NaN."""
+    ``device`` (the reference's pose_stack_from_canonical_form, for the
+    canonical amino acids).  The stack's tensors are returned on the host,
+    coordinates included (the op argument builders move them to the device
+    of the call); the reference returns its stack on the device.  Missing
+    non-leaf atoms raise ValueError; an atom whose ancestors are all missing
+    stays NaN."""
     layouts = block_layout_from_canonical_form(co, chain_id, res_types, coords,
                                                res_not_connected)
     ps = pose_stack_from_layouts(layouts, dtype)
