@@ -102,13 +102,16 @@ def test_ingest_heavy_atoms_then_score(co, pdb):
             if name.startswith("H") or (len(name) > 1 and name[0].isdigit() and name[1] == "H"):
                 heavy[0, r, k] = float("nan")
     ps = ingest.pose_stack_from_canonical_form(co, ch, rt, heavy, device="cuda")
+    assert ps.coords.is_cuda and ps.block_type_ind.is_cuda
     assert ps.coords.shape == full.coords.shape
-    dev = (ps.coords[0] - full.coords[0]).norm(dim=-1)
+    dev = (ps.coords[0].cpu() - full.coords[0]).norm(dim=-1)
     assert not torch.isnan(ps.coords).any()
     assert float(dev.max()) < 1e-1 and float(dev.mean()) < 1e-3
-    term = LJLKEnergyTerm()
-    e_full, _ = ljlk_pose_scores(*_dev(pose_op_args(full, term)))
-    e_built, _ = ljlk_pose_scores(*_dev(pose_op_args(ps, term)))
+    from tmol_amd.pose import PoseStack
+
+    term = LJLKEnergyTerm(device="cuda")
+    e_full, _ = ljlk_pose_scores(*pose_op_args(PoseStack(full.poses, device="cuda"), term))
+    e_built, _ = ljlk_pose_scores(*pose_op_args(ps, term))
     assert torch.allclose(e_built.cpu(), e_full.cpu(), rtol=2e-3, atol=2e-2)
 
 

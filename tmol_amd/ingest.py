@@ -517,11 +517,9 @@ def pose_stack_from_canonical_form(co, chain_id, res_types, coords, res_not_conn
                                    device="cuda", dtype=torch.float32):
     """A PoseStack from a canonical form, the missing leaf atoms built on
     ``device`` (the reference's pose_stack_from_canonical_form, for the
-    canonical amino acids).  The stack's tensors are returned on the host,
-    coordinates included (the op argument builders move them to the device
-    of the call); the reference returns its stack on the device.  Missing
-    non-leaf atoms raise ValueError; an atom whose ancestors are all missing
-    stays NaN."""
+    canonical amino acids), returned on ``device`` as the reference's is.
+    Missing non-leaf atoms raise ValueError; an atom whose ancestors are all
+    missing stays NaN."""
     layouts = block_layout_from_canonical_form(co, chain_id, res_types, coords,
                                                res_not_connected)
     ps = pose_stack_from_layouts(layouts, dtype)
@@ -541,7 +539,9 @@ def pose_stack_from_canonical_form(co, chain_id, res_types, coords, res_not_conn
         for b, rt in enumerate(pose.residue_types):
             pose.coords[b][:] = ps.coords[k, off: off + rt.n_atoms].double().numpy()
             off += rt.n_atoms
-    return ps
+    if dev.type == "cpu":
+        return ps
+    return PoseStack(ps.poses, device=dev, dtype=dtype)
 
 
 def pose_stack_from_layouts(layouts, dtype=torch.float32) -> PoseStack:
